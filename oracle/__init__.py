@@ -1,0 +1,285 @@
+"""ctypes front-end of the oracle (CPU restatement of nbodysim's step).
+
+Test infrastructure only: imported by tests/, by __graft_entry__.smoke() and
+by bench.py's cpu_baseline leg, never by the product package.  Each wrapper
+names the reference code it restates (see nbs_oracle.cpp for file:line).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(_HERE, "_build", "libnbs_oracle.so")
+REF_DIR = os.path.join(_HERE, "_ref")
+
+GEN_TYPES = {"cube": 0, "sphere": 1, "shell": 2}
+DISTS = {"linear": 0, "exp": 1, "normal": 2}
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_ip = ctypes.POINTER(ctypes.c_int)
+_lp = ctypes.POINTER(ctypes.c_long)
+
+
+class GenSettings(ctypes.Structure):
+    _fields_ = [("gen_type", ctypes.c_int), ("mass_dist", ctypes.c_int),
+                ("vel_dist", ctypes.c_int), ("pad", ctypes.c_int),
+                ("size", ctypes.c_double), ("min_mass", ctypes.c_double),
+                ("max_mass", ctypes.c_double), ("min_vel", ctypes.c_double),
+                ("max_vel", ctypes.c_double), ("r_sphere", ctypes.c_double),
+                ("rotation_magnitude", ctypes.c_double),
+                ("rot_x", ctypes.c_double), ("rot_y", ctypes.c_double),
+                ("rot_z", ctypes.c_double), ("scale_x", ctypes.c_double),
+                ("scale_y", ctypes.c_double), ("scale_z", ctypes.c_double)]
+
+
+def _p(a, t=_dp):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+_lib = None
+
+
+def lib():
+    """Loads the oracle library, building it first if it is missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            import subprocess
+            subprocess.check_call(["sh", os.path.join(_HERE, "build_oracle.sh")])
+        L = ctypes.CDLL(ORACLE_SO)
+        L.orc_generate.restype = ctypes.c_int
+        L.orc_generate.argtypes = [ctypes.c_int, ctypes.POINTER(GenSettings),
+                                   ctypes.c_ulong, ctypes.c_size_t, _dp, _dp, _dp]
+        L.orc_minstd_raw.argtypes = [ctypes.c_ulong, ctypes.c_int, ctypes.c_size_t, _u64p]
+        L.orc_uniform_raw.argtypes = [ctypes.c_int, ctypes.c_ulong, ctypes.c_double,
+                                      ctypes.c_double, ctypes.c_size_t, _dp]
+        L.orc_root.argtypes = [ctypes.c_int, ctypes.c_size_t, _dp, _dp, _dp]
+        L.orc_keys.argtypes = [ctypes.c_int, ctypes.c_size_t, _dp, _dp, ctypes.c_double,
+                               _u64p, _u64p]
+        L.orc_tree_build.restype = ctypes.c_void_p
+        L.orc_tree_build.argtypes = [ctypes.c_int, ctypes.c_size_t, _dp, _dp, ctypes.c_int,
+                                     _dp, ctypes.c_double]
+        L.orc_tree_free.argtypes = [ctypes.c_void_p]
+        L.orc_tree_size.restype = ctypes.c_size_t
+        L.orc_tree_size.argtypes = [ctypes.c_void_p]
+        L.orc_tree_max_depth.restype = ctypes.c_int
+        L.orc_tree_max_depth.argtypes = [ctypes.c_void_p]
+        L.orc_tree_dump.argtypes = [ctypes.c_void_p, _ip, _u64p, _u64p, _ip, _i64p, _i64p,
+                                    _dp, _dp, _dp, _dp, _ip]
+        L.orc_tree_forces.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int,
+                                      ctypes.c_int, _dp, _i64p]
+        L.orc_tree_forces_subset.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int,
+                                             ctypes.c_int, ctypes.c_size_t, _i64p, _dp, _i64p]
+        L.orc_run.restype = ctypes.c_int
+        L.orc_run.argtypes = [ctypes.c_int, ctypes.c_size_t, _dp, _dp, _dp, _dp,
+                              ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                              ctypes.c_int]
+        L.orc_direct.argtypes = [ctypes.c_size_t, _dp, _dp, ctypes.c_int, ctypes.c_size_t,
+                                 _i64p, ctypes.c_int, _dp]
+        _lib = L
+    return _lib
+
+
+def prec_code(prec):
+    return {"f64": 0, "double": 0, "f32": 1, "float": 1}[prec]
+
+
+def generate(n, gen_type="cube", seed=0xDEADBEEF, prec="f64", size=512.0,
+             min_mass=5e10, max_mass=5e11, min_vel=0.0, max_vel=0.0, r_sphere=100.0,
+             rotation_magnitude=0.1, rotation_vector=(0.0, 0.0, 1.0),
+             scale=(1.0, 1.0, 1.0), mass_dist="linear", vel_dist="linear"):
+    """IC generation of src/main.cpp:184-275 (engine seeded with `seed`).
+
+    Returns (pos[n,3], vel[n,3], mass[n]) as float64 (float values widened
+    exactly when prec is f32).
+    """
+    s = GenSettings(GEN_TYPES[gen_type], DISTS[mass_dist], DISTS[vel_dist], 0,
+                    size, min_mass, max_mass, min_vel, max_vel, r_sphere,
+                    rotation_magnitude, *rotation_vector, *scale)
+    pos = np.zeros((n, 3))
+    vel = np.zeros((n, 3))
+    mass = np.zeros(n)
+    rc = lib().orc_generate(prec_code(prec), ctypes.byref(s), seed, n, _p(pos), _p(vel),
+                            _p(mass))
+    if rc != 0:
+        raise ValueError("orc_generate failed: %d" % rc)
+    return pos, vel, mass
+
+
+def minstd_raw(count, seed=None):
+    out = np.zeros(count, dtype=np.uint64)
+    lib().orc_minstd_raw(seed or 0, 1 if seed is None else 0, count, _p(out, _u64p))
+    return out
+
+
+def uniform_raw(count, a, b, seed, prec="f64"):
+    out = np.zeros(count)
+    lib().orc_uniform_raw(prec_code(prec), seed, a, b, count, _p(out))
+    return out
+
+
+def root(pos, prec="f64"):
+    """gen_root's centre and side (src/main.cpp:887-921)."""
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    c = np.zeros(3)
+    s = np.zeros(1)
+    lib().orc_root(prec_code(prec), len(pos), _p(pos), _p(c), _p(s))
+    return c, float(s[0])
+
+
+def keys(pos, center, side, prec="f64"):
+    """42-level child-index paths by the reference's centre chain."""
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    c = np.ascontiguousarray(center, dtype=np.float64)
+    hi = np.zeros(len(pos), dtype=np.uint64)
+    lo = np.zeros(len(pos), dtype=np.uint64)
+    lib().orc_keys(prec_code(prec), len(pos), _p(pos), _p(c), side, _p(hi, _u64p),
+                   _p(lo, _u64p))
+    return hi, lo
+
+
+class Tree:
+    """The reference's one-particle-per-leaf octree with mass and COM."""
+
+    def __init__(self, pos, mass, prec="f64", center=None, side=None):
+        self.pos = np.ascontiguousarray(pos, dtype=np.float64)
+        self.mass = np.ascontiguousarray(mass, dtype=np.float64)
+        mode = 0 if center is None else 1
+        c = np.ascontiguousarray(center if center is not None else np.zeros(3),
+                                 dtype=np.float64)
+        h = lib().orc_tree_build(prec_code(prec), len(self.pos), _p(self.pos),
+                                 _p(self.mass), mode, _p(c), side or 0.0)
+        if not h:
+            raise ValueError("tree overflow: coincident particles")
+        self.h = ctypes.c_void_p(h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_tree_free(self.h)
+            self.h = None
+
+    def max_depth(self):
+        return lib().orc_tree_max_depth(self.h)
+
+    def dump(self):
+        """Pre-order node list (children 0..7) as a dict of arrays."""
+        m = lib().orc_tree_size(self.h)
+        n = len(self.pos)
+        d = dict(depth=np.zeros(m, np.int32), khi=np.zeros(m, np.uint64),
+                 klo=np.zeros(m, np.uint64), leaf=np.zeros(m, np.int32),
+                 first=np.zeros(m, np.int64), count=np.zeros(m, np.int64),
+                 side=np.zeros(m), center=np.zeros((m, 3)), mass=np.zeros(m),
+                 com=np.zeros((m, 3)), order=np.zeros(n, np.int32))
+        lib().orc_tree_dump(self.h, _p(d["depth"], _ip), _p(d["khi"], _u64p),
+                            _p(d["klo"], _u64p), _p(d["leaf"], _ip), _p(d["first"], _i64p),
+                            _p(d["count"], _i64p), _p(d["side"]), _p(d["center"]),
+                            _p(d["mass"]), _p(d["com"]), _p(d["order"], _ip))
+        return d
+
+    def forces(self, theta=0.5, damping=False, threads=1):
+        n = len(self.pos)
+        acc = np.zeros((n, 3))
+        cnt = np.zeros(n, np.int64)
+        lib().orc_tree_forces(self.h, theta, int(damping), threads, _p(acc), _p(cnt, _i64p))
+        return acc, cnt
+
+    def forces_subset(self, idx, theta=0.5, damping=False, threads=1):
+        idx = np.ascontiguousarray(idx, dtype=np.int64)
+        acc = np.zeros((len(idx), 3))
+        cnt = np.zeros(len(idx), np.int64)
+        lib().orc_tree_forces_subset(self.h, theta, int(damping), threads, len(idx),
+                                     _p(idx, _i64p), _p(acc), _p(cnt, _i64p))
+        return acc, cnt
+
+
+def run(pos, vel, mass, theta=0.5, dt=0.0333, frames=1, damping=False, prec="f64",
+        threads=1):
+    """`frames` serial frames of the reference loop; returns (pos, vel, acc)."""
+    pos = np.array(pos, dtype=np.float64, order="C")
+    vel = np.array(vel, dtype=np.float64, order="C")
+    mass = np.ascontiguousarray(mass, dtype=np.float64)
+    acc = np.zeros_like(pos)
+    rc = lib().orc_run(prec_code(prec), len(pos), _p(pos), _p(vel), _p(acc), _p(mass),
+                       theta, dt, int(damping), frames, threads)
+    if rc:
+        raise ValueError("orc_run failed (coincident particles)")
+    return pos, vel, acc
+
+
+def direct(pos, mass, idx=None, damping=False, threads=8):
+    """FP64 direct sum (long-double accumulation) on the particles in idx."""
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    mass = np.ascontiguousarray(mass, dtype=np.float64)
+    if idx is None:
+        idx = np.arange(len(pos))
+    idx = np.ascontiguousarray(idx, dtype=np.int64)
+    acc = np.zeros((len(idx), 3))
+    lib().orc_direct(len(pos), _p(pos), _p(mass), int(damping), len(idx), _p(idx, _i64p),
+                     threads, _p(acc))
+    return acc
+
+
+# ----------------------------------------------------------------- _ref --
+_ref = {}
+
+
+def ref_available(prec="f64"):
+    return os.path.exists(os.path.join(REF_DIR, "libnbsref_%s.so" % prec))
+
+
+def ref_lib(prec="f64"):
+    """The reference's own octree/particle/vector code + harness (oracle/_ref)."""
+    if prec not in _ref:
+        L = ctypes.CDLL(os.path.join(REF_DIR, "libnbsref_%s.so" % prec))
+        L.ref_tree_dump.restype = ctypes.c_long
+        L.ref_tree_dump.argtypes = [ctypes.c_size_t, _dp, _dp, ctypes.c_long, _ip, _ip, _lp,
+                                    _lp, _dp, _dp, _dp, _ip]
+        L.ref_forces.argtypes = [ctypes.c_size_t, _dp, _dp, ctypes.c_double, ctypes.c_int,
+                                 _dp, _lp]
+        L.ref_run.argtypes = [ctypes.c_size_t, _dp, _dp, _dp, _dp, ctypes.c_double,
+                              ctypes.c_double, ctypes.c_int, ctypes.c_int]
+        _ref[prec] = L
+    return _ref[prec]
+
+
+def ref_tree_dump(pos, mass, prec="f64"):
+    L = ref_lib(prec)
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    mass = np.ascontiguousarray(mass, dtype=np.float64)
+    n = len(pos)
+    cap = 64 * n + 64
+    d = dict(depth=np.zeros(cap, np.int32), leaf=np.zeros(cap, np.int32),
+             first=np.zeros(cap, np.int64), count=np.zeros(cap, np.int64),
+             side=np.zeros(cap), mass=np.zeros(cap), com=np.zeros((cap, 3)),
+             order=np.zeros(n, np.int32))
+    m = L.ref_tree_dump(n, _p(pos), _p(mass), cap, _p(d["depth"], _ip), _p(d["leaf"], _ip),
+                        _p(d["first"], _lp), _p(d["count"], _lp), _p(d["side"]),
+                        _p(d["mass"]), _p(d["com"]), _p(d["order"], _ip))
+    assert m <= cap
+    for k in list(d):
+        if k != "order":
+            d[k] = d[k][:m].copy()
+    return d
+
+
+def ref_forces(pos, mass, theta=0.5, damping=False, prec="f64"):
+    L = ref_lib(prec)
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    mass = np.ascontiguousarray(mass, dtype=np.float64)
+    acc = np.zeros_like(pos)
+    cnt = np.zeros(len(pos), np.int64)
+    L.ref_forces(len(pos), _p(pos), _p(mass), theta, int(damping), _p(acc), _p(cnt, _lp))
+    return acc, cnt
+
+
+def ref_run(pos, vel, mass, theta=0.5, dt=0.0333, frames=1, damping=False, prec="f64"):
+    L = ref_lib(prec)
+    pos = np.array(pos, dtype=np.float64, order="C")
+    vel = np.array(vel, dtype=np.float64, order="C")
+    mass = np.ascontiguousarray(mass, dtype=np.float64)
+    acc = np.zeros_like(pos)
+    L.ref_run(len(pos), _p(pos), _p(vel), _p(acc), _p(mass), theta, dt, int(damping), frames)
+    return pos, vel, acc
