@@ -1,0 +1,166 @@
+/* nbs.h -- C ABI of the MI355X Barnes-Hut step engine (libnbs.so).
+ *
+ * Drop-in for nbodysim's per-frame step.  The reference crosses the
+ * host/device boundary once per frame, at
+ *     void barnes_hut_cuda(std::unordered_set<particle*>*, octree*,
+ *                          bool damping, datatype theta);
+ *     (reference src/cuda_helper.h:8, called from src/main.cpp:1449-1452)
+ * with the octree built on the host (src/main.cpp:887-927,
+ * src/octree.cpp:134-313) and the update done on the host
+ * (src/main.cpp:929-973).  Here the whole frame -- root, octree, mass/COM,
+ * walk, update -- runs on the GPU, so the boundary moves to step
+ * granularity: the caller hands over particle state once and asks for
+ * forces (nbs_forces, barnes_hut_cuda's role) or whole frames (nbs_step).
+ *
+ * Conventions: every function returns NBS_OK (0) or an NBS_ERR_* code and
+ * never exits the process (the reference's cuda_assert calls exit,
+ * src/cuda_code.cu:29-37); nbs_last_error() gives the message.  Host
+ * arrays are particle-major [n][3] doubles (x, y, z), in the caller's
+ * particle order.  One context = one device, one stream, one host thread;
+ * calls on one context must not overlap.
+ */
+#ifndef NBS_H_
+#define NBS_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct nbs_ctx nbs_ctx;
+
+enum {
+  NBS_OK = 0,
+  NBS_ERR_ARG = 1,      /* bad argument */
+  NBS_ERR_HIP = 2,      /* HIP runtime failure */
+  NBS_ERR_STATE = 3,    /* call out of order (e.g. no state set) */
+  NBS_ERR_OVERFLOW = 4, /* tree or sort scratch exceeded its capacity */
+  NBS_ERR_NOMEM = 5,    /* device allocation failed */
+  NBS_ERR_IO = 6        /* file I/O (dump / resume) */
+};
+
+/* Force arithmetic.  State (positions, velocities), the octree keys and the
+ * centre-of-mass reduction are FP64 in both modes. */
+enum { NBS_FORCE_F64 = 0, NBS_FORCE_F32 = 1 };
+
+/* Root cell.  BBOX is the reference's gen_root (bounding-box midpoint,
+ * largest extent + 2, src/main.cpp:887-921); FIXED uses root_center and
+ * root_size. */
+enum { NBS_ROOT_BBOX = 0, NBS_ROOT_FIXED = 1 };
+
+typedef struct nbs_config {
+  double theta;          /* settings.cfg `theta` (src/main.cpp:1287) */
+  double dt;             /* settings.cfg `dt` (src/main.cpp:1288) */
+  int damping;           /* settings.cfg `damping` (src/main.cpp:1193-1198) */
+  int force_precision;   /* NBS_FORCE_F64 | NBS_FORCE_F32 */
+  int root_mode;         /* NBS_ROOT_BBOX | NBS_ROOT_FIXED */
+  int adaptive;          /* adaptive dt (documented in the reference README,
+                            README.md:54-60; not implemented there) */
+  double root_center[3]; /* NBS_ROOT_FIXED only */
+  double root_size;      /* NBS_ROOT_FIXED only */
+  double min_adaptive_dt;
+  double max_pos_change;
+  double max_vel_change;
+  int device;            /* HIP device ordinal */
+  int node_factor;       /* node capacity = node_factor * n + 4096 (0: 4) */
+  int reserved[6];
+} nbs_config;
+
+/* Tree summary of the last build. */
+typedef struct nbs_tree_info {
+  int64_t n_particles;
+  int64_t n_nodes;       /* internal nodes + leaves, all systems */
+  int64_t n_internal;
+  int32_t max_depth;
+  int32_t n_buckets;     /* depth-42 cells holding >= 2 particles (the
+                            reference would split deeper); 0 = exact */
+  int64_t node_capacity;
+} nbs_tree_info;
+
+/* Defaults of src/main.cpp:1312-1357 that the step uses (theta 0.5,
+ * dt 0.0333, no damping) plus FP64 forces and the bbox root. */
+void nbs_config_default(nbs_config* cfg);
+
+int nbs_create(const nbs_config* cfg, nbs_ctx** out);
+void nbs_destroy(nbs_ctx* ctx);
+const char* nbs_last_error(const nbs_ctx* ctx);
+
+/* Replaces the configuration (theta, dt, precision, ...) of a context. */
+int nbs_configure(nbs_ctx* ctx, const nbs_config* cfg);
+
+/* Uploads n particles (host arrays; vel may be NULL = zero).  Replaces the
+ * reference's particle_set contents (src/data_structures.h:9). */
+int nbs_set_state(nbs_ctx* ctx, size_t n, const double* pos,
+                  const double* vel, const double* mass);
+
+/* Batched mode: splits the uploaded particles into nsys independent
+ * systems, system s = caller indices [offsets[s], offsets[s+1]).
+ * offsets has nsys + 1 entries, offsets[0] = 0, offsets[nsys] = n. */
+int nbs_set_systems(nbs_ctx* ctx, int nsys, const int64_t* offsets);
+
+/* Downloads state in the caller's order; any pointer may be NULL.  acc is
+ * the acceleration of the last force evaluation (what the reference keeps
+ * in particle::acceleration, src/particle.h:35). */
+int nbs_get_state(nbs_ctx* ctx, double* pos, double* vel, double* acc);
+
+/* Accelerations for the current positions: barnes_hut_cuda's role
+ * (src/cuda_helper.cpp:218-246) including the tree build it was handed.
+ * acc (host, may be NULL to keep the result on the device). */
+int nbs_forces(nbs_ctx* ctx, double* acc);
+
+/* nsteps frames of the reference loop body (src/main.cpp:1422-1480):
+ * root, octree, mass, COM, walk, kick-drift update. */
+int nbs_step(nbs_ctx* ctx, int nsteps);
+
+/* Blocks until the context's stream has drained. */
+int nbs_synchronize(nbs_ctx* ctx);
+
+/* FP64 direct sum (no tree) at the current positions for the m particles
+ * whose caller indices are in ids; acc is host [m][3].  The accuracy
+ * reference for the tree walk. */
+int nbs_direct(nbs_ctx* ctx, size_t m, const int64_t* ids, double* acc);
+
+/* Interaction count (accepted nodes per particle) of the last walk, summed
+ * over all particles; per_particle (host, caller order) may be NULL. */
+int nbs_interactions(nbs_ctx* ctx, int64_t* total, int64_t* per_particle);
+
+/* Simulation time advanced so far, summed over steps (differs from
+ * steps * dt only with adaptive dt); per system when batched. */
+int nbs_time(nbs_ctx* ctx, double* t, int nsys_cap);
+
+int nbs_tree_info_get(nbs_ctx* ctx, nbs_tree_info* info);
+
+/* Test hook: the last tree in pre-order, children 0..7, every system after
+ * the other.  Arrays have n_nodes entries (order: n_particles):
+ *   depth, leaf (caller index of a leaf's particle, -1 internal,
+ *   -2 bucket), first / count (position range in `order`), side, mass,
+ *   com [n_nodes][3], key_hi / key_lo (node path, 3 bits per level,
+ *   level 0 in bits 62..60 of key_hi), order (caller indices in DFS leaf
+ *   order).  Any pointer may be NULL. */
+int nbs_tree_dump(nbs_ctx* ctx, int32_t* depth, int32_t* leaf,
+                  int64_t* first, int64_t* count, double* side,
+                  double* mass, double* com, uint64_t* key_hi,
+                  uint64_t* key_lo, int32_t* order);
+
+/* Per-particle 42-level keys (caller order) of the last build. */
+int nbs_keys(nbs_ctx* ctx, uint64_t* key_hi, uint64_t* key_lo);
+
+/* Root (centre, side) per system of the last build; center [nsys][3]. */
+int nbs_roots(nbs_ctx* ctx, double* center, double* side, int nsys_cap);
+
+/* Phase timing on the context's stream with HIP events (ms, summed since
+ * the last reset): [0] tree build (bbox..mass/COM), [1] walk (incl. fused
+ * update), [2] separate update/adaptive-dt kernels, [3] walk launches. */
+int nbs_profile_enable(nbs_ctx* ctx, int on);
+int nbs_profile_read(nbs_ctx* ctx, double* ms4, int reset);
+
+/* Raw device stream (hipStream_t) of the context, for interop. */
+void* nbs_stream(nbs_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NBS_H_ */

@@ -1,0 +1,213 @@
+"""nbodysim_amd -- MI355X Barnes-Hut step engine, a drop-in for
+fosted3/nbodysim's per-frame step.
+
+Python front-end over the C ABI of libnbs.so (include/nbs.h):
+
+    sim = nbodysim_amd.Simulation(theta=0.5, dt=0.0333, precision="f32")
+    sim.set_state(pos, vel, mass)      # float64 [n,3], [n,3], [n]
+    acc = sim.forces()                 # barnes_hut_cuda's role
+    sim.step(10)                       # whole frames on the GPU
+    pos, vel, acc = sim.state()
+
+plus the host contract of the reference (settings.cfg reader, seeded
+generator, .dat/.txt frame files) in nbodysim_amd.host.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _native
+from ._native import (NBS_FORCE_F32, NBS_FORCE_F64, NBS_ROOT_BBOX, NBS_ROOT_FIXED,
+                      Config, TreeInfo, lib)
+
+__all__ = ["Simulation", "NbsError", "lib", "NBS_FORCE_F32", "NBS_FORCE_F64"]
+
+
+class NbsError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("nbs error %d: %s" % (code, msg))
+        self.code = code
+
+
+def _dp(a):
+    return a.ctypes.data_as(_native.c_dp) if a is not None else None
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if shape is not None and a.shape != shape:
+        raise ValueError("expected shape %s, got %s" % (shape, a.shape))
+    return a
+
+
+class Simulation:
+    """One GPU context: particle state resident in HBM, one stream."""
+
+    def __init__(self, theta=0.5, dt=0.0333, damping=False, precision="f64",
+                 root_center=None, root_size=None, adaptive=False, min_adaptive_dt=0.00333,
+                 max_pos_change=3.0, max_vel_change=3.0, device=0, node_factor=4):
+        self._L = lib()
+        self.cfg = Config()
+        self._L.nbs_config_default(ctypes.byref(self.cfg))
+        self._fill_config(theta, dt, damping, precision, root_center, root_size, adaptive,
+                          min_adaptive_dt, max_pos_change, max_vel_change, device,
+                          node_factor)
+        h = ctypes.c_void_p()
+        rc = self._L.nbs_create(ctypes.byref(self.cfg), ctypes.byref(h))
+        if rc != 0:
+            raise NbsError(rc, "nbs_create failed")
+        self._h = h
+        self.n = 0
+
+    def _fill_config(self, theta, dt, damping, precision, root_center, root_size, adaptive,
+                     min_adaptive_dt, max_pos_change, max_vel_change, device, node_factor):
+        c = self.cfg
+        c.theta = theta
+        c.dt = dt
+        c.damping = int(bool(damping))
+        c.force_precision = {"f64": NBS_FORCE_F64, "f32": NBS_FORCE_F32}[precision]
+        if root_center is not None:
+            c.root_mode = NBS_ROOT_FIXED
+            for k in range(3):
+                c.root_center[k] = root_center[k]
+            c.root_size = root_size
+        else:
+            c.root_mode = NBS_ROOT_BBOX
+        c.adaptive = int(bool(adaptive))
+        c.min_adaptive_dt = min_adaptive_dt
+        c.max_pos_change = max_pos_change
+        c.max_vel_change = max_vel_change
+        c.device = device
+        c.node_factor = node_factor
+
+    def configure(self, **kw):
+        """Changes theta / dt / precision / ... of this context."""
+        c = self.cfg
+        args = dict(theta=c.theta, dt=c.dt, damping=bool(c.damping),
+                    precision="f32" if c.force_precision == NBS_FORCE_F32 else "f64",
+                    root_center=list(c.root_center) if c.root_mode == NBS_ROOT_FIXED else None,
+                    root_size=c.root_size, adaptive=bool(c.adaptive),
+                    min_adaptive_dt=c.min_adaptive_dt, max_pos_change=c.max_pos_change,
+                    max_vel_change=c.max_vel_change, device=c.device,
+                    node_factor=c.node_factor)
+        args.update(kw)
+        self._fill_config(**args)
+        self._check(self._L.nbs_configure(self._h, ctypes.byref(self.cfg)))
+
+    def _check(self, rc):
+        if rc != 0:
+            raise NbsError(rc, self._L.nbs_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.nbs_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---------------------------------------------------------------- state
+    def set_state(self, pos, vel, mass):
+        mass = _f64(mass)
+        n = len(mass)
+        pos = _f64(pos, (n, 3))
+        vel = None if vel is None else _f64(vel, (n, 3))
+        self._check(self._L.nbs_set_state(self._h, n, _dp(pos), _dp(vel), _dp(mass)))
+        self.n = n
+
+    def set_systems(self, offsets):
+        off = np.ascontiguousarray(offsets, dtype=np.int64)
+        self._check(self._L.nbs_set_systems(self._h, len(off) - 1,
+                                            off.ctypes.data_as(_native.c_i64p)))
+
+    def state(self):
+        n = self.n
+        pos, vel, acc = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
+        self._check(self._L.nbs_get_state(self._h, _dp(pos), _dp(vel), _dp(acc)))
+        return pos, vel, acc
+
+    # ----------------------------------------------------------------- step
+    def forces(self, download=True):
+        if not download:
+            self._check(self._L.nbs_forces(self._h, None))
+            return None
+        acc = np.zeros((self.n, 3))
+        self._check(self._L.nbs_forces(self._h, _dp(acc)))
+        return acc
+
+    def step(self, nsteps=1):
+        self._check(self._L.nbs_step(self._h, int(nsteps)))
+
+    def synchronize(self):
+        self._check(self._L.nbs_synchronize(self._h))
+
+    def direct(self, ids):
+        ids = np.ascontiguousarray(ids, dtype=np.int64)
+        acc = np.zeros((len(ids), 3))
+        self._check(self._L.nbs_direct(self._h, len(ids), ids.ctypes.data_as(_native.c_i64p),
+                                       _dp(acc)))
+        return acc
+
+    def interactions(self, per_particle=False):
+        tot = ctypes.c_int64()
+        per = np.zeros(self.n, np.int64) if per_particle else None
+        self._check(self._L.nbs_interactions(
+            self._h, ctypes.byref(tot),
+            per.ctypes.data_as(_native.c_i64p) if per is not None else None))
+        return (tot.value, per) if per_particle else tot.value
+
+    def time(self, nsys=1):
+        t = np.zeros(nsys)
+        self._check(self._L.nbs_time(self._h, _dp(t), nsys))
+        return t
+
+    # ----------------------------------------------------------------- tree
+    def tree_info(self):
+        info = TreeInfo()
+        self._check(self._L.nbs_tree_info_get(self._h, ctypes.byref(info)))
+        return {k: getattr(info, k) for k, _ in TreeInfo._fields_}
+
+    def tree_dump(self):
+        info = self.tree_info()
+        m, n = info["n_nodes"], self.n
+        d = dict(depth=np.zeros(m, np.int32), leaf=np.zeros(m, np.int32),
+                 first=np.zeros(m, np.int64), count=np.zeros(m, np.int64),
+                 side=np.zeros(m), mass=np.zeros(m), com=np.zeros((m, 3)),
+                 khi=np.zeros(m, np.uint64), klo=np.zeros(m, np.uint64),
+                 order=np.zeros(n, np.int32))
+        P = lambda a, t: a.ctypes.data_as(t)
+        self._check(self._L.nbs_tree_dump(
+            self._h, P(d["depth"], _native.c_i32p), P(d["leaf"], _native.c_i32p),
+            P(d["first"], _native.c_i64p), P(d["count"], _native.c_i64p), _dp(d["side"]),
+            _dp(d["mass"]), _dp(d["com"]), P(d["khi"], _native.c_u64p),
+            P(d["klo"], _native.c_u64p), P(d["order"], _native.c_i32p)))
+        return d
+
+    def keys(self):
+        hi = np.zeros(self.n, np.uint64)
+        lo = np.zeros(self.n, np.uint64)
+        self._check(self._L.nbs_keys(self._h, hi.ctypes.data_as(_native.c_u64p),
+                                     lo.ctypes.data_as(_native.c_u64p)))
+        return hi, lo
+
+    def roots(self, nsys=1):
+        c = np.zeros((nsys, 3))
+        s = np.zeros(nsys)
+        self._check(self._L.nbs_roots(self._h, _dp(c), _dp(s), nsys))
+        return c, s
+
+    # ------------------------------------------------------------ profiling
+    def profile(self, on=True):
+        self._check(self._L.nbs_profile_enable(self._h, int(on)))
+
+    def profile_read(self, reset=True):
+        ms = np.zeros(4)
+        self._check(self._L.nbs_profile_read(self._h, _dp(ms), int(reset)))
+        return {"build_ms": ms[0], "walk_ms": ms[1], "update_ms": ms[2],
+                "walk_launches": int(ms[3])}

@@ -1,0 +1,836 @@
+// api.hip -- the C ABI of include/nbs.h: context, device buffers and the
+// step pipeline
+//   [bbox] -> root -> keys -> radix sort -> fix-up -> permute ->
+//   emit -> link -> bottom-up mass/COM -> walk (+ fused kick-drift) ->
+//   [adaptive dt -> update]
+// all enqueued on the context's stream with no host synchronisation inside
+// a step.  Errors are returned, never exit()ed (the reference's
+// cuda_assert exits, src/cuda_code.cu:29-37).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/nbs.h"
+#include "nbs_device_util.h"
+#include "nbs_internal.h"
+
+using namespace nbs;
+
+namespace {
+
+constexpr int kPhaseBuild = 0, kPhaseWalk = 1, kPhaseUpdate = 2;
+
+__global__ void deinterleave_kernel(const double* __restrict__ pos,
+                                    const double* __restrict__ vel,
+                                    const double* __restrict__ mass, int n,
+                                    State st, int has_vel) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  st.x[i] = pos[3 * i];
+  st.y[i] = pos[3 * i + 1];
+  st.z[i] = pos[3 * i + 2];
+  st.vx[i] = has_vel ? vel[3 * i] : 0.0;
+  st.vy[i] = has_vel ? vel[3 * i + 1] : 0.0;
+  st.vz[i] = has_vel ? vel[3 * i + 2] : 0.0;
+  st.m[i] = mass[i];
+  st.id[i] = (unsigned)i;
+}
+
+// SoA in slot order -> AoS in caller order.
+__global__ void interleave_kernel(const double* __restrict__ a,
+                                  const double* __restrict__ b,
+                                  const double* __restrict__ c,
+                                  const unsigned* __restrict__ id, int n,
+                                  double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const unsigned k = id[i];
+  out[3 * (size_t)k] = a[i];
+  out[3 * (size_t)k + 1] = b[i];
+  out[3 * (size_t)k + 2] = c[i];
+}
+
+__global__ void gather_slots_kernel(const long long* __restrict__ ids, int m,
+                                    const int* __restrict__ slot_of_id,
+                                    int* __restrict__ slots) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < m) slots[k] = slot_of_id[ids[k]];
+}
+
+template <class T>
+T* dalloc(std::vector<void*>& owner, size_t count, hipError_t* err) {
+  void* p = nullptr;
+  size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+  bytes = (bytes + 255) & ~size_t(255);
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) {
+    *err = e;
+    return nullptr;
+  }
+  owner.push_back(p);
+  return (T*)p;
+}
+
+void free_all(std::vector<void*>& v) {
+  for (void* p : v) (void)hipFree(p);
+  v.clear();
+}
+
+}  // namespace
+
+struct nbs_ctx {
+  nbs_config cfg;
+  StepParams p;
+  std::string err;
+  int device = 0;
+  hipStream_t stream = nullptr;
+
+  int n = 0, cap_n = 0;
+  int node_cap = 0;
+  std::vector<void*> part_bufs, node_bufs, sys_bufs, misc_bufs;
+
+  State st[2];
+  int cur = 0;
+  double *ax = nullptr, *ay = nullptr, *az = nullptr;
+  int* inter = nullptr;
+  double* stage = nullptr;
+  double* stage_v = nullptr;
+  double* stage_m = nullptr;
+  int* slot_of_id = nullptr;
+  BuildBuffers bb{};
+  WalkNode* nodes = nullptr;
+  TreeAux aux{};
+  Systems sys{};
+  std::vector<int> h_start, h_wave_start;
+  int* flags = nullptr;
+  int* steps_done = nullptr;
+  int* stats = nullptr;
+
+  // direct-sum scratch
+  int direct_cap = 0;
+  long long* d_ids = nullptr;
+  int* d_slots = nullptr;
+  double* d_direct_tmp = nullptr;
+  double* d_direct_out = nullptr;
+
+  bool has_state = false, bbox_valid = false, tree_valid = false;
+  bool acc_valid = false;
+  int nsys_req = 1;
+  std::vector<int64_t> sys_off_req;
+
+  bool prof = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[4];
+  double prof_ms[4] = {0, 0, 0, 0};
+};
+
+namespace {
+
+int fail(nbs_ctx* c, int code, const std::string& msg) {
+  c->err = msg;
+  return code;
+}
+
+int hip_fail(nbs_ctx* c, hipError_t e, const char* what) {
+  return fail(c, e == hipErrorOutOfMemory ? NBS_ERR_NOMEM : NBS_ERR_HIP,
+              std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(c, call)                                   \
+  do {                                                    \
+    hipError_t e_ = (call);                               \
+    if (e_ != hipSuccess) return hip_fail((c), e_, #call); \
+  } while (0)
+
+StepParams params_of(const nbs_config& c) {
+  StepParams p{};
+  p.theta = c.theta;
+  p.dt = c.dt;
+  p.damping = c.damping;
+  p.root_mode = c.root_mode;
+  for (int k = 0; k < 3; k++) p.root_center[k] = c.root_center[k];
+  p.root_size = c.root_size;
+  p.adaptive = c.adaptive;
+  p.min_adaptive_dt = c.min_adaptive_dt;
+  p.max_pos_change = c.max_pos_change;
+  p.max_vel_change = c.max_vel_change;
+  return p;
+}
+
+int check_config(nbs_ctx* c, const nbs_config* cfg) {
+  if (!(cfg->theta >= 0)) return fail(c, NBS_ERR_ARG, "theta must be >= 0");
+  if (!(cfg->dt >= 0)) return fail(c, NBS_ERR_ARG, "dt must be >= 0");
+  if (cfg->force_precision != NBS_FORCE_F64 &&
+      cfg->force_precision != NBS_FORCE_F32)
+    return fail(c, NBS_ERR_ARG, "force_precision must be NBS_FORCE_F64/F32");
+  if (cfg->root_mode != NBS_ROOT_BBOX && cfg->root_mode != NBS_ROOT_FIXED)
+    return fail(c, NBS_ERR_ARG, "root_mode must be NBS_ROOT_BBOX/FIXED");
+  if (cfg->root_mode == NBS_ROOT_FIXED && !(cfg->root_size > 0))
+    return fail(c, NBS_ERR_ARG, "root_size must be > 0 for NBS_ROOT_FIXED");
+  return NBS_OK;
+}
+
+int alloc_particles(nbs_ctx* c, int n) {
+  if (n <= c->cap_n && c->cap_n > 0) return NBS_OK;
+  free_all(c->part_bufs);
+  const int cap = std::max(n, 64);
+  hipError_t e = hipSuccess;
+  auto& o = c->part_bufs;
+  for (int b = 0; b < 2; b++) {
+    State& s = c->st[b];
+    s.x = dalloc<double>(o, cap, &e);
+    s.y = dalloc<double>(o, cap, &e);
+    s.z = dalloc<double>(o, cap, &e);
+    s.vx = dalloc<double>(o, cap, &e);
+    s.vy = dalloc<double>(o, cap, &e);
+    s.vz = dalloc<double>(o, cap, &e);
+    s.m = dalloc<double>(o, cap, &e);
+    s.id = dalloc<unsigned>(o, cap, &e);
+  }
+  c->ax = dalloc<double>(o, cap, &e);
+  c->ay = dalloc<double>(o, cap, &e);
+  c->az = dalloc<double>(o, cap, &e);
+  c->inter = dalloc<int>(o, cap, &e);
+  c->stage = dalloc<double>(o, 3 * (size_t)cap, &e);
+  c->stage_v = dalloc<double>(o, 3 * (size_t)cap, &e);
+  c->stage_m = dalloc<double>(o, cap, &e);
+  c->slot_of_id = dalloc<int>(o, cap, &e);
+  BuildBuffers& b = c->bb;
+  b.prim0 = dalloc<unsigned long long>(o, cap, &e);
+  b.prim1 = dalloc<unsigned long long>(o, cap, &e);
+  b.val0 = dalloc<unsigned>(o, cap, &e);
+  b.val1 = dalloc<unsigned>(o, cap, &e);
+  b.khi_u = dalloc<unsigned long long>(o, cap, &e);
+  b.klo_u = dalloc<unsigned long long>(o, cap, &e);
+  b.khi = dalloc<unsigned long long>(o, cap, &e);
+  b.klo = dalloc<unsigned long long>(o, cap, &e);
+  b.counts = dalloc<int>(o, cap, &e);
+  b.offsets = dalloc<int>(o, (size_t)cap + 1, &e);
+  b.sort_tmp = dalloc<int>(o, radix_tmp_ints(cap), &e);
+  b.scan_tmp = dalloc<int>(o, scan_tmp_ints((size_t)cap + 1), &e);
+  if (e != hipSuccess) {
+    free_all(c->part_bufs);
+    c->cap_n = 0;
+    return hip_fail(c, e, "hipMalloc(particle buffers)");
+  }
+  c->cap_n = cap;
+  return NBS_OK;
+}
+
+int alloc_nodes(nbs_ctx* c, int cap) {
+  free_all(c->node_bufs);
+  hipError_t e = hipSuccess;
+  auto& o = c->node_bufs;
+  c->nodes = dalloc<WalkNode>(o, cap, &e);
+  c->aux.depth = dalloc<int>(o, cap, &e);
+  c->aux.first = dalloc<int>(o, cap, &e);
+  c->aux.count = dalloc<int>(o, cap, &e);
+  c->aux.parent = dalloc<int>(o, cap, &e);
+  c->aux.nchild = dalloc<int>(o, cap, &e);
+  c->aux.arrive = dalloc<int>(o, cap, &e);
+  if (e != hipSuccess) {
+    free_all(c->node_bufs);
+    c->node_cap = 0;
+    return hip_fail(c, e, "hipMalloc(node buffers)");
+  }
+  c->node_cap = cap;
+  return NBS_OK;
+}
+
+// Per-system arrays from caller offsets (one system when offsets empty).
+int setup_systems(nbs_ctx* c) {
+  const int nsys = c->nsys_req;
+  free_all(c->sys_bufs);
+  hipError_t e = hipSuccess;
+  auto& o = c->sys_bufs;
+  Systems& s = c->sys;
+  s.nsys = nsys;
+  s.start = dalloc<int>(o, nsys + 1, &e);
+  s.wave_start = dalloc<int>(o, nsys + 1, &e);
+  s.node_start = dalloc<int>(o, nsys + 1, &e);
+  s.bbox = dalloc<unsigned long long>(o, 6 * (size_t)nsys, &e);
+  s.root = dalloc<double>(o, 4 * (size_t)nsys, &e);
+  s.dt = dalloc<double>(o, nsys, &e);
+  s.time = dalloc<double>(o, nsys, &e);
+  s.amax = dalloc<unsigned long long>(o, nsys, &e);
+  s.vmax = dalloc<unsigned long long>(o, nsys, &e);
+  if (e != hipSuccess) return hip_fail(c, e, "hipMalloc(system buffers)");
+  c->h_start.assign(nsys + 1, 0);
+  c->h_wave_start.assign(nsys + 1, 0);
+  for (int k = 0; k <= nsys; k++)
+    c->h_start[k] = (int)(c->sys_off_req.empty() ? (k == 0 ? 0 : c->n)
+                                                 : c->sys_off_req[k]);
+  for (int k = 0; k < nsys; k++) {
+    const int cnt = c->h_start[k + 1] - c->h_start[k];
+    c->h_wave_start[k + 1] = c->h_wave_start[k] + (cnt + kWave - 1) / kWave;
+  }
+  s.n_waves = c->h_wave_start[nsys];
+  HIPCHK(c, hipMemcpyAsync(s.start, c->h_start.data(), (nsys + 1) * sizeof(int),
+                           hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(s.wave_start, c->h_wave_start.data(),
+                           (nsys + 1) * sizeof(int), hipMemcpyHostToDevice,
+                           c->stream));
+  std::vector<double> zeros(nsys, 0.0), dts(nsys, c->cfg.dt);
+  HIPCHK(c, hipMemcpyAsync(s.time, zeros.data(), nsys * sizeof(double),
+                           hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(s.dt, dts.data(), nsys * sizeof(double),
+                           hipMemcpyHostToDevice, c->stream));
+  launch_fill_u64(s.amax, 0, nsys, c->stream);
+  launch_fill_u64(s.vmax, 0, nsys, c->stream);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->bbox_valid = false;
+  c->tree_valid = false;
+  return NBS_OK;
+}
+
+void prof_begin(nbs_ctx* c, int phase, hipEvent_t* ev) {
+  if (!c->prof) return;
+  (void)phase;
+  hipEventCreate(ev);
+  hipEventRecord(*ev, c->stream);
+}
+void prof_end(nbs_ctx* c, int phase, hipEvent_t ev0) {
+  if (!c->prof) return;
+  hipEvent_t ev1;
+  hipEventCreate(&ev1);
+  hipEventRecord(ev1, c->stream);
+  c->ev[phase].push_back({ev0, ev1});
+}
+
+int ensure_nodes(nbs_ctx* c) {
+  const int want = std::max(c->cfg.node_factor > 0 ? c->cfg.node_factor : 4, 2) * c->n + 4096;
+  if (c->node_cap >= want) return NBS_OK;
+  return alloc_nodes(c, want);
+}
+
+// Enqueues one tree build of the current state.
+void enqueue_build(nbs_ctx* c) {
+  hipStream_t s = c->stream;
+  hipEvent_t e0 = nullptr;
+  prof_begin(c, kPhaseBuild, &e0);
+  State& cur = c->st[c->cur];
+  State& nxt = c->st[1 - c->cur];
+  if (!c->bbox_valid) {
+    launch_bbox_reset(c->sys, s);
+    launch_bbox(cur, c->sys, s);
+  }
+  launch_root(c->sys, c->p, c->flags, s);
+  const int nbits = launch_keys(cur, c->sys, c->bb, s);
+  const int which = radix_sort_pairs(c->bb.prim0, c->bb.val0, c->bb.prim1,
+                                     c->bb.val1, c->n, nbits, c->bb.sort_tmp, s);
+  unsigned long long* prim = which ? c->bb.prim1 : c->bb.prim0;
+  unsigned* perm = which ? c->bb.val1 : c->bb.val0;
+  launch_fixup(c->bb, prim, perm, cur, c->n, s);
+  launch_permute(cur, nxt, perm, c->n, s);
+  c->cur ^= 1;
+  launch_emit(c->bb, c->sys, c->p, c->nodes, c->aux, c->n, c->node_cap, s);
+  launch_link(c->bb, c->nodes, c->aux, c->n, c->node_cap, s);
+  launch_bottom_up(c->bb, c->st[c->cur], c->nodes, c->aux, c->n, s);
+  c->bbox_valid = false;
+  prof_end(c, kPhaseBuild, e0);
+}
+
+void enqueue_walk(nbs_ctx* c, int fused) {
+  hipEvent_t e0 = nullptr;
+  prof_begin(c, kPhaseWalk, &e0);
+  State& st = c->st[c->cur];
+  WalkArgs a{};
+  a.nodes = c->nodes;
+  a.node_start = c->sys.node_start;
+  a.sys_start = c->sys.start;
+  a.wave_start = c->sys.wave_start;
+  a.nsys = c->sys.nsys;
+  a.n_waves = c->sys.n_waves;
+  a.x = st.x;
+  a.y = st.y;
+  a.z = st.z;
+  a.vx = st.vx;
+  a.vy = st.vy;
+  a.vz = st.vz;
+  a.ax = c->ax;
+  a.ay = c->ay;
+  a.az = c->az;
+  a.interactions = c->inter;
+  a.flags = c->flags;
+  a.flags_out = c->flags;
+  a.theta = c->p.theta;
+  a.dt = c->p.dt;
+  a.bbox = c->sys.bbox;
+  a.sys_dt = c->sys.dt;
+  a.sys_time = c->sys.time;
+  a.steps_done = c->steps_done;
+  a.fused_update = fused;
+  launch_walk(a, c->cfg.force_precision == NBS_FORCE_F32, c->p.damping,
+              c->stream);
+  if (fused) c->bbox_valid = true;
+  prof_end(c, kPhaseWalk, e0);
+}
+
+void enqueue_adaptive_update(nbs_ctx* c) {
+  hipEvent_t e0 = nullptr;
+  prof_begin(c, kPhaseUpdate, &e0);
+  State& st = c->st[c->cur];
+  launch_adaptive_dt(st, c->ax, c->ay, c->az, c->n, c->sys, c->p, c->flags,
+                     c->stream);
+  launch_bbox_reset(c->sys, c->stream);
+  launch_update(st, c->ax, c->ay, c->az, c->n, c->sys, c->flags, c->stream);
+  c->bbox_valid = true;
+  prof_end(c, kPhaseUpdate, e0);
+}
+
+int read_flags(nbs_ctx* c, int* flags, int* done) {
+  int h[2] = {0, 0};
+  HIPCHK(c, hipMemcpyAsync(&h[0], c->flags, sizeof(int), hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipMemcpyAsync(&h[1], c->steps_done, sizeof(int),
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  *flags = h[0];
+  *done = h[1];
+  return NBS_OK;
+}
+
+void reset_flags(nbs_ctx* c) {
+  launch_fill_int(c->flags, 0, 1, c->stream);
+  launch_fill_int(c->steps_done, 0, 1, c->stream);
+}
+
+// Runs `nsteps` steps (or one force evaluation when nsteps == 0), growing
+// the node arrays and redoing the remaining steps on overflow.
+int run(nbs_ctx* c, int nsteps) {
+  if (!c->has_state) return fail(c, NBS_ERR_STATE, "no particle state set");
+  if (c->n == 0) return NBS_OK;
+  int r = ensure_nodes(c);
+  if (r) return r;
+  const bool forces_only = nsteps == 0;
+  int remaining = forces_only ? 1 : nsteps;
+  for (int attempt = 0; attempt < 6; attempt++) {
+    reset_flags(c);
+    for (int k = 0; k < remaining; k++) {
+      enqueue_build(c);
+      if (forces_only) {
+        enqueue_walk(c, 0);
+      } else if (c->p.adaptive) {
+        enqueue_walk(c, 0);
+        enqueue_adaptive_update(c);
+      } else {
+        enqueue_walk(c, 1);
+      }
+    }
+    int flags = 0, done = 0;
+    r = read_flags(c, &flags, &done);
+    if (r) return r;
+    if (!flags) {
+      c->tree_valid = true;
+      c->acc_valid = true;
+      return NBS_OK;
+    }
+    c->bbox_valid = false;
+    c->tree_valid = false;
+    if (flags & kFlagRunOverflow)
+      return fail(c, NBS_ERR_OVERFLOW,
+                  "more than 8192 particles share one 21-level cell");
+    remaining -= done;
+    r = alloc_nodes(c, c->node_cap * 2);
+    if (r) return r;
+  }
+  return fail(c, NBS_ERR_OVERFLOW, "node capacity overflow after retries");
+}
+
+}  // namespace
+
+extern "C" {
+
+void nbs_config_default(nbs_config* cfg) {
+  std::memset(cfg, 0, sizeof(*cfg));
+  cfg->theta = 0.5;
+  cfg->dt = 0.0333;
+  cfg->damping = 0;
+  cfg->force_precision = NBS_FORCE_F64;
+  cfg->root_mode = NBS_ROOT_BBOX;
+  cfg->adaptive = 0;
+  cfg->root_size = 512;
+  cfg->min_adaptive_dt = 0.00333;
+  cfg->max_pos_change = 3;
+  cfg->max_vel_change = 3;
+  cfg->device = 0;
+  cfg->node_factor = 4;
+}
+
+int nbs_create(const nbs_config* cfg, nbs_ctx** out) {
+  if (!cfg || !out) return NBS_ERR_ARG;
+  *out = nullptr;
+  nbs_ctx* c = new nbs_ctx();
+  int r = check_config(c, cfg);
+  if (r) {
+    delete c;
+    return r;
+  }
+  c->cfg = *cfg;
+  c->p = params_of(*cfg);
+  c->device = cfg->device;
+  hipError_t e = hipSetDevice(c->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return NBS_ERR_HIP;
+  }
+  e = hipSuccess;
+  c->flags = dalloc<int>(c->misc_bufs, 1, &e);
+  c->steps_done = dalloc<int>(c->misc_bufs, 1, &e);
+  c->bb.flags = c->flags;
+  c->bb.stats = dalloc<int>(c->misc_bufs, 4, &e);
+  c->stats = c->bb.stats;
+  if (e != hipSuccess) {
+    free_all(c->misc_bufs);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return NBS_ERR_NOMEM;
+  }
+  launch_fill_int(c->flags, 0, 1, c->stream);
+  launch_fill_int(c->stats, 0, 4, c->stream);
+  *out = c;
+  return NBS_OK;
+}
+
+void nbs_destroy(nbs_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  for (auto& v : c->ev)
+    for (auto& p : v) {
+      (void)hipEventDestroy(p.first);
+      (void)hipEventDestroy(p.second);
+    }
+  free_all(c->part_bufs);
+  free_all(c->node_bufs);
+  free_all(c->sys_bufs);
+  free_all(c->misc_bufs);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* nbs_last_error(const nbs_ctx* c) {
+  return c ? c->err.c_str() : "null context";
+}
+
+int nbs_configure(nbs_ctx* c, const nbs_config* cfg) {
+  if (!c || !cfg) return NBS_ERR_ARG;
+  int r = check_config(c, cfg);
+  if (r) return r;
+  const int old_node_factor = c->cfg.node_factor;
+  c->cfg = *cfg;
+  c->p = params_of(*cfg);
+  if (cfg->node_factor != old_node_factor) c->node_cap = 0;
+  c->tree_valid = false;
+  return NBS_OK;
+}
+
+int nbs_set_state(nbs_ctx* c, size_t n, const double* pos, const double* vel,
+                  const double* mass) {
+  if (!c) return NBS_ERR_ARG;
+  if (n > 0 && (!pos || !mass)) return fail(c, NBS_ERR_ARG, "pos and mass are required");
+  if (n > (size_t)0x3fffffff) return fail(c, NBS_ERR_ARG, "n too large");
+  HIPCHK(c, hipSetDevice(c->device));
+  int r = alloc_particles(c, (int)n);
+  if (r) return r;
+  c->n = (int)n;
+  c->cur = 0;
+  if (n > 0) {
+    HIPCHK(c, hipMemcpyAsync(c->stage, pos, 3 * n * sizeof(double),
+                             hipMemcpyHostToDevice, c->stream));
+    if (vel)
+      HIPCHK(c, hipMemcpyAsync(c->stage_v, vel, 3 * n * sizeof(double),
+                               hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->stage_m, mass, n * sizeof(double),
+                             hipMemcpyHostToDevice, c->stream));
+    deinterleave_kernel<<<(int)((n + 255) / 256), 256, 0, c->stream>>>(
+        c->stage, c->stage_v, c->stage_m, (int)n, c->st[0], vel ? 1 : 0);
+    launch_fill_int(c->inter, 0, n, c->stream);
+  }
+  c->nsys_req = 1;
+  c->sys_off_req.clear();
+  r = setup_systems(c);
+  if (r) return r;
+  c->has_state = true;
+  c->acc_valid = false;
+  return NBS_OK;
+}
+
+int nbs_set_systems(nbs_ctx* c, int nsys, const int64_t* off) {
+  if (!c || nsys < 1 || !off) return NBS_ERR_ARG;
+  if (!c->has_state) return fail(c, NBS_ERR_STATE, "set state before systems");
+  if (off[0] != 0 || off[nsys] != c->n)
+    return fail(c, NBS_ERR_ARG, "offsets must run from 0 to n");
+  for (int k = 0; k < nsys; k++)
+    if (off[k + 1] < off[k]) return fail(c, NBS_ERR_ARG, "offsets must not decrease");
+  // Particles are in caller order only until the first build.
+  if (c->cur != 0 || c->tree_valid)
+    return fail(c, NBS_ERR_STATE, "set systems right after set_state");
+  c->nsys_req = nsys;
+  c->sys_off_req.assign(off, off + nsys + 1);
+  return setup_systems(c);
+}
+
+int nbs_get_state(nbs_ctx* c, double* pos, double* vel, double* acc) {
+  if (!c) return NBS_ERR_ARG;
+  if (!c->has_state) return fail(c, NBS_ERR_STATE, "no particle state set");
+  HIPCHK(c, hipSetDevice(c->device));
+  const int n = c->n;
+  if (n == 0) return NBS_OK;
+  State& st = c->st[c->cur];
+  const int blocks = (n + 255) / 256;
+  if (pos) {
+    interleave_kernel<<<blocks, 256, 0, c->stream>>>(st.x, st.y, st.z, st.id, n, c->stage);
+    HIPCHK(c, hipMemcpyAsync(pos, c->stage, 3 * (size_t)n * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+  }
+  if (vel) {
+    interleave_kernel<<<blocks, 256, 0, c->stream>>>(st.vx, st.vy, st.vz, st.id, n,
+                                                     c->stage_v);
+    HIPCHK(c, hipMemcpyAsync(vel, c->stage_v, 3 * (size_t)n * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+  }
+  if (acc) {
+    if (!c->acc_valid) {
+      std::memset(acc, 0, 3 * (size_t)n * sizeof(double));
+    } else {
+      interleave_kernel<<<blocks, 256, 0, c->stream>>>(c->ax, c->ay, c->az, st.id, n,
+                                                       c->stage);
+      HIPCHK(c, hipMemcpyAsync(acc, c->stage, 3 * (size_t)n * sizeof(double),
+                               hipMemcpyDeviceToHost, c->stream));
+    }
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return NBS_OK;
+}
+
+int nbs_forces(nbs_ctx* c, double* acc) {
+  if (!c) return NBS_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  int r = run(c, 0);
+  if (r) return r;
+  if (acc) return nbs_get_state(c, nullptr, nullptr, acc);
+  return NBS_OK;
+}
+
+int nbs_step(nbs_ctx* c, int nsteps) {
+  if (!c) return NBS_ERR_ARG;
+  if (nsteps < 0) return fail(c, NBS_ERR_ARG, "nsteps must be >= 0");
+  if (nsteps == 0) return NBS_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  return run(c, nsteps);
+}
+
+int nbs_synchronize(nbs_ctx* c) {
+  if (!c) return NBS_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return NBS_OK;
+}
+
+int nbs_direct(nbs_ctx* c, size_t m, const int64_t* ids, double* acc) {
+  if (!c) return NBS_ERR_ARG;
+  if (!c->has_state) return fail(c, NBS_ERR_STATE, "no particle state set");
+  if (m == 0) return NBS_OK;
+  if (!ids || !acc) return fail(c, NBS_ERR_ARG, "ids and acc are required");
+  for (size_t k = 0; k < m; k++)
+    if (ids[k] < 0 || ids[k] >= c->n) return fail(c, NBS_ERR_ARG, "id out of range");
+  HIPCHK(c, hipSetDevice(c->device));
+  if ((int)m > c->direct_cap) {
+    for (void* p : {(void*)c->d_ids, (void*)c->d_slots, (void*)c->d_direct_tmp,
+                    (void*)c->d_direct_out})
+      if (p) {
+        (void)hipFree(p);
+        c->misc_bufs.erase(std::remove(c->misc_bufs.begin(), c->misc_bufs.end(), p),
+                           c->misc_bufs.end());
+      }
+    hipError_t e = hipSuccess;
+    c->d_ids = dalloc<long long>(c->misc_bufs, m, &e);
+    c->d_slots = dalloc<int>(c->misc_bufs, m, &e);
+    c->d_direct_tmp = dalloc<double>(c->misc_bufs, direct_tmp_doubles(m), &e);
+    c->d_direct_out = dalloc<double>(c->misc_bufs, 3 * m, &e);
+    if (e != hipSuccess) {
+      c->direct_cap = 0;
+      return hip_fail(c, e, "hipMalloc(direct)");
+    }
+    c->direct_cap = (int)m;
+  }
+  State& st = c->st[c->cur];
+  launch_inverse_perm(st.id, c->n, c->slot_of_id, c->stream);
+  HIPCHK(c, hipMemcpyAsync(c->d_ids, ids, m * sizeof(long long), hipMemcpyHostToDevice,
+                           c->stream));
+  gather_slots_kernel<<<(int)((m + 255) / 256), 256, 0, c->stream>>>(
+      c->d_ids, (int)m, c->slot_of_id, c->d_slots);
+  launch_direct(st, c->n, c->d_slots, (int)m, c->p.damping, c->d_direct_tmp,
+                c->d_direct_out, c->stream);
+  HIPCHK(c, hipMemcpyAsync(acc, c->d_direct_out, 3 * m * sizeof(double),
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return NBS_OK;
+}
+
+int nbs_interactions(nbs_ctx* c, int64_t* total, int64_t* per_particle) {
+  if (!c) return NBS_ERR_ARG;
+  if (!c->acc_valid) return fail(c, NBS_ERR_STATE, "no walk has run");
+  const int n = c->n;
+  std::vector<int> h(n);
+  std::vector<unsigned> id(n);
+  HIPCHK(c, hipMemcpyAsync(h.data(), c->inter, n * sizeof(int), hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipMemcpyAsync(id.data(), c->st[c->cur].id, n * sizeof(unsigned),
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  int64_t t = 0;
+  for (int i = 0; i < n; i++) {
+    t += h[i];
+    if (per_particle) per_particle[id[i]] = h[i];
+  }
+  if (total) *total = t;
+  return NBS_OK;
+}
+
+int nbs_time(nbs_ctx* c, double* t, int cap) {
+  if (!c || !t) return NBS_ERR_ARG;
+  if (!c->has_state) return fail(c, NBS_ERR_STATE, "no particle state set");
+  const int k = std::min(cap, c->sys.nsys);
+  HIPCHK(c, hipMemcpyAsync(t, c->sys.time, k * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return NBS_OK;
+}
+
+int nbs_tree_info_get(nbs_ctx* c, nbs_tree_info* info) {
+  if (!c || !info) return NBS_ERR_ARG;
+  if (!c->tree_valid) return fail(c, NBS_ERR_STATE, "no tree built");
+  int s[4];
+  HIPCHK(c, hipMemcpyAsync(s, c->stats, sizeof(s), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  info->n_particles = c->n;
+  info->n_nodes = s[0];
+  info->n_internal = s[1];
+  info->max_depth = s[2];
+  info->n_buckets = s[3];
+  info->node_capacity = c->node_cap;
+  return NBS_OK;
+}
+
+int nbs_tree_dump(nbs_ctx* c, int32_t* depth, int32_t* leaf, int64_t* first,
+                  int64_t* count, double* side, double* mass, double* com,
+                  uint64_t* key_hi, uint64_t* key_lo, int32_t* order) {
+  nbs_tree_info info;
+  int r = nbs_tree_info_get(c, &info);
+  if (r) return r;
+  const int T = (int)info.n_nodes, n = c->n;
+  std::vector<WalkNode> nd(T);
+  std::vector<int> dep(T), fst(T), cnt(T);
+  std::vector<unsigned> id(n);
+  std::vector<unsigned long long> khi(n), klo(n);
+  hipStream_t s = c->stream;
+  HIPCHK(c, hipMemcpyAsync(nd.data(), c->nodes, T * sizeof(WalkNode), hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipMemcpyAsync(dep.data(), c->aux.depth, T * sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipMemcpyAsync(fst.data(), c->aux.first, T * sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipMemcpyAsync(cnt.data(), c->aux.count, T * sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipMemcpyAsync(id.data(), c->st[c->cur].id, n * sizeof(unsigned),
+                           hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipMemcpyAsync(khi.data(), c->bb.khi, n * sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipMemcpyAsync(klo.data(), c->bb.klo, n * sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  for (int k = 0; k < T; k++) {
+    const int d = dep[k];
+    if (depth) depth[k] = d;
+    if (leaf) leaf[k] = nd[k].leaf >= 0 ? (int32_t)id[nd[k].leaf] : nd[k].leaf;
+    if (first) first[k] = fst[k];
+    if (count) count[k] = cnt[k];
+    if (side) side[k] = nd[k].side;
+    if (mass) mass[k] = nd[k].m;
+    if (com) {
+      com[3 * k] = nd[k].cx;
+      com[3 * k + 1] = nd[k].cy;
+      com[3 * k + 2] = nd[k].cz;
+    }
+    const int dd = std::min(d, kLevels);
+    const unsigned long long h = khi[fst[k]], l = klo[fst[k]];
+    unsigned long long mh, ml;
+    if (dd <= 21) {
+      mh = dd == 0 ? 0 : (h & (~0ull << (63 - 3 * dd)));
+      ml = 0;
+    } else {
+      mh = h;
+      ml = l & (~0ull << (63 - 3 * (dd - 21)));
+    }
+    if (key_hi) key_hi[k] = mh;
+    if (key_lo) key_lo[k] = ml;
+  }
+  if (order)
+    for (int i = 0; i < n; i++) order[i] = (int32_t)id[i];
+  return NBS_OK;
+}
+
+int nbs_keys(nbs_ctx* c, uint64_t* key_hi, uint64_t* key_lo) {
+  if (!c) return NBS_ERR_ARG;
+  if (!c->tree_valid) return fail(c, NBS_ERR_STATE, "no tree built");
+  const int n = c->n;
+  std::vector<unsigned> id(n);
+  std::vector<unsigned long long> khi(n), klo(n);
+  HIPCHK(c, hipMemcpyAsync(id.data(), c->st[c->cur].id, n * sizeof(unsigned),
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(khi.data(), c->bb.khi, n * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(klo.data(), c->bb.klo, n * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int i = 0; i < n; i++) {
+    if (key_hi) key_hi[id[i]] = khi[i];
+    if (key_lo) key_lo[id[i]] = klo[i];
+  }
+  return NBS_OK;
+}
+
+int nbs_roots(nbs_ctx* c, double* center, double* side, int cap) {
+  if (!c) return NBS_ERR_ARG;
+  if (!c->tree_valid) return fail(c, NBS_ERR_STATE, "no tree built");
+  const int k = std::min(cap, c->sys.nsys);
+  std::vector<double> r(4 * (size_t)c->sys.nsys);
+  HIPCHK(c, hipMemcpyAsync(r.data(), c->sys.root, r.size() * sizeof(double),
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int s = 0; s < k; s++) {
+    if (center)
+      for (int d = 0; d < 3; d++) center[3 * s + d] = r[4 * s + d];
+    if (side) side[s] = r[4 * s + 3];
+  }
+  return NBS_OK;
+}
+
+int nbs_profile_enable(nbs_ctx* c, int on) {
+  if (!c) return NBS_ERR_ARG;
+  c->prof = on != 0;
+  return NBS_OK;
+}
+
+int nbs_profile_read(nbs_ctx* c, double* ms4, int reset) {
+  if (!c || !ms4) return NBS_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int ph = 0; ph < 3; ph++) {
+    for (auto& p : c->ev[ph]) {
+      float ms = 0;
+      if (hipEventElapsedTime(&ms, p.first, p.second) == hipSuccess) c->prof_ms[ph] += ms;
+      (void)hipEventDestroy(p.first);
+      (void)hipEventDestroy(p.second);
+      if (ph == kPhaseWalk) c->prof_ms[3] += 1;
+    }
+    c->ev[ph].clear();
+  }
+  for (int k = 0; k < 4; k++) ms4[k] = c->prof_ms[k];
+  if (reset)
+    for (int k = 0; k < 4; k++) c->prof_ms[k] = 0;
+  return NBS_OK;
+}
+
+void* nbs_stream(nbs_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+}  // extern "C"

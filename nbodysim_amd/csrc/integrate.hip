@@ -1,0 +1,152 @@
+// integrate.hip -- adaptive step length and the separate kick-drift update
+// (the fixed-dt path fuses the update into the walk kernel).
+//
+// Adaptive dt: documented by the reference README (README.md:54-60:
+// min_adaptive_dt, max_pos_change, max_vel_change) but not implemented in
+// its code; the only hook is particle::update(dv, dp) returning |a| and |v|
+// (src/particle.cpp:76-80).  This build defines it per system as
+//   dt_s = max(min(dt, max_vel_change / max|a|, max_pos_change / max|v|),
+//              min_adaptive_dt)
+// with |a| from the current forces and |v| before the kick.
+#include <cmath>
+
+#include "nbs_device_util.h"
+#include "nbs_internal.h"
+
+namespace nbs {
+namespace {
+
+constexpr int kBlock = 256;
+
+__global__ void __launch_bounds__(kBlock)
+    adaptive_max_kernel(const double* __restrict__ vx,
+                        const double* __restrict__ vy,
+                        const double* __restrict__ vz,
+                        const double* __restrict__ ax,
+                        const double* __restrict__ ay,
+                        const double* __restrict__ az,
+                        const int* __restrict__ start,
+                        const int* __restrict__ wstart, int nsys, int nwaves,
+                        unsigned long long* __restrict__ amax,
+                        unsigned long long* __restrict__ vmax,
+                        const int* __restrict__ flags) {
+  if (*flags) return;
+  const int w = (blockIdx.x * kBlock + threadIdx.x) >> 6;
+  if (w >= nwaves) return;
+  const int s = nsys > 1 ? sys_of_wave(w, wstart, nsys) : 0;
+  const int i = start[s] + (w - wstart[s]) * 64 + lane_id();
+  const bool v = i < start[s + 1];
+  double a2 = 0, v2 = 0;
+  if (v) {
+    a2 = sqrt(ax[i] * ax[i] + ay[i] * ay[i] + az[i] * az[i]);
+    v2 = sqrt(vx[i] * vx[i] + vy[i] * vy[i] + vz[i] * vz[i]);
+  }
+  a2 = wave_max(a2);
+  v2 = wave_max(v2);
+  if (lane_id() == 0) {
+    // non-negative doubles order like their bit patterns
+    atomicMax(amax + s, (unsigned long long)__double_as_longlong(a2));
+    atomicMax(vmax + s, (unsigned long long)__double_as_longlong(v2));
+  }
+}
+
+__global__ void adaptive_dt_kernel(unsigned long long* __restrict__ amax,
+                                   unsigned long long* __restrict__ vmax,
+                                   int nsys, double dt, double min_dt,
+                                   double max_dp, double max_dv,
+                                   double* __restrict__ sys_dt,
+                                   double* __restrict__ sys_time,
+                                   const int* __restrict__ flags) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsys || *flags) return;
+  const double am = __longlong_as_double((long long)amax[s]);
+  const double vm = __longlong_as_double((long long)vmax[s]);
+  double h = dt;
+  if (am > 0) h = fmin(h, max_dv / am);
+  if (vm > 0) h = fmin(h, max_dp / vm);
+  h = fmax(h, min_dt);
+  sys_dt[s] = h;
+  sys_time[s] += h;
+  amax[s] = 0;
+  vmax[s] = 0;
+}
+
+__global__ void __launch_bounds__(kBlock)
+    update_kernel(State st, const double* __restrict__ ax,
+                  const double* __restrict__ ay, const double* __restrict__ az,
+                  const int* __restrict__ start,
+                  const int* __restrict__ wstart, int nsys, int nwaves,
+                  const double* __restrict__ sys_dt,
+                  unsigned long long* __restrict__ bbox,
+                  const int* __restrict__ flags) {
+#pragma clang fp contract(off)
+  if (*flags) return;
+  const int w = (blockIdx.x * kBlock + threadIdx.x) >> 6;
+  if (w >= nwaves) return;
+  const int s = nsys > 1 ? sys_of_wave(w, wstart, nsys) : 0;
+  const int i = start[s] + (w - wstart[s]) * 64 + lane_id();
+  const bool v = i < start[s + 1];
+  const double dt = sys_dt[s];
+  double x = 0, y = 0, z = 0;
+  if (v) {
+    // src/particle.cpp:66-74
+    double vx = st.vx[i], vy = st.vy[i], vz = st.vz[i];
+    const double tx = ax[i] * dt, ty = ay[i] * dt, tz = az[i] * dt;
+    vx = vx + tx;
+    vy = vy + ty;
+    vz = vz + tz;
+    const double px = vx * dt, py = vy * dt, pz = vz * dt;
+    x = st.x[i] + px;
+    y = st.y[i] + py;
+    z = st.z[i] + pz;
+    st.vx[i] = vx;
+    st.vy[i] = vy;
+    st.vz[i] = vz;
+    st.x[i] = x;
+    st.y[i] = y;
+    st.z[i] = z;
+  }
+  const double mnx = wave_min(v ? x : INFINITY), mny = wave_min(v ? y : INFINITY),
+               mnz = wave_min(v ? z : INFINITY);
+  const double mxx = wave_max(v ? x : -INFINITY),
+               mxy = wave_max(v ? y : -INFINITY),
+               mxz = wave_max(v ? z : -INFINITY);
+  if (lane_id() == 0) {
+    unsigned long long* b = bbox + 6 * s;
+    atomicMin(b + 0, ord_enc(mnx));
+    atomicMin(b + 1, ord_enc(mny));
+    atomicMin(b + 2, ord_enc(mnz));
+    atomicMax(b + 3, ord_enc(mxx));
+    atomicMax(b + 4, ord_enc(mxy));
+    atomicMax(b + 5, ord_enc(mxz));
+  }
+}
+
+}  // namespace
+
+void launch_adaptive_dt(const State& st, const double* ax, const double* ay,
+                        const double* az, int n, Systems& sys,
+                        const StepParams& p, const int* flags, hipStream_t s) {
+  (void)n;
+  const int nw = sys.n_waves;
+  if (nw > 0)
+    adaptive_max_kernel<<<(nw * 64 + kBlock - 1) / kBlock, kBlock, 0, s>>>(
+        st.vx, st.vy, st.vz, ax, ay, az, sys.start, sys.wave_start, sys.nsys,
+        nw, sys.amax, sys.vmax, flags);
+  adaptive_dt_kernel<<<(sys.nsys + 63) / 64, 64, 0, s>>>(
+      sys.amax, sys.vmax, sys.nsys, p.dt, p.min_adaptive_dt, p.max_pos_change,
+      p.max_vel_change, sys.dt, sys.time, flags);
+}
+
+void launch_update(State& st, const double* ax, const double* ay,
+                   const double* az, int n, Systems& sys, const int* flags,
+                   hipStream_t s) {
+  (void)n;
+  const int nw = sys.n_waves;
+  if (nw > 0)
+    update_kernel<<<(nw * 64 + kBlock - 1) / kBlock, kBlock, 0, s>>>(
+        st, ax, ay, az, sys.start, sys.wave_start, sys.nsys, nw, sys.dt,
+        sys.bbox, flags);
+}
+
+}  // namespace nbs

@@ -1,0 +1,180 @@
+// nbs_internal.h -- device data layout and kernel launchers of libnbs.
+//
+// Data layout in HBM (per context, FP64 state):
+//   particles, structure-of-arrays, in octree (Morton / DFS-leaf) order of
+//   the last build, double-buffered so a build can permute into the other
+//   half:  x y z vx vy vz m (f64), id (u32 caller index), ax ay az (f64).
+//   nodes: one 64-byte WalkNode per tree entry (internal cells and leaves)
+//   in DFS pre-order with children 0..7 -- exactly the order of the
+//   reference octree's recursive traversal (src/octree.cpp:203-313) -- so a
+//   subtree is a contiguous run and `skip` jumps over it.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace nbs {
+
+constexpr double kG = 6.67384e-11;  // src/main.cpp:322
+constexpr int kLevels = 42;         // key depth: 21 levels per u64 half
+constexpr int kWave = 64;
+
+// One tree entry.  Read by the walk with wave-uniform scalar loads, so the
+// whole record is one 64-byte line.
+struct alignas(64) WalkNode {
+  double cx, cy, cz;  // centre of mass (src/octree.cpp:252-278)
+  double m;           // mass (src/octree.cpp:203-221)
+  double side;        // cell side: root side / 2^depth
+  double gm;          // G * m
+  float gm32;         // G * m rounded to f32
+  float r2open32;     // (side / theta)^2 in f32: open if d^2 < this
+  int skip;           // pre-order index after this subtree
+  int leaf;           // >= 0 leaf (particle slot); -1 internal; -2 bucket
+};
+static_assert(sizeof(WalkNode) == 64, "WalkNode must be one 64 B line");
+
+// Per-build tree bookkeeping beside the WalkNode array.
+struct TreeAux {
+  int* depth;   // [cap]
+  int* first;   // [cap] first particle slot
+  int* count;   // [cap] particle count
+  int* parent;  // [cap]
+  int* nchild;  // [cap]
+  int* arrive;  // [cap] bottom-up arrival counters
+};
+
+// Device-side per-system data (batched mode; one system otherwise).
+struct Systems {
+  int nsys;
+  int n_waves;       // host copy of wave_start[nsys]
+  int* start;        // [nsys+1] particle slots
+  int* wave_start;   // [nsys+1] walk waves (64 particles each, per system)
+  int* node_start;   // [nsys+1] pre-order node offsets (written by emit)
+  unsigned long long* bbox;  // [nsys*6] ordered-encoded min xyz, max xyz
+  double* root;      // [nsys*4] centre xyz, side
+  double* dt;        // [nsys] step length used by the last update
+  double* time;      // [nsys] simulated time
+  unsigned long long* amax;  // [nsys] adaptive: max |a| (ordered bits)
+  unsigned long long* vmax;  // [nsys] adaptive: max |v|
+};
+
+struct State {
+  double *x, *y, *z, *vx, *vy, *vz, *m;
+  unsigned* id;
+};
+
+// Flags word bits (device int).
+enum : int { kFlagNodeOverflow = 1, kFlagRunOverflow = 2, kFlagStack = 4 };
+
+struct StepParams {
+  double theta;
+  double dt;
+  int damping;
+  int root_mode;
+  double root_center[3];
+  double root_size;
+  int adaptive;
+  double min_adaptive_dt, max_pos_change, max_vel_change;
+};
+
+// ---------------------------------------------------------------- scan --
+// Exclusive prefix sum of n ints into out (n+1 entries; out[n] = total).
+// tmp needs scan_tmp_ints(n) ints.
+size_t scan_tmp_ints(size_t n);
+void exclusive_scan(const int* in, int* out, int n, int* tmp,
+                    hipStream_t s);
+
+// ----------------------------------------------------------- radix sort --
+// Stable LSD sort of (u64 key, u32 value) pairs over bits [0, nbits).
+// Ping-pongs between (k0,v0) and (k1,v1); returns 0 if the result ends in
+// (k0, v0), 1 if in (k1, v1).  tmp needs radix_tmp_ints(n) ints.
+size_t radix_tmp_ints(size_t n);
+int radix_sort_pairs(unsigned long long* k0, unsigned* v0,
+                     unsigned long long* k1, unsigned* v1, int n, int nbits,
+                     int* tmp, hipStream_t s);
+
+// ----------------------------------------------------------- tree build --
+struct BuildBuffers {
+  unsigned long long *prim0, *prim1;  // sort keys
+  unsigned *val0, *val1;              // sort values (source slot)
+  unsigned long long *khi_u, *klo_u;  // keys in unsorted (current) order
+  unsigned long long *khi, *klo;      // keys in sorted order
+  int* counts;                        // [n]
+  int* offsets;                       // [n+1] node offsets per slot
+  int* sort_tmp;
+  int* scan_tmp;
+  int* flags;                         // device flags word
+  int* stats;                         // [4]: nodes, internal, maxdepth, buckets
+};
+
+void launch_bbox_reset(Systems& sys, hipStream_t s);
+void launch_bbox(const State& st, Systems& sys, hipStream_t s);
+void launch_root(Systems& sys, const StepParams& p, const int* flags,
+                 hipStream_t s);
+// Keys of the particles in `st` (current order) into prim0/val0 and
+// khi_u/klo_u.  Returns the number of significant primary-key bits.
+int launch_keys(const State& st, Systems& sys, BuildBuffers& b,
+                hipStream_t s);
+// Gathers the full keys into sorted order and sorts runs of equal primary
+// keys by (full key, caller index).  perm is updated in place.
+void launch_fixup(BuildBuffers& b, const unsigned long long* prim,
+                  unsigned* perm, const State& st, int n, hipStream_t s);
+void launch_permute(const State& src, State& dst, const unsigned* perm,
+                    int n, hipStream_t s);
+// Emits the pre-order node array; node capacity `cap`.
+void launch_emit(BuildBuffers& b, Systems& sys, const StepParams& p,
+                 WalkNode* nodes, TreeAux& aux, int n, int cap,
+                 hipStream_t s);
+void launch_link(BuildBuffers& b, WalkNode* nodes, TreeAux& aux, int n,
+                 int cap, hipStream_t s);
+void launch_bottom_up(BuildBuffers& b, const State& st, WalkNode* nodes,
+                      TreeAux& aux, int n, hipStream_t s);
+
+// ----------------------------------------------------------------- walk --
+struct WalkArgs {
+  const WalkNode* nodes;
+  const int* node_start;
+  const int* sys_start;
+  const int* wave_start;
+  int nsys;
+  int n_waves;
+  double *x, *y, *z, *vx, *vy, *vz;
+  double *ax, *ay, *az;
+  int* interactions;
+  const int* flags;
+  int* flags_out;
+  double theta;
+  double dt;
+  unsigned long long* bbox;  // next step's bbox accumulators (fused update)
+  double* sys_dt;
+  double* sys_time;
+  int* steps_done;  // incremented once per completed walk
+  int fused_update;
+};
+
+void launch_walk(const WalkArgs& a, int f32, int damping, hipStream_t s);
+
+// FP64 direct sum on sample slots; partial buffer of
+// direct_tmp_doubles(m) doubles.
+size_t direct_tmp_doubles(size_t m);
+void launch_direct(const State& st, int n, const int* sample_slots, int m,
+                   int damping, double* tmp, double* acc_out, hipStream_t s);
+void launch_inverse_perm(const unsigned* id, int n, int* slot_of_id,
+                         hipStream_t s);
+
+// ------------------------------------------------------------ integrate --
+void launch_adaptive_dt(const State& st, const double* ax, const double* ay,
+                        const double* az, int n, Systems& sys,
+                        const StepParams& p, const int* flags, hipStream_t s);
+void launch_update(State& st, const double* ax, const double* ay,
+                   const double* az, int n, Systems& sys, const int* flags,
+                   hipStream_t s);
+
+// Small fills (no hipMemsetAsync: these run inside captured graphs too).
+void launch_fill_int(int* p, int v, size_t n, hipStream_t s);
+void launch_fill_u64(unsigned long long* p, unsigned long long v, size_t n,
+                     hipStream_t s);
+
+}  // namespace nbs

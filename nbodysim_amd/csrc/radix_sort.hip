@@ -1,0 +1,170 @@
+// radix_sort.hip -- stable LSD radix sort of (u64 key, u32 value) pairs,
+// 8-bit digits.  Per pass: an upsweep writes each tile's digit histogram
+// (digit-major), a device scan turns them into global offsets, and a
+// downsweep ranks the tile's keys stably with wave64 ballot matching,
+// sorts the tile in LDS and writes digit runs out contiguously.
+#include "nbs_device_util.h"
+#include "nbs_internal.h"
+
+namespace nbs {
+namespace {
+
+constexpr int kRsBlock = 256;
+constexpr int kRsItems = 16;
+constexpr int kRsTile = kRsBlock * kRsItems;
+constexpr int kRadix = 256;
+constexpr int kRsWaves = kRsBlock / 64;
+
+__global__ void __launch_bounds__(kRsBlock)
+    upsweep_kernel(const unsigned long long* __restrict__ keys, int n,
+                   int shift, int ntiles, int* __restrict__ hist) {
+  __shared__ int h[kRadix];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const int base = blockIdx.x * kRsTile;
+#pragma unroll 4
+  for (int k = 0; k < kRsItems; k++) {
+    int i = base + k * kRsBlock + threadIdx.x;
+    if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255], 1);
+  }
+  __syncthreads();
+  hist[threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(kRsBlock)
+    downsweep_kernel(const unsigned long long* __restrict__ kin,
+                     const unsigned* __restrict__ vin,
+                     unsigned long long* __restrict__ kout,
+                     unsigned* __restrict__ vout, int n, int shift,
+                     int ntiles, const int* __restrict__ goff) {
+  __shared__ unsigned long long sk[kRsTile];
+  __shared__ unsigned sv[kRsTile];
+  __shared__ int tile_start[kRadix];
+  __shared__ int gbase[kRadix];
+  __shared__ int run[kRadix];
+  __shared__ int wcnt[kRsWaves][kRadix];
+  __shared__ int wsum[kRsWaves];
+
+  const int t = threadIdx.x;
+  const int lane = lane_id();
+  const int w = t >> 6;
+  const int base = blockIdx.x * kRsTile;
+  const int tile_n = min(kRsTile, n - base);
+
+  unsigned long long key[kRsItems];
+  unsigned val[kRsItems];
+  int dig[kRsItems];
+
+  run[t] = 0;
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kRsItems; k++) {
+    int j = k * kRsBlock + t;
+    if (j < tile_n) {
+      key[k] = kin[base + j];
+      val[k] = vin[base + j];
+      dig[k] = (int)((key[k] >> shift) & 255);
+      atomicAdd(&run[dig[k]], 1);
+    } else {
+      key[k] = 0;
+      val[k] = 0;
+      dig[k] = kRadix;
+    }
+  }
+  __syncthreads();
+  // Tile-local exclusive scan of the digit counts (one digit per thread).
+  {
+    int c = run[t];
+    int inc = wave_inclusive_scan(c);
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    int wb = 0;
+    for (int k = 0; k < w; k++) wb += wsum[k];
+    tile_start[t] = wb + inc - c;
+    gbase[t] = goff[t * ntiles + blockIdx.x];
+    run[t] = 0;
+  }
+  __syncthreads();
+
+  for (int k = 0; k < kRsItems; k++) {
+    wcnt[w][lane] = 0;
+    wcnt[w][lane + 64] = 0;
+    wcnt[w][lane + 128] = 0;
+    wcnt[w][lane + 192] = 0;
+    const int d = dig[k];
+    const bool valid = d < kRadix;
+    unsigned long long peers = ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+      bool bit = (d >> b) & 1;
+      unsigned long long bb = ballot(bit);
+      peers &= bit ? bb : ~bb;
+    }
+    const int rank = mbcnt(peers);
+    if (valid && rank == 0) wcnt[w][d] = __popcll(peers);
+    __syncthreads();
+    if (valid) {
+      int pos = tile_start[d] + run[d] + rank;
+      for (int q = 0; q < w; q++) pos += wcnt[q][d];
+      sk[pos] = key[k];
+      sv[pos] = val[k];
+    }
+    __syncthreads();
+    int add = 0;
+#pragma unroll
+    for (int q = 0; q < kRsWaves; q++) add += wcnt[q][t];
+    run[t] += add;
+    __syncthreads();
+  }
+
+  for (int j = t; j < tile_n; j += kRsBlock) {
+    unsigned long long kk = sk[j];
+    int d = (int)((kk >> shift) & 255);
+    const int o = gbase[d] + (j - tile_start[d]);
+    if (o >= 0 && o < n) {
+      kout[o] = kk;
+      vout[o] = sv[j];
+    }
+  }
+}
+
+}  // namespace
+
+size_t radix_tmp_ints(size_t n) {
+  size_t ntiles = (n + kRsTile - 1) / kRsTile;
+  if (ntiles == 0) ntiles = 1;
+  size_t h = kRadix * ntiles;
+  return 2 * h + 1 + scan_tmp_ints(h) + 64;
+}
+
+int radix_sort_pairs(unsigned long long* k0, unsigned* v0,
+                     unsigned long long* k1, unsigned* v1, int n, int nbits,
+                     int* tmp, hipStream_t s) {
+  if (n <= 1 || nbits <= 0) return 0;
+  int ntiles = (n + kRsTile - 1) / kRsTile;
+  int h = kRadix * ntiles;
+  int* hist = tmp;
+  int* goff = tmp + h;
+  int* stmp = tmp + 2 * h + 1;
+  unsigned long long* ka = k0;
+  unsigned* va = v0;
+  unsigned long long* kb = k1;
+  unsigned* vb = v1;
+  int which = 0;
+  for (int shift = 0; shift < nbits; shift += 8) {
+    upsweep_kernel<<<ntiles, kRsBlock, 0, s>>>(ka, n, shift, ntiles, hist);
+    exclusive_scan(hist, goff, h, stmp, s);
+    downsweep_kernel<<<ntiles, kRsBlock, 0, s>>>(ka, va, kb, vb, n, shift,
+                                                 ntiles, goff);
+    unsigned long long* tk = ka;
+    ka = kb;
+    kb = tk;
+    unsigned* tv = va;
+    va = vb;
+    vb = tv;
+    which ^= 1;
+  }
+  return which;
+}
+
+}  // namespace nbs

@@ -1,0 +1,554 @@
+// tree_build.hip -- Morton-keyed octree build on the GPU that reproduces
+// the reference's pointer octree node for node.
+//
+// Reference tree (src/octree.cpp:54-196): one particle per leaf; a cell
+// splits exactly when it holds two or more particles; child index
+// (x>cx) + 2(y>cy) + 4(z>cz); child centre = centre + (+-1 * half) / 2
+// with half = side / 2, every addition rounded in the datatype.  That tree
+// is canonical (independent of insertion order), so it is fully described
+// by each particle's path of child indices computed with the same rounded
+// centre chain (keys_kernel), sorted (radix_sort.hip), plus the lengths of
+// the common prefixes of sorted neighbours:
+//   * l_i = levels shared by keys i and i+1 (-1 across a system boundary);
+//   * internal cells starting at slot i have depths (l_{i-1}, l_i];
+//   * particle i is a leaf at depth 1 + max(l_{i-1}, l_i).
+// Cells are emitted in DFS pre-order (children 0..7), the order of the
+// reference's recursive calc_mass/calc_com, so a cell's subtree is the run
+// up to its `skip`.  Mass and COM are then reduced bottom-up, each cell
+// summing its children in child order with the reference's operation
+// sequence (src/octree.cpp:203-221, :252-278), so they match bit for bit.
+#include <cmath>
+
+#include "nbs_device_util.h"
+#include "nbs_internal.h"
+
+namespace nbs {
+namespace {
+
+constexpr int kBlock = 256;
+
+// Common prefix of two 42-level keys, in levels (0..42).
+__device__ __forceinline__ int cpl(unsigned long long ahi,
+                                   unsigned long long alo,
+                                   unsigned long long bhi,
+                                   unsigned long long blo) {
+  unsigned long long x = ahi ^ bhi;
+  if (x) return (__clzll(x) - 1) / 3;
+  x = alo ^ blo;
+  if (x) return 21 + (__clzll(x) - 1) / 3;
+  return kLevels;
+}
+
+__global__ void fill_int_kernel(int* p, int v, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void fill_u64_kernel(unsigned long long* p, unsigned long long v,
+                                size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void bbox_reset_kernel(unsigned long long* bbox, int nsys) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nsys * 6)
+    bbox[i] = (i % 6) < 3 ? ord_enc(INFINITY) : ord_enc(-INFINITY);
+}
+
+// Bounding box per system: wave min/max, then one atomic per wave.
+__global__ void __launch_bounds__(kBlock)
+    bbox_kernel(const double* __restrict__ x, const double* __restrict__ y,
+                const double* __restrict__ z, const int* __restrict__ start,
+                const int* __restrict__ wstart, int nsys, int nwaves,
+                unsigned long long* __restrict__ bbox) {
+  const int w = (blockIdx.x * kBlock + threadIdx.x) >> 6;
+  if (w >= nwaves) return;
+  const int s = nsys > 1 ? sys_of_wave(w, wstart, nsys) : 0;
+  const int i = start[s] + (w - wstart[s]) * 64 + lane_id();
+  const bool v = i < start[s + 1];
+  double px = v ? x[i] : 0, py = v ? y[i] : 0, pz = v ? z[i] : 0;
+  double mnx = wave_min(v ? px : INFINITY), mny = wave_min(v ? py : INFINITY),
+         mnz = wave_min(v ? pz : INFINITY);
+  double mxx = wave_max(v ? px : -INFINITY),
+         mxy = wave_max(v ? py : -INFINITY),
+         mxz = wave_max(v ? pz : -INFINITY);
+  if (lane_id() == 0) {
+    unsigned long long* b = bbox + 6 * s;
+    atomicMin(b + 0, ord_enc(mnx));
+    atomicMin(b + 1, ord_enc(mny));
+    atomicMin(b + 2, ord_enc(mnz));
+    atomicMax(b + 3, ord_enc(mxx));
+    atomicMax(b + 4, ord_enc(mxy));
+    atomicMax(b + 5, ord_enc(mxz));
+  }
+}
+
+// Root cell per system: src/main.cpp:908-920 in FP64 (midpoint of the
+// bbox; side = largest extent + 2.0, ties falling through to z).  Resets
+// the bbox accumulators for the next step.
+__global__ void root_kernel(unsigned long long* __restrict__ bbox,
+                            double* __restrict__ root, int nsys, int mode,
+                            double cx, double cy, double cz, double side,
+                            const int* __restrict__ flags) {
+#pragma clang fp contract(off)
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsys || *flags) return;
+  unsigned long long* b = bbox + 6 * s;
+  if (mode == 1) {
+    root[4 * s] = cx;
+    root[4 * s + 1] = cy;
+    root[4 * s + 2] = cz;
+    root[4 * s + 3] = side;
+  } else {
+    double min_x = ord_dec(b[0]), min_y = ord_dec(b[1]), min_z = ord_dec(b[2]);
+    double max_x = ord_dec(b[3]), max_y = ord_dec(b[4]), max_z = ord_dec(b[5]);
+    root[4 * s] = (min_x + max_x) / 2.0;
+    root[4 * s + 1] = (min_y + max_y) / 2.0;
+    root[4 * s + 2] = (min_z + max_z) / 2.0;
+    double ex = max_x - min_x, ey = max_y - min_y, ez = max_z - min_z;
+    double sz;
+    if (ex > ey && ex > ez)
+      sz = ex + 2.0;
+    else if (ey > ex && ey > ez)
+      sz = ey + 2.0;
+    else
+      sz = ez + 2.0;
+    root[4 * s + 3] = sz;
+  }
+  b[0] = b[1] = b[2] = ord_enc(INFINITY);
+  b[3] = b[4] = b[5] = ord_enc(-INFINITY);
+}
+
+// 42-level keys by the reference's rounded centre chain.  The primary sort
+// key is the key's top levels behind the system index (batched mode).
+__global__ void __launch_bounds__(kBlock)
+    keys_kernel(const double* __restrict__ x, const double* __restrict__ y,
+                const double* __restrict__ z, const int* __restrict__ start,
+                const int* __restrict__ wstart, int nsys, int nwaves,
+                const double* __restrict__ root, int sys_shift,
+                int prim_levels, unsigned long long* __restrict__ prim,
+                unsigned* __restrict__ val,
+                unsigned long long* __restrict__ khi,
+                unsigned long long* __restrict__ klo) {
+#pragma clang fp contract(off)
+  const int w = (blockIdx.x * kBlock + threadIdx.x) >> 6;
+  if (w >= nwaves) return;
+  const int s = nsys > 1 ? sys_of_wave(w, wstart, nsys) : 0;
+  const int i = start[s] + (w - wstart[s]) * 64 + lane_id();
+  if (i >= start[s + 1]) return;
+  const double px = x[i], py = y[i], pz = z[i];
+  double cx = root[4 * s], cy = root[4 * s + 1], cz = root[4 * s + 2];
+  double sd = root[4 * s + 3];
+  unsigned long long hi = 0, lo = 0;
+#pragma unroll
+  for (int d = 0; d < kLevels; d++) {
+    const bool bx = px > cx, by = py > cy, bz = pz > cz;
+    const unsigned o = (unsigned)bx | ((unsigned)by << 1) | ((unsigned)bz << 2);
+    const double half = sd * 0.5;  // side / 2 (exact)
+    const double q = half * 0.5;   // (+-1 * half) / 2 (exact)
+    cx = cx + (bx ? q : -q);
+    cy = cy + (by ? q : -q);
+    cz = cz + (bz ? q : -q);
+    sd = half;
+    if (d < 21)
+      hi |= (unsigned long long)o << (60 - 3 * d);
+    else
+      lo |= (unsigned long long)o << (60 - 3 * (d - 21));
+  }
+  khi[i] = hi;
+  klo[i] = lo;
+  unsigned long long p = hi;
+  if (sys_shift > 0)
+    p = ((unsigned long long)s << sys_shift) | (hi >> (63 - 3 * prim_levels));
+  prim[i] = p;
+  val[i] = (unsigned)i;
+}
+
+__global__ void gather_keys_kernel(const unsigned* __restrict__ perm, int n,
+                                   const unsigned long long* __restrict__ hu,
+                                   const unsigned long long* __restrict__ lu,
+                                   unsigned long long* __restrict__ h,
+                                   unsigned long long* __restrict__ l) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const unsigned p = perm[j];
+  if (p >= (unsigned)n) return;
+  h[j] = hu[p];
+  l[j] = lu[p];
+}
+
+// Runs of equal primary keys (rare: particles closer than the primary
+// key's resolution) are sorted by (full key, caller index), one thread per
+// run, insertion sort in place.
+__global__ void fixup_kernel(const unsigned long long* __restrict__ prim,
+                             unsigned* __restrict__ perm,
+                             unsigned long long* __restrict__ khi,
+                             unsigned long long* __restrict__ klo,
+                             const unsigned* __restrict__ id, int n,
+                             int* __restrict__ flags) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j + 1 >= n) return;
+  if (prim[j] != prim[j + 1]) return;
+  if (j > 0 && prim[j - 1] == prim[j]) return;
+  int e = j + 1;
+  while (e + 1 < n && prim[e + 1] == prim[j]) e++;
+  if (e - j + 1 > 8192) {
+    atomicOr(flags, kFlagRunOverflow);
+    return;
+  }
+  for (int a = j + 1; a <= e; a++) {
+    unsigned long long h = khi[a], l = klo[a];
+    unsigned p = perm[a];
+    unsigned pid = id[p];
+    int b = a - 1;
+    while (b >= j) {
+      unsigned long long bh = khi[b], bl = klo[b];
+      unsigned bid = id[perm[b]];
+      bool greater = bh > h || (bh == h && (bl > l || (bl == l && bid > pid)));
+      if (!greater) break;
+      khi[b + 1] = bh;
+      klo[b + 1] = bl;
+      perm[b + 1] = perm[b];
+      b--;
+    }
+    khi[b + 1] = h;
+    klo[b + 1] = l;
+    perm[b + 1] = p;
+  }
+}
+
+__global__ void permute_kernel(State src, State dst,
+                               const unsigned* __restrict__ perm, int n) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const unsigned p = perm[j];
+  if (p >= (unsigned)n) return;
+  dst.x[j] = src.x[p];
+  dst.y[j] = src.y[p];
+  dst.z[j] = src.z[p];
+  dst.vx[j] = src.vx[p];
+  dst.vy[j] = src.vy[p];
+  dst.vz[j] = src.vz[p];
+  dst.m[j] = src.m[p];
+  dst.id[j] = src.id[p];
+}
+
+struct Neigh {
+  int s, a, e, lp, ln;
+};
+
+__device__ __forceinline__ Neigh neighbours(
+    int i, const int* __restrict__ start, int nsys,
+    const unsigned long long* __restrict__ khi,
+    const unsigned long long* __restrict__ klo) {
+  Neigh r;
+  r.s = nsys > 1 ? sys_of_slot(i, start, nsys) : 0;
+  r.a = start[r.s];
+  r.e = start[r.s + 1];
+  const unsigned long long hi = khi[i], lo = klo[i];
+  r.lp = i > r.a ? cpl(khi[i - 1], klo[i - 1], hi, lo) : -1;
+  r.ln = i + 1 < r.e ? cpl(hi, lo, khi[i + 1], klo[i + 1]) : -1;
+  return r;
+}
+
+// Entries (internal cells + the leaf) that start at slot i.
+__global__ void __launch_bounds__(kBlock)
+    emit_count_kernel(const int* __restrict__ start, int nsys, int n,
+                      const unsigned long long* __restrict__ khi,
+                      const unsigned long long* __restrict__ klo,
+                      int* __restrict__ counts) {
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  Neigh g = neighbours(i, start, nsys, khi, klo);
+  counts[i] = max(0, g.ln - g.lp) + 1;
+}
+
+__global__ void sys_nodes_kernel(const int* __restrict__ start,
+                                 const int* __restrict__ offsets, int nsys,
+                                 int* __restrict__ node_start) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s <= nsys) node_start[s] = offsets[start[s]];
+}
+
+// Last slot j in [i, e) whose key shares >= d levels with key i.
+__device__ __forceinline__ int range_end(int i, int d, int e,
+                                         const unsigned long long* khi,
+                                         const unsigned long long* klo) {
+  const unsigned long long hi = khi[i], lo = klo[i];
+  int good = i, bad = e, step = 1;
+  for (;;) {
+    int probe = good + step;
+    if (probe >= e) break;
+    if (cpl(hi, lo, khi[probe], klo[probe]) >= d) {
+      good = probe;
+      step <<= 1;
+    } else {
+      bad = probe;
+      break;
+    }
+  }
+  while (bad - good > 1) {
+    int mid = good + ((bad - good) >> 1);
+    if (cpl(hi, lo, khi[mid], klo[mid]) >= d)
+      good = mid;
+    else
+      bad = mid;
+  }
+  return good;
+}
+
+__global__ void __launch_bounds__(kBlock)
+    emit_kernel(const int* __restrict__ start, int nsys, int n,
+                const unsigned long long* __restrict__ khi,
+                const unsigned long long* __restrict__ klo,
+                const int* __restrict__ offsets,
+                const double* __restrict__ root, double theta, int cap,
+                WalkNode* __restrict__ nodes, TreeAux aux,
+                int* __restrict__ flags, int* __restrict__ stats) {
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  const int total = offsets[n];
+  if (total > cap) {
+    if (i == 0) atomicOr(flags, kFlagNodeOverflow);
+    return;
+  }
+  int my_depth = 0, my_bucket = 0;
+  if (i < n) {
+    Neigh g = neighbours(i, start, nsys, khi, klo);
+    const double s0 = root[4 * g.s + 3];
+    const int base = offsets[i];
+    const int nint = max(0, g.ln - g.lp);
+    for (int k = 0; k < nint; k++) {
+      const int d = g.lp + 1 + k;
+      const int node = base + k;
+      const int b = range_end(i, d, g.e, khi, klo);
+      WalkNode w;
+      w.cx = w.cy = w.cz = 0;
+      w.m = w.gm = 0;
+      w.gm32 = 0;
+      w.side = ldexp(s0, -d);
+      const bool bucket = d >= kLevels;
+      const double ro = w.side / theta;
+      w.r2open32 = bucket ? INFINITY : (float)(ro * ro);
+      w.skip = offsets[b + 1];
+      w.leaf = bucket ? -2 : -1;
+      nodes[node] = w;
+      aux.depth[node] = d;
+      aux.first[node] = i;
+      aux.count[node] = b - i + 1;
+      if (bucket) my_bucket++;
+    }
+    const int leafn = base + nint;
+    const int ld = 1 + max(g.lp, g.ln);
+    WalkNode w;
+    w.cx = w.cy = w.cz = 0;
+    w.m = w.gm = 0;
+    w.gm32 = 0;
+    w.side = ldexp(s0, -ld);
+    w.r2open32 = -1.0f;
+    w.skip = leafn + 1;
+    w.leaf = i;
+    nodes[leafn] = w;
+    aux.depth[leafn] = ld;
+    aux.first[leafn] = i;
+    aux.count[leafn] = 1;
+    my_depth = ld;
+  }
+  // tree statistics: nodes, internal cells, max depth, buckets
+  int md = my_depth;
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) md = max(md, __shfl_xor(md, d, 64));
+  int nb = wave_sum(my_bucket);
+  if (lane_id() == 0) {
+    atomicMax(&stats[2], md);
+    if (nb) atomicAdd(&stats[3], nb);
+  }
+  if (i == 0) {
+    stats[0] = total;
+    stats[1] = total - n;
+  }
+}
+
+// Parent links and child counts; clears the bottom-up arrival counters.
+__global__ void __launch_bounds__(kBlock)
+    link_kernel(const int* __restrict__ offsets, int n,
+                const WalkNode* __restrict__ nodes, TreeAux aux,
+                const int* __restrict__ flags, int* flags_out) {
+  if (*flags) return;
+  const int total = offsets[n];
+  for (int k = blockIdx.x * kBlock + threadIdx.x; k < total;
+       k += gridDim.x * kBlock) {
+    aux.arrive[k] = 0;
+    if (aux.depth[k] == 0) aux.parent[k] = -1;
+    if (nodes[k].leaf >= 0) {
+      aux.nchild[k] = 0;
+      continue;
+    }
+    const int end = min(nodes[k].skip, total);
+    int c = k + 1, cnt = 0;
+    while (c < end) {
+      aux.parent[c] = k;
+      cnt++;
+      const int nx = nodes[c].skip;
+      if (nx <= c) {  // corrupt tree: stop rather than loop
+        atomicOr(flags_out, kFlagNodeOverflow);
+        break;
+      }
+      c = nx;
+    }
+    aux.nchild[k] = cnt;
+  }
+}
+
+// Leaves take their particle; the last child to finish reduces its parent
+// (children in order 0..7, the reference's operation sequence), climbing
+// while it keeps arriving last.
+__global__ void __launch_bounds__(kBlock)
+    bottom_up_kernel(const int* __restrict__ offsets, int n,
+                     const double* __restrict__ x, const double* __restrict__ y,
+                     const double* __restrict__ z, const double* __restrict__ m,
+                     WalkNode* nodes, TreeAux aux, const int* __restrict__ flags) {
+#pragma clang fp contract(off)
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n || *flags) return;
+  const int leafn = offsets[i + 1] - 1;
+  const double mi = m[i];
+  nodes[leafn].cx = x[i];
+  nodes[leafn].cy = y[i];
+  nodes[leafn].cz = z[i];
+  nodes[leafn].m = mi;
+  nodes[leafn].gm = kG * mi;
+  nodes[leafn].gm32 = (float)(kG * mi);
+  __threadfence();
+  int p = aux.parent[leafn];
+  for (int level = 0; p >= 0 && level <= kLevels + 2; level++) {
+    const int old = atomicAdd(&aux.arrive[p], 1);
+    if (old != aux.nchild[p] - 1) break;
+    __threadfence();
+    const int end = nodes[p].skip;
+    double M = 0, tx = 0, ty = 0, tz = 0;
+    int c = p + 1;
+    while (c < end) {
+      const double cm = nodes[c].m;
+      M = M + cm;
+      c = max(nodes[c].skip, c + 1);
+    }
+    double cxv, cyv, czv;
+    if (M == 0) {
+      cxv = cyv = czv = 0;  // src/octree.cpp:259-262
+    } else {
+      c = p + 1;
+      while (c < end) {
+        const double cm = nodes[c].m;
+        const double px = nodes[c].cx * cm;
+        const double py = nodes[c].cy * cm;
+        const double pz = nodes[c].cz * cm;
+        tx = tx + px;
+        ty = ty + py;
+        tz = tz + pz;
+        c = max(nodes[c].skip, c + 1);
+      }
+      cxv = tx / M;
+      cyv = ty / M;
+      czv = tz / M;
+    }
+    nodes[p].cx = cxv;
+    nodes[p].cy = cyv;
+    nodes[p].cz = czv;
+    nodes[p].m = M;
+    nodes[p].gm = kG * M;
+    nodes[p].gm32 = (float)(kG * M);
+    __threadfence();
+    p = aux.parent[p];
+  }
+}
+
+inline int blocks_for(size_t n, int b) { return (int)((n + b - 1) / b); }
+
+}  // namespace
+
+void launch_fill_int(int* p, int v, size_t n, hipStream_t s) {
+  if (n) fill_int_kernel<<<blocks_for(n, kBlock), kBlock, 0, s>>>(p, v, n);
+}
+void launch_fill_u64(unsigned long long* p, unsigned long long v, size_t n,
+                     hipStream_t s) {
+  if (n) fill_u64_kernel<<<blocks_for(n, kBlock), kBlock, 0, s>>>(p, v, n);
+}
+
+void launch_bbox_reset(Systems& sys, hipStream_t s) {
+  bbox_reset_kernel<<<blocks_for(6 * sys.nsys, 64), 64, 0, s>>>(sys.bbox,
+                                                                sys.nsys);
+}
+
+void launch_bbox(const State& st, Systems& sys, hipStream_t s) {
+  const int nw = sys.n_waves;
+  if (nw <= 0) return;
+  bbox_kernel<<<blocks_for((size_t)nw * 64, kBlock), kBlock, 0, s>>>(
+      st.x, st.y, st.z, sys.start, sys.wave_start, sys.nsys, nw, sys.bbox);
+}
+
+void launch_root(Systems& sys, const StepParams& p, const int* flags,
+                 hipStream_t s) {
+  root_kernel<<<blocks_for(sys.nsys, 64), 64, 0, s>>>(
+      sys.bbox, sys.root, sys.nsys, p.root_mode, p.root_center[0],
+      p.root_center[1], p.root_center[2], p.root_size, flags);
+}
+
+int launch_keys(const State& st, Systems& sys, BuildBuffers& b,
+                hipStream_t s) {
+  const int nw = sys.n_waves;
+  int sys_bits = 0;
+  while ((1 << sys_bits) < sys.nsys) sys_bits++;
+  int levels = sys_bits > 0 ? (64 - sys_bits) / 3 : 21;
+  if (levels > 21) levels = 21;
+  const int sys_shift = sys_bits > 0 ? 3 * levels : 0;
+  if (nw > 0)
+    keys_kernel<<<blocks_for((size_t)nw * 64, kBlock), kBlock, 0, s>>>(
+        st.x, st.y, st.z, sys.start, sys.wave_start, sys.nsys, nw, sys.root,
+        sys_shift, levels, b.prim0, b.val0, b.khi_u, b.klo_u);
+  return sys_bits > 0 ? sys_bits + 3 * levels : 63;
+}
+
+void launch_fixup(BuildBuffers& b, const unsigned long long* prim,
+                  unsigned* perm, const State& st, int n, hipStream_t s) {
+  if (n <= 0) return;
+  gather_keys_kernel<<<blocks_for(n, kBlock), kBlock, 0, s>>>(
+      perm, n, b.khi_u, b.klo_u, b.khi, b.klo);
+  fixup_kernel<<<blocks_for(n, kBlock), kBlock, 0, s>>>(prim, perm, b.khi,
+                                                        b.klo, st.id, n,
+                                                        b.flags);
+}
+
+void launch_permute(const State& src, State& dst, const unsigned* perm, int n,
+                    hipStream_t s) {
+  if (n) permute_kernel<<<blocks_for(n, kBlock), kBlock, 0, s>>>(src, dst, perm, n);
+}
+
+void launch_emit(BuildBuffers& b, Systems& sys, const StepParams& p,
+                 WalkNode* nodes, TreeAux& aux, int n, int cap,
+                 hipStream_t s) {
+  emit_count_kernel<<<blocks_for(n, kBlock), kBlock, 0, s>>>(
+      sys.start, sys.nsys, n, b.khi, b.klo, b.counts);
+  exclusive_scan(b.counts, b.offsets, n, b.scan_tmp, s);
+  sys_nodes_kernel<<<blocks_for(sys.nsys + 1, 64), 64, 0, s>>>(
+      sys.start, b.offsets, sys.nsys, sys.node_start);
+  launch_fill_int(b.stats, 0, 4, s);
+  emit_kernel<<<blocks_for(n, kBlock), kBlock, 0, s>>>(
+      sys.start, sys.nsys, n, b.khi, b.klo, b.offsets, sys.root, p.theta, cap,
+      nodes, aux, b.flags, b.stats);
+}
+
+void launch_link(BuildBuffers& b, WalkNode* nodes, TreeAux& aux, int n,
+                 int cap, hipStream_t s) {
+  int blocks = blocks_for(cap, kBlock);
+  if (blocks > 4096) blocks = 4096;
+  link_kernel<<<blocks, kBlock, 0, s>>>(b.offsets, n, nodes, aux, b.flags,
+                                       b.flags);
+}
+
+void launch_bottom_up(BuildBuffers& b, const State& st, WalkNode* nodes,
+                      TreeAux& aux, int n, hipStream_t s) {
+  bottom_up_kernel<<<blocks_for(n, kBlock), kBlock, 0, s>>>(
+      b.offsets, n, st.x, st.y, st.z, st.m, nodes, aux, b.flags);
+}
+
+}  // namespace nbs

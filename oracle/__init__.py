@@ -245,6 +245,21 @@ def ref_lib(prec="f64"):
     return _ref[prec]
 
 
+def ref_particle_raw(pos, vel, acc, mass, prec="f64"):
+    """Raw bytes of the reference's particle objects (what .dat holds)."""
+    L = ref_lib(prec)
+    L.ref_particle_bytes.restype = ctypes.c_int
+    rec = L.ref_particle_bytes()
+    pos, vel, acc, mass = (np.ascontiguousarray(a, dtype=np.float64)
+                           for a in (pos, vel, acc, mass))
+    out = np.zeros(len(mass) * rec, np.uint8)
+    L.ref_particle_raw.argtypes = [ctypes.c_size_t, _dp, _dp, _dp, _dp,
+                                   ctypes.POINTER(ctypes.c_uint8)]
+    L.ref_particle_raw(len(mass), _p(pos), _p(vel), _p(acc), _p(mass),
+                       out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+    return out.tobytes()
+
+
 def ref_tree_dump(pos, mass, prec="f64"):
     L = ref_lib(prec)
     pos = np.ascontiguousarray(pos, dtype=np.float64)

@@ -18,6 +18,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <cstring>
 #include <limits>
 #include <queue>
 #include <unordered_map>
@@ -133,6 +134,20 @@ void walk(RefState& s, particle* curr, datatype theta, bool damping,
 extern "C" {
 
 int ref_datatype_bytes(void) { return (int)sizeof(datatype); }
+
+// The bytes src/main.cpp:737 writes per particle: the raw object.
+void ref_particle_raw(size_t n, const double* pos, const double* vel,
+                      const double* acc, const double* mass,
+                      unsigned char* out) {
+  for (size_t i = 0; i < n; i++) {
+    vector p(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+    vector v(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2]);
+    vector a(acc[3 * i], acc[3 * i + 1], acc[3 * i + 2]);
+    particle par(&p, &v, &a, mass[i]);
+    std::memcpy(out + i * sizeof(particle), (const void*)&par,
+                sizeof(particle));
+  }
+}
 int ref_particle_bytes(void) { return (int)sizeof(particle); }
 int ref_octree_bytes(void) { return (int)sizeof(octree); }
 

@@ -1,0 +1,178 @@
+"""The host contract kept by libnbs.so (include/nbs_host.h): settings.cfg
+keys and parsing quirks, the seeded generator (bit-exact with the oracle),
+and the .dat / .txt frame files (bytes of the reference's particle records,
+ostream's default number format).  CPU only: no GPU call is made."""
+import ctypes
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from nbodysim_amd import host
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                   "reference_golden.json")))
+
+
+def write(tmp_path, text, name="s.cfg"):
+    p = tmp_path / name
+    p.write_text(text)
+    return str(p)
+
+
+def test_defaults_match_reference():
+    s = host.default_settings()
+    # src/main.cpp:1312-1357
+    assert s["num_particles"] == 5000 and s["num_frames"] == 300
+    assert s["theta"] == 0.5 and s["dt"] == 0.0333 and s["size"] == 512
+    assert s["seed"] == 0xDEADBEEF and s["min_mass"] == 5e10 and s["max_mass"] == 5e11
+    assert s["gen_type"] == "cube" and s["mass_dist"] == "linear"
+    assert s["dump_binary"] and s["dump_text"] and s["dump_image"]
+    assert not s["overwrite_data"] and s["keep_previous_text"]
+    assert list(s["rotation_vector"]) == [0, 0, 1] and s["rotation_magnitude"] == 0.1
+    assert s["threads"] == 1 and s["projection"] == "iso" and s["img_w"] == 1920
+
+
+def test_reference_sample_settings(tmp_path):
+    # the reference's shipped settings.cfg (settings.cfg:1-22)
+    text = """read_existing true
+display_progress true
+dump_binary true
+overwrite_data true
+num_particles 50000
+num_frames 300
+size 512
+theta 0.5
+dt 0.0333
+min_mass 2e9
+max_mass 2e10
+projection iso
+brightness 16
+gen_type sphere
+rotation_vector 0 0 1
+rotation_magnitude 0.05
+r_sphere 100
+scale_z 0.2
+mass_dist exp
+damping true
+threads 8
+verbose true
+"""
+    s, msg = host.read_settings(write(tmp_path, text))
+    assert msg == ""
+    assert s["read_existing"] and s["overwrite_data"] and s["damping"] and s["verbose"]
+    assert s["num_particles"] == 50000 and s["min_mass"] == 2e9 and s["max_mass"] == 2e10
+    assert s["gen_type"] == "sphere" and s["mass_dist"] == "exp"
+    assert s["rotation_magnitude"] == 0.05 and s["scale_z"] == 0.2 and s["threads"] == 8
+
+
+def test_parser_quirks(tmp_path):
+    s, msg = host.read_settings(write(tmp_path, "use_seed yes\nfoo 3\ntrue\nseed 0xDEADBEEF\n"))
+    assert not s["use_seed"]               # only the literal "true" is true
+    assert "Unrecognized variable foo" in msg
+    assert "Unrecognized variable 3" in msg
+    assert s["seed"] == 0                   # decimal read stops at 'x'
+    assert "Unrecognized variable xDEADBEEF" in msg
+
+
+def test_invalid_enum_rejected(tmp_path):
+    with pytest.raises(ValueError):
+        host.read_settings(write(tmp_path, "gen_type torus\n"))
+
+
+def test_missing_file():
+    with pytest.raises(FileNotFoundError):
+        host.read_settings("/nonexistent/settings.cfg")
+
+
+def test_extension_keys(tmp_path):
+    s, msg = host.read_settings(write(tmp_path, "adaptive true\nmin_adaptive_dt 0.001\n"
+                                               "max_pos_change 2\nforce_precision f32\n"
+                                               "shell_jitter 0.01\n"))
+    assert msg == "" and s["adaptive"] and s["min_adaptive_dt"] == 0.001
+    assert s["force_precision"] == "f32" and s["shell_jitter"] == 0.01
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("gen,extra", [
+    ("cube", {}), ("cube", {"min_vel": 1.0, "max_vel": 5.0}),
+    ("sphere", {}), ("shell", {}),
+    ("cube", {"mass_dist": "exp"}), ("sphere", {"mass_dist": "normal"}),
+    ("cube", {"vel_dist": "normal", "min_vel": 0.5, "max_vel": 2.0}),
+    ("sphere", {"scale_z": 0.2, "rotation_vector": (1.0, 2.0, 3.0),
+                "rotation_magnitude": 0.05}),
+])
+def test_generator_bit_exact_with_oracle(prec, gen, extra):
+    kw = dict(extra)
+    scale = (1.0, 1.0, kw.pop("scale_z", 1.0))
+    p, v, m = host.generate(3000, gen_type=gen, seed=0xDEADBEEF, prec=prec, scale=scale, **kw)
+    op, ov, om = O.generate(3000, gen, seed=0xDEADBEEF, prec=prec, scale=scale, **kw)
+    np.testing.assert_array_equal(p, op)
+    np.testing.assert_array_equal(v, ov)
+    np.testing.assert_array_equal(m, om)
+
+
+def test_generator_matches_golden_digest():
+    c = GOLD["cases"]["cube5000_f64"]
+    p, v, m = host.generate(5000, gen_type="cube", seed=c["seed"])
+    import hashlib
+    assert hashlib.sha256(p.tobytes()).hexdigest() == c["ic"]["pos"]
+    assert hashlib.sha256(m.tobytes()).hexdigest() == c["ic"]["mass"]
+
+
+def test_shell_jitter_zero_is_reference():
+    p0, _, _ = host.generate(100, gen_type="shell", shell_jitter=0.0)
+    p1, _, _ = host.generate(100, gen_type="shell", shell_jitter=0.01)
+    r0 = np.linalg.norm(p0, axis=1)
+    r1 = np.linalg.norm(p1, axis=1)
+    np.testing.assert_allclose(r0, 100.0, rtol=1e-14)
+    assert (np.abs(r1 / 100.0 - 1) <= 0.0100001).all() and r1.std() > 0.1
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_binary_records_are_reference_particle_bytes(tmp_path, prec):
+    recs = np.array(GOLD["dat_records"]["values"])
+    path = str(tmp_path / "0000.dat")
+    host.write_binary(path, recs[:, 0:3], recs[:, 3:6], recs[:, 6:9], recs[:, 9], prec=prec)
+    want = bytes.fromhex(GOLD["dat_records"][prec + "_hex"])
+    assert open(path, "rb").read() == want
+    n = host.binary_count(path, prec=prec)
+    assert n == 3
+    p, v, a, m = host.read_binary(path, prec=prec)
+    if prec == "f64":
+        np.testing.assert_array_equal(p, recs[:, 0:3])
+        np.testing.assert_array_equal(m, recs[:, 9])
+
+
+def test_partial_trailing_record_dropped(tmp_path):
+    # src/main.cpp:802 floors size / sizeof(particle)
+    path = str(tmp_path / "0001.dat")
+    pos = np.ones((4, 3))
+    host.write_binary(path, pos, pos, pos, np.ones(4))
+    with open(path, "ab") as f:
+        f.write(b"\0" * 17)
+    assert host.binary_count(path) == 4
+
+
+def test_text_format(tmp_path):
+    pos = np.array([[123.456789, -1e-7, 0.0], [1e20, 2.5, -300000.0], [1.0 / 3, 1e6, 1234567.0]])
+    path = str(tmp_path / "0000.txt")
+    host.write_text(path, pos)
+    got = open(path).read()
+    want = "".join("%s, %s, %s\n" % tuple("%g" % x for x in row) for row in pos)
+    assert got == want
+    assert got.splitlines()[0] == "123.457, -1e-07, 0"
+
+
+def test_frame_names_and_resume(tmp_path):
+    d = str(tmp_path)
+    assert host.frame_filename(d, 7, True).endswith("/0007.dat")
+    assert host.frame_filename(d, 12345, False).endswith("/12345.txt")
+    assert host.resume_frame(d, 300) == 0
+    for f in (3, 9):
+        host.write_binary(host.frame_filename(d, f, True), np.zeros((2, 3)), None, None,
+                          np.ones(2))
+    assert host.resume_frame(d, 300) == 10
+    assert host.resume_frame(d, 5) == 4  # searches down from num_frames
