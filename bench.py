@@ -1,0 +1,192 @@
+"""Benchmark of the Barnes-Hut step on one MI355X.
+
+Workload: gen_type sphere, N = 1,048,576 uniform in a ball of r = 100,
+masses uniform in [5e10, 5e11], rigid rotation 0.1 about z, theta = 0.5,
+dt = 0.0333 (root from the bounding box + 2, the reference's gen_root).
+One step = root, octree build, mass/COM, walk and kick-drift update on the
+GPU with the state resident in HBM.
+
+Prints ONE JSON line:
+  value     particle-steps/s of the FP32-force / FP64-state path
+  fp64      the all-FP64 path on the same workload, and the speed ratio
+  accuracy  rms relative acceleration errors at the initial state on 4096
+            seeded samples: e_prec (FP32 tree vs FP64 tree), e_trunc (FP64
+            tree vs FP64 direct sum), e_total (FP32 tree vs direct sum)
+  roofline  the walk kernel: 20 FLOP per body-node interaction (interactions
+            counted by the kernel) over its HIP-event time, against the FP32
+            vector peak
+  cpu_baseline  the oracle (CPU restatement of the reference's step) on the
+            host cores: full tree build + mass/COM of all N particles and the
+            walk of a seeded sample, extrapolated to one step of N
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP32_PEAK_TFLOPS = 157.3   # MI355X vector FP32 (AMD spec)
+FP64_PEAK_TFLOPS = 78.6    # MI355X vector FP64
+HBM_PEAK_GBS = 8000.0
+FLOP_PER_INTERACTION = 20
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def time_path(sim, pos, vel, mass, precision, steps, warmup):
+    sim.configure(precision=precision)
+    sim.set_state(pos, vel, mass)
+    if warmup:
+        sim.step(warmup)
+    sim.synchronize()
+    sim.interactions_total(reset=True)
+    sim.profile(True)
+    sim.profile_read(reset=True)
+    t0 = time.perf_counter()
+    sim.step(steps)
+    sim.synchronize()
+    t1 = time.perf_counter()
+    prof = sim.profile_read(reset=True)
+    sim.profile(False)
+    inter = sim.interactions_total(reset=True)
+    info = sim.tree_info()
+    return {"ms_per_step": (t1 - t0) * 1e3 / steps, "prof": prof, "interactions": inter,
+            "tree": info}
+
+
+def cpu_baseline(pos, mass, theta, sample, threads):
+    import oracle as O
+    n = len(pos)
+    t0 = time.perf_counter()
+    tree = O.Tree(pos, mass)           # gen_root + add_particle + mass + COM
+    t1 = time.perf_counter()
+    rng = np.random.default_rng(0xBEEF)
+    idx = np.sort(rng.choice(n, sample, replace=False))
+    t2 = time.perf_counter()
+    _, cnt = tree.forces_subset(idx, theta=theta, threads=threads)
+    t3 = time.perf_counter()
+    t_step = (t1 - t0) + (t3 - t2) * n / sample
+    return {
+        "value": n / t_step, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+        "sample": "oracle (C++ restatement of the reference pthreads path, FP64): tree build "
+                  "+ mass/COM of all %d particles (%.2f s, serial like gen_root) and the "
+                  "BFS walk of a seeded %d-particle sample on %d threads (%.2f s), "
+                  "extrapolated to one step of N" % (n, t1 - t0, sample, threads, t3 - t2),
+        "interactions_per_particle": float(cnt.mean()),
+        "seconds": t3 - t0,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=1 << 20)
+    ap.add_argument("--fp64-steps", type=int, default=20)
+    ap.add_argument("--samples", type=int, default=4096)
+    ap.add_argument("--cpu-sample", type=int, default=16384)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import nbodysim_amd
+    from nbodysim_amd import host
+
+    n, theta, dt = args.n, 0.5, 0.0333
+    pos, vel, mass = host.generate(n, gen_type="sphere", seed=0xDEADBEEF, r_sphere=100.0,
+                                   rotation_magnitude=0.1, rotation_vector=(0.0, 0.0, 1.0),
+                                   min_mass=5e10, max_mass=5e11)
+    log("generated %d particles" % n)
+    sim = nbodysim_amd.Simulation(theta=theta, dt=dt, precision="f32")
+
+    # ---- accuracy at the initial state
+    sim.set_state(pos, vel, mass)
+    a32 = sim.forces()
+    sim.configure(precision="f64")
+    a64 = sim.forces()
+    ids = np.random.default_rng(0xDEADBEEF).choice(n, args.samples, replace=False)
+    ref = sim.direct(ids)
+
+    def rms(a, b):
+        return float(np.sqrt(np.mean((np.linalg.norm(a - b, axis=1) /
+                                      np.linalg.norm(b, axis=1)) ** 2)))
+    acc = {"e_prec": rms(a32[ids], a64[ids]), "e_trunc": rms(a64[ids], ref),
+           "e_total": rms(a32[ids], ref), "samples": args.samples,
+           "where": "initial state, FP64 direct sum on the samples as reference"}
+    log("accuracy", acc)
+
+    # ---- timed paths
+    r32 = time_path(sim, pos, vel, mass, "f32", args.steps, args.warmup)
+    log("f32", r32)
+    r64 = time_path(sim, pos, vel, mass, "f64", args.fp64_steps, args.warmup)
+    log("f64", r64)
+    sim.close()
+
+    value = n * 1e3 / r32["ms_per_step"]
+    value64 = n * 1e3 / r64["ms_per_step"]
+    walk_ms = r32["prof"]["walk_ms"] / max(r32["prof"]["walk_launches"], 1)
+    inter_per_launch = r32["interactions"] / args.steps
+    achieved = FLOP_PER_INTERACTION * inter_per_launch / (walk_ms * 1e-3) / 1e12
+    walk64_ms = r64["prof"]["walk_ms"] / max(r64["prof"]["walk_launches"], 1)
+    achieved64 = FLOP_PER_INTERACTION * (r64["interactions"] / args.fp64_steps) / \
+        (walk64_ms * 1e-3) / 1e12
+    build_ms = r32["prof"]["build_ms"] / args.steps
+    nodes = r32["tree"]["n_nodes"]
+    build_bytes = 56 * n + 32 * nodes
+    out = {
+        "metric": "particle-steps/s @N=1M theta=0.5, FP32 vs FP64 GPU at rms accel err <=1e-4",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r32["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded gen_type sphere, the reference's generator)",
+        "config": {"workload": "gen_type sphere N=%d r=100 rotation 0.1 about z, masses "
+                               "U[5e10,5e11], theta=0.5, dt=0.0333, bbox+2 root; FP32 walk "
+                               "forces, FP64 state/keys/COM" % n,
+                   "n": n, "theta": theta, "dt": dt, "parallelism": "single GPU"},
+        "interactions_per_s": r32["interactions"] / (r32["ms_per_step"] * args.steps * 1e-3),
+        "interactions_per_particle": inter_per_launch / n,
+        "fp64": {"value": value64, "ms_per_step": r64["ms_per_step"],
+                 "steps": args.fp64_steps, "walk_ms": walk64_ms,
+                 "walk_tflops": achieved64, "walk_frac_fp64_peak": achieved64 / FP64_PEAK_TFLOPS,
+                 "speedup_fp32_over_fp64": value / value64},
+        "accuracy": acc,
+        "phases_ms_per_step": {"build": build_ms, "walk": walk_ms,
+                               "update": r32["prof"]["update_ms"] / args.steps},
+        "tree_build": {"ms": build_ms, "algorithmic_bytes": build_bytes,
+                       "gbps": build_bytes / (build_ms * 1e-3) / 1e9,
+                       "frac_hbm": build_bytes / (build_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                       "nodes": nodes, "max_depth": r32["tree"]["max_depth"],
+                       "buckets": r32["tree"]["n_buckets"]},
+        "roofline": {"bound": "valu", "kernel": "walk_kernel<f32>",
+                     "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "flop_per_interaction": FLOP_PER_INTERACTION,
+                     "avg_launch_ms": walk_ms},
+    }
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(pos, mass, theta, args.cpu_sample, threads)
+        log("cpu", out["cpu_baseline"])
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

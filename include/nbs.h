@@ -114,6 +114,13 @@ int nbs_forces(nbs_ctx* ctx, double* acc);
  * root, octree, mass, COM, walk, kick-drift update. */
 int nbs_step(nbs_ctx* ctx, int nsteps);
 
+/* nbs_step through a captured HIP graph: two steps are captured once
+ * (kernels only, no memset or copy nodes) and replayed, which removes the
+ * per-kernel launch cost of the ~35 kernels of a step.  Results are
+ * identical to nbs_step.  The graph is rebuilt after any change of state
+ * size, systems, configuration or node capacity. */
+int nbs_step_graph(nbs_ctx* ctx, int nsteps);
+
 /* Blocks until the context's stream has drained. */
 int nbs_synchronize(nbs_ctx* ctx);
 
@@ -125,6 +132,10 @@ int nbs_direct(nbs_ctx* ctx, size_t m, const int64_t* ids, double* acc);
 /* Interaction count (accepted nodes per particle) of the last walk, summed
  * over all particles; per_particle (host, caller order) may be NULL. */
 int nbs_interactions(nbs_ctx* ctx, int64_t* total, int64_t* per_particle);
+
+/* Interactions summed over every walk since the last reset (reset != 0
+ * clears the counter after reading). */
+int nbs_interactions_total(nbs_ctx* ctx, int64_t* total, int reset);
 
 /* Simulation time advanced so far, summed over steps (differs from
  * steps * dt only with adaptive dt); per system when batched. */
@@ -149,6 +160,27 @@ int nbs_keys(nbs_ctx* ctx, uint64_t* key_hi, uint64_t* key_lo);
 
 /* Root (centre, side) per system of the last build; center [nsys][3]. */
 int nbs_roots(nbs_ctx* ctx, double* center, double* side, int nsys_cap);
+
+/* Reverse mode (no counterpart in the reference).  nbs_adjoint_forward
+ * runs nsteps frames from the current state (fixed dt, one system),
+ * keeping (x, v) checkpoints every `every` steps and, with record_all, the
+ * positions of every step.  nbs_adjoint_backward takes dL/dx_T and
+ * dL/dv_T (host [n][3], NULL = 0) and returns dL/dx_0, dL/dv_0 (host
+ * [n][3]) and dL/dm (host [n]): each segment is recomputed from its
+ * checkpoint and reversed with the walk's cell decisions of the forward.
+ * The context is left at the forward's final state.  The gradient is that
+ * of the step with the tree topology and accepted cells held fixed (the
+ * decisions are step functions of the positions). */
+int nbs_adjoint_forward(nbs_ctx* ctx, int nsteps, int every, int record_all);
+int nbs_adjoint_backward(nbs_ctx* ctx, const double* gx_T, const double* gv_T,
+                         double* gx0, double* gv0, double* gm);
+
+/* Replays the recorded forward from perturbed initial data with the tree
+ * topology and cell decisions of the recorded trajectory (FP64 only): the
+ * smooth map the adjoint differentiates, for finite-difference checks.
+ * Outputs host [n][3]; the context is left at the forward's final state. */
+int nbs_frozen_forward(nbs_ctx* ctx, const double* pos0, const double* vel0,
+                       const double* mass, double* posT, double* velT);
 
 /* Phase timing on the context's stream with HIP events (ms, summed since
  * the last reset): [0] tree build (bbox..mass/COM), [1] walk (incl. fused

@@ -141,8 +141,10 @@ class Simulation:
         self._check(self._L.nbs_forces(self._h, _dp(acc)))
         return acc
 
-    def step(self, nsteps=1):
-        self._check(self._L.nbs_step(self._h, int(nsteps)))
+    def step(self, nsteps=1, graph=False):
+        """nsteps frames; graph=True replays a captured HIP graph."""
+        f = self._L.nbs_step_graph if graph else self._L.nbs_step
+        self._check(f(self._h, int(nsteps)))
 
     def synchronize(self):
         self._check(self._L.nbs_synchronize(self._h))
@@ -162,10 +164,43 @@ class Simulation:
             per.ctypes.data_as(_native.c_i64p) if per is not None else None))
         return (tot.value, per) if per_particle else tot.value
 
+    def interactions_total(self, reset=True):
+        """Interactions summed over every walk since the last reset."""
+        tot = ctypes.c_int64()
+        self._check(self._L.nbs_interactions_total(self._h, ctypes.byref(tot), int(reset)))
+        return tot.value
+
     def time(self, nsys=1):
         t = np.zeros(nsys)
         self._check(self._L.nbs_time(self._h, _dp(t), nsys))
         return t
+
+    # -------------------------------------------------------------- adjoint
+    def adjoint_forward(self, nsteps, every=20, record_all=False):
+        """nsteps frames with checkpoints every `every` steps (reverse mode)."""
+        self._check(self._L.nbs_adjoint_forward(self._h, int(nsteps), int(every),
+                                                int(bool(record_all))))
+
+    def adjoint_backward(self, gx_T=None, gv_T=None):
+        """Returns (dL/dx0, dL/dv0, dL/dm) for the given dL/dx_T, dL/dv_T."""
+        n = self.n
+        gx = None if gx_T is None else _f64(gx_T, (n, 3))
+        gv = None if gv_T is None else _f64(gv_T, (n, 3))
+        gx0, gv0, gm = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros(n)
+        self._check(self._L.nbs_adjoint_backward(self._h, _dp(gx), _dp(gv), _dp(gx0),
+                                                 _dp(gv0), _dp(gm)))
+        return gx0, gv0, gm
+
+    def frozen_forward(self, pos0, vel0, mass):
+        """Replays the recorded forward at perturbed inputs (FP64, fixed
+        cell decisions); returns (x_T, v_T)."""
+        n = self.n
+        pos0, mass = _f64(pos0, (n, 3)), _f64(mass, (n,))
+        vel0 = None if vel0 is None else _f64(vel0, (n, 3))
+        xT, vT = np.zeros((n, 3)), np.zeros((n, 3))
+        self._check(self._L.nbs_frozen_forward(self._h, _dp(pos0), _dp(vel0), _dp(mass),
+                                               _dp(xT), _dp(vT)))
+        return xT, vT
 
     # ----------------------------------------------------------------- tree
     def tree_info(self):

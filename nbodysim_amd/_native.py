@@ -69,15 +69,21 @@ SIGNATURES = [
     ("nbs_get_state", ctypes.c_int, [ctypes.c_void_p, c_dp, c_dp, c_dp]),
     ("nbs_forces", ctypes.c_int, [ctypes.c_void_p, c_dp]),
     ("nbs_step", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("nbs_step_graph", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("nbs_synchronize", ctypes.c_int, [ctypes.c_void_p]),
     ("nbs_direct", ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, c_i64p, c_dp]),
     ("nbs_interactions", ctypes.c_int, [ctypes.c_void_p, c_i64p, c_i64p]),
+    ("nbs_interactions_total", ctypes.c_int, [ctypes.c_void_p, c_i64p, ctypes.c_int]),
     ("nbs_time", ctypes.c_int, [ctypes.c_void_p, c_dp, ctypes.c_int]),
     ("nbs_tree_info_get", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(TreeInfo)]),
     ("nbs_tree_dump", ctypes.c_int, [ctypes.c_void_p, c_i32p, c_i32p, c_i64p, c_i64p, c_dp,
                                      c_dp, c_dp, c_u64p, c_u64p, c_i32p]),
     ("nbs_keys", ctypes.c_int, [ctypes.c_void_p, c_u64p, c_u64p]),
     ("nbs_roots", ctypes.c_int, [ctypes.c_void_p, c_dp, c_dp, ctypes.c_int]),
+    ("nbs_adjoint_forward", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_int]),
+    ("nbs_adjoint_backward", ctypes.c_int, [ctypes.c_void_p, c_dp, c_dp, c_dp, c_dp, c_dp]),
+    ("nbs_frozen_forward", ctypes.c_int, [ctypes.c_void_p, c_dp, c_dp, c_dp, c_dp, c_dp]),
     ("nbs_profile_enable", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("nbs_profile_read", ctypes.c_int, [ctypes.c_void_p, c_dp, ctypes.c_int]),
     ("nbs_stream", ctypes.c_void_p, [ctypes.c_void_p]),

@@ -30,7 +30,7 @@ CXX_FLAGS = ["-std=c++17", "-O2", "-ffp-contract=off", "-fPIC", "-Wall",
              "-I" + os.path.join(ROOT, "include")]
 
 HIP_SOURCES = ["scan.hip", "radix_sort.hip", "tree_build.hip", "walk.hip",
-               "integrate.hip", "api.hip"]
+               "integrate.hip", "adjoint.hip", "api.hip"]
 HOST_SOURCES = ["host/settings.cpp", "host/ic.cpp", "host/dump.cpp"]
 CLI_SOURCES = ["host/nbodysim_main.cpp"]
 

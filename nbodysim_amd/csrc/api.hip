@@ -62,6 +62,50 @@ __global__ void gather_slots_kernel(const long long* __restrict__ ids, int m,
   if (k < m) slots[k] = slot_of_id[ids[k]];
 }
 
+// State from caller-order [3][n] arrays (vel may be null), identity order.
+__global__ void load_soa_kernel(const double* __restrict__ pos,
+                                const double* __restrict__ vel,
+                                const double* __restrict__ mass, int n,
+                                State st) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  st.x[i] = pos[i];
+  st.y[i] = pos[n + i];
+  st.z[i] = pos[2 * (size_t)n + i];
+  st.vx[i] = vel ? vel[i] : 0.0;
+  st.vy[i] = vel ? vel[n + i] : 0.0;
+  st.vz[i] = vel ? vel[2 * (size_t)n + i] : 0.0;
+  st.m[i] = mass[i];
+  st.id[i] = (unsigned)i;
+}
+
+// slot-order triple -> caller-order [3][n]
+__global__ void scatter3_kernel(const double* __restrict__ a,
+                                const double* __restrict__ b,
+                                const double* __restrict__ c,
+                                const unsigned* __restrict__ id, int n,
+                                double* __restrict__ dst) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const unsigned j = id[s];
+  dst[j] = a[s];
+  dst[n + j] = b[s];
+  dst[2 * (size_t)n + j] = c[s];
+}
+
+// caller-order [3][n] -> slot-order triple
+__global__ void gather3_kernel(const double* __restrict__ src,
+                               const unsigned* __restrict__ id, int n,
+                               double* __restrict__ a, double* __restrict__ b,
+                               double* __restrict__ c) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const unsigned j = id[s];
+  a[s] = src[j];
+  b[s] = src[n + j];
+  c[s] = src[2 * (size_t)n + j];
+}
+
 template <class T>
 T* dalloc(std::vector<void*>& owner, size_t count, hipError_t* err) {
   void* p = nullptr;
@@ -110,6 +154,7 @@ struct nbs_ctx {
   int* flags = nullptr;
   int* steps_done = nullptr;
   int* stats = nullptr;
+  unsigned long long* inter_total = nullptr;
 
   // direct-sum scratch
   int direct_cap = 0;
@@ -126,6 +171,32 @@ struct nbs_ctx {
   bool prof = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[4];
   double prof_ms[4] = {0, 0, 0, 0};
+
+  // Two captured steps (the state ping-pongs between its two buffers, so
+  // a pair of steps returns to the same buffer); valid while the
+  // configuration, state size and node capacity do not change.
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  int graph_cur = -1;
+
+  // Adjoint: checkpoints (caller order, [3][n] SoA) every `adj_every`
+  // steps, the recomputed positions of one segment, optionally every base
+  // position (frozen replays), and the adjoint accumulators.
+  std::vector<void*> adj_bufs;
+  int adj_steps = 0, adj_every = 0, adj_n = 0;
+  bool adj_record = false;
+  double adj_dt = 0;
+  std::vector<double*> ck_x, ck_v;
+  double *adj_m = nullptr, *seg_x = nullptr, *base_x = nullptr;
+  double *fin_x = nullptr, *fin_v = nullptr;
+  double *g_x = nullptr, *g_v = nullptr, *g_m = nullptr, *g_u = nullptr;
+  double *s_u = nullptr, *s_gx = nullptr, *s_gm = nullptr;
+  double* node_g = nullptr;
+  int node_g_cap = 0;
+  WalkNode* mac_nodes = nullptr;
+  int mac_nodes_cap = 0;
+  double* mac_x = nullptr;
+  double *p_x = nullptr, *p_v = nullptr, *p_m = nullptr;
 };
 
 namespace {
@@ -173,6 +244,8 @@ int check_config(nbs_ctx* c, const nbs_config* cfg) {
     return fail(c, NBS_ERR_ARG, "root_size must be > 0 for NBS_ROOT_FIXED");
   return NBS_OK;
 }
+
+void drop_graph(nbs_ctx* c);
 
 int alloc_particles(nbs_ctx* c, int n) {
   if (n <= c->cap_n && c->cap_n > 0) return NBS_OK;
@@ -222,6 +295,7 @@ int alloc_particles(nbs_ctx* c, int n) {
 }
 
 int alloc_nodes(nbs_ctx* c, int cap) {
+  drop_graph(c);
   free_all(c->node_bufs);
   hipError_t e = hipSuccess;
   auto& o = c->node_bufs;
@@ -244,6 +318,7 @@ int alloc_nodes(nbs_ctx* c, int cap) {
 // Per-system arrays from caller offsets (one system when offsets empty).
 int setup_systems(nbs_ctx* c) {
   const int nsys = c->nsys_req;
+  drop_graph(c);
   free_all(c->sys_bufs);
   hipError_t e = hipSuccess;
   auto& o = c->sys_bufs;
@@ -363,6 +438,7 @@ void enqueue_walk(nbs_ctx* c, int fused) {
   a.sys_dt = c->sys.dt;
   a.sys_time = c->sys.time;
   a.steps_done = c->steps_done;
+  a.inter_total = c->inter_total;
   a.fused_update = fused;
   launch_walk(a, c->cfg.force_precision == NBS_FORCE_F32, c->p.damping,
               c->stream);
@@ -380,6 +456,14 @@ void enqueue_adaptive_update(nbs_ctx* c) {
   launch_update(st, c->ax, c->ay, c->az, c->n, c->sys, c->flags, c->stream);
   c->bbox_valid = true;
   prof_end(c, kPhaseUpdate, e0);
+}
+
+void drop_graph(nbs_ctx* c) {
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->graph_exec = nullptr;
+  c->graph = nullptr;
+  c->graph_cur = -1;
 }
 
 int read_flags(nbs_ctx* c, int* flags, int* done) {
@@ -441,6 +525,43 @@ int run(nbs_ctx* c, int nsteps) {
   return fail(c, NBS_ERR_OVERFLOW, "node capacity overflow after retries");
 }
 
+// Enqueues the two-step graph replays for an even number of steps.  The
+// first call captures the graph (steps enqueued while capturing are not
+// executed by the capture itself).
+int run_graph_pairs(nbs_ctx* c, int pairs) {
+  if (!c->graph_exec || c->graph_cur != c->cur) {
+    drop_graph(c);
+    const bool prof = c->prof;
+    c->prof = false;  // no event records inside a captured graph
+    const int cur0 = c->cur;
+    const bool bbox0 = c->bbox_valid;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < 2; k++) {
+      enqueue_build(c);
+      if (c->p.adaptive) {
+        enqueue_walk(c, 0);
+        enqueue_adaptive_update(c);
+      } else {
+        enqueue_walk(c, 1);
+      }
+    }
+    hipError_t e = hipStreamEndCapture(c->stream, &c->graph);
+    c->prof = prof;
+    c->cur = cur0;
+    c->bbox_valid = bbox0;
+    if (e != hipSuccess) return hip_fail(c, e, "hipStreamEndCapture");
+    e = hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      drop_graph(c);
+      return hip_fail(c, e, "hipGraphInstantiate");
+    }
+    c->graph_cur = cur0;
+  }
+  for (int k = 0; k < pairs; k++) HIPCHK(c, hipGraphLaunch(c->graph_exec, c->stream));
+  c->bbox_valid = true;
+  return NBS_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -485,6 +606,7 @@ int nbs_create(const nbs_config* cfg, nbs_ctx** out) {
   c->bb.flags = c->flags;
   c->bb.stats = dalloc<int>(c->misc_bufs, 4, &e);
   c->stats = c->bb.stats;
+  c->inter_total = dalloc<unsigned long long>(c->misc_bufs, 1, &e);
   if (e != hipSuccess) {
     free_all(c->misc_bufs);
     (void)hipStreamDestroy(c->stream);
@@ -493,6 +615,7 @@ int nbs_create(const nbs_config* cfg, nbs_ctx** out) {
   }
   launch_fill_int(c->flags, 0, 1, c->stream);
   launch_fill_int(c->stats, 0, 4, c->stream);
+  launch_fill_u64(c->inter_total, 0, 1, c->stream);
   *out = c;
   return NBS_OK;
 }
@@ -501,6 +624,8 @@ void nbs_destroy(nbs_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
+  drop_graph(c);
+  free_all(c->adj_bufs);
   for (auto& v : c->ev)
     for (auto& p : v) {
       (void)hipEventDestroy(p.first);
@@ -523,6 +648,7 @@ int nbs_configure(nbs_ctx* c, const nbs_config* cfg) {
   int r = check_config(c, cfg);
   if (r) return r;
   const int old_node_factor = c->cfg.node_factor;
+  drop_graph(c);
   c->cfg = *cfg;
   c->p = params_of(*cfg);
   if (cfg->node_factor != old_node_factor) c->node_cap = 0;
@@ -536,6 +662,7 @@ int nbs_set_state(nbs_ctx* c, size_t n, const double* pos, const double* vel,
   if (n > 0 && (!pos || !mass)) return fail(c, NBS_ERR_ARG, "pos and mass are required");
   if (n > (size_t)0x3fffffff) return fail(c, NBS_ERR_ARG, "n too large");
   HIPCHK(c, hipSetDevice(c->device));
+  drop_graph(c);
   int r = alloc_particles(c, (int)n);
   if (r) return r;
   c->n = (int)n;
@@ -626,6 +753,58 @@ int nbs_step(nbs_ctx* c, int nsteps) {
   return run(c, nsteps);
 }
 
+int nbs_step_graph(nbs_ctx* c, int nsteps) {
+  if (!c) return NBS_ERR_ARG;
+  if (nsteps < 0) return fail(c, NBS_ERR_ARG, "nsteps must be >= 0");
+  if (!c->has_state) return fail(c, NBS_ERR_STATE, "no particle state set");
+  if (nsteps == 0 || c->n == 0) return NBS_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  int r = ensure_nodes(c);
+  if (r) return r;
+  reset_flags(c);
+  int left = nsteps;
+  // The graph assumes the bbox accumulators were filled by the previous
+  // step's update; a first eager step provides that.
+  if (!c->bbox_valid) {
+    enqueue_build(c);
+    if (c->p.adaptive) {
+      enqueue_walk(c, 0);
+      enqueue_adaptive_update(c);
+    } else {
+      enqueue_walk(c, 1);
+    }
+    left--;
+  }
+  r = run_graph_pairs(c, left / 2);
+  if (r) return r;
+  if (left % 2) {
+    enqueue_build(c);
+    if (c->p.adaptive) {
+      enqueue_walk(c, 0);
+      enqueue_adaptive_update(c);
+    } else {
+      enqueue_walk(c, 1);
+    }
+  }
+  int flags = 0, done = 0;
+  r = read_flags(c, &flags, &done);
+  if (r) return r;
+  if (!flags) {
+    c->tree_valid = true;
+    c->acc_valid = true;
+    return NBS_OK;
+  }
+  // overflow: the failing step and all after it did nothing
+  drop_graph(c);
+  c->bbox_valid = false;
+  c->tree_valid = false;
+  if (flags & kFlagRunOverflow)
+    return fail(c, NBS_ERR_OVERFLOW, "more than 8192 particles share one 21-level cell");
+  r = alloc_nodes(c, c->node_cap * 2);
+  if (r) return r;
+  return run(c, nsteps - done);
+}
+
 int nbs_synchronize(nbs_ctx* c) {
   if (!c) return NBS_ERR_ARG;
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -690,6 +869,17 @@ int nbs_interactions(nbs_ctx* c, int64_t* total, int64_t* per_particle) {
     if (per_particle) per_particle[id[i]] = h[i];
   }
   if (total) *total = t;
+  return NBS_OK;
+}
+
+int nbs_interactions_total(nbs_ctx* c, int64_t* total, int reset) {
+  if (!c || !total) return NBS_ERR_ARG;
+  unsigned long long h = 0;
+  HIPCHK(c, hipMemcpyAsync(&h, c->inter_total, sizeof(h), hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  *total = (int64_t)h;
+  if (reset) launch_fill_u64(c->inter_total, 0, 1, c->stream);
   return NBS_OK;
 }
 
@@ -832,5 +1022,317 @@ int nbs_profile_read(nbs_ctx* c, double* ms4, int reset) {
 }
 
 void* nbs_stream(nbs_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- adjoint --
+namespace {
+
+void adj_free(nbs_ctx* c) {
+  free_all(c->adj_bufs);
+  c->ck_x.clear();
+  c->ck_v.clear();
+  c->adj_steps = c->adj_every = c->adj_n = 0;
+  c->node_g = nullptr;
+  c->node_g_cap = 0;
+  c->mac_nodes = nullptr;
+  c->mac_nodes_cap = 0;
+  c->mac_x = nullptr;
+  c->base_x = nullptr;
+  c->adj_record = false;
+}
+
+// Current state (slot order) -> caller-order [3][n] positions / velocities.
+void save_state(nbs_ctx* c, double* x3, double* v3) {
+  State& st = c->st[c->cur];
+  const int b = (c->n + 255) / 256;
+  if (x3) scatter3_kernel<<<b, 256, 0, c->stream>>>(st.x, st.y, st.z, st.id, c->n, x3);
+  if (v3) scatter3_kernel<<<b, 256, 0, c->stream>>>(st.vx, st.vy, st.vz, st.id, c->n, v3);
+}
+
+// Caller-order arrays -> state buffer 0 in caller order (the next build
+// sorts it); invalidates the bbox and the tree.
+void load_state(nbs_ctx* c, const double* x3, const double* v3, const double* m) {
+  load_soa_kernel<<<(c->n + 255) / 256, 256, 0, c->stream>>>(x3, v3, m, c->n, c->st[0]);
+  c->cur = 0;
+  c->bbox_valid = false;
+  c->tree_valid = false;
+}
+
+int ensure_node_adjoints(nbs_ctx* c) {
+  if (c->node_g && c->node_g_cap >= c->node_cap) return NBS_OK;
+  hipError_t e = hipSuccess;
+  c->node_g = dalloc<double>(c->adj_bufs, 4 * (size_t)c->node_cap, &e);
+  if (e != hipSuccess) return hip_fail(c, e, "hipMalloc(node adjoints)");
+  c->node_g_cap = c->node_cap;
+  return NBS_OK;
+}
+
+// One build of the current state with an overflow check.
+int build_checked(nbs_ctx* c) {
+  for (int attempt = 0; attempt < 6; attempt++) {
+    reset_flags(c);
+    enqueue_build(c);
+    int flags = 0, done = 0;
+    int r = read_flags(c, &flags, &done);
+    if (r) return r;
+    if (!flags) return NBS_OK;
+    if (flags & kFlagRunOverflow)
+      return fail(c, NBS_ERR_OVERFLOW, "more than 8192 particles share one cell");
+    // state was permuted but not changed: rebuild with room to spare
+    c->bbox_valid = false;
+    r = alloc_nodes(c, c->node_cap * 2);
+    if (r) return r;
+  }
+  return fail(c, NBS_ERR_OVERFLOW, "node capacity overflow after retries");
+}
+
+// Reverse step at positions x3 (caller order): g_x += dF/dx^T u,
+// g_m += dF/dm^T u, with u = dt * (g_v + dt g_x) and g_v updated.
+int adj_reverse_step(nbs_ctx* c, const double* x3) {
+  const int n = c->n;
+  hipStream_t s = c->stream;
+  launch_adj_kick(c->g_v, c->g_x, c->g_u, c->adj_dt, n, s);
+  load_state(c, x3, nullptr, c->adj_m);
+  int r = build_checked(c);
+  if (r) return r;
+  r = ensure_node_adjoints(c);
+  if (r) return r;
+  State& st = c->st[c->cur];
+  gather3_kernel<<<(n + 255) / 256, 256, 0, s>>>(c->g_u, st.id, n, c->s_u, c->s_u + n,
+                                                 c->s_u + 2 * (size_t)n);
+  launch_fill_u64((unsigned long long*)c->node_g, 0, 4 * (size_t)c->node_cap, s);
+  AdjArgs a{};
+  a.nodes = c->nodes;
+  a.node_start = c->sys.node_start;
+  a.sys_start = c->sys.start;
+  a.wave_start = c->sys.wave_start;
+  a.nsys = c->sys.nsys;
+  a.n_waves = c->sys.n_waves;
+  a.x = st.x;
+  a.y = st.y;
+  a.z = st.z;
+  a.ux = c->s_u;
+  a.uy = c->s_u + n;
+  a.uz = c->s_u + 2 * (size_t)n;
+  a.gx = c->s_gx;
+  a.gy = c->s_gx + n;
+  a.gz = c->s_gx + 2 * (size_t)n;
+  a.gcx = c->node_g;
+  a.gcy = c->node_g + c->node_cap;
+  a.gcz = c->node_g + 2 * (size_t)c->node_cap;
+  a.gm_node = c->node_g + 3 * (size_t)c->node_cap;
+  a.flags = c->flags;
+  a.theta = c->p.theta;
+  launch_adj_walk(a, c->cfg.force_precision == NBS_FORCE_F32, c->p.damping, s);
+  launch_adj_propagate(a, c->bb.offsets, c->aux.parent, st.m, c->s_gm, n, s);
+  launch_scatter_comp(c->s_gx, c->g_x, st.id, n, 3, 1, s);
+  launch_scatter_comp(c->s_gm, c->g_m, st.id, n, 1, 1, s);
+  return NBS_OK;
+}
+
+void to_soa(const double* aos, std::vector<double>& soa, size_t n) {
+  soa.resize(3 * n);
+  for (size_t i = 0; i < n; i++)
+    for (int k = 0; k < 3; k++) soa[k * n + i] = aos[3 * i + k];
+}
+void to_aos(const std::vector<double>& soa, double* aos, size_t n) {
+  for (size_t i = 0; i < n; i++)
+    for (int k = 0; k < 3; k++) aos[3 * i + k] = soa[k * n + i];
+}
+
+}  // namespace
+
+extern "C" {
+
+int nbs_adjoint_forward(nbs_ctx* c, int nsteps, int every, int record_all) {
+  if (!c) return NBS_ERR_ARG;
+  if (!c->has_state) return fail(c, NBS_ERR_STATE, "no particle state set");
+  if (nsteps < 1 || every < 1) return fail(c, NBS_ERR_ARG, "nsteps and every must be >= 1");
+  if (c->p.adaptive) return fail(c, NBS_ERR_ARG, "the adjoint needs a fixed dt");
+  if (c->sys.nsys != 1) return fail(c, NBS_ERR_ARG, "the adjoint runs one system");
+  HIPCHK(c, hipSetDevice(c->device));
+  adj_free(c);
+  const int n = c->n;
+  const int nck = (nsteps + every - 1) / every;
+  hipError_t e = hipSuccess;
+  auto& o = c->adj_bufs;
+  for (int k = 0; k < nck; k++) {
+    c->ck_x.push_back(dalloc<double>(o, 3 * (size_t)n, &e));
+    c->ck_v.push_back(dalloc<double>(o, 3 * (size_t)n, &e));
+  }
+  c->adj_m = dalloc<double>(o, n, &e);
+  c->seg_x = dalloc<double>(o, 3 * (size_t)n * every, &e);
+  c->fin_x = dalloc<double>(o, 3 * (size_t)n, &e);
+  c->fin_v = dalloc<double>(o, 3 * (size_t)n, &e);
+  c->g_x = dalloc<double>(o, 3 * (size_t)n, &e);
+  c->g_v = dalloc<double>(o, 3 * (size_t)n, &e);
+  c->g_u = dalloc<double>(o, 3 * (size_t)n, &e);
+  c->g_m = dalloc<double>(o, n, &e);
+  c->s_u = dalloc<double>(o, 3 * (size_t)n, &e);
+  c->s_gx = dalloc<double>(o, 3 * (size_t)n, &e);
+  c->s_gm = dalloc<double>(o, n, &e);
+  c->p_x = dalloc<double>(o, 3 * (size_t)n, &e);
+  c->p_v = dalloc<double>(o, 3 * (size_t)n, &e);
+  c->p_m = dalloc<double>(o, n, &e);
+  if (record_all) c->base_x = dalloc<double>(o, 3 * (size_t)n * nsteps, &e);
+  if (e != hipSuccess) {
+    adj_free(c);
+    return hip_fail(c, e, "hipMalloc(adjoint buffers)");
+  }
+  {
+    State& st = c->st[c->cur];
+    launch_scatter_comp(st.m, c->adj_m, st.id, n, 1, 0, c->stream);
+  }
+  for (int t = 0; t < nsteps; t++) {
+    if (t % every == 0) save_state(c, c->ck_x[t / every], c->ck_v[t / every]);
+    if (record_all) save_state(c, c->base_x + 3 * (size_t)n * t, nullptr);
+    int r = run(c, 1);
+    if (r) return r;
+  }
+  save_state(c, c->fin_x, c->fin_v);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->adj_steps = nsteps;
+  c->adj_every = every;
+  c->adj_n = n;
+  c->adj_record = record_all != 0;
+  c->adj_dt = c->p.dt;
+  return NBS_OK;
+}
+
+int nbs_adjoint_backward(nbs_ctx* c, const double* gx_T, const double* gv_T,
+                         double* gx0, double* gv0, double* gm) {
+  if (!c) return NBS_ERR_ARG;
+  if (!c->adj_steps || c->adj_n != c->n)
+    return fail(c, NBS_ERR_STATE, "run nbs_adjoint_forward first");
+  HIPCHK(c, hipSetDevice(c->device));
+  const size_t n = c->n;
+  hipStream_t s = c->stream;
+  std::vector<double> h(3 * n, 0.0);
+  if (gx_T) to_soa(gx_T, h, n); else std::fill(h.begin(), h.end(), 0.0);
+  HIPCHK(c, hipMemcpyAsync(c->g_x, h.data(), 3 * n * 8, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  if (gv_T) to_soa(gv_T, h, n); else std::fill(h.begin(), h.end(), 0.0);
+  HIPCHK(c, hipMemcpyAsync(c->g_v, h.data(), 3 * n * 8, hipMemcpyHostToDevice, s));
+  launch_fill_u64((unsigned long long*)c->g_m, 0, n, s);
+  const int T = c->adj_steps, K = c->adj_every;
+  const int nck = (T + K - 1) / K;
+  for (int seg = nck - 1; seg >= 0; seg--) {
+    const int t0 = seg * K, t1 = std::min(T, t0 + K);
+    // recompute the segment's positions from its checkpoint (bit-exact:
+    // the build sorts canonically, so every step repeats the forward)
+    load_state(c, c->ck_x[seg], c->ck_v[seg], c->adj_m);
+    for (int t = t0; t < t1; t++) {
+      save_state(c, c->seg_x + 3 * n * (t - t0), nullptr);
+      if (t + 1 < t1) {
+        int r = run(c, 1);
+        if (r) return r;
+      }
+    }
+    for (int t = t1 - 1; t >= t0; t--) {
+      int r = adj_reverse_step(c, c->seg_x + 3 * n * (t - t0));
+      if (r) return r;
+    }
+  }
+  std::vector<double> hx(3 * n), hv(3 * n), hm(n);
+  HIPCHK(c, hipMemcpyAsync(hx.data(), c->g_x, 3 * n * 8, hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipMemcpyAsync(hv.data(), c->g_v, 3 * n * 8, hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipMemcpyAsync(hm.data(), c->g_m, n * 8, hipMemcpyDeviceToHost, s));
+  // leave the context at the forward's final state
+  load_state(c, c->fin_x, c->fin_v, c->adj_m);
+  HIPCHK(c, hipStreamSynchronize(s));
+  if (gx0) to_aos(hx, gx0, n);
+  if (gv0) to_aos(hv, gv0, n);
+  if (gm) std::memcpy(gm, hm.data(), n * 8);
+  return NBS_OK;
+}
+
+int nbs_frozen_forward(nbs_ctx* c, const double* pos0, const double* vel0,
+                       const double* mass, double* posT, double* velT) {
+  if (!c || !pos0 || !mass) return NBS_ERR_ARG;
+  if (!c->adj_steps || !c->adj_record || c->adj_n != c->n)
+    return fail(c, NBS_ERR_STATE, "run nbs_adjoint_forward with record_all first");
+  if (c->cfg.force_precision != NBS_FORCE_F64)
+    return fail(c, NBS_ERR_ARG, "frozen replays are FP64 only");
+  HIPCHK(c, hipSetDevice(c->device));
+  const size_t n = c->n;
+  hipStream_t s = c->stream;
+  hipError_t e = hipSuccess;
+  if (!c->mac_x) c->mac_x = dalloc<double>(c->adj_bufs, 3 * n, &e);
+  if (e != hipSuccess) return hip_fail(c, e, "hipMalloc(frozen)");
+  std::vector<double> h;
+  to_soa(pos0, h, n);
+  HIPCHK(c, hipMemcpyAsync(c->p_x, h.data(), 3 * n * 8, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  if (vel0) to_soa(vel0, h, n); else h.assign(3 * n, 0.0);
+  HIPCHK(c, hipMemcpyAsync(c->p_v, h.data(), 3 * n * 8, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(c->p_m, mass, n * 8, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  for (int t = 0; t < c->adj_steps; t++) {
+    load_state(c, c->base_x + 3 * n * t, nullptr, c->adj_m);
+    int r = build_checked(c);
+    if (r) return r;
+    if (!c->mac_nodes || c->mac_nodes_cap < c->node_cap) {
+      c->mac_nodes = dalloc<WalkNode>(c->adj_bufs, c->node_cap, &e);
+      if (e != hipSuccess) return hip_fail(c, e, "hipMalloc(frozen nodes)");
+      c->mac_nodes_cap = c->node_cap;
+    }
+    State& st = c->st[c->cur];
+    const int b = (int)((n + 255) / 256);
+    HIPCHK(c, hipMemcpyAsync(c->mac_nodes, c->nodes, (size_t)c->node_cap * sizeof(WalkNode),
+                             hipMemcpyDeviceToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(c->mac_x, st.x, n * 8, hipMemcpyDeviceToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(c->mac_x + n, st.y, n * 8, hipMemcpyDeviceToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(c->mac_x + 2 * n, st.z, n * 8, hipMemcpyDeviceToDevice, s));
+    // perturbed state into slot order, COM of the same topology
+    gather3_kernel<<<b, 256, 0, s>>>(c->p_x, st.id, (int)n, st.x, st.y, st.z);
+    gather3_kernel<<<b, 256, 0, s>>>(c->p_v, st.id, (int)n, st.vx, st.vy, st.vz);
+    launch_gather_comp(c->p_m, st.m, st.id, (int)n, 1, s);
+    launch_fill_int(c->aux.arrive, 0, c->node_cap, s);
+    launch_bottom_up(c->bb, st, c->nodes, c->aux, (int)n, s);
+    WalkArgs a{};
+    a.nodes = c->nodes;
+    a.node_start = c->sys.node_start;
+    a.sys_start = c->sys.start;
+    a.wave_start = c->sys.wave_start;
+    a.nsys = c->sys.nsys;
+    a.n_waves = c->sys.n_waves;
+    a.x = st.x;
+    a.y = st.y;
+    a.z = st.z;
+    a.vx = st.vx;
+    a.vy = st.vy;
+    a.vz = st.vz;
+    a.ax = c->ax;
+    a.ay = c->ay;
+    a.az = c->az;
+    a.interactions = c->inter;
+    a.flags = c->flags;
+    a.flags_out = c->flags;
+    a.theta = c->p.theta;
+    a.dt = c->adj_dt;
+    a.bbox = c->sys.bbox;
+    a.sys_dt = c->sys.dt;
+    a.sys_time = c->sys.time;
+    a.steps_done = c->steps_done;
+    a.inter_total = c->inter_total;
+    a.fused_update = 1;
+    a.mac_nodes = c->mac_nodes;
+    a.mac_x = c->mac_x;
+    a.mac_y = c->mac_x + n;
+    a.mac_z = c->mac_x + 2 * n;
+    launch_walk(a, 0, c->p.damping, s);
+    scatter3_kernel<<<b, 256, 0, s>>>(st.x, st.y, st.z, st.id, (int)n, c->p_x);
+    scatter3_kernel<<<b, 256, 0, s>>>(st.vx, st.vy, st.vz, st.id, (int)n, c->p_v);
+  }
+  std::vector<double> hx(3 * n), hv(3 * n);
+  HIPCHK(c, hipMemcpyAsync(hx.data(), c->p_x, 3 * n * 8, hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipMemcpyAsync(hv.data(), c->p_v, 3 * n * 8, hipMemcpyDeviceToHost, s));
+  load_state(c, c->fin_x, c->fin_v, c->adj_m);
+  HIPCHK(c, hipStreamSynchronize(s));
+  if (posT) to_aos(hx, posT, n);
+  if (velT) to_aos(hv, velT, n);
+  return NBS_OK;
+}
 
 }  // extern "C"

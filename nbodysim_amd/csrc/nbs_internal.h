@@ -151,7 +151,11 @@ struct WalkArgs {
   double* sys_dt;
   double* sys_time;
   int* steps_done;  // incremented once per completed walk
+  unsigned long long* inter_total;  // running sum of interactions
   int fused_update;
+  // frozen decisions (FP64 only; null otherwise): base tree + positions
+  const WalkNode* mac_nodes;
+  const double *mac_x, *mac_y, *mac_z;
 };
 
 void launch_walk(const WalkArgs& a, int f32, int damping, hipStream_t s);
@@ -163,6 +167,37 @@ void launch_direct(const State& st, int n, const int* sample_slots, int m,
                    int damping, double* tmp, double* acc_out, hipStream_t s);
 void launch_inverse_perm(const unsigned* id, int n, int* slot_of_id,
                          hipStream_t s);
+
+// -------------------------------------------------------------- adjoint --
+struct AdjArgs {
+  const WalkNode* nodes;
+  const int* node_start;
+  const int* sys_start;
+  const int* wave_start;
+  int nsys;
+  int n_waves;
+  const double *x, *y, *z;       // slot order
+  const double *ux, *uy, *uz;    // adjoint of the acceleration, slot order
+  double *gx, *gy, *gz;          // out: position adjoint, slot order
+  double *gcx, *gcy, *gcz;       // per node: adjoint of the COM
+  double* gm_node;               // per node: adjoint of the mass
+  const int* flags;
+  double theta;
+};
+
+void launch_adj_walk(const AdjArgs& a, int mac32, int damping, hipStream_t s);
+void launch_adj_propagate(const AdjArgs& a, const int* offsets,
+                          const int* parent, const double* m, double* gm,
+                          int n, hipStream_t s);
+// gv += dt gx; u = dt gv (caller order, 3 components of n each)
+void launch_adj_kick(double* gv, const double* gx, double* u, double dt,
+                     int n, hipStream_t s);
+// dst[c][s] = src[c][id[s]] for c < ncomp (caller -> slot order)
+void launch_gather_comp(const double* src, double* dst, const unsigned* id,
+                        int n, int ncomp, hipStream_t s);
+// dst[c][id[s]] (+)= src[c][s] (slot -> caller order)
+void launch_scatter_comp(const double* src, double* dst, const unsigned* id,
+                         int n, int ncomp, int add, hipStream_t s);
 
 // ------------------------------------------------------------ integrate --
 void launch_adaptive_dt(const State& st, const double* ax, const double* ay,

@@ -80,7 +80,12 @@ __device__ __forceinline__ void kick_drift(double& x, double& v, double a,
   x = x + t2;
 }
 
-template <bool F32, bool DAMP>
+// FROZEN (FP64 only): the MAC is evaluated on a second tree (mac_nodes,
+// same topology) against second positions (mac_x/y/z) -- a recorded base
+// trajectory -- while forces use this step's tree and positions.  That
+// makes the step a smooth function of the perturbed state, for finite
+// differences of the adjoint.
+template <bool F32, bool DAMP, bool FROZEN = false>
 __global__ void __launch_bounds__(kWalkBlock) walk_kernel(WalkArgs a) {
   if (*a.flags) return;
   const int lane = lane_id();
@@ -94,6 +99,10 @@ __global__ void __launch_bounds__(kWalkBlock) walk_kernel(WalkArgs a) {
   const int pend = a.sys_start[s + 1];
   const int i = pbeg + lane;
   const bool valid = i < pend;
+  const double mx = FROZEN && valid ? a.mac_x[i] : 0.0;
+  const double my = FROZEN && valid ? a.mac_y[i] : 0.0;
+  const double mz = FROZEN && valid ? a.mac_z[i] : 0.0;
+  ConstNodes mac_nodes = (ConstNodes)a.mac_nodes;
   double x = valid ? a.x[i] : 0.0;
   double y = valid ? a.y[i] : 0.0;
   double z = valid ? a.z[i] : 0.0;
@@ -142,7 +151,11 @@ __global__ void __launch_bounds__(kWalkBlock) walk_kernel(WalkArgs a) {
       else if (F32)
         open = fd2 < nodes[n].r2open32;
       else
-        open = mac_open_f64(d2, nodes[n].side, theta, theta2);
+        open = mac_open_f64(
+            FROZEN ? sq_sum_ref(mac_nodes[n].cx - mx, mac_nodes[n].cy - my,
+                                mac_nodes[n].cz - mz)
+                   : d2,
+            nodes[n].side, theta, theta2);
       open = open && in;
       const unsigned long long O = ballot(open);
       inter = in && !open;
@@ -249,6 +262,8 @@ __global__ void __launch_bounds__(kWalkBlock) walk_kernel(WalkArgs a) {
       }
     }
   }
+  const int wtot = wave_sum(valid ? cnt : 0);
+  if (lane == 0) atomicAdd(a.inter_total, (unsigned long long)wtot);
   if (wave == 0 && lane == 0) atomicAdd(a.steps_done, 1);
 }
 
@@ -331,9 +346,11 @@ void launch_walk(const WalkArgs& a, int f32, int damping, hipStream_t s) {
       walk_kernel<true, false><<<blocks, kWalkBlock, 0, s>>>(a);
   } else {
     if (damping)
-      walk_kernel<false, true><<<blocks, kWalkBlock, 0, s>>>(a);
+      a.mac_nodes ? walk_kernel<false, true, true><<<blocks, kWalkBlock, 0, s>>>(a)
+                  : walk_kernel<false, true><<<blocks, kWalkBlock, 0, s>>>(a);
     else
-      walk_kernel<false, false><<<blocks, kWalkBlock, 0, s>>>(a);
+      a.mac_nodes ? walk_kernel<false, false, true><<<blocks, kWalkBlock, 0, s>>>(a)
+                  : walk_kernel<false, false><<<blocks, kWalkBlock, 0, s>>>(a);
   }
 }
 

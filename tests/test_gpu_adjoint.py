@@ -1,0 +1,147 @@
+"""The hand-written adjoint of the step against FP64 central finite
+differences.  At theta = 0 the walk is the direct sum (a smooth map); at
+theta = 0.5 the finite differences run through nbs_frozen_forward, which
+holds the tree topology and cell decisions of the recorded trajectory (the
+map the adjoint differentiates).  Bar: 1e-3 relative per checked
+coordinate (plus a small absolute floor for near-zero gradients)."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+ROT30 = np.array([[np.cos(np.pi / 6), -np.sin(np.pi / 6), 0.0],
+                  [np.sin(np.pi / 6), np.cos(np.pi / 6), 0.0],
+                  [0.0, 0.0, 1.0]])
+
+
+def sim(**kw):
+    import nbodysim_amd
+    return nbodysim_amd.Simulation(**kw)
+
+
+def loss(xT, tgt):
+    return float(np.mean(np.sum((xT - tgt) ** 2, axis=1)))
+
+
+def check_fd(adj, fd):
+    adj, fd = np.asarray(adj), np.asarray(fd)
+    floor = 1e-6 * np.abs(fd).max()
+    err = np.abs(adj - fd) / (np.abs(fd) + floor)
+    assert err.max() < 1e-3, (err.max(), adj, fd)
+    return err.max()
+
+
+def run_case(n, steps, every, theta, frozen, seed=3, ncoord=12):
+    pos, vel, mass = O.generate(n, "sphere", seed=seed)
+    tgt = pos @ ROT30.T
+    with sim(theta=theta, dt=0.0333, precision="f64") as s:
+        s.set_state(pos, vel, mass)
+        s.adjoint_forward(steps, every, record_all=frozen)
+        xT, vT, _ = s.state()
+        gx0, gv0, gm = s.adjoint_backward(2.0 * (xT - tgt) / n, None)
+
+        def fwd(p, v, m):
+            if frozen:
+                return s.frozen_forward(p, v, m)[0]
+            with sim(theta=theta, dt=0.0333, precision="f64") as t:
+                t.set_state(p, v, m)
+                t.step(steps)
+                return t.state()[0]
+
+        # the frozen replay of the unperturbed inputs is the trajectory
+        if frozen:
+            np.testing.assert_array_equal(fwd(pos, vel, mass), xT)
+        rng = np.random.default_rng(seed)
+        adj, fd = [], []
+        for k in range(ncoord):
+            kind = ("x", "v", "m")[k % 3]
+            i = int(rng.integers(n))
+            c = int(rng.integers(3))
+            p, v, m = pos.copy(), vel.copy(), mass.copy()
+            if kind == "x":
+                h = 1e-5
+                p[i, c] += h
+                lp = loss(fwd(p, v, m), tgt)
+                p[i, c] -= 2 * h
+                lm = loss(fwd(p, v, m), tgt)
+                adj.append(gx0[i, c])
+            elif kind == "v":
+                h = 1e-6
+                v[i, c] += h
+                lp = loss(fwd(p, v, m), tgt)
+                v[i, c] -= 2 * h
+                lm = loss(fwd(p, v, m), tgt)
+                adj.append(gv0[i, c])
+            else:
+                h = 1e-6 * mass[i]
+                m[i] += h
+                lp = loss(fwd(p, v, m), tgt)
+                m[i] -= 2 * h
+                lm = loss(fwd(p, v, m), tgt)
+                adj.append(gm[i])
+            fd.append((lp - lm) / (2 * h))
+    return check_fd(adj, fd), (gx0, gv0, gm)
+
+
+def test_adjoint_matches_fd_direct_sum_dynamics():
+    run_case(n=384, steps=12, every=5, theta=0.0, frozen=False)
+
+
+def test_adjoint_matches_fd_frozen_theta05():
+    run_case(n=2048, steps=20, every=6, theta=0.5, frozen=True)
+
+
+def test_checkpoint_interval_does_not_change_the_gradient():
+    pos, vel, mass = O.generate(3000, "sphere", seed=9)
+    out = []
+    for every in (1, 7, 20):
+        with sim(theta=0.5, dt=0.0333, precision="f32") as s:
+            s.set_state(pos, vel, mass)
+            s.adjoint_forward(20, every)
+            xT, _, _ = s.state()
+            out.append(s.adjoint_backward(xT, None))
+    for a, b in zip(out[0], out[1]):
+        np.testing.assert_array_equal(a, b)
+    for a, b in zip(out[0], out[2]):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_tree_gradient_close_to_direct_sum_gradient():
+    # theta = 0.5 adjoint vs the theta = 0 (direct sum) adjoint at small N
+    pos, vel, mass = O.generate(1024, "sphere", seed=11)
+    tgt = pos @ ROT30.T
+    g = []
+    for theta in (0.5, 0.0):
+        with sim(theta=theta, dt=0.0333, precision="f64") as s:
+            s.set_state(pos, vel, mass)
+            s.adjoint_forward(10, 5)
+            xT, _, _ = s.state()
+            g.append(s.adjoint_backward(2.0 * (xT - tgt) / len(pos), None))
+    for a, b in zip(g[0], g[1]):
+        rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+        assert rel < 5e-2, rel
+
+
+def test_torch_autograd_function():
+    torch = pytest.importorskip("torch")
+    import nbodysim_amd
+    from nbodysim_amd.autograd import trajectory
+    pos, vel, mass = O.generate(1000, "sphere", seed=5)
+    tgt = torch.from_numpy(pos @ ROT30.T)
+    x0 = torch.from_numpy(pos).requires_grad_(True)
+    v0 = torch.from_numpy(vel).requires_grad_(True)
+    m = torch.from_numpy(mass).requires_grad_(True)
+    with nbodysim_amd.Simulation(theta=0.5, dt=0.0333, precision="f32") as s:
+        xT, vT = trajectory(x0, v0, m, s, nsteps=10, every=5)
+        L = ((xT - tgt) ** 2).sum(1).mean()
+        L.backward()
+    with sim(theta=0.5, dt=0.0333, precision="f32") as s:
+        s.set_state(pos, vel, mass)
+        s.adjoint_forward(10, 5)
+        xT2, _, _ = s.state()
+        gx0, gv0, gm = s.adjoint_backward(2.0 * (xT2 - tgt.numpy()) / len(pos), None)
+    np.testing.assert_allclose(x0.grad.numpy(), gx0, rtol=1e-12, atol=0)
+    np.testing.assert_allclose(v0.grad.numpy(), gv0, rtol=1e-12, atol=0)
+    np.testing.assert_allclose(m.grad.numpy(), gm, rtol=1e-12, atol=0)

@@ -224,3 +224,47 @@ def test_large_sphere_invariants_and_precision():
     e_trunc = np.sqrt(np.mean(rel_err(a64[ids], ref) ** 2))
     assert e_prec < 1e-5
     assert e_trunc < 1e-2
+
+
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+def test_graph_replays_equal_eager_steps(precision):
+    pos, vel, mass = O.generate(20000, "sphere")
+    with sim(theta=0.5, precision=precision) as s:
+        s.set_state(pos, vel, mass)
+        s.step(7)
+        want = s.state()
+    with sim(theta=0.5, precision=precision) as s:
+        s.set_state(pos, vel, mass)
+        # several replays with eager work and synchronisation in between
+        s.step(3, graph=True)
+        mid = s.state()
+        s.step(2, graph=True)
+        s.synchronize()
+        s.step(2, graph=True)
+        got = s.state()
+    with sim(theta=0.5, precision=precision) as s:
+        s.set_state(pos, vel, mass)
+        s.step(3)
+        np.testing.assert_array_equal(s.state()[0], mid[0])
+    for a, b in zip(got, want):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_graph_batched_adaptive():
+    parts = [O.generate(n, g, seed=1 + k) for k, (n, g) in
+             enumerate([(3000, "cube"), (1025, "sphere"), (64, "shell"), (4000, "sphere")])]
+    pos = np.concatenate([p[0] for p in parts])
+    vel = np.concatenate([p[1] for p in parts])
+    mass = np.concatenate([p[2] for p in parts])
+    off = np.cumsum([0] + [len(p[2]) for p in parts])
+    res = []
+    for graph in (False, True):
+        with sim(theta=0.5, precision="f32", adaptive=True) as s:
+            s.set_state(pos, vel, mass)
+            s.set_systems(off)
+            s.step(6, graph=graph)
+            res.append((s.state(), s.time(len(parts))))
+    for a, b in zip(res[0][0], res[1][0]):
+        np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(res[0][1], res[1][1])
+    assert (res[0][1] > 0).all()
