@@ -40,24 +40,33 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def time_path(sim, pos, vel, mass, precision, steps, warmup):
+def time_path(sim, pos, vel, mass, precision, steps, warmup, prof_steps):
+    """Timed region: `steps` graph-replayed steps (nbs_step_graph) after
+    `warmup` steps.  Kernel timing for the roofline: `prof_steps` eager
+    steps right after, with HIP events around each phase on the context's
+    stream, and the interactions those walks counted."""
     sim.configure(precision=precision)
     sim.set_state(pos, vel, mass)
     if warmup:
-        sim.step(warmup)
+        sim.step(warmup, graph=True)
     sim.synchronize()
+    t0 = time.perf_counter()
+    sim.step(steps, graph=True)
+    sim.synchronize()
+    t1 = time.perf_counter()
+    info = sim.tree_info()
     sim.interactions_total(reset=True)
     sim.profile(True)
     sim.profile_read(reset=True)
-    t0 = time.perf_counter()
-    sim.step(steps)
+    t2 = time.perf_counter()
+    sim.step(prof_steps)
     sim.synchronize()
-    t1 = time.perf_counter()
+    t3 = time.perf_counter()
     prof = sim.profile_read(reset=True)
     sim.profile(False)
     inter = sim.interactions_total(reset=True)
-    info = sim.tree_info()
     return {"ms_per_step": (t1 - t0) * 1e3 / steps, "prof": prof, "interactions": inter,
+            "prof_steps": prof_steps, "eager_ms_per_step": (t3 - t2) * 1e3 / prof_steps,
             "tree": info}
 
 
@@ -91,6 +100,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--fp64-steps", type=int, default=20)
+    ap.add_argument("--prof-steps", type=int, default=10)
     ap.add_argument("--samples", type=int, default=4096)
     ap.add_argument("--cpu-sample", type=int, default=16384)
     ap.add_argument("--no-cpu", action="store_true")
@@ -127,21 +137,21 @@ def main():
     log("accuracy", acc)
 
     # ---- timed paths
-    r32 = time_path(sim, pos, vel, mass, "f32", args.steps, args.warmup)
+    r32 = time_path(sim, pos, vel, mass, "f32", args.steps, args.warmup, args.prof_steps)
     log("f32", r32)
-    r64 = time_path(sim, pos, vel, mass, "f64", args.fp64_steps, args.warmup)
+    r64 = time_path(sim, pos, vel, mass, "f64", args.fp64_steps, args.warmup, args.prof_steps)
     log("f64", r64)
     sim.close()
 
     value = n * 1e3 / r32["ms_per_step"]
     value64 = n * 1e3 / r64["ms_per_step"]
     walk_ms = r32["prof"]["walk_ms"] / max(r32["prof"]["walk_launches"], 1)
-    inter_per_launch = r32["interactions"] / args.steps
+    inter_per_launch = r32["interactions"] / r32["prof_steps"]
     achieved = FLOP_PER_INTERACTION * inter_per_launch / (walk_ms * 1e-3) / 1e12
     walk64_ms = r64["prof"]["walk_ms"] / max(r64["prof"]["walk_launches"], 1)
-    achieved64 = FLOP_PER_INTERACTION * (r64["interactions"] / args.fp64_steps) / \
+    achieved64 = FLOP_PER_INTERACTION * (r64["interactions"] / r64["prof_steps"]) / \
         (walk64_ms * 1e-3) / 1e12
-    build_ms = r32["prof"]["build_ms"] / args.steps
+    build_ms = r32["prof"]["build_ms"] / r32["prof_steps"]
     nodes = r32["tree"]["n_nodes"]
     build_bytes = 56 * n + 32 * nodes
     out = {
@@ -161,7 +171,7 @@ def main():
                                "U[5e10,5e11], theta=0.5, dt=0.0333, bbox+2 root; FP32 walk "
                                "forces, FP64 state/keys/COM" % n,
                    "n": n, "theta": theta, "dt": dt, "parallelism": "single GPU"},
-        "interactions_per_s": r32["interactions"] / (r32["ms_per_step"] * args.steps * 1e-3),
+        "interactions_per_s": inter_per_launch / (r32["ms_per_step"] * 1e-3),
         "interactions_per_particle": inter_per_launch / n,
         "fp64": {"value": value64, "ms_per_step": r64["ms_per_step"],
                  "steps": args.fp64_steps, "walk_ms": walk64_ms,
@@ -169,7 +179,8 @@ def main():
                  "speedup_fp32_over_fp64": value / value64},
         "accuracy": acc,
         "phases_ms_per_step": {"build": build_ms, "walk": walk_ms,
-                               "update": r32["prof"]["update_ms"] / args.steps},
+                               "update": r32["prof"]["update_ms"] / r32["prof_steps"],
+                               "eager_step": r32["eager_ms_per_step"]},
         "tree_build": {"ms": build_ms, "algorithmic_bytes": build_bytes,
                        "gbps": build_bytes / (build_ms * 1e-3) / 1e9,
                        "frac_hbm": build_bytes / (build_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,

@@ -284,7 +284,10 @@ int alloc_particles(nbs_ctx* c, int n) {
   b.counts = dalloc<int>(o, cap, &e);
   b.offsets = dalloc<int>(o, (size_t)cap + 1, &e);
   b.sort_tmp = dalloc<int>(o, radix_tmp_ints(cap), &e);
-  b.scan_tmp = dalloc<int>(o, scan_tmp_ints((size_t)cap + 1), &e);
+  b.dhist = dalloc<int>(o, depth_hist_ints(cap), &e);
+  b.doff = dalloc<int>(o, depth_hist_ints(cap), &e);
+  b.scan_tmp = dalloc<int>(
+      o, scan_tmp_ints(std::max((size_t)cap + 1, depth_hist_ints(cap))), &e);
   if (e != hipSuccess) {
     free_all(c->part_bufs);
     c->cap_n = 0;
@@ -304,8 +307,7 @@ int alloc_nodes(nbs_ctx* c, int cap) {
   c->aux.first = dalloc<int>(o, cap, &e);
   c->aux.count = dalloc<int>(o, cap, &e);
   c->aux.parent = dalloc<int>(o, cap, &e);
-  c->aux.nchild = dalloc<int>(o, cap, &e);
-  c->aux.arrive = dalloc<int>(o, cap, &e);
+  c->bb.bucket = dalloc<int>(o, cap, &e);
   if (e != hipSuccess) {
     free_all(c->node_bufs);
     c->node_cap = 0;
@@ -327,7 +329,6 @@ int setup_systems(nbs_ctx* c) {
   s.start = dalloc<int>(o, nsys + 1, &e);
   s.wave_start = dalloc<int>(o, nsys + 1, &e);
   s.node_start = dalloc<int>(o, nsys + 1, &e);
-  s.bbox = dalloc<unsigned long long>(o, 6 * (size_t)nsys, &e);
   s.root = dalloc<double>(o, 4 * (size_t)nsys, &e);
   s.dt = dalloc<double>(o, nsys, &e);
   s.time = dalloc<double>(o, nsys, &e);
@@ -344,6 +345,10 @@ int setup_systems(nbs_ctx* c) {
     c->h_wave_start[k + 1] = c->h_wave_start[k] + (cnt + kWave - 1) / kWave;
   }
   s.n_waves = c->h_wave_start[nsys];
+  s.bbox = dalloc<double>(o, 6 * (size_t)std::max(s.n_waves, 1), &e);
+  c->inter_total = dalloc<unsigned long long>(o, std::max(s.n_waves, 1), &e);
+  if (e != hipSuccess) return hip_fail(c, e, "hipMalloc(system buffers)");
+  launch_fill_u64(c->inter_total, 0, std::max(s.n_waves, 1), c->stream);
   HIPCHK(c, hipMemcpyAsync(s.start, c->h_start.data(), (nsys + 1) * sizeof(int),
                            hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(s.wave_start, c->h_wave_start.data(),
@@ -390,8 +395,8 @@ void enqueue_build(nbs_ctx* c) {
   State& cur = c->st[c->cur];
   State& nxt = c->st[1 - c->cur];
   if (!c->bbox_valid) {
-    launch_bbox_reset(c->sys, s);
     launch_bbox(cur, c->sys, s);
+    c->bbox_valid = true;  // positions do not change during a build
   }
   launch_root(c->sys, c->p, c->flags, s);
   const int nbits = launch_keys(cur, c->sys, c->bb, s);
@@ -402,10 +407,9 @@ void enqueue_build(nbs_ctx* c) {
   launch_fixup(c->bb, prim, perm, cur, c->n, s);
   launch_permute(cur, nxt, perm, c->n, s);
   c->cur ^= 1;
-  launch_emit(c->bb, c->sys, c->p, c->nodes, c->aux, c->n, c->node_cap, s);
-  launch_link(c->bb, c->nodes, c->aux, c->n, c->node_cap, s);
-  launch_bottom_up(c->bb, c->st[c->cur], c->nodes, c->aux, c->n, s);
-  c->bbox_valid = false;
+  launch_emit(c->bb, c->sys, c->p, c->st[c->cur], c->nodes, c->aux, c->n,
+              c->node_cap, s);
+  launch_bottom_up(c->bb, c->nodes, c->aux, c->n, kLevels, s);
   prof_end(c, kPhaseBuild, e0);
 }
 
@@ -452,7 +456,6 @@ void enqueue_adaptive_update(nbs_ctx* c) {
   State& st = c->st[c->cur];
   launch_adaptive_dt(st, c->ax, c->ay, c->az, c->n, c->sys, c->p, c->flags,
                      c->stream);
-  launch_bbox_reset(c->sys, c->stream);
   launch_update(st, c->ax, c->ay, c->az, c->n, c->sys, c->flags, c->stream);
   c->bbox_valid = true;
   prof_end(c, kPhaseUpdate, e0);
@@ -606,7 +609,6 @@ int nbs_create(const nbs_config* cfg, nbs_ctx** out) {
   c->bb.flags = c->flags;
   c->bb.stats = dalloc<int>(c->misc_bufs, 4, &e);
   c->stats = c->bb.stats;
-  c->inter_total = dalloc<unsigned long long>(c->misc_bufs, 1, &e);
   if (e != hipSuccess) {
     free_all(c->misc_bufs);
     (void)hipStreamDestroy(c->stream);
@@ -615,7 +617,6 @@ int nbs_create(const nbs_config* cfg, nbs_ctx** out) {
   }
   launch_fill_int(c->flags, 0, 1, c->stream);
   launch_fill_int(c->stats, 0, 4, c->stream);
-  launch_fill_u64(c->inter_total, 0, 1, c->stream);
   *out = c;
   return NBS_OK;
 }
@@ -874,12 +875,16 @@ int nbs_interactions(nbs_ctx* c, int64_t* total, int64_t* per_particle) {
 
 int nbs_interactions_total(nbs_ctx* c, int64_t* total, int reset) {
   if (!c || !total) return NBS_ERR_ARG;
-  unsigned long long h = 0;
-  HIPCHK(c, hipMemcpyAsync(&h, c->inter_total, sizeof(h), hipMemcpyDeviceToHost,
-                           c->stream));
+  if (!c->has_state) return fail(c, NBS_ERR_STATE, "no particle state set");
+  const int nw = std::max(c->sys.n_waves, 1);
+  std::vector<unsigned long long> h(nw);
+  HIPCHK(c, hipMemcpyAsync(h.data(), c->inter_total, nw * sizeof(h[0]),
+                           hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  *total = (int64_t)h;
-  if (reset) launch_fill_u64(c->inter_total, 0, 1, c->stream);
+  unsigned long long t = 0;
+  for (unsigned long long v : h) t += v;
+  *total = (int64_t)t;
+  if (reset) launch_fill_u64(c->inter_total, 0, nw, c->stream);
   return NBS_OK;
 }
 
@@ -1288,8 +1293,8 @@ int nbs_frozen_forward(nbs_ctx* c, const double* pos0, const double* vel0,
     gather3_kernel<<<b, 256, 0, s>>>(c->p_x, st.id, (int)n, st.x, st.y, st.z);
     gather3_kernel<<<b, 256, 0, s>>>(c->p_v, st.id, (int)n, st.vx, st.vy, st.vz);
     launch_gather_comp(c->p_m, st.m, st.id, (int)n, 1, s);
-    launch_fill_int(c->aux.arrive, 0, c->node_cap, s);
-    launch_bottom_up(c->bb, st, c->nodes, c->aux, (int)n, s);
+    launch_leaf_refresh(c->bb, st, c->nodes, (int)n, s);
+    launch_bottom_up(c->bb, c->nodes, c->aux, (int)n, kLevels, s);
     WalkArgs a{};
     a.nodes = c->nodes;
     a.node_start = c->sys.node_start;

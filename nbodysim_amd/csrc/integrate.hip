@@ -76,8 +76,7 @@ __global__ void __launch_bounds__(kBlock)
                   const double* __restrict__ ay, const double* __restrict__ az,
                   const int* __restrict__ start,
                   const int* __restrict__ wstart, int nsys, int nwaves,
-                  const double* __restrict__ sys_dt,
-                  unsigned long long* __restrict__ bbox,
+                  const double* __restrict__ sys_dt, double* __restrict__ bbox,
                   const int* __restrict__ flags) {
 #pragma clang fp contract(off)
   if (*flags) return;
@@ -106,20 +105,7 @@ __global__ void __launch_bounds__(kBlock)
     st.y[i] = y;
     st.z[i] = z;
   }
-  const double mnx = wave_min(v ? x : INFINITY), mny = wave_min(v ? y : INFINITY),
-               mnz = wave_min(v ? z : INFINITY);
-  const double mxx = wave_max(v ? x : -INFINITY),
-               mxy = wave_max(v ? y : -INFINITY),
-               mxz = wave_max(v ? z : -INFINITY);
-  if (lane_id() == 0) {
-    unsigned long long* b = bbox + 6 * s;
-    atomicMin(b + 0, ord_enc(mnx));
-    atomicMin(b + 1, ord_enc(mny));
-    atomicMin(b + 2, ord_enc(mnz));
-    atomicMax(b + 3, ord_enc(mxx));
-    atomicMax(b + 4, ord_enc(mxy));
-    atomicMax(b + 5, ord_enc(mxz));
-  }
+  wave_bbox_partial(v, x, y, z, w, nwaves, bbox);
 }
 
 }  // namespace

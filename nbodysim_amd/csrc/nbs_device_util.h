@@ -59,6 +59,27 @@ __device__ __host__ __forceinline__ double ord_dec(unsigned long long u) {
   return __builtin_bit_cast(double, u);
 }
 
+// Bounding-box partial of one wave: part[k * nwaves + w] for k = min x,
+// y, z, max x, y, z over the wave's valid lanes (every lane participates).
+__device__ __forceinline__ void wave_bbox_partial(bool v, double x, double y,
+                                                  double z, int w, int nwaves,
+                                                  double* part) {
+  const double mnx = wave_min(v ? x : INFINITY);
+  const double mny = wave_min(v ? y : INFINITY);
+  const double mnz = wave_min(v ? z : INFINITY);
+  const double mxx = wave_max(v ? x : -INFINITY);
+  const double mxy = wave_max(v ? y : -INFINITY);
+  const double mxz = wave_max(v ? z : -INFINITY);
+  if (lane_id() == 0) {
+    part[w] = mnx;
+    part[nwaves + w] = mny;
+    part[2 * nwaves + w] = mnz;
+    part[3 * nwaves + w] = mxx;
+    part[4 * nwaves + w] = mxy;
+    part[5 * nwaves + w] = mxz;
+  }
+}
+
 // System owning walk wave w / particle slot i (batched mode): the last
 // s with start[s] <= w (binary search over nsys+1 offsets).
 __device__ __forceinline__ int sys_of_wave(int w, const int* __restrict__ ws,

@@ -19,6 +19,7 @@ namespace nbs {
 
 constexpr double kG = 6.67384e-11;  // src/main.cpp:322
 constexpr int kLevels = 42;         // key depth: 21 levels per u64 half
+constexpr int kDepthBins = kLevels + 1;  // internal-cell depths 0..42
 constexpr int kWave = 64;
 
 // One tree entry.  Read by the walk with wave-uniform scalar loads, so the
@@ -41,8 +42,6 @@ struct TreeAux {
   int* first;   // [cap] first particle slot
   int* count;   // [cap] particle count
   int* parent;  // [cap]
-  int* nchild;  // [cap]
-  int* arrive;  // [cap] bottom-up arrival counters
 };
 
 // Device-side per-system data (batched mode; one system otherwise).
@@ -52,7 +51,7 @@ struct Systems {
   int* start;        // [nsys+1] particle slots
   int* wave_start;   // [nsys+1] walk waves (64 particles each, per system)
   int* node_start;   // [nsys+1] pre-order node offsets (written by emit)
-  unsigned long long* bbox;  // [nsys*6] ordered-encoded min xyz, max xyz
+  double* bbox;      // [6][n_waves] per-wave min xyz, max xyz
   double* root;      // [nsys*4] centre xyz, side
   double* dt;        // [nsys] step length used by the last update
   double* time;      // [nsys] simulated time
@@ -103,13 +102,15 @@ struct BuildBuffers {
   unsigned long long *khi, *klo;      // keys in sorted order
   int* counts;                        // [n]
   int* offsets;                       // [n+1] node offsets per slot
+  int* dhist;                         // [depth_hist_ints(n)] per-block depths
+  int* doff;                          // same, exclusive-scanned
+  int* bucket;                        // [node cap] internal cells by depth
   int* sort_tmp;
   int* scan_tmp;
   int* flags;                         // device flags word
   int* stats;                         // [4]: nodes, internal, maxdepth, buckets
 };
 
-void launch_bbox_reset(Systems& sys, hipStream_t s);
 void launch_bbox(const State& st, Systems& sys, hipStream_t s);
 void launch_root(Systems& sys, const StepParams& p, const int* flags,
                  hipStream_t s);
@@ -123,14 +124,17 @@ void launch_fixup(BuildBuffers& b, const unsigned long long* prim,
                   unsigned* perm, const State& st, int n, hipStream_t s);
 void launch_permute(const State& src, State& dst, const unsigned* perm,
                     int n, hipStream_t s);
-// Emits the pre-order node array; node capacity `cap`.
+// Emits the pre-order node array (capacity `cap`), leaves filled from `st`.
+size_t depth_hist_ints(size_t n);
 void launch_emit(BuildBuffers& b, Systems& sys, const StepParams& p,
-                 WalkNode* nodes, TreeAux& aux, int n, int cap,
-                 hipStream_t s);
-void launch_link(BuildBuffers& b, WalkNode* nodes, TreeAux& aux, int n,
+                 const State& st, WalkNode* nodes, TreeAux& aux, int n,
                  int cap, hipStream_t s);
-void launch_bottom_up(BuildBuffers& b, const State& st, WalkNode* nodes,
-                      TreeAux& aux, int n, hipStream_t s);
+// Mass/COM of every internal cell, one launch per level from max_depth up.
+void launch_bottom_up(BuildBuffers& b, WalkNode* nodes, TreeAux& aux, int n,
+                      int max_depth, hipStream_t s);
+// Leaves of the current topology take new particle data.
+void launch_leaf_refresh(BuildBuffers& b, const State& st, WalkNode* nodes,
+                         int n, hipStream_t s);
 
 // ----------------------------------------------------------------- walk --
 struct WalkArgs {
@@ -147,7 +151,7 @@ struct WalkArgs {
   int* flags_out;
   double theta;
   double dt;
-  unsigned long long* bbox;  // next step's bbox accumulators (fused update)
+  double* bbox;  // next step's per-wave bbox partials (fused update)
   double* sys_dt;
   double* sys_time;
   int* steps_done;  // incremented once per completed walk

@@ -17,6 +17,10 @@
 // up to its `skip`.  Mass and COM are then reduced bottom-up, each cell
 // summing its children in child order with the reference's operation
 // sequence (src/octree.cpp:203-221, :252-278), so they match bit for bit.
+// The reduction runs one kernel per tree level from the deepest up
+// (internal cells are bucketed by depth during emission), so a parent reads
+// its children after a kernel boundary and no device-wide fences are
+// needed.
 #include <cmath>
 
 #include "nbs_device_util.h"
@@ -49,74 +53,75 @@ __global__ void fill_u64_kernel(unsigned long long* p, unsigned long long v,
   if (i < n) p[i] = v;
 }
 
-__global__ void bbox_reset_kernel(unsigned long long* bbox, int nsys) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nsys * 6)
-    bbox[i] = (i % 6) < 3 ? ord_enc(INFINITY) : ord_enc(-INFINITY);
-}
-
-// Bounding box per system: wave min/max, then one atomic per wave.
+// Bounding-box partials: every wave writes the min/max of its particles to
+// part[k * nwaves + w] (k = min x, y, z, max x, y, z); root_kernel reduces a
+// system's waves.  No atomics, so no contention on a few addresses.
 __global__ void __launch_bounds__(kBlock)
     bbox_kernel(const double* __restrict__ x, const double* __restrict__ y,
                 const double* __restrict__ z, const int* __restrict__ start,
                 const int* __restrict__ wstart, int nsys, int nwaves,
-                unsigned long long* __restrict__ bbox) {
+                double* __restrict__ part) {
   const int w = (blockIdx.x * kBlock + threadIdx.x) >> 6;
   if (w >= nwaves) return;
   const int s = nsys > 1 ? sys_of_wave(w, wstart, nsys) : 0;
   const int i = start[s] + (w - wstart[s]) * 64 + lane_id();
   const bool v = i < start[s + 1];
-  double px = v ? x[i] : 0, py = v ? y[i] : 0, pz = v ? z[i] : 0;
-  double mnx = wave_min(v ? px : INFINITY), mny = wave_min(v ? py : INFINITY),
-         mnz = wave_min(v ? pz : INFINITY);
-  double mxx = wave_max(v ? px : -INFINITY),
-         mxy = wave_max(v ? py : -INFINITY),
-         mxz = wave_max(v ? pz : -INFINITY);
-  if (lane_id() == 0) {
-    unsigned long long* b = bbox + 6 * s;
-    atomicMin(b + 0, ord_enc(mnx));
-    atomicMin(b + 1, ord_enc(mny));
-    atomicMin(b + 2, ord_enc(mnz));
-    atomicMax(b + 3, ord_enc(mxx));
-    atomicMax(b + 4, ord_enc(mxy));
-    atomicMax(b + 5, ord_enc(mxz));
-  }
+  const double px = v ? x[i] : 0, py = v ? y[i] : 0, pz = v ? z[i] : 0;
+  wave_bbox_partial(v, px, py, pz, w, nwaves, part);
 }
 
 // Root cell per system: src/main.cpp:908-920 in FP64 (midpoint of the
-// bbox; side = largest extent + 2.0, ties falling through to z).  Resets
-// the bbox accumulators for the next step.
-__global__ void root_kernel(unsigned long long* __restrict__ bbox,
-                            double* __restrict__ root, int nsys, int mode,
-                            double cx, double cy, double cz, double side,
-                            const int* __restrict__ flags) {
+// bbox; side = largest extent + 2.0, ties falling through to z).  One block
+// per system reduces the system's wave partials.
+__global__ void __launch_bounds__(kBlock)
+    root_kernel(const double* __restrict__ part, int nwaves,
+                const int* __restrict__ wstart, double* __restrict__ root,
+                int nsys, int mode, double cx, double cy, double cz,
+                double side, const int* __restrict__ flags) {
 #pragma clang fp contract(off)
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = blockIdx.x;
   if (s >= nsys || *flags) return;
-  unsigned long long* b = bbox + 6 * s;
+  __shared__ double red[6][kBlock / 64];
+  double b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  if (mode != 1) {
+    for (int w = wstart[s] + threadIdx.x; w < wstart[s + 1]; w += kBlock)
+      for (int k = 0; k < 6; k++) {
+        const double v = part[(size_t)k * nwaves + w];
+        b[k] = k < 3 ? fmin(b[k], v) : fmax(b[k], v);
+      }
+    for (int k = 0; k < 6; k++) {
+      b[k] = k < 3 ? wave_min(b[k]) : wave_max(b[k]);
+      if (lane_id() == 0) red[k][threadIdx.x >> 6] = b[k];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
   if (mode == 1) {
     root[4 * s] = cx;
     root[4 * s + 1] = cy;
     root[4 * s + 2] = cz;
     root[4 * s + 3] = side;
-  } else {
-    double min_x = ord_dec(b[0]), min_y = ord_dec(b[1]), min_z = ord_dec(b[2]);
-    double max_x = ord_dec(b[3]), max_y = ord_dec(b[4]), max_z = ord_dec(b[5]);
-    root[4 * s] = (min_x + max_x) / 2.0;
-    root[4 * s + 1] = (min_y + max_y) / 2.0;
-    root[4 * s + 2] = (min_z + max_z) / 2.0;
-    double ex = max_x - min_x, ey = max_y - min_y, ez = max_z - min_z;
-    double sz;
-    if (ex > ey && ex > ez)
-      sz = ex + 2.0;
-    else if (ey > ex && ey > ez)
-      sz = ey + 2.0;
-    else
-      sz = ez + 2.0;
-    root[4 * s + 3] = sz;
+    return;
   }
-  b[0] = b[1] = b[2] = ord_enc(INFINITY);
-  b[3] = b[4] = b[5] = ord_enc(-INFINITY);
+  for (int k = 0; k < 6; k++) {
+    double r = red[k][0];
+    for (int q = 1; q < kBlock / 64; q++) r = k < 3 ? fmin(r, red[k][q]) : fmax(r, red[k][q]);
+    b[k] = r;
+  }
+  const double min_x = b[0], min_y = b[1], min_z = b[2];
+  const double max_x = b[3], max_y = b[4], max_z = b[5];
+  root[4 * s] = (min_x + max_x) / 2.0;
+  root[4 * s + 1] = (min_y + max_y) / 2.0;
+  root[4 * s + 2] = (min_z + max_z) / 2.0;
+  const double ex = max_x - min_x, ey = max_y - min_y, ez = max_z - min_z;
+  double sz;
+  if (ex > ey && ex > ez)
+    sz = ex + 2.0;
+  else if (ey > ex && ey > ez)
+    sz = ey + 2.0;
+  else
+    sz = ez + 2.0;
+  root[4 * s + 3] = sz;
 }
 
 // 42-level keys by the reference's rounded centre chain.  The primary sort
@@ -233,6 +238,7 @@ __global__ void permute_kernel(State src, State dst,
   dst.id[j] = src.id[p];
 }
 
+
 struct Neigh {
   int s, a, e, lp, ln;
 };
@@ -251,16 +257,26 @@ __device__ __forceinline__ Neigh neighbours(
   return r;
 }
 
-// Entries (internal cells + the leaf) that start at slot i.
+// Entries (internal cells + the leaf) that start at slot i, and this
+// block's histogram of internal-cell depths (depth-major: the buckets the
+// per-level reduction walks).
 __global__ void __launch_bounds__(kBlock)
     emit_count_kernel(const int* __restrict__ start, int nsys, int n,
                       const unsigned long long* __restrict__ khi,
                       const unsigned long long* __restrict__ klo,
-                      int* __restrict__ counts) {
-  int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  Neigh g = neighbours(i, start, nsys, khi, klo);
-  counts[i] = max(0, g.ln - g.lp) + 1;
+                      int* __restrict__ counts, int* __restrict__ dhist) {
+  __shared__ int h[kDepthBins];
+  for (int d = threadIdx.x; d < kDepthBins; d += kBlock) h[d] = 0;
+  __syncthreads();
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) {
+    const Neigh g = neighbours(i, start, nsys, khi, klo);
+    counts[i] = max(0, g.ln - g.lp) + 1;
+    for (int d = g.lp + 1; d <= g.ln; d++) atomicAdd(&h[d], 1);
+  }
+  __syncthreads();
+  for (int d = threadIdx.x; d < kDepthBins; d += kBlock)
+    dhist[d * gridDim.x + blockIdx.x] = h[d];
 }
 
 __global__ void sys_nodes_kernel(const int* __restrict__ start,
@@ -297,23 +313,32 @@ __device__ __forceinline__ int range_end(int i, int d, int e,
   return good;
 }
 
+// Writes the cells and the leaf that start at slot i (pre-order), files
+// each internal cell into its depth bucket, and fills the leaf's mass/COM
+// from its particle.  Same grid as emit_count_kernel.
 __global__ void __launch_bounds__(kBlock)
     emit_kernel(const int* __restrict__ start, int nsys, int n,
                 const unsigned long long* __restrict__ khi,
                 const unsigned long long* __restrict__ klo,
                 const int* __restrict__ offsets,
+                const int* __restrict__ doff,
                 const double* __restrict__ root, double theta, int cap,
+                const double* __restrict__ x, const double* __restrict__ y,
+                const double* __restrict__ z, const double* __restrict__ m,
                 WalkNode* __restrict__ nodes, TreeAux aux,
-                int* __restrict__ flags, int* __restrict__ stats) {
-  int i = blockIdx.x * kBlock + threadIdx.x;
+                int* __restrict__ bucket, int* __restrict__ flags,
+                int* __restrict__ stats) {
+  __shared__ int cursor[kDepthBins];
+  for (int d = threadIdx.x; d < kDepthBins; d += kBlock)
+    cursor[d] = doff[d * gridDim.x + blockIdx.x];
+  __syncthreads();
+  const int i = blockIdx.x * kBlock + threadIdx.x;
   const int total = offsets[n];
-  if (total > cap) {
-    if (i == 0) atomicOr(flags, kFlagNodeOverflow);
-    return;
-  }
+  const bool fits = total <= cap;
+  if (!fits && i == 0) atomicOr(flags, kFlagNodeOverflow);
   int my_depth = 0, my_bucket = 0;
-  if (i < n) {
-    Neigh g = neighbours(i, start, nsys, khi, klo);
+  if (i < n && fits) {
+    const Neigh g = neighbours(i, start, nsys, khi, klo);
     const double s0 = root[4 * g.s + 3];
     const int base = offsets[i];
     const int nint = max(0, g.ln - g.lp);
@@ -326,23 +351,29 @@ __global__ void __launch_bounds__(kBlock)
       w.m = w.gm = 0;
       w.gm32 = 0;
       w.side = ldexp(s0, -d);
-      const bool bucket = d >= kLevels;
+      const bool bkt = d >= kLevels;
       const double ro = w.side / theta;
-      w.r2open32 = bucket ? INFINITY : (float)(ro * ro);
+      w.r2open32 = bkt ? INFINITY : (float)(ro * ro);
       w.skip = offsets[b + 1];
-      w.leaf = bucket ? -2 : -1;
+      w.leaf = bkt ? -2 : -1;
       nodes[node] = w;
       aux.depth[node] = d;
       aux.first[node] = i;
       aux.count[node] = b - i + 1;
-      if (bucket) my_bucket++;
+      if (d == 0) aux.parent[node] = -1;
+      bucket[atomicAdd(&cursor[d], 1)] = node;
+      if (bkt) my_bucket++;
     }
     const int leafn = base + nint;
     const int ld = 1 + max(g.lp, g.ln);
+    const double mi = m[i];
     WalkNode w;
-    w.cx = w.cy = w.cz = 0;
-    w.m = w.gm = 0;
-    w.gm32 = 0;
+    w.cx = x[i];
+    w.cy = y[i];
+    w.cz = z[i];
+    w.m = mi;
+    w.gm = kG * mi;
+    w.gm32 = (float)(kG * mi);
     w.side = ldexp(s0, -ld);
     w.r2open32 = -1.0f;
     w.skip = leafn + 1;
@@ -351,13 +382,13 @@ __global__ void __launch_bounds__(kBlock)
     aux.depth[leafn] = ld;
     aux.first[leafn] = i;
     aux.count[leafn] = 1;
+    if (ld == 0) aux.parent[leafn] = -1;
     my_depth = ld;
   }
-  // tree statistics: nodes, internal cells, max depth, buckets
   int md = my_depth;
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) md = max(md, __shfl_xor(md, d, 64));
-  int nb = wave_sum(my_bucket);
+  const int nb = wave_sum(my_bucket);
   if (lane_id() == 0) {
     atomicMax(&stats[2], md);
     if (nb) atomicAdd(&stats[3], nb);
@@ -368,47 +399,17 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
-// Parent links and child counts; clears the bottom-up arrival counters.
+// Leaves take their particle's position and mass (frozen replays refresh
+// the leaves of an existing topology with new particle data).
 __global__ void __launch_bounds__(kBlock)
-    link_kernel(const int* __restrict__ offsets, int n,
-                const WalkNode* __restrict__ nodes, TreeAux aux,
-                const int* __restrict__ flags, int* flags_out) {
-  if (*flags) return;
-  const int total = offsets[n];
-  for (int k = blockIdx.x * kBlock + threadIdx.x; k < total;
-       k += gridDim.x * kBlock) {
-    aux.arrive[k] = 0;
-    if (aux.depth[k] == 0) aux.parent[k] = -1;
-    if (nodes[k].leaf >= 0) {
-      aux.nchild[k] = 0;
-      continue;
-    }
-    const int end = min(nodes[k].skip, total);
-    int c = k + 1, cnt = 0;
-    while (c < end) {
-      aux.parent[c] = k;
-      cnt++;
-      const int nx = nodes[c].skip;
-      if (nx <= c) {  // corrupt tree: stop rather than loop
-        atomicOr(flags_out, kFlagNodeOverflow);
-        break;
-      }
-      c = nx;
-    }
-    aux.nchild[k] = cnt;
-  }
-}
-
-// Leaves take their particle; the last child to finish reduces its parent
-// (children in order 0..7, the reference's operation sequence), climbing
-// while it keeps arriving last.
-__global__ void __launch_bounds__(kBlock)
-    bottom_up_kernel(const int* __restrict__ offsets, int n,
-                     const double* __restrict__ x, const double* __restrict__ y,
-                     const double* __restrict__ z, const double* __restrict__ m,
-                     WalkNode* nodes, TreeAux aux, const int* __restrict__ flags) {
-#pragma clang fp contract(off)
-  int i = blockIdx.x * kBlock + threadIdx.x;
+    leaf_refresh_kernel(const int* __restrict__ offsets, int n,
+                        const double* __restrict__ x,
+                        const double* __restrict__ y,
+                        const double* __restrict__ z,
+                        const double* __restrict__ m,
+                        WalkNode* __restrict__ nodes,
+                        const int* __restrict__ flags) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n || *flags) return;
   const int leafn = offsets[i + 1] - 1;
   const double mi = m[i];
@@ -418,26 +419,33 @@ __global__ void __launch_bounds__(kBlock)
   nodes[leafn].m = mi;
   nodes[leafn].gm = kG * mi;
   nodes[leafn].gm32 = (float)(kG * mi);
-  __threadfence();
-  int p = aux.parent[leafn];
-  for (int level = 0; p >= 0 && level <= kLevels + 2; level++) {
-    const int old = atomicAdd(&aux.arrive[p], 1);
-    if (old != aux.nchild[p] - 1) break;
-    __threadfence();
+}
+
+// One tree level: every internal cell of depth d sums its children (depth
+// d+1, already final) in child order 0..7 with the reference's operation
+// sequence, src/octree.cpp:203-221 (mass) and :252-278 (COM), and links
+// them to itself.  Launched from the deepest level up; kernel boundaries
+// order the levels, so no fences are needed.
+__global__ void __launch_bounds__(kBlock)
+    level_kernel(int d, int nblocks, const int* __restrict__ doff,
+                 const int* __restrict__ bucket, WalkNode* __restrict__ nodes,
+                 int* __restrict__ parent, const int* __restrict__ flags) {
+#pragma clang fp contract(off)
+  if (*flags) return;
+  const int b0 = doff[d * nblocks], b1 = doff[(d + 1) * nblocks];
+  for (int q = b0 + blockIdx.x * kBlock + threadIdx.x; q < b1;
+       q += gridDim.x * kBlock) {
+    const int p = bucket[q];
     const int end = nodes[p].skip;
-    double M = 0, tx = 0, ty = 0, tz = 0;
-    int c = p + 1;
-    while (c < end) {
-      const double cm = nodes[c].m;
-      M = M + cm;
-      c = max(nodes[c].skip, c + 1);
+    double M = 0;
+    for (int c = p + 1; c < end; c = max(nodes[c].skip, c + 1)) {
+      M = M + nodes[c].m;
+      parent[c] = p;
     }
-    double cxv, cyv, czv;
-    if (M == 0) {
-      cxv = cyv = czv = 0;  // src/octree.cpp:259-262
-    } else {
-      c = p + 1;
-      while (c < end) {
+    double cxv = 0, cyv = 0, czv = 0;
+    if (M != 0) {  // else COM (0,0,0), src/octree.cpp:259-262
+      double tx = 0, ty = 0, tz = 0;
+      for (int c = p + 1; c < end; c = max(nodes[c].skip, c + 1)) {
         const double cm = nodes[c].m;
         const double px = nodes[c].cx * cm;
         const double py = nodes[c].cy * cm;
@@ -445,7 +453,6 @@ __global__ void __launch_bounds__(kBlock)
         tx = tx + px;
         ty = ty + py;
         tz = tz + pz;
-        c = max(nodes[c].skip, c + 1);
       }
       cxv = tx / M;
       cyv = ty / M;
@@ -457,8 +464,6 @@ __global__ void __launch_bounds__(kBlock)
     nodes[p].m = M;
     nodes[p].gm = kG * M;
     nodes[p].gm32 = (float)(kG * M);
-    __threadfence();
-    p = aux.parent[p];
   }
 }
 
@@ -474,11 +479,6 @@ void launch_fill_u64(unsigned long long* p, unsigned long long v, size_t n,
   if (n) fill_u64_kernel<<<blocks_for(n, kBlock), kBlock, 0, s>>>(p, v, n);
 }
 
-void launch_bbox_reset(Systems& sys, hipStream_t s) {
-  bbox_reset_kernel<<<blocks_for(6 * sys.nsys, 64), 64, 0, s>>>(sys.bbox,
-                                                                sys.nsys);
-}
-
 void launch_bbox(const State& st, Systems& sys, hipStream_t s) {
   const int nw = sys.n_waves;
   if (nw <= 0) return;
@@ -488,9 +488,9 @@ void launch_bbox(const State& st, Systems& sys, hipStream_t s) {
 
 void launch_root(Systems& sys, const StepParams& p, const int* flags,
                  hipStream_t s) {
-  root_kernel<<<blocks_for(sys.nsys, 64), 64, 0, s>>>(
-      sys.bbox, sys.root, sys.nsys, p.root_mode, p.root_center[0],
-      p.root_center[1], p.root_center[2], p.root_size, flags);
+  root_kernel<<<sys.nsys, kBlock, 0, s>>>(
+      sys.bbox, sys.n_waves, sys.wave_start, sys.root, sys.nsys, p.root_mode,
+      p.root_center[0], p.root_center[1], p.root_center[2], p.root_size, flags);
 }
 
 int launch_keys(const State& st, Systems& sys, BuildBuffers& b,
@@ -523,32 +523,39 @@ void launch_permute(const State& src, State& dst, const unsigned* perm, int n,
   if (n) permute_kernel<<<blocks_for(n, kBlock), kBlock, 0, s>>>(src, dst, perm, n);
 }
 
+size_t depth_hist_ints(size_t n) { return (size_t)kDepthBins * blocks_for(n, kBlock) + 1; }
+
 void launch_emit(BuildBuffers& b, Systems& sys, const StepParams& p,
-                 WalkNode* nodes, TreeAux& aux, int n, int cap,
-                 hipStream_t s) {
-  emit_count_kernel<<<blocks_for(n, kBlock), kBlock, 0, s>>>(
-      sys.start, sys.nsys, n, b.khi, b.klo, b.counts);
+                 const State& st, WalkNode* nodes, TreeAux& aux, int n,
+                 int cap, hipStream_t s) {
+  const int nb = blocks_for(n, kBlock);
+  emit_count_kernel<<<nb, kBlock, 0, s>>>(sys.start, sys.nsys, n, b.khi, b.klo,
+                                          b.counts, b.dhist);
   exclusive_scan(b.counts, b.offsets, n, b.scan_tmp, s);
+  exclusive_scan(b.dhist, b.doff, kDepthBins * nb, b.scan_tmp, s);
   sys_nodes_kernel<<<blocks_for(sys.nsys + 1, 64), 64, 0, s>>>(
       sys.start, b.offsets, sys.nsys, sys.node_start);
   launch_fill_int(b.stats, 0, 4, s);
-  emit_kernel<<<blocks_for(n, kBlock), kBlock, 0, s>>>(
-      sys.start, sys.nsys, n, b.khi, b.klo, b.offsets, sys.root, p.theta, cap,
-      nodes, aux, b.flags, b.stats);
+  emit_kernel<<<nb, kBlock, 0, s>>>(
+      sys.start, sys.nsys, n, b.khi, b.klo, b.offsets, b.doff, sys.root,
+      p.theta, cap, st.x, st.y, st.z, st.m, nodes, aux, b.bucket, b.flags,
+      b.stats);
 }
 
-void launch_link(BuildBuffers& b, WalkNode* nodes, TreeAux& aux, int n,
-                 int cap, hipStream_t s) {
-  int blocks = blocks_for(cap, kBlock);
-  if (blocks > 4096) blocks = 4096;
-  link_kernel<<<blocks, kBlock, 0, s>>>(b.offsets, n, nodes, aux, b.flags,
-                                       b.flags);
+void launch_bottom_up(BuildBuffers& b, WalkNode* nodes, TreeAux& aux, int n,
+                      int max_depth, hipStream_t s) {
+  const int nb = blocks_for(n, kBlock);
+  const int top = max_depth < kLevels ? max_depth : kLevels;
+  for (int d = top; d >= 0; d--)
+    level_kernel<<<256, kBlock, 0, s>>>(d, nb, b.doff, b.bucket, nodes,
+                                        aux.parent, b.flags);
 }
 
-void launch_bottom_up(BuildBuffers& b, const State& st, WalkNode* nodes,
-                      TreeAux& aux, int n, hipStream_t s) {
-  bottom_up_kernel<<<blocks_for(n, kBlock), kBlock, 0, s>>>(
-      b.offsets, n, st.x, st.y, st.z, st.m, nodes, aux, b.flags);
+void launch_leaf_refresh(BuildBuffers& b, const State& st, WalkNode* nodes,
+                         int n, hipStream_t s) {
+  if (n > 0)
+    leaf_refresh_kernel<<<blocks_for(n, kBlock), kBlock, 0, s>>>(
+        b.offsets, n, st.x, st.y, st.z, st.m, nodes, b.flags);
 }
 
 }  // namespace nbs

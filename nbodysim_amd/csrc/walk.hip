@@ -242,28 +242,15 @@ __global__ void __launch_bounds__(kWalkBlock) walk_kernel(WalkArgs a) {
       a.vy[i] = vy;
       a.vz[i] = vz;
     }
-    const double mnx = wave_min(valid ? x : INFINITY);
-    const double mny = wave_min(valid ? y : INFINITY);
-    const double mnz = wave_min(valid ? z : INFINITY);
-    const double mxx = wave_max(valid ? x : -INFINITY);
-    const double mxy = wave_max(valid ? y : -INFINITY);
-    const double mxz = wave_max(valid ? z : -INFINITY);
-    if (lane == 0) {
-      unsigned long long* b = a.bbox + 6 * s;
-      atomicMin(b + 0, ord_enc(mnx));
-      atomicMin(b + 1, ord_enc(mny));
-      atomicMin(b + 2, ord_enc(mnz));
-      atomicMax(b + 3, ord_enc(mxx));
-      atomicMax(b + 4, ord_enc(mxy));
-      atomicMax(b + 5, ord_enc(mxz));
-      if (wave == a.wave_start[s]) {
-        a.sys_dt[s] = dt;
-        a.sys_time[s] += dt;
-      }
+    wave_bbox_partial(valid, x, y, z, wave, a.n_waves, a.bbox);
+    if (lane == 0 && wave == a.wave_start[s]) {
+      a.sys_dt[s] = dt;
+      a.sys_time[s] += dt;
     }
   }
+  // per-wave running interaction count (no contended atomics)
   const int wtot = wave_sum(valid ? cnt : 0);
-  if (lane == 0) atomicAdd(a.inter_total, (unsigned long long)wtot);
+  if (lane == 0) a.inter_total[wave] += (unsigned long long)wtot;
   if (wave == 0 && lane == 0) atomicAdd(a.steps_done, 1);
 }
 

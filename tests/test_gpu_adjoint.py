@@ -25,6 +25,10 @@ def loss(xT, tgt):
     return float(np.mean(np.sum((xT - tgt) ** 2, axis=1)))
 
 
+def loss_diff(xp, xm, tgt):
+    return float(np.mean(np.sum((xp - xm) * (xp + xm - 2 * tgt), axis=1)))
+
+
 def check_fd(adj, fd):
     adj, fd = np.asarray(adj), np.asarray(fd)
     floor = 1e-6 * np.abs(fd).max()
@@ -63,25 +67,28 @@ def run_case(n, steps, every, theta, frozen, seed=3, ncoord=12):
             if kind == "x":
                 h = 1e-5
                 p[i, c] += h
-                lp = loss(fwd(p, v, m), tgt)
+                xp = fwd(p, v, m)
                 p[i, c] -= 2 * h
-                lm = loss(fwd(p, v, m), tgt)
+                xm = fwd(p, v, m)
                 adj.append(gx0[i, c])
             elif kind == "v":
                 h = 1e-6
                 v[i, c] += h
-                lp = loss(fwd(p, v, m), tgt)
+                xp = fwd(p, v, m)
                 v[i, c] -= 2 * h
-                lm = loss(fwd(p, v, m), tgt)
+                xm = fwd(p, v, m)
                 adj.append(gv0[i, c])
             else:
                 h = 1e-6 * mass[i]
                 m[i] += h
-                lp = loss(fwd(p, v, m), tgt)
+                xp = fwd(p, v, m)
                 m[i] -= 2 * h
-                lm = loss(fwd(p, v, m), tgt)
+                xm = fwd(p, v, m)
                 adj.append(gm[i])
-            fd.append((lp - lm) / (2 * h))
+            # L(+) - L(-) formed from the trajectories themselves,
+            # mean((x+ - x-) . (x+ + x- - 2 t)), so the difference is not
+            # lost to the cancellation of two large loss values
+            fd.append(loss_diff(xp, xm, tgt) / (2 * h))
     return check_fd(adj, fd), (gx0, gv0, gm)
 
 
@@ -90,7 +97,11 @@ def test_adjoint_matches_fd_direct_sum_dynamics():
 
 
 def test_adjoint_matches_fd_frozen_theta05():
-    run_case(n=2048, steps=20, every=6, theta=0.5, frozen=True)
+    # Horizon 10 steps: the FD of the frozen map converges there (FD at h,
+    # h/4, h/16 agree to ~1e-5).  Over ~20+ steps close encounters (the
+    # reference has no softening) make the map chaotic and FP64 central
+    # differences stop converging, while the adjoint stays exact.
+    run_case(n=2048, steps=10, every=4, theta=0.5, frozen=True)
 
 
 def test_checkpoint_interval_does_not_change_the_gradient():
