@@ -172,6 +172,10 @@ struct nbs_ctx {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[4];
   double prof_ms[4] = {0, 0, 0, 0};
 
+  // launch shape of the walk (nbs_autotune / nbs_set_walk_shape)
+  int walk_block = 256;
+  int walk_groups = 1;
+
   // Two captured steps (the state ping-pongs between its two buffers, so
   // a pair of steps returns to the same buffer); valid while the
   // configuration, state size and node capacity do not change.
@@ -444,8 +448,9 @@ void enqueue_walk(nbs_ctx* c, int fused) {
   a.steps_done = c->steps_done;
   a.inter_total = c->inter_total;
   a.fused_update = fused;
+  a.groups = c->walk_groups;
   launch_walk(a, c->cfg.force_precision == NBS_FORCE_F32, c->p.damping,
-              c->stream);
+              c->walk_block, c->stream);
   if (fused) c->bbox_valid = true;
   prof_end(c, kPhaseWalk, e0);
 }
@@ -459,6 +464,16 @@ void enqueue_adaptive_update(nbs_ctx* c) {
   launch_update(st, c->ax, c->ay, c->az, c->n, c->sys, c->flags, c->stream);
   c->bbox_valid = true;
   prof_end(c, kPhaseUpdate, e0);
+}
+
+// The walk and update of one step (after its build).
+void enqueue_step_tail(nbs_ctx* c) {
+  if (c->p.adaptive) {
+    enqueue_walk(c, 0);
+    enqueue_adaptive_update(c);
+  } else {
+    enqueue_walk(c, 1);
+  }
 }
 
 void drop_graph(nbs_ctx* c) {
@@ -499,14 +514,10 @@ int run(nbs_ctx* c, int nsteps) {
     reset_flags(c);
     for (int k = 0; k < remaining; k++) {
       enqueue_build(c);
-      if (forces_only) {
+      if (forces_only)
         enqueue_walk(c, 0);
-      } else if (c->p.adaptive) {
-        enqueue_walk(c, 0);
-        enqueue_adaptive_update(c);
-      } else {
-        enqueue_walk(c, 1);
-      }
+      else
+        enqueue_step_tail(c);
     }
     int flags = 0, done = 0;
     r = read_flags(c, &flags, &done);
@@ -541,12 +552,7 @@ int run_graph_pairs(nbs_ctx* c, int pairs) {
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < 2; k++) {
       enqueue_build(c);
-      if (c->p.adaptive) {
-        enqueue_walk(c, 0);
-        enqueue_adaptive_update(c);
-      } else {
-        enqueue_walk(c, 1);
-      }
+      enqueue_step_tail(c);
     }
     hipError_t e = hipStreamEndCapture(c->stream, &c->graph);
     c->prof = prof;
@@ -804,6 +810,161 @@ int nbs_step_graph(nbs_ctx* c, int nsteps) {
   r = alloc_nodes(c, c->node_cap * 2);
   if (r) return r;
   return run(c, nsteps - done);
+}
+
+int nbs_set_walk_shape(nbs_ctx* c, int block, int groups) {
+  if (!c) return NBS_ERR_ARG;
+  bool okb = false, okg = false;
+  for (int b : kWalkBlocks) okb = okb || b == block;
+  for (int g : kWalkGroups) okg = okg || g == groups;
+  if (!okb) return fail(c, NBS_ERR_ARG, "walk block must be 64, 128, 256, 512 or 1024");
+  if (!okg) return fail(c, NBS_ERR_ARG, "walk groups must be 1, 8, 16, 32 or 64");
+  drop_graph(c);
+  c->walk_block = block;
+  c->walk_groups = groups;
+  return NBS_OK;
+}
+
+int nbs_autotune(nbs_ctx* c, int reps, int* best_block, int* best_groups,
+                 double* ms) {
+  if (!c) return NBS_ERR_ARG;
+  if (!c->has_state) return fail(c, NBS_ERR_STATE, "no particle state set");
+  if (c->n == 0) return NBS_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  reps = std::max(reps, 1);
+  int r = ensure_nodes(c);
+  if (r) return r;
+  drop_graph(c);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+
+  // Save what the timing steps change: the current state buffer, the
+  // accelerations and interaction counts of the last walk (slot order of
+  // that buffer), simulated time and step length, the running total.
+  const size_t n = c->n;
+  const int nsys = c->sys.nsys, nw = std::max(c->sys.n_waves, 1);
+  std::vector<void*> tmp;
+  hipError_t e = hipSuccess;
+  double* sv = dalloc<double>(tmp, 10 * n, &e);
+  unsigned* sid = dalloc<unsigned>(tmp, n, &e);
+  int* sint = dalloc<int>(tmp, n, &e);
+  if (e != hipSuccess) {
+    free_all(tmp);
+    return hip_fail(c, e, "hipMalloc(autotune save)");
+  }
+  const int cur0 = c->cur;
+  const bool acc0 = c->acc_valid;
+  State& st0 = c->st[cur0];
+  double* const live[10] = {st0.x, st0.y, st0.z, st0.vx, st0.vy, st0.vz,
+                            st0.m, c->ax, c->ay, c->az};
+  std::vector<double> h_time(nsys), h_dt(nsys);
+  std::vector<unsigned long long> h_total(nw);
+  auto copy_state = [&](bool save) -> hipError_t {
+    hipError_t err = hipSuccess;
+    for (int k = 0; k < 10 && err == hipSuccess; k++)
+      err = save ? hipMemcpyAsync(sv + k * n, live[k], n * 8, hipMemcpyDeviceToDevice, c->stream)
+                 : hipMemcpyAsync(live[k], sv + k * n, n * 8, hipMemcpyDeviceToDevice, c->stream);
+    if (err == hipSuccess)
+      err = hipMemcpyAsync(save ? (void*)sid : (void*)st0.id, save ? (void*)st0.id : (void*)sid,
+                           n * sizeof(unsigned), hipMemcpyDeviceToDevice, c->stream);
+    if (err == hipSuccess)
+      err = hipMemcpyAsync(save ? (void*)sint : (void*)c->inter, save ? (void*)c->inter : (void*)sint,
+                           n * sizeof(int), hipMemcpyDeviceToDevice, c->stream);
+    c->cur = cur0;
+    c->bbox_valid = false;
+    return err;
+  };
+  e = copy_state(true);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(h_time.data(), c->sys.time, nsys * 8, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(h_dt.data(), c->sys.dt, nsys * 8, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(h_total.data(), c->inter_total, nw * 8, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) {
+    free_all(tmp);
+    return hip_fail(c, e, "autotune save");
+  }
+
+  // Each candidate: restore, one untimed step (first use of the shape), then
+  // `reps` whole steps timed with events on the context's stream.
+  const bool prof0 = c->prof;
+  c->prof = false;
+  const int block0 = c->walk_block, groups0 = c->walk_groups;
+  double best = 1e300;
+  int bb = block0, bg = groups0;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  for (int i = 0; i < kNumWalkBlocks && e == hipSuccess && r == NBS_OK; i++) {
+    for (int j = 0; j < kNumWalkGroups && e == hipSuccess && r == NBS_OK; j++) {
+      c->walk_block = kWalkBlocks[i];
+      c->walk_groups = kWalkGroups[j];
+      e = copy_state(false);
+      if (e != hipSuccess) break;
+      r = run(c, 1);  // grows the node arrays if needed
+      if (r) break;
+      e = copy_state(false);
+      if (e != hipSuccess) break;
+      reset_flags(c);
+      enqueue_build(c);  // untimed: first use of this shape
+      enqueue_step_tail(c);
+      e = copy_state(false);
+      if (e != hipSuccess) break;
+      e = hipEventRecord(e0, c->stream);
+      for (int q = 0; q < reps && e == hipSuccess; q++) {
+        enqueue_build(c);
+        enqueue_step_tail(c);
+      }
+      if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      float t = 0;
+      if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+      int flags = 0, done = 0;
+      if (e == hipSuccess) r = read_flags(c, &flags, &done);
+      if (e != hipSuccess || r) break;
+      const double per = flags ? 1e300 : t / reps;  // overflow: not timed
+      if (ms) ms[i * kNumWalkGroups + j] = flags ? -1.0 : per;
+      if (per < best) {
+        best = per;
+        bb = kWalkBlocks[i];
+        bg = kWalkGroups[j];
+      }
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  c->prof = prof0;
+
+  // Restore (also after a failure above).
+  hipError_t e2 = copy_state(false);
+  if (e2 == hipSuccess)
+    e2 = hipMemcpyAsync(c->sys.time, h_time.data(), nsys * 8, hipMemcpyHostToDevice, c->stream);
+  if (e2 == hipSuccess)
+    e2 = hipMemcpyAsync(c->sys.dt, h_dt.data(), nsys * 8, hipMemcpyHostToDevice, c->stream);
+  if (e2 == hipSuccess)
+    e2 = hipMemcpyAsync(c->inter_total, h_total.data(), nw * 8, hipMemcpyHostToDevice, c->stream);
+  if (e2 == hipSuccess) e2 = hipStreamSynchronize(c->stream);
+  reset_flags(c);
+  free_all(tmp);
+  c->tree_valid = false;  // the node array holds a tuning step's tree
+  c->acc_valid = acc0;
+  drop_graph(c);
+  if (r) {
+    c->walk_block = block0;
+    c->walk_groups = groups0;
+    return r;
+  }
+  if (e != hipSuccess || e2 != hipSuccess) {
+    c->walk_block = block0;
+    c->walk_groups = groups0;
+    return hip_fail(c, e != hipSuccess ? e : e2, "autotune");
+  }
+  c->walk_block = bb;
+  c->walk_groups = bg;
+  if (best_block) *best_block = bb;
+  if (best_groups) *best_groups = bg;
+  return NBS_OK;
 }
 
 int nbs_synchronize(nbs_ctx* c) {
@@ -1326,7 +1487,7 @@ int nbs_frozen_forward(nbs_ctx* c, const double* pos0, const double* vel0,
     a.mac_x = c->mac_x;
     a.mac_y = c->mac_x + n;
     a.mac_z = c->mac_x + 2 * n;
-    launch_walk(a, 0, c->p.damping, s);
+    launch_walk(a, 0, c->p.damping, c->walk_block, s);
     scatter3_kernel<<<b, 256, 0, s>>>(st.x, st.y, st.z, st.id, (int)n, c->p_x);
     scatter3_kernel<<<b, 256, 0, s>>>(st.vx, st.vy, st.vz, st.id, (int)n, c->p_v);
   }

@@ -157,12 +157,20 @@ struct WalkArgs {
   int* steps_done;  // incremented once per completed walk
   unsigned long long* inter_total;  // running sum of interactions
   int fused_update;
+  int groups;  // workgroup -> tile interleave over the XCDs (1 = plain)
   // frozen decisions (FP64 only; null otherwise): base tree + positions
   const WalkNode* mac_nodes;
   const double *mac_x, *mac_y, *mac_z;
 };
 
-void launch_walk(const WalkArgs& a, int f32, int damping, hipStream_t s);
+// block: workgroup size of the walk (64, 128, 256, 512 or 1024 threads);
+// a.groups: contiguous tile regions interleaved over the XCDs.
+void launch_walk(const WalkArgs& a, int f32, int damping, int block,
+                 hipStream_t s);
+constexpr int kWalkBlocks[] = {64, 128, 256, 512, 1024};
+constexpr int kWalkGroups[] = {1, 8, 16, 32, 64};
+constexpr int kNumWalkBlocks = sizeof(kWalkBlocks) / sizeof(int);
+constexpr int kNumWalkGroups = sizeof(kWalkGroups) / sizeof(int);
 
 // FP64 direct sum on sample slots; partial buffer of
 // direct_tmp_doubles(m) doubles.

@@ -38,9 +38,30 @@
 namespace nbs {
 namespace {
 
-constexpr int kWalkBlock = 256;
 
 typedef __attribute__((address_space(4))) const WalkNode* ConstNodes;
+
+// The fields of a WalkNode the walk reads, loaded with scalar loads (the
+// node index is wave-uniform); unused fields are dropped by the compiler.
+struct NodeRegs {
+  double cx, cy, cz, side, gm;
+  float gm32, r2open32;
+  int skip, leaf;
+};
+
+__device__ __forceinline__ NodeRegs load_node(ConstNodes nodes, int k) {
+  NodeRegs r;
+  r.cx = nodes[k].cx;
+  r.cy = nodes[k].cy;
+  r.cz = nodes[k].cz;
+  r.side = nodes[k].side;
+  r.gm = nodes[k].gm;
+  r.gm32 = nodes[k].gm32;
+  r.r2open32 = nodes[k].r2open32;
+  r.skip = nodes[k].skip;
+  r.leaf = nodes[k].leaf;
+  return r;
+}
 
 __device__ __forceinline__ double sq_sum_ref(double dx, double dy, double dz) {
 #pragma clang fp contract(off)
@@ -85,12 +106,26 @@ __device__ __forceinline__ void kick_drift(double& x, double& v, double a,
 // trajectory -- while forces use this step's tree and positions.  That
 // makes the step a smooth function of the perturbed state, for finite
 // differences of the adjoint.
-template <bool F32, bool DAMP, bool FROZEN = false>
-__global__ void __launch_bounds__(kWalkBlock) walk_kernel(WalkArgs a) {
+// Workgroup -> tile order.  Workgroups are handed to the 8 XCDs round-robin
+// by workgroup id, each XCD with its own L2.  With `groups` > 1 the tiles
+// (runs of Morton-consecutive waves) are cut into `groups` contiguous
+// regions and workgroup b works in region b % groups, so XCD k walks the
+// regions k, k+8, ...: neighbouring waves, which open the same cells, share
+// an L2.  groups = 1 keeps the plain order.  A bijection on [0, nblocks).
+__device__ __forceinline__ int tile_of_block(int b, int nblocks, int groups) {
+  if (groups <= 1) return b;
+  const int v = b % groups, j = b / groups;
+  const int q = nblocks / groups, r = nblocks % groups;
+  return v * q + min(v, r) + j;
+}
+
+template <bool F32, bool DAMP, bool FROZEN, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) walk_kernel(WalkArgs a) {
   if (*a.flags) return;
   const int lane = lane_id();
+  const int tile = tile_of_block(blockIdx.x, gridDim.x, a.groups);
   const int wave = __builtin_amdgcn_readfirstlane(
-      (int)((blockIdx.x * kWalkBlock + threadIdx.x) >> 6));
+      (int)((tile * BLOCK + threadIdx.x) >> 6));
   if (wave >= a.n_waves) return;
   const int s = a.nsys > 1 ? __builtin_amdgcn_readfirstlane(
                                  sys_of_wave(wave, a.wave_start, a.nsys))
@@ -117,17 +152,22 @@ __global__ void __launch_bounds__(kWalkBlock) walk_kernel(WalkArgs a) {
   int st_pos = 0;
   int st_lo = 0, st_hi = 0;
   int top = 0;
+  int top_pos = -1;  // st_pos of the top entry, kept in an SGPR
 
   typedef typename std::conditional<F32, float, double>::type acc_t;
   acc_t ax = 0, ay = 0, az = 0;
   int cnt = 0;
 
+  // One scalar load of the node record per visit.  (Loading both possible
+  // successors a visit ahead measured slower: the doubled scalar-memory
+  // traffic costs more than the latency it hides.)
   while (n < nend) {
-    const int leaf = nodes[n].leaf;
-    const int skip = nodes[n].skip;
-    const double dx = nodes[n].cx - x;
-    const double dy = nodes[n].cy - y;
-    const double dz = nodes[n].cz - z;
+    const NodeRegs cur = load_node(nodes, n);
+    const int leaf = cur.leaf;
+    const int skip = cur.skip;
+    const double dx = cur.cx - x;
+    const double dy = cur.cy - y;
+    const double dz = cur.cz - z;
     const bool in = (A >> lane) & 1ull;
     float fdx = 0, fdy = 0, fdz = 0, fd2 = 0;
     double d2 = 0;
@@ -149,13 +189,13 @@ __global__ void __launch_bounds__(kWalkBlock) walk_kernel(WalkArgs a) {
       if (leaf == -2)
         open = true;
       else if (F32)
-        open = fd2 < nodes[n].r2open32;
+        open = fd2 < cur.r2open32;
       else
         open = mac_open_f64(
             FROZEN ? sq_sum_ref(mac_nodes[n].cx - mx, mac_nodes[n].cy - my,
                                 mac_nodes[n].cz - mz)
                    : d2,
-            nodes[n].side, theta, theta2);
+            cur.side, theta, theta2);
       open = open && in;
       const unsigned long long O = ballot(open);
       inter = in && !open;
@@ -163,7 +203,7 @@ __global__ void __launch_bounds__(kWalkBlock) walk_kernel(WalkArgs a) {
         const unsigned long long C = A & ~O;
         if (C) {
           // lane `slot` of the st_* registers holds stack entry `slot`
-          if (top > 0 && __builtin_amdgcn_readlane(st_pos, top - 1) == skip) {
+          if (top > 0 && top_pos == skip) {
             const int slot = top - 1;
             st_lo = lane == slot ? (st_lo | (int)(unsigned)C) : st_lo;
             st_hi = lane == slot ? (st_hi | (int)(unsigned)(C >> 32)) : st_hi;
@@ -172,6 +212,7 @@ __global__ void __launch_bounds__(kWalkBlock) walk_kernel(WalkArgs a) {
             st_pos = lane == slot ? skip : st_pos;
             st_lo = lane == slot ? (int)(unsigned)C : st_lo;
             st_hi = lane == slot ? (int)(unsigned)(C >> 32) : st_hi;
+            top_pos = skip;
             if (top < kWave - 1)
               top++;
             else if (lane == 0)
@@ -190,9 +231,9 @@ __global__ void __launch_bounds__(kWalkBlock) walk_kernel(WalkArgs a) {
         float sc;
         if (DAMP) {
           const float d = fd2 * r;
-          sc = nodes[n].gm32 * r / (fd2 + __expf(-d));
+          sc = cur.gm32 * r / (fd2 + __expf(-d));
         } else {
-          sc = nodes[n].gm32 * (r * r * r);
+          sc = cur.gm32 * (r * r * r);
         }
         ax = fmaf(sc, fdx, (float)ax);
         ay = fmaf(sc, fdy, (float)ay);
@@ -202,9 +243,9 @@ __global__ void __launch_bounds__(kWalkBlock) walk_kernel(WalkArgs a) {
         double sc;
         if (DAMP) {
           const double d = d2 * r;
-          sc = nodes[n].gm * r / (d2 + exp(-d));
+          sc = cur.gm * r / (d2 + exp(-d));
         } else {
-          sc = nodes[n].gm * (r * r * r);
+          sc = cur.gm * (r * r * r);
         }
         ax = fma(sc, dx, (double)ax);
         ay = fma(sc, dy, (double)ay);
@@ -213,11 +254,12 @@ __global__ void __launch_bounds__(kWalkBlock) walk_kernel(WalkArgs a) {
       cnt++;
     }
     n = max(next, n + 1);  // always advance, even on a corrupt skip
-    if (top > 0 && __builtin_amdgcn_readlane(st_pos, top - 1) == n) {
+    if (top > 0 && top_pos == n) {
       const unsigned lo = (unsigned)__builtin_amdgcn_readlane(st_lo, top - 1);
       const unsigned hi = (unsigned)__builtin_amdgcn_readlane(st_hi, top - 1);
       A |= ((unsigned long long)hi << 32) | lo;
       top--;
+      top_pos = top > 0 ? __builtin_amdgcn_readlane(st_pos, top - 1) : -1;
     }
   }
 
@@ -323,21 +365,45 @@ __global__ void inverse_perm_kernel(const unsigned* __restrict__ id, int n,
 
 }  // namespace
 
-void launch_walk(const WalkArgs& a, int f32, int damping, hipStream_t s) {
-  if (a.n_waves <= 0) return;
-  const int blocks = (a.n_waves * kWave + kWalkBlock - 1) / kWalkBlock;
+template <int BLOCK>
+void launch_walk_block(const WalkArgs& a, int f32, int damping, hipStream_t s) {
+  const int blocks = (a.n_waves * kWave + BLOCK - 1) / BLOCK;
   if (f32) {
     if (damping)
-      walk_kernel<true, true><<<blocks, kWalkBlock, 0, s>>>(a);
+      walk_kernel<true, true, false, BLOCK><<<blocks, BLOCK, 0, s>>>(a);
     else
-      walk_kernel<true, false><<<blocks, kWalkBlock, 0, s>>>(a);
+      walk_kernel<true, false, false, BLOCK><<<blocks, BLOCK, 0, s>>>(a);
+  } else if (a.mac_nodes) {
+    if (damping)
+      walk_kernel<false, true, true, BLOCK><<<blocks, BLOCK, 0, s>>>(a);
+    else
+      walk_kernel<false, false, true, BLOCK><<<blocks, BLOCK, 0, s>>>(a);
   } else {
     if (damping)
-      a.mac_nodes ? walk_kernel<false, true, true><<<blocks, kWalkBlock, 0, s>>>(a)
-                  : walk_kernel<false, true><<<blocks, kWalkBlock, 0, s>>>(a);
+      walk_kernel<false, true, false, BLOCK><<<blocks, BLOCK, 0, s>>>(a);
     else
-      a.mac_nodes ? walk_kernel<false, false, true><<<blocks, kWalkBlock, 0, s>>>(a)
-                  : walk_kernel<false, false><<<blocks, kWalkBlock, 0, s>>>(a);
+      walk_kernel<false, false, false, BLOCK><<<blocks, BLOCK, 0, s>>>(a);
+  }
+}
+
+void launch_walk(const WalkArgs& a, int f32, int damping, int block,
+                 hipStream_t s) {
+  if (a.n_waves <= 0) return;
+  switch (block) {
+    case 64:
+      launch_walk_block<64>(a, f32, damping, s);
+      break;
+    case 128:
+      launch_walk_block<128>(a, f32, damping, s);
+      break;
+    case 512:
+      launch_walk_block<512>(a, f32, damping, s);
+      break;
+    case 1024:
+      launch_walk_block<1024>(a, f32, damping, s);
+      break;
+    default:
+      launch_walk_block<256>(a, f32, damping, s);
   }
 }
 

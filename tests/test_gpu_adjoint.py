@@ -113,10 +113,14 @@ def test_checkpoint_interval_does_not_change_the_gradient():
             s.adjoint_forward(20, every)
             xT, _, _ = s.state()
             out.append(s.adjoint_backward(xT, None))
-    for a, b in zip(out[0], out[1]):
-        np.testing.assert_array_equal(a, b)
-    for a, b in zip(out[0], out[2]):
-        np.testing.assert_array_equal(a, b)
+    # Segments are recomputed bit-exactly from their checkpoints; the only
+    # run-to-run freedom is the order of the FP64 atomic adds that gather
+    # node adjoints from many waves (~1e-14 relative, measured); a recompute
+    # taking a different tree or cell decision moves the gradient by O(1).
+    for k in (1, 2):
+        for a, b in zip(out[0], out[k]):
+            err = np.abs(a - b).max() / np.abs(b).max()
+            assert err < 1e-11, (k, err)
 
 
 def test_tree_gradient_close_to_direct_sum_gradient():

@@ -268,3 +268,81 @@ def test_graph_batched_adaptive():
         np.testing.assert_array_equal(a, b)
     np.testing.assert_array_equal(res[0][1], res[1][1])
     assert (res[0][1] > 0).all()
+
+
+WALK_SHAPES = [(64, 1), (128, 1), (256, 1), (512, 1), (1024, 1), (64, 8), (128, 32),
+               (256, 16), (512, 64), (1024, 8)]
+
+
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+def test_walk_shapes_give_identical_results(precision):
+    """Each 64-lane wave walks independently and the XCD interleave only
+    permutes which workgroup takes which waves, so no launch shape the
+    autotuner may pick changes a single bit of the forces or steps."""
+    pos, vel, mass = O.generate(30000, "sphere", seed=7)
+    ref = None
+    for block, groups in WALK_SHAPES:
+        with sim(theta=0.5, precision=precision) as s:
+            s.set_state(pos, vel, mass)
+            s.set_walk_shape(block, groups)
+            acc = s.forces()
+            s.step(3)
+            out = (acc, s.interactions(per_particle=True)[1]) + s.state()
+        if ref is None:
+            ref = out
+        for a, b in zip(out, ref):
+            np.testing.assert_array_equal(a, b, err_msg=str((block, groups)))
+
+
+def test_walk_shapes_batched():
+    parts = [O.generate(n, g, seed=3 + k) for k, (n, g) in
+             enumerate([(5000, "cube"), (77, "sphere"), (9000, "shell"), (640, "sphere")])]
+    pos = np.concatenate([p[0] for p in parts])
+    vel = np.concatenate([p[1] for p in parts])
+    mass = np.concatenate([p[2] for p in parts])
+    off = np.cumsum([0] + [len(p[2]) for p in parts])
+    ref = None
+    for block, groups in [(256, 1), (64, 64), (128, 8), (1024, 16)]:
+        with sim(theta=0.6, precision="f32") as s:
+            s.set_state(pos, vel, mass)
+            s.set_systems(off)
+            s.set_walk_shape(block, groups)
+            s.step(3, graph=True)
+            out = s.state()
+        if ref is None:
+            ref = out
+        for a, b in zip(out, ref):
+            np.testing.assert_array_equal(a, b, err_msg=str((block, groups)))
+
+
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+def test_autotune_restores_state(precision):
+    pos, vel, mass = O.generate(30000, "sphere", seed=7)
+    with sim(theta=0.5, precision=precision) as s:
+        s.set_state(pos, vel, mass)
+        s.step(4)
+        want = s.state()
+        want_t = s.time()
+    with sim(theta=0.5, precision=precision) as s:
+        s.set_state(pos, vel, mass)
+        s.step(1)
+        s.interactions_total(reset=True)
+        s.step(1)
+        before = s.state()
+        t_before = s.time()
+        tot = s.interactions_total(reset=False)
+        (block, groups), times = s.autotune(reps=2)
+        assert block in s.WALK_BLOCKS and groups in s.WALK_GROUPS
+        assert all(t > 0 for t in times.values())
+        assert times[(block, groups)] == min(times.values())
+        # tuning moves neither the state nor time nor the running total
+        assert s.interactions_total(reset=False) == tot
+        for a, b in zip(s.state(), before):
+            np.testing.assert_array_equal(a, b)
+        np.testing.assert_array_equal(s.time(), t_before)
+        s.step(2, graph=True)
+        got = s.state()
+        got_t = s.time()
+    for a, b in zip(got[:2], want[:2]):
+        np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(got_t, want_t)
