@@ -40,18 +40,25 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def time_path(sim, pos, vel, mass, precision, steps, warmup, prof_steps):
-    """Timed region: `steps` graph-replayed steps (nbs_step_graph) after
-    `warmup` steps.  Kernel timing for the roofline: `prof_steps` eager
-    steps right after, with HIP events around each phase on the context's
-    stream, and the interactions those walks counted."""
+def time_path(sim, pos, vel, mass, precision, steps, warmup, prof_steps, shape, graph):
+    """Timed region: `steps` graph-replayed steps (nbs_step_graph; eager
+    launches with graph=False) after `warmup` steps.  Before it (untimed)
+    the walk's launch shape is tuned on the initial state (nbs_autotune,
+    which restores the state) unless `shape` fixes it.  Kernel
+    timing for the roofline: `prof_steps` eager steps right after, with HIP
+    events around each phase on the context's stream, and the interactions
+    those walks counted."""
     sim.configure(precision=precision)
     sim.set_state(pos, vel, mass)
+    if shape is None:
+        shape, _ = sim.autotune(reps=3)
+    else:
+        sim.set_walk_shape(*shape)
     if warmup:
-        sim.step(warmup, graph=True)
+        sim.step(warmup, graph=graph)
     sim.synchronize()
     t0 = time.perf_counter()
-    sim.step(steps, graph=True)
+    sim.step(steps, graph=graph)
     sim.synchronize()
     t1 = time.perf_counter()
     info = sim.tree_info()
@@ -67,7 +74,7 @@ def time_path(sim, pos, vel, mass, precision, steps, warmup, prof_steps):
     inter = sim.interactions_total(reset=True)
     return {"ms_per_step": (t1 - t0) * 1e3 / steps, "prof": prof, "interactions": inter,
             "prof_steps": prof_steps, "eager_ms_per_step": (t3 - t2) * 1e3 / prof_steps,
-            "tree": info}
+            "tree": info, "walk_shape": list(shape)}
 
 
 def cpu_baseline(pos, mass, theta, sample, threads):
@@ -93,6 +100,20 @@ def cpu_baseline(pos, mass, theta, sample, threads):
     }
 
 
+def copy_calibration(mib, reps):
+    """A plain device-to-device copy of known size, the yardstick for the
+    FETCH_SIZE / WRITE_SIZE counters (counter KiB per copied KiB)."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.arange(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration": {"bytes_read": n * 4, "bytes_written": n * 4,
+                                           "launches": reps}}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,11 +125,20 @@ def main():
     ap.add_argument("--samples", type=int, default=4096)
     ap.add_argument("--cpu-sample", type=int, default=16384)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--walk-shape", default=None, metavar="BLOCK,GROUPS",
+                    help="fixed walk launch shape instead of the autotuner's")
+    ap.add_argument("--eager", action="store_true",
+                    help="timed steps launched one kernel at a time, not by graph replay")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
+                    help="only run the counter yardstick: copy MIB MiB 5 times")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.copy_calib:
+        copy_calibration(args.copy_calib, 5)
+        return
 
     import nbodysim_amd
     from nbodysim_amd import host
@@ -137,9 +167,13 @@ def main():
     log("accuracy", acc)
 
     # ---- timed paths
-    r32 = time_path(sim, pos, vel, mass, "f32", args.steps, args.warmup, args.prof_steps)
+    shape = tuple(int(v) for v in args.walk_shape.split(",")) if args.walk_shape else None
+    graph = not args.eager
+    r32 = time_path(sim, pos, vel, mass, "f32", args.steps, args.warmup, args.prof_steps,
+                    shape, graph)
     log("f32", r32)
-    r64 = time_path(sim, pos, vel, mass, "f64", args.fp64_steps, args.warmup, args.prof_steps)
+    r64 = time_path(sim, pos, vel, mass, "f64", args.fp64_steps, args.warmup, args.prof_steps,
+                    shape, graph)
     log("f64", r64)
     sim.close()
 
@@ -170,11 +204,13 @@ def main():
         "config": {"workload": "gen_type sphere N=%d r=100 rotation 0.1 about z, masses "
                                "U[5e10,5e11], theta=0.5, dt=0.0333, bbox+2 root; FP32 walk "
                                "forces, FP64 state/keys/COM" % n,
-                   "n": n, "theta": theta, "dt": dt, "parallelism": "single GPU"},
+                   "n": n, "theta": theta, "dt": dt, "parallelism": "single GPU",
+                   "walk_shape": r32["walk_shape"], "launch": "graph" if graph else "eager"},
         "interactions_per_s": inter_per_launch / (r32["ms_per_step"] * 1e-3),
         "interactions_per_particle": inter_per_launch / n,
         "fp64": {"value": value64, "ms_per_step": r64["ms_per_step"],
                  "steps": args.fp64_steps, "walk_ms": walk64_ms,
+                 "walk_shape": r64["walk_shape"],
                  "walk_tflops": achieved64, "walk_frac_fp64_peak": achieved64 / FP64_PEAK_TFLOPS,
                  "speedup_fp32_over_fp64": value / value64},
         "accuracy": acc,

@@ -123,19 +123,20 @@ int nbs_step_graph(nbs_ctx* ctx, int nsteps);
 
 /* Launch-shape tuning of the walk (no counterpart in the reference, whose
  * kernel shape was fixed, src/cuda_helper.cpp:183-188).  The shape is the
- * workgroup size (64, 128, 256, 512, 1024 threads) and the number of
- * contiguous particle regions interleaved over the 8 XCDs (1 = plain
- * order, 8, 16, 32, 64); it changes the speed only, never a result bit.
- * nbs_autotune times `reps` whole steps from the current state for each of
- * the 25 shapes, keeps the fastest for this context, reports it
- * (best_block, best_groups; may be NULL) and the times in ms per step
- * (ms[5 * block_index + groups_index], may be NULL; -1 = not timed), and
- * restores the state, accelerations, time and interaction counters as they
- * were (the tree must be rebuilt: nbs_tree_dump etc. need a new step or
- * nbs_forces).  nbs_set_walk_shape sets the shape directly. */
+ * workgroup size (block: 64, 128, 256, 512, 1024 threads), the number of
+ * contiguous particle regions interleaved over the 8 XCDs (groups: 1 =
+ * plain order, 8, 16, 32, 64) and the particles per lane (pack: 1, 2); it
+ * changes the speed only, never a result bit.  nbs_autotune times `reps`
+ * whole steps from the current state for each of the 50 shapes, keeps the
+ * fastest for this context, reports it (best_*; may be NULL) and the times
+ * in ms per step (ms[(pack_index * 5 + block_index) * 5 + groups_index],
+ * may be NULL; -1 = not timed), and restores the state, accelerations, time
+ * and interaction counters as they were (the tree must be rebuilt:
+ * nbs_tree_dump etc. need a new step or nbs_forces).  nbs_set_walk_shape
+ * sets the shape directly. */
 int nbs_autotune(nbs_ctx* ctx, int reps, int* best_block, int* best_groups,
-                 double* ms);
-int nbs_set_walk_shape(nbs_ctx* ctx, int block, int groups);
+                 int* best_pack, double* ms);
+int nbs_set_walk_shape(nbs_ctx* ctx, int block, int groups, int pack);
 
 /* Blocks until the context's stream has drained. */
 int nbs_synchronize(nbs_ctx* ctx);

@@ -148,20 +148,23 @@ class Simulation:
 
     WALK_BLOCKS = (64, 128, 256, 512, 1024)
     WALK_GROUPS = (1, 8, 16, 32, 64)
+    WALK_PACKS = (1, 2)
 
     def autotune(self, reps=3):
         """Times `reps` steps from the current state for every walk shape
-        (workgroup size x XCD interleave), keeps the fastest and restores
-        the state; returns ((block, groups), {(block, groups): ms/step})."""
-        bb, bg = ctypes.c_int32(), ctypes.c_int32()
-        ms = np.zeros(len(self.WALK_BLOCKS) * len(self.WALK_GROUPS))
-        self._check(self._L.nbs_autotune(self._h, int(reps), ctypes.byref(bb),
-                                         ctypes.byref(bg), _dp(ms)))
-        shapes = [(b, g) for b in self.WALK_BLOCKS for g in self.WALK_GROUPS]
-        return (bb.value, bg.value), dict(zip(shapes, ms.tolist()))
+        (workgroup size x XCD interleave x particles per lane), keeps the
+        fastest and restores the state; returns ((block, groups, pack),
+        {(block, groups, pack): ms/step})."""
+        bb, bg, bp = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        shapes = [(b, g, p) for p in self.WALK_PACKS for b in self.WALK_BLOCKS
+                  for g in self.WALK_GROUPS]
+        ms = np.zeros(len(shapes))
+        self._check(self._L.nbs_autotune(self._h, int(reps), ctypes.byref(bb), ctypes.byref(bg),
+                                         ctypes.byref(bp), _dp(ms)))
+        return (bb.value, bg.value, bp.value), dict(zip(shapes, ms.tolist()))
 
-    def set_walk_shape(self, block, groups=1):
-        self._check(self._L.nbs_set_walk_shape(self._h, int(block), int(groups)))
+    def set_walk_shape(self, block, groups=1, pack=1):
+        self._check(self._L.nbs_set_walk_shape(self._h, int(block), int(groups), int(pack)))
 
     def synchronize(self):
         self._check(self._L.nbs_synchronize(self._h))

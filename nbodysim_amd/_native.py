@@ -70,8 +70,10 @@ SIGNATURES = [
     ("nbs_forces", ctypes.c_int, [ctypes.c_void_p, c_dp]),
     ("nbs_step", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("nbs_step_graph", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
-    ("nbs_autotune", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_i32p, c_i32p, c_dp]),
-    ("nbs_set_walk_shape", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    ("nbs_autotune", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_i32p, c_i32p, c_i32p,
+                                    c_dp]),
+    ("nbs_set_walk_shape", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_int]),
     ("nbs_synchronize", ctypes.c_int, [ctypes.c_void_p]),
     ("nbs_direct", ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, c_i64p, c_dp]),
     ("nbs_interactions", ctypes.c_int, [ctypes.c_void_p, c_i64p, c_i64p]),

@@ -150,7 +150,7 @@ struct nbs_ctx {
   WalkNode* nodes = nullptr;
   TreeAux aux{};
   Systems sys{};
-  std::vector<int> h_start, h_wave_start;
+  std::vector<int> h_start, h_wave_start, h_wave2_start;
   int* flags = nullptr;
   int* steps_done = nullptr;
   int* stats = nullptr;
@@ -175,6 +175,7 @@ struct nbs_ctx {
   // launch shape of the walk (nbs_autotune / nbs_set_walk_shape)
   int walk_block = 256;
   int walk_groups = 1;
+  int walk_pack = 1;
 
   // Two captured steps (the state ping-pongs between its two buffers, so
   // a pair of steps returns to the same buffer); valid while the
@@ -332,6 +333,7 @@ int setup_systems(nbs_ctx* c) {
   s.nsys = nsys;
   s.start = dalloc<int>(o, nsys + 1, &e);
   s.wave_start = dalloc<int>(o, nsys + 1, &e);
+  s.wave2_start = dalloc<int>(o, nsys + 1, &e);
   s.node_start = dalloc<int>(o, nsys + 1, &e);
   s.root = dalloc<double>(o, 4 * (size_t)nsys, &e);
   s.dt = dalloc<double>(o, nsys, &e);
@@ -341,14 +343,18 @@ int setup_systems(nbs_ctx* c) {
   if (e != hipSuccess) return hip_fail(c, e, "hipMalloc(system buffers)");
   c->h_start.assign(nsys + 1, 0);
   c->h_wave_start.assign(nsys + 1, 0);
+  c->h_wave2_start.assign(nsys + 1, 0);
   for (int k = 0; k <= nsys; k++)
     c->h_start[k] = (int)(c->sys_off_req.empty() ? (k == 0 ? 0 : c->n)
                                                  : c->sys_off_req[k]);
   for (int k = 0; k < nsys; k++) {
     const int cnt = c->h_start[k + 1] - c->h_start[k];
     c->h_wave_start[k + 1] = c->h_wave_start[k] + (cnt + kWave - 1) / kWave;
+    c->h_wave2_start[k + 1] =
+        c->h_wave2_start[k] + (cnt + 2 * kWave - 1) / (2 * kWave);
   }
   s.n_waves = c->h_wave_start[nsys];
+  s.n_waves2 = c->h_wave2_start[nsys];
   s.bbox = dalloc<double>(o, 6 * (size_t)std::max(s.n_waves, 1), &e);
   c->inter_total = dalloc<unsigned long long>(o, std::max(s.n_waves, 1), &e);
   if (e != hipSuccess) return hip_fail(c, e, "hipMalloc(system buffers)");
@@ -356,6 +362,9 @@ int setup_systems(nbs_ctx* c) {
   HIPCHK(c, hipMemcpyAsync(s.start, c->h_start.data(), (nsys + 1) * sizeof(int),
                            hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(s.wave_start, c->h_wave_start.data(),
+                           (nsys + 1) * sizeof(int), hipMemcpyHostToDevice,
+                           c->stream));
+  HIPCHK(c, hipMemcpyAsync(s.wave2_start, c->h_wave2_start.data(),
                            (nsys + 1) * sizeof(int), hipMemcpyHostToDevice,
                            c->stream));
   std::vector<double> zeros(nsys, 0.0), dts(nsys, c->cfg.dt);
@@ -426,8 +435,10 @@ void enqueue_walk(nbs_ctx* c, int fused) {
   a.node_start = c->sys.node_start;
   a.sys_start = c->sys.start;
   a.wave_start = c->sys.wave_start;
+  a.wave2_start = c->sys.wave2_start;
   a.nsys = c->sys.nsys;
   a.n_waves = c->sys.n_waves;
+  a.n_waves2 = c->sys.n_waves2;
   a.x = st.x;
   a.y = st.y;
   a.z = st.z;
@@ -450,7 +461,7 @@ void enqueue_walk(nbs_ctx* c, int fused) {
   a.fused_update = fused;
   a.groups = c->walk_groups;
   launch_walk(a, c->cfg.force_precision == NBS_FORCE_F32, c->p.damping,
-              c->walk_block, c->stream);
+              c->walk_block, c->walk_pack, c->stream);
   if (fused) c->bbox_valid = true;
   prof_end(c, kPhaseWalk, e0);
 }
@@ -812,21 +823,24 @@ int nbs_step_graph(nbs_ctx* c, int nsteps) {
   return run(c, nsteps - done);
 }
 
-int nbs_set_walk_shape(nbs_ctx* c, int block, int groups) {
+int nbs_set_walk_shape(nbs_ctx* c, int block, int groups, int pack) {
   if (!c) return NBS_ERR_ARG;
-  bool okb = false, okg = false;
+  bool okb = false, okg = false, okp = false;
   for (int b : kWalkBlocks) okb = okb || b == block;
   for (int g : kWalkGroups) okg = okg || g == groups;
+  for (int q : kWalkPacks) okp = okp || q == pack;
   if (!okb) return fail(c, NBS_ERR_ARG, "walk block must be 64, 128, 256, 512 or 1024");
   if (!okg) return fail(c, NBS_ERR_ARG, "walk groups must be 1, 8, 16, 32 or 64");
+  if (!okp) return fail(c, NBS_ERR_ARG, "walk pack must be 1 or 2");
   drop_graph(c);
   c->walk_block = block;
   c->walk_groups = groups;
+  c->walk_pack = pack;
   return NBS_OK;
 }
 
 int nbs_autotune(nbs_ctx* c, int reps, int* best_block, int* best_groups,
-                 double* ms) {
+                 int* best_pack, double* ms) {
   if (!c) return NBS_ERR_ARG;
   if (!c->has_state) return fail(c, NBS_ERR_STATE, "no particle state set");
   if (c->n == 0) return NBS_OK;
@@ -891,15 +905,21 @@ int nbs_autotune(nbs_ctx* c, int reps, int* best_block, int* best_groups,
   const bool prof0 = c->prof;
   c->prof = false;
   const int block0 = c->walk_block, groups0 = c->walk_groups;
+  const int pack0 = c->walk_pack;
   double best = 1e300;
-  int bb = block0, bg = groups0;
+  int bb = block0, bg = groups0, bp = pack0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   e = hipEventCreate(&e0);
   if (e == hipSuccess) e = hipEventCreate(&e1);
-  for (int i = 0; i < kNumWalkBlocks && e == hipSuccess && r == NBS_OK; i++) {
-    for (int j = 0; j < kNumWalkGroups && e == hipSuccess && r == NBS_OK; j++) {
+  const int nshapes = kNumWalkPacks * kNumWalkBlocks * kNumWalkGroups;
+  for (int sh = 0; sh < nshapes && e == hipSuccess && r == NBS_OK; sh++) {
+    {
+      const int j = sh % kNumWalkGroups;
+      const int i = (sh / kNumWalkGroups) % kNumWalkBlocks;
+      const int q = sh / (kNumWalkGroups * kNumWalkBlocks);
       c->walk_block = kWalkBlocks[i];
       c->walk_groups = kWalkGroups[j];
+      c->walk_pack = kWalkPacks[q];
       e = copy_state(false);
       if (e != hipSuccess) break;
       r = run(c, 1);  // grows the node arrays if needed
@@ -924,11 +944,12 @@ int nbs_autotune(nbs_ctx* c, int reps, int* best_block, int* best_groups,
       if (e == hipSuccess) r = read_flags(c, &flags, &done);
       if (e != hipSuccess || r) break;
       const double per = flags ? 1e300 : t / reps;  // overflow: not timed
-      if (ms) ms[i * kNumWalkGroups + j] = flags ? -1.0 : per;
+      if (ms) ms[sh] = flags ? -1.0 : per;
       if (per < best) {
         best = per;
         bb = kWalkBlocks[i];
         bg = kWalkGroups[j];
+        bp = kWalkPacks[q];
       }
     }
   }
@@ -950,20 +971,18 @@ int nbs_autotune(nbs_ctx* c, int reps, int* best_block, int* best_groups,
   c->tree_valid = false;  // the node array holds a tuning step's tree
   c->acc_valid = acc0;
   drop_graph(c);
-  if (r) {
+  if (r || e != hipSuccess || e2 != hipSuccess) {
     c->walk_block = block0;
     c->walk_groups = groups0;
-    return r;
-  }
-  if (e != hipSuccess || e2 != hipSuccess) {
-    c->walk_block = block0;
-    c->walk_groups = groups0;
-    return hip_fail(c, e != hipSuccess ? e : e2, "autotune");
+    c->walk_pack = pack0;
+    return r ? r : hip_fail(c, e != hipSuccess ? e : e2, "autotune");
   }
   c->walk_block = bb;
   c->walk_groups = bg;
+  c->walk_pack = bp;
   if (best_block) *best_block = bb;
   if (best_groups) *best_groups = bg;
+  if (best_pack) *best_pack = bp;
   return NBS_OK;
 }
 
@@ -1461,8 +1480,10 @@ int nbs_frozen_forward(nbs_ctx* c, const double* pos0, const double* vel0,
     a.node_start = c->sys.node_start;
     a.sys_start = c->sys.start;
     a.wave_start = c->sys.wave_start;
+    a.wave2_start = c->sys.wave2_start;
     a.nsys = c->sys.nsys;
     a.n_waves = c->sys.n_waves;
+    a.n_waves2 = c->sys.n_waves2;
     a.x = st.x;
     a.y = st.y;
     a.z = st.z;
@@ -1487,7 +1508,7 @@ int nbs_frozen_forward(nbs_ctx* c, const double* pos0, const double* vel0,
     a.mac_x = c->mac_x;
     a.mac_y = c->mac_x + n;
     a.mac_z = c->mac_x + 2 * n;
-    launch_walk(a, 0, c->p.damping, c->walk_block, s);
+    launch_walk(a, 0, c->p.damping, c->walk_block, c->walk_pack, s);
     scatter3_kernel<<<b, 256, 0, s>>>(st.x, st.y, st.z, st.id, (int)n, c->p_x);
     scatter3_kernel<<<b, 256, 0, s>>>(st.vx, st.vy, st.vz, st.id, (int)n, c->p_v);
   }

@@ -22,17 +22,18 @@ constexpr int kLevels = 42;         // key depth: 21 levels per u64 half
 constexpr int kDepthBins = kLevels + 1;  // internal-cell depths 0..42
 constexpr int kWave = 64;
 
-// One tree entry.  Read by the walk with wave-uniform scalar loads, so the
-// whole record is one 64-byte line.
+// One tree entry, one 64-byte line.  The walk reads it with wave-uniform
+// scalar loads: the FP32 walk reads bytes 0-31 and 48-55, the FP64 walk
+// bytes 0-47, two loads each.
 struct alignas(64) WalkNode {
   double cx, cy, cz;  // centre of mass (src/octree.cpp:252-278)
-  double m;           // mass (src/octree.cpp:203-221)
-  double side;        // cell side: root side / 2^depth
-  double gm;          // G * m
-  float gm32;         // G * m rounded to f32
-  float r2open32;     // (side / theta)^2 in f32: open if d^2 < this
   int skip;           // pre-order index after this subtree
   int leaf;           // >= 0 leaf (particle slot); -1 internal; -2 bucket
+  double gm;          // G * m
+  double side;        // cell side: root side / 2^depth
+  float gm32;         // G * m rounded to f32
+  float r2open32;     // (side / theta)^2 in f32: open if d^2 < this
+  double m;           // mass (src/octree.cpp:203-221)
 };
 static_assert(sizeof(WalkNode) == 64, "WalkNode must be one 64 B line");
 
@@ -50,6 +51,8 @@ struct Systems {
   int n_waves;       // host copy of wave_start[nsys]
   int* start;        // [nsys+1] particle slots
   int* wave_start;   // [nsys+1] walk waves (64 particles each, per system)
+  int n_waves2;      // host copy of wave2_start[nsys]
+  int* wave2_start;  // [nsys+1] wavefronts of 128 particles (2 per lane)
   int* node_start;   // [nsys+1] pre-order node offsets (written by emit)
   double* bbox;      // [6][n_waves] per-wave min xyz, max xyz
   double* root;      // [nsys*4] centre xyz, side
@@ -142,8 +145,10 @@ struct WalkArgs {
   const int* node_start;
   const int* sys_start;
   const int* wave_start;
+  const int* wave2_start;  // per-system wavefront offsets, 2 particles/lane
   int nsys;
   int n_waves;
+  int n_waves2;
   double *x, *y, *z, *vx, *vy, *vz;
   double *ax, *ay, *az;
   int* interactions;
@@ -163,14 +168,17 @@ struct WalkArgs {
   const double *mac_x, *mac_y, *mac_z;
 };
 
-// block: workgroup size of the walk (64, 128, 256, 512 or 1024 threads);
-// a.groups: contiguous tile regions interleaved over the XCDs.
-void launch_walk(const WalkArgs& a, int f32, int damping, int block,
+// Launch shape of the walk: block = workgroup size (64, 128, 256, 512 or
+// 1024 threads); a.groups = contiguous tile regions interleaved over the
+// XCDs; pack = particles per lane (1 or 2).
+void launch_walk(const WalkArgs& a, int f32, int damping, int block, int pack,
                  hipStream_t s);
 constexpr int kWalkBlocks[] = {64, 128, 256, 512, 1024};
 constexpr int kWalkGroups[] = {1, 8, 16, 32, 64};
+constexpr int kWalkPacks[] = {1, 2};
 constexpr int kNumWalkBlocks = sizeof(kWalkBlocks) / sizeof(int);
 constexpr int kNumWalkGroups = sizeof(kWalkGroups) / sizeof(int);
+constexpr int kNumWalkPacks = sizeof(kWalkPacks) / sizeof(int);
 
 // FP64 direct sum on sample slots; partial buffer of
 // direct_tmp_doubles(m) doubles.

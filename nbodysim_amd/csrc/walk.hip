@@ -41,25 +41,39 @@ namespace {
 
 typedef __attribute__((address_space(4))) const WalkNode* ConstNodes;
 
-// The fields of a WalkNode the walk reads, loaded with scalar loads (the
-// node index is wave-uniform); unused fields are dropped by the compiler.
+// The fields of a WalkNode the walk reads, loaded as 64-bit words with
+// scalar loads (the node index is wave-uniform); the compiler merges the
+// adjacent words of a visit into two wide loads.  (Loading the record as
+// one vector value and extracting its elements miscompiled with this
+// toolchain: extractions from the 512-bit register tuple read the wrong
+// registers.  Separate scalar loads merged by the compiler do not.)
 struct NodeRegs {
   double cx, cy, cz, side, gm;
   float gm32, r2open32;
   int skip, leaf;
 };
 
+typedef __attribute__((address_space(4))) const unsigned long long* ConstQwords;
+
+// 64-bit words of the record: 0-2 cx cy cz, 3 (skip, leaf), 4 gm, 5 side,
+// 6 (gm32, r2open32).
 __device__ __forceinline__ NodeRegs load_node(ConstNodes nodes, int k) {
+  static_assert(offsetof(WalkNode, skip) == 24 && offsetof(WalkNode, gm) == 32 &&
+                    offsetof(WalkNode, side) == 40 && offsetof(WalkNode, gm32) == 48,
+                "load_node follows the WalkNode layout");
+  const ConstQwords q = (ConstQwords)(nodes + k);
   NodeRegs r;
-  r.cx = nodes[k].cx;
-  r.cy = nodes[k].cy;
-  r.cz = nodes[k].cz;
-  r.side = nodes[k].side;
-  r.gm = nodes[k].gm;
-  r.gm32 = nodes[k].gm32;
-  r.r2open32 = nodes[k].r2open32;
-  r.skip = nodes[k].skip;
-  r.leaf = nodes[k].leaf;
+  r.cx = __builtin_bit_cast(double, q[0]);
+  r.cy = __builtin_bit_cast(double, q[1]);
+  r.cz = __builtin_bit_cast(double, q[2]);
+  const unsigned long long link = q[3];
+  r.skip = (int)(unsigned)link;
+  r.leaf = (int)(unsigned)(link >> 32);
+  r.gm = __builtin_bit_cast(double, q[4]);
+  r.side = __builtin_bit_cast(double, q[5]);
+  const unsigned long long mac = q[6];
+  r.gm32 = __builtin_bit_cast(float, (unsigned)mac);
+  r.r2open32 = __builtin_bit_cast(float, (unsigned)(mac >> 32));
   return r;
 }
 
@@ -68,15 +82,23 @@ __device__ __forceinline__ double sq_sum_ref(double dx, double dy, double dz) {
   return dx * dx + dy * dy + dz * dz;  // src/vector.cpp:146 rounding
 }
 
+// Called wave-uniformly (every lane of the wave, divergence-free).  The
+// exact test runs only when some lane is within the margin; the empty asm
+// keeps the compiler from hoisting the sqrt and division (cheap-looking
+// in IR, ~30 FP64 instructions in ISA) out of that rare branch.
 __device__ __forceinline__ bool mac_open_f64(double d2, double side,
                                              double theta, double theta2) {
 #pragma clang fp contract(off)
   const double s2 = side * side;
   const double t = s2 - theta2 * d2;
   const double tol = 1e-9 * s2;
-  if (t > tol) return true;
-  if (t < -tol) return false;
-  return side / __builtin_sqrt(d2) > theta;
+  bool open = t > tol;
+  const bool tie = !(t > tol) && !(t < -tol);
+  if (ballot(tie)) {
+    __asm__ volatile("");
+    if (tie) open = side / __builtin_sqrt(d2) > theta;
+  }
+  return open;
 }
 
 __device__ __forceinline__ double rsqrt_f64(double d2) {
@@ -101,11 +123,6 @@ __device__ __forceinline__ void kick_drift(double& x, double& v, double a,
   x = x + t2;
 }
 
-// FROZEN (FP64 only): the MAC is evaluated on a second tree (mac_nodes,
-// same topology) against second positions (mac_x/y/z) -- a recorded base
-// trajectory -- while forces use this step's tree and positions.  That
-// makes the step a smooth function of the perturbed state, for finite
-// differences of the adjoint.
 // Workgroup -> tile order.  Workgroups are handed to the 8 XCDs round-robin
 // by workgroup id, each XCD with its own L2.  With `groups` > 1 the tiles
 // (runs of Morton-consecutive waves) are cut into `groups` contiguous
@@ -119,44 +136,74 @@ __device__ __forceinline__ int tile_of_block(int b, int nblocks, int groups) {
   return v * q + min(v, r) + j;
 }
 
-template <bool F32, bool DAMP, bool FROZEN, int BLOCK>
+// One wavefront walks P * 64 particles that are consecutive in octree
+// order (lane l holds particles pbeg + l, pbeg + 64 + l, ...), each set
+// with its own lane mask; a node is visited if any particle of any set
+// needs it.  P = 2 halves the node loads and traversal bookkeeping per
+// particle at the price of a larger union of paths.
+//
+// FROZEN (FP64 only): the MAC is evaluated on a second tree (mac_nodes,
+// same topology) against second positions (mac_x/y/z) -- a recorded base
+// trajectory -- while forces use this step's tree and positions.  That
+// makes the step a smooth function of the perturbed state, for finite
+// differences of the adjoint.
+template <bool F32, bool DAMP, bool FROZEN, int BLOCK, int P>
 __global__ void __launch_bounds__(BLOCK) walk_kernel(WalkArgs a) {
   if (*a.flags) return;
   const int lane = lane_id();
   const int tile = tile_of_block(blockIdx.x, gridDim.x, a.groups);
-  const int wave = __builtin_amdgcn_readfirstlane(
+  const int hw = __builtin_amdgcn_readfirstlane(
       (int)((tile * BLOCK + threadIdx.x) >> 6));
-  if (wave >= a.n_waves) return;
+  const int* hw_start = P == 1 ? a.wave_start : a.wave2_start;
+  if (hw >= (P == 1 ? a.n_waves : a.n_waves2)) return;
   const int s = a.nsys > 1 ? __builtin_amdgcn_readfirstlane(
-                                 sys_of_wave(wave, a.wave_start, a.nsys))
+                                 sys_of_wave(hw, hw_start, a.nsys))
                            : 0;
-  const int pbeg = a.sys_start[s] + (wave - a.wave_start[s]) * kWave;
+  const int k = hw - hw_start[s];          // wavefront within the system
+  const int w0 = a.wave_start[s] + k * P;  // its first 64-particle wave
+  const int wend = a.wave_start[s + 1];
+  const int pbeg = a.sys_start[s] + k * (kWave * P);
   const int pend = a.sys_start[s + 1];
-  const int i = pbeg + lane;
-  const bool valid = i < pend;
-  const double mx = FROZEN && valid ? a.mac_x[i] : 0.0;
-  const double my = FROZEN && valid ? a.mac_y[i] : 0.0;
-  const double mz = FROZEN && valid ? a.mac_z[i] : 0.0;
-  ConstNodes mac_nodes = (ConstNodes)a.mac_nodes;
-  double x = valid ? a.x[i] : 0.0;
-  double y = valid ? a.y[i] : 0.0;
-  double z = valid ? a.z[i] : 0.0;
 
+  int i[P];
+  bool valid[P];
+  double x[P], y[P], z[P], mx[P], my[P], mz[P];
+  unsigned long long A[P];  // lanes of set p that take the current node
+#pragma unroll
+  for (int p = 0; p < P; p++) {
+    i[p] = pbeg + p * kWave + lane;
+    valid[p] = i[p] < pend;
+    x[p] = valid[p] ? a.x[i[p]] : 0.0;
+    y[p] = valid[p] ? a.y[i[p]] : 0.0;
+    z[p] = valid[p] ? a.z[i[p]] : 0.0;
+    mx[p] = FROZEN && valid[p] ? a.mac_x[i[p]] : 0.0;
+    my[p] = FROZEN && valid[p] ? a.mac_y[i[p]] : 0.0;
+    mz[p] = FROZEN && valid[p] ? a.mac_z[i[p]] : 0.0;
+    A[p] = ballot(valid[p]);
+  }
   ConstNodes nodes = (ConstNodes)a.nodes;
-  unsigned long long A = ballot(valid);
+  ConstNodes mac_nodes = (ConstNodes)a.mac_nodes;
   int n = a.node_start[s];
   const int nend = a.node_start[s + 1];
   const double theta = a.theta;
   const double theta2 = theta * theta;
 
+  // waiting lanes: stack entry `slot` lives in lane `slot` of these
   int st_pos = 0;
-  int st_lo = 0, st_hi = 0;
+  int st_lo[P], st_hi[P];
+#pragma unroll
+  for (int p = 0; p < P; p++) st_lo[p] = st_hi[p] = 0;
   int top = 0;
   int top_pos = -1;  // st_pos of the top entry, kept in an SGPR
 
   typedef typename std::conditional<F32, float, double>::type acc_t;
-  acc_t ax = 0, ay = 0, az = 0;
-  int cnt = 0;
+  acc_t ax[P], ay[P], az[P];
+  int cnt[P];
+#pragma unroll
+  for (int p = 0; p < P; p++) {
+    ax[p] = ay[p] = az[p] = 0;
+    cnt[p] = 0;
+  }
 
   // One scalar load of the node record per visit.  (Loading both possible
   // successors a visit ahead measured slower: the doubled scalar-memory
@@ -165,53 +212,74 @@ __global__ void __launch_bounds__(BLOCK) walk_kernel(WalkArgs a) {
     const NodeRegs cur = load_node(nodes, n);
     const int leaf = cur.leaf;
     const int skip = cur.skip;
-    const double dx = cur.cx - x;
-    const double dy = cur.cy - y;
-    const double dz = cur.cz - z;
-    const bool in = (A >> lane) & 1ull;
-    float fdx = 0, fdy = 0, fdz = 0, fd2 = 0;
-    double d2 = 0;
-    if (F32) {
-      fdx = (float)dx;
-      fdy = (float)dy;
-      fdz = (float)dz;
-      fd2 = fmaf(fdx, fdx, fmaf(fdy, fdy, fdz * fdz));
-    } else {
-      d2 = sq_sum_ref(dx, dy, dz);
+    double dx[P], dy[P], dz[P], d2[P];
+    float fdx[P], fdy[P], fdz[P], fd2[P];
+    bool in[P], inter[P];
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+      dx[p] = cur.cx - x[p];
+      dy[p] = cur.cy - y[p];
+      dz[p] = cur.cz - z[p];
+      in[p] = (A[p] >> lane) & 1ull;
+      if (F32) {
+        fdx[p] = (float)dx[p];
+        fdy[p] = (float)dy[p];
+        fdz[p] = (float)dz[p];
+        fd2[p] = fmaf(fdx[p], fdx[p], fmaf(fdy[p], fdy[p], fdz[p] * fdz[p]));
+        d2[p] = 0;
+      } else {
+        fdx[p] = fdy[p] = fdz[p] = fd2[p] = 0;
+        d2[p] = sq_sum_ref(dx[p], dy[p], dz[p]);
+      }
     }
-    bool inter;
     int next;
     if (leaf >= 0) {
-      inter = in && leaf != i;
+#pragma unroll
+      for (int p = 0; p < P; p++) inter[p] = in[p] && leaf != i[p];
       next = n + 1;
     } else {
-      bool open;
-      if (leaf == -2)
-        open = true;
-      else if (F32)
-        open = fd2 < cur.r2open32;
-      else
-        open = mac_open_f64(
-            FROZEN ? sq_sum_ref(mac_nodes[n].cx - mx, mac_nodes[n].cy - my,
-                                mac_nodes[n].cz - mz)
-                   : d2,
-            cur.side, theta, theta2);
-      open = open && in;
-      const unsigned long long O = ballot(open);
-      inter = in && !open;
-      if (O) {
-        const unsigned long long C = A & ~O;
-        if (C) {
-          // lane `slot` of the st_* registers holds stack entry `slot`
+      unsigned long long O[P], any_open = 0;
+#pragma unroll
+      for (int p = 0; p < P; p++) {
+        bool open;
+        if (leaf == -2)
+          open = true;
+        else if (F32)
+          open = fd2[p] < cur.r2open32;
+        else
+          open = mac_open_f64(
+              FROZEN ? sq_sum_ref(mac_nodes[n].cx - mx[p], mac_nodes[n].cy - my[p],
+                                  mac_nodes[n].cz - mz[p])
+                     : d2[p],
+              cur.side, theta, theta2);
+        open = open && in[p];
+        O[p] = ballot(open);
+        inter[p] = in[p] && !open;
+        any_open |= O[p];
+      }
+      if (any_open) {
+        unsigned long long C[P], any_wait = 0;
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+          C[p] = A[p] & ~O[p];
+          any_wait |= C[p];
+        }
+        if (any_wait) {
           if (top > 0 && top_pos == skip) {
             const int slot = top - 1;
-            st_lo = lane == slot ? (st_lo | (int)(unsigned)C) : st_lo;
-            st_hi = lane == slot ? (st_hi | (int)(unsigned)(C >> 32)) : st_hi;
+#pragma unroll
+            for (int p = 0; p < P; p++) {
+              st_lo[p] = lane == slot ? (st_lo[p] | (int)(unsigned)C[p]) : st_lo[p];
+              st_hi[p] = lane == slot ? (st_hi[p] | (int)(unsigned)(C[p] >> 32)) : st_hi[p];
+            }
           } else {
             const int slot = top;
             st_pos = lane == slot ? skip : st_pos;
-            st_lo = lane == slot ? (int)(unsigned)C : st_lo;
-            st_hi = lane == slot ? (int)(unsigned)(C >> 32) : st_hi;
+#pragma unroll
+            for (int p = 0; p < P; p++) {
+              st_lo[p] = lane == slot ? (int)(unsigned)C[p] : st_lo[p];
+              st_hi[p] = lane == slot ? (int)(unsigned)(C[p] >> 32) : st_hi[p];
+            }
             top_pos = skip;
             if (top < kWave - 1)
               top++;
@@ -219,81 +287,96 @@ __global__ void __launch_bounds__(BLOCK) walk_kernel(WalkArgs a) {
               atomicOr(a.flags_out, kFlagStack);
           }
         }
-        A = O;
+#pragma unroll
+        for (int p = 0; p < P; p++) A[p] = O[p];
         next = n + 1;
       } else {
         next = skip;
       }
     }
-    if (inter) {
-      if (F32) {
-        const float r = __builtin_amdgcn_rsqf(fd2);
-        float sc;
-        if (DAMP) {
-          const float d = fd2 * r;
-          sc = cur.gm32 * r / (fd2 + __expf(-d));
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+      if (inter[p]) {
+        if (F32) {
+          const float r = __builtin_amdgcn_rsqf(fd2[p]);
+          float sc;
+          if (DAMP) {
+            const float d = fd2[p] * r;
+            sc = cur.gm32 * r / (fd2[p] + __expf(-d));
+          } else {
+            sc = cur.gm32 * (r * r * r);
+          }
+          ax[p] = fmaf(sc, fdx[p], (float)ax[p]);
+          ay[p] = fmaf(sc, fdy[p], (float)ay[p]);
+          az[p] = fmaf(sc, fdz[p], (float)az[p]);
         } else {
-          sc = cur.gm32 * (r * r * r);
+          const double r = rsqrt_f64(d2[p]);
+          double sc;
+          if (DAMP) {
+            const double d = d2[p] * r;
+            sc = cur.gm * r / (d2[p] + exp(-d));
+          } else {
+            sc = cur.gm * (r * r * r);
+          }
+          ax[p] = fma(sc, dx[p], (double)ax[p]);
+          ay[p] = fma(sc, dy[p], (double)ay[p]);
+          az[p] = fma(sc, dz[p], (double)az[p]);
         }
-        ax = fmaf(sc, fdx, (float)ax);
-        ay = fmaf(sc, fdy, (float)ay);
-        az = fmaf(sc, fdz, (float)az);
-      } else {
-        const double r = rsqrt_f64(d2);
-        double sc;
-        if (DAMP) {
-          const double d = d2 * r;
-          sc = cur.gm * r / (d2 + exp(-d));
-        } else {
-          sc = cur.gm * (r * r * r);
-        }
-        ax = fma(sc, dx, (double)ax);
-        ay = fma(sc, dy, (double)ay);
-        az = fma(sc, dz, (double)az);
+        cnt[p]++;
       }
-      cnt++;
     }
     n = max(next, n + 1);  // always advance, even on a corrupt skip
     if (top > 0 && top_pos == n) {
-      const unsigned lo = (unsigned)__builtin_amdgcn_readlane(st_lo, top - 1);
-      const unsigned hi = (unsigned)__builtin_amdgcn_readlane(st_hi, top - 1);
-      A |= ((unsigned long long)hi << 32) | lo;
+#pragma unroll
+      for (int p = 0; p < P; p++) {
+        const unsigned lo = (unsigned)__builtin_amdgcn_readlane(st_lo[p], top - 1);
+        const unsigned hi = (unsigned)__builtin_amdgcn_readlane(st_hi[p], top - 1);
+        A[p] |= ((unsigned long long)hi << 32) | lo;
+      }
       top--;
       top_pos = top > 0 ? __builtin_amdgcn_readlane(st_pos, top - 1) : -1;
     }
   }
 
-  const double fax = (double)ax, fay = (double)ay, faz = (double)az;
-  if (valid) {
-    a.ax[i] = fax;
-    a.ay[i] = fay;
-    a.az[i] = faz;
-    a.interactions[i] = cnt;
+  int wtot = 0;
+#pragma unroll
+  for (int p = 0; p < P; p++) {
+    const double fax = (double)ax[p], fay = (double)ay[p], faz = (double)az[p];
+    const int ip = i[p];
+    if (valid[p]) {
+      a.ax[ip] = fax;
+      a.ay[ip] = fay;
+      a.az[ip] = faz;
+      a.interactions[ip] = cnt[p];
+    }
+    if (a.fused_update) {
+      if (valid[p]) {
+        const double dt = a.dt;
+        double vx = a.vx[ip], vy = a.vy[ip], vz = a.vz[ip];
+        kick_drift(x[p], vx, fax, dt);
+        kick_drift(y[p], vy, fay, dt);
+        kick_drift(z[p], vz, faz, dt);
+        a.x[ip] = x[p];
+        a.y[ip] = y[p];
+        a.z[ip] = z[p];
+        a.vx[ip] = vx;
+        a.vy[ip] = vy;
+        a.vz[ip] = vz;
+      }
+      // the next step's bbox partial of 64-particle wave w0 + p (if this
+      // system has it)
+      if (w0 + p < wend)
+        wave_bbox_partial(valid[p], x[p], y[p], z[p], w0 + p, a.n_waves, a.bbox);
+    }
+    wtot += wave_sum(valid[p] ? cnt[p] : 0);
   }
-  if (a.fused_update) {
-    const double dt = a.dt;
-    if (valid) {
-      double vx = a.vx[i], vy = a.vy[i], vz = a.vz[i];
-      kick_drift(x, vx, fax, dt);
-      kick_drift(y, vy, fay, dt);
-      kick_drift(z, vz, faz, dt);
-      a.x[i] = x;
-      a.y[i] = y;
-      a.z[i] = z;
-      a.vx[i] = vx;
-      a.vy[i] = vy;
-      a.vz[i] = vz;
-    }
-    wave_bbox_partial(valid, x, y, z, wave, a.n_waves, a.bbox);
-    if (lane == 0 && wave == a.wave_start[s]) {
-      a.sys_dt[s] = dt;
-      a.sys_time[s] += dt;
-    }
+  if (a.fused_update && lane == 0 && k == 0) {
+    a.sys_dt[s] = a.dt;
+    a.sys_time[s] += a.dt;
   }
   // per-wave running interaction count (no contended atomics)
-  const int wtot = wave_sum(valid ? cnt : 0);
-  if (lane == 0) a.inter_total[wave] += (unsigned long long)wtot;
-  if (wave == 0 && lane == 0) atomicAdd(a.steps_done, 1);
+  if (lane == 0) a.inter_total[w0] += (unsigned long long)wtot;
+  if (hw == 0 && lane == 0) atomicAdd(a.steps_done, 1);
 }
 
 // ------------------------------------------------------------ direct sum --
@@ -365,46 +448,57 @@ __global__ void inverse_perm_kernel(const unsigned* __restrict__ id, int n,
 
 }  // namespace
 
-template <int BLOCK>
-void launch_walk_block(const WalkArgs& a, int f32, int damping, hipStream_t s) {
-  const int blocks = (a.n_waves * kWave + BLOCK - 1) / BLOCK;
+template <int BLOCK, int P>
+void launch_walk_shape(const WalkArgs& a, int f32, int damping, hipStream_t s) {
+  const int nhw = P == 1 ? a.n_waves : a.n_waves2;
+  const int blocks = (nhw * kWave + BLOCK - 1) / BLOCK;
   if (f32) {
     if (damping)
-      walk_kernel<true, true, false, BLOCK><<<blocks, BLOCK, 0, s>>>(a);
+      walk_kernel<true, true, false, BLOCK, P><<<blocks, BLOCK, 0, s>>>(a);
     else
-      walk_kernel<true, false, false, BLOCK><<<blocks, BLOCK, 0, s>>>(a);
+      walk_kernel<true, false, false, BLOCK, P><<<blocks, BLOCK, 0, s>>>(a);
   } else if (a.mac_nodes) {
     if (damping)
-      walk_kernel<false, true, true, BLOCK><<<blocks, BLOCK, 0, s>>>(a);
+      walk_kernel<false, true, true, BLOCK, 1><<<blocks, BLOCK, 0, s>>>(a);
     else
-      walk_kernel<false, false, true, BLOCK><<<blocks, BLOCK, 0, s>>>(a);
+      walk_kernel<false, false, true, BLOCK, 1><<<blocks, BLOCK, 0, s>>>(a);
   } else {
     if (damping)
-      walk_kernel<false, true, false, BLOCK><<<blocks, BLOCK, 0, s>>>(a);
+      walk_kernel<false, true, false, BLOCK, P><<<blocks, BLOCK, 0, s>>>(a);
     else
-      walk_kernel<false, false, false, BLOCK><<<blocks, BLOCK, 0, s>>>(a);
+      walk_kernel<false, false, false, BLOCK, P><<<blocks, BLOCK, 0, s>>>(a);
   }
 }
 
-void launch_walk(const WalkArgs& a, int f32, int damping, int block,
-                 hipStream_t s) {
-  if (a.n_waves <= 0) return;
+template <int P>
+void launch_walk_pack(const WalkArgs& a, int f32, int damping, int block,
+                      hipStream_t s) {
   switch (block) {
     case 64:
-      launch_walk_block<64>(a, f32, damping, s);
+      launch_walk_shape<64, P>(a, f32, damping, s);
       break;
     case 128:
-      launch_walk_block<128>(a, f32, damping, s);
+      launch_walk_shape<128, P>(a, f32, damping, s);
       break;
     case 512:
-      launch_walk_block<512>(a, f32, damping, s);
+      launch_walk_shape<512, P>(a, f32, damping, s);
       break;
     case 1024:
-      launch_walk_block<1024>(a, f32, damping, s);
+      launch_walk_shape<1024, P>(a, f32, damping, s);
       break;
     default:
-      launch_walk_block<256>(a, f32, damping, s);
+      launch_walk_shape<256, P>(a, f32, damping, s);
   }
+}
+
+void launch_walk(const WalkArgs& a, int f32, int damping, int block, int pack,
+                 hipStream_t s) {
+  if (a.n_waves <= 0) return;
+  // the frozen-decision walk has a one-particle-per-lane form only
+  if (pack == 2 && !a.mac_nodes)
+    launch_walk_pack<2>(a, f32, damping, block, s);
+  else
+    launch_walk_pack<1>(a, f32, damping, block, s);
 }
 
 size_t direct_tmp_doubles(size_t m) { return (size_t)kDirectChunks * m * 3; }

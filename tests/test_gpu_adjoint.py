@@ -157,6 +157,7 @@ def test_torch_autograd_function():
         s.adjoint_forward(10, 5)
         xT2, _, _ = s.state()
         gx0, gv0, gm = s.adjoint_backward(2.0 * (xT2 - tgt.numpy()) / len(pos), None)
-    np.testing.assert_allclose(x0.grad.numpy(), gx0, rtol=1e-12, atol=0)
-    np.testing.assert_allclose(v0.grad.numpy(), gv0, rtol=1e-12, atol=0)
-    np.testing.assert_allclose(m.grad.numpy(), gm, rtol=1e-12, atol=0)
+    # same computation through two entry points; the FP64 atomic adds that
+    # gather node adjoints may sum in another order (~1e-14 of the scale)
+    for got, want in ((x0.grad.numpy(), gx0), (v0.grad.numpy(), gv0), (m.grad.numpy(), gm)):
+        np.testing.assert_allclose(got, want, rtol=1e-10, atol=1e-12 * np.abs(want).max())

@@ -270,28 +270,32 @@ def test_graph_batched_adaptive():
     assert (res[0][1] > 0).all()
 
 
-WALK_SHAPES = [(64, 1), (128, 1), (256, 1), (512, 1), (1024, 1), (64, 8), (128, 32),
-               (256, 16), (512, 64), (1024, 8)]
+WALK_SHAPES = [(64, 1, 1), (128, 1, 1), (256, 1, 1), (512, 1, 1), (1024, 1, 1), (64, 8, 1),
+               (128, 32, 1), (256, 16, 1), (512, 64, 1), (1024, 8, 1), (64, 1, 2), (256, 8, 2),
+               (1024, 64, 2), (128, 16, 2)]
 
 
 @pytest.mark.parametrize("precision", ["f32", "f64"])
 def test_walk_shapes_give_identical_results(precision):
-    """Each 64-lane wave walks independently and the XCD interleave only
-    permutes which workgroup takes which waves, so no launch shape the
-    autotuner may pick changes a single bit of the forces or steps."""
-    pos, vel, mass = O.generate(30000, "sphere", seed=7)
+    """Each particle's accepted cells and its summation order are fixed by
+    the traversal order of the pre-order array, whatever set of lanes shares
+    the traversal; the XCD interleave only permutes which workgroup takes
+    which waves.  So no launch shape the autotuner may pick changes a single
+    bit of the forces or steps.  30001 particles: the last 128-particle
+    wavefront is partly empty."""
+    pos, vel, mass = O.generate(30001, "sphere", seed=7)
     ref = None
-    for block, groups in WALK_SHAPES:
+    for shape in WALK_SHAPES:
         with sim(theta=0.5, precision=precision) as s:
             s.set_state(pos, vel, mass)
-            s.set_walk_shape(block, groups)
+            s.set_walk_shape(*shape)
             acc = s.forces()
             s.step(3)
             out = (acc, s.interactions(per_particle=True)[1]) + s.state()
         if ref is None:
             ref = out
         for a, b in zip(out, ref):
-            np.testing.assert_array_equal(a, b, err_msg=str((block, groups)))
+            np.testing.assert_array_equal(a, b, err_msg=str(shape))
 
 
 def test_walk_shapes_batched():
@@ -302,17 +306,24 @@ def test_walk_shapes_batched():
     mass = np.concatenate([p[2] for p in parts])
     off = np.cumsum([0] + [len(p[2]) for p in parts])
     ref = None
-    for block, groups in [(256, 1), (64, 64), (128, 8), (1024, 16)]:
+    for shape in [(256, 1, 1), (64, 64, 1), (128, 8, 1), (1024, 16, 1), (256, 1, 2),
+                  (64, 8, 2)]:
+        with sim(theta=0.6, precision="f32", adaptive=True) as s:
+            s.set_state(pos, vel, mass)
+            s.set_systems(off)
+            s.set_walk_shape(*shape)
+            s.step(3, graph=True)
+            out = s.state() + (s.time(len(parts)),)
         with sim(theta=0.6, precision="f32") as s:
             s.set_state(pos, vel, mass)
             s.set_systems(off)
-            s.set_walk_shape(block, groups)
-            s.step(3, graph=True)
-            out = s.state()
+            s.set_walk_shape(*shape)
+            s.step(2)
+            out = out + s.state() + (s.time(len(parts)),)
         if ref is None:
             ref = out
         for a, b in zip(out, ref):
-            np.testing.assert_array_equal(a, b, err_msg=str((block, groups)))
+            np.testing.assert_array_equal(a, b, err_msg=str(shape))
 
 
 @pytest.mark.parametrize("precision", ["f32", "f64"])
@@ -331,10 +342,10 @@ def test_autotune_restores_state(precision):
         before = s.state()
         t_before = s.time()
         tot = s.interactions_total(reset=False)
-        (block, groups), times = s.autotune(reps=2)
-        assert block in s.WALK_BLOCKS and groups in s.WALK_GROUPS
+        best, times = s.autotune(reps=2)
+        assert best in times and len(times) == 50
         assert all(t > 0 for t in times.values())
-        assert times[(block, groups)] == min(times.values())
+        assert times[best] == min(times.values())
         # tuning moves neither the state nor time nor the running total
         assert s.interactions_total(reset=False) == tot
         for a, b in zip(s.state(), before):
