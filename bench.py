@@ -34,6 +34,22 @@ FP32_PEAK_TFLOPS = 157.3   # MI355X vector FP32 (AMD spec)
 FP64_PEAK_TFLOPS = 78.6    # MI355X vector FP64
 HBM_PEAK_GBS = 8000.0
 FLOP_PER_INTERACTION = 20
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "walk_traffic_sphere1M.md")
+
+
+def measured_traffic(key):
+    """HBM bytes per walk launch for `key` (precision and shape, e.g.
+    "f32 64,8,1") from the committed rocprofv3 counter summary, whose lines
+    read `traffic_bytes_per_launch[<key>]: <bytes>`; None if absent."""
+    try:
+        with open(TRAFFIC_SUMMARY) as f:
+            for line in f:
+                line = line.strip().strip("`")
+                if line.startswith("traffic_bytes_per_launch[%s]:" % key):
+                    return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def log(*a):
@@ -62,6 +78,7 @@ def time_path(sim, pos, vel, mass, precision, steps, warmup, prof_steps, shape, 
     sim.synchronize()
     t1 = time.perf_counter()
     info = sim.tree_info()
+    end_pos, end_vel, _ = sim.state()
     sim.interactions_total(reset=True)
     sim.profile(True)
     sim.profile_read(reset=True)
@@ -74,29 +91,38 @@ def time_path(sim, pos, vel, mass, precision, steps, warmup, prof_steps, shape, 
     inter = sim.interactions_total(reset=True)
     return {"ms_per_step": (t1 - t0) * 1e3 / steps, "prof": prof, "interactions": inter,
             "prof_steps": prof_steps, "eager_ms_per_step": (t3 - t2) * 1e3 / prof_steps,
-            "tree": info, "walk_shape": list(shape)}
+            "tree": info, "walk_shape": list(shape), "end_state": (end_pos, end_vel)}
 
 
 def cpu_baseline(pos, mass, theta, sample, threads):
+    """The oracle's step (gen_root + octree + mass/COM, then the BFS walk)
+    on the host cores, at the particle state the GPU's timed region ended
+    in, so both sides do the same interactions.  With sample < N the walk
+    covers a seeded subset and is extrapolated to N."""
     import oracle as O
     n = len(pos)
     t0 = time.perf_counter()
     tree = O.Tree(pos, mass)           # gen_root + add_particle + mass + COM
     t1 = time.perf_counter()
-    rng = np.random.default_rng(0xBEEF)
-    idx = np.sort(rng.choice(n, sample, replace=False))
+    if sample >= n:
+        _, cnt = tree.forces(theta=theta, threads=threads)
+        walked = n
+    else:
+        rng = np.random.default_rng(0xBEEF)
+        idx = np.sort(rng.choice(n, sample, replace=False))
+        _, cnt = tree.forces_subset(idx, theta=theta, threads=threads)
+        walked = sample
     t2 = time.perf_counter()
-    _, cnt = tree.forces_subset(idx, theta=theta, threads=threads)
-    t3 = time.perf_counter()
-    t_step = (t1 - t0) + (t3 - t2) * n / sample
+    t_step = (t1 - t0) + (t2 - t1) * n / walked
     return {
         "value": n / t_step, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-        "sample": "oracle (C++ restatement of the reference pthreads path, FP64): tree build "
-                  "+ mass/COM of all %d particles (%.2f s, serial like gen_root) and the "
-                  "BFS walk of a seeded %d-particle sample on %d threads (%.2f s), "
-                  "extrapolated to one step of N" % (n, t1 - t0, sample, threads, t3 - t2),
+        "sample": "oracle (C++ restatement of the reference step, FP64) at the state the "
+                  "timed GPU steps ended in: tree build + mass/COM of all %d particles "
+                  "(%.2f s, serial like gen_root) and the BFS walk of %d particles on %d "
+                  "threads (%.2f s)%s" % (n, t1 - t0, walked, threads, t2 - t1,
+                                          "" if walked == n else ", extrapolated to N"),
         "interactions_per_particle": float(cnt.mean()),
-        "seconds": t3 - t0,
+        "seconds": t2 - t0,
     }
 
 
@@ -123,10 +149,14 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=20)
     ap.add_argument("--prof-steps", type=int, default=10)
     ap.add_argument("--samples", type=int, default=4096)
-    ap.add_argument("--cpu-sample", type=int, default=16384)
+    ap.add_argument("--cpu-sample", type=int, default=0,
+                    help="particles the CPU baseline walks (0 = all)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--walk-shape", default=None, metavar="BLOCK,GROUPS",
-                    help="fixed walk launch shape instead of the autotuner's")
+    ap.add_argument("--walk-shape", default="64,8,1", metavar="BLOCK,GROUPS,PACK",
+                    help="walk launch shape (default: the one nbs_autotune picks on this "
+                         "workload, so the committed counter profiles apply)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="let nbs_autotune pick the walk shape before the timed steps")
     ap.add_argument("--eager", action="store_true",
                     help="timed steps launched one kernel at a time, not by graph replay")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
@@ -167,14 +197,14 @@ def main():
     log("accuracy", acc)
 
     # ---- timed paths
-    shape = tuple(int(v) for v in args.walk_shape.split(",")) if args.walk_shape else None
+    shape = None if args.autotune else tuple(int(v) for v in args.walk_shape.split(","))
     graph = not args.eager
     r32 = time_path(sim, pos, vel, mass, "f32", args.steps, args.warmup, args.prof_steps,
                     shape, graph)
-    log("f32", r32)
+    log("f32", {k: v for k, v in r32.items() if k != "end_state"})
     r64 = time_path(sim, pos, vel, mass, "f64", args.fp64_steps, args.warmup, args.prof_steps,
                     shape, graph)
-    log("f64", r64)
+    log("f64", {k: v for k, v in r64.items() if k != "end_state"})
     sim.close()
 
     value = n * 1e3 / r32["ms_per_step"]
@@ -224,13 +254,15 @@ def main():
                        "buckets": r32["tree"]["n_buckets"]},
         "roofline": {"bound": "valu", "kernel": "walk_kernel<f32>",
                      "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP32_PEAK_TFLOPS,
+                     "traffic": measured_traffic("f32 %s" % ",".join(map(str, r32["walk_shape"]))),
                      "flop_per_interaction": FLOP_PER_INTERACTION,
                      "avg_launch_ms": walk_ms},
     }
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(pos, mass, theta, args.cpu_sample, threads)
+        end_pos = r32["end_state"][0]
+        out["cpu_baseline"] = cpu_baseline(end_pos, mass, theta, args.cpu_sample or n, threads)
         log("cpu", out["cpu_baseline"])
     print(json.dumps(out), flush=True)
 
