@@ -125,9 +125,10 @@ int nbs_step_graph(nbs_ctx* ctx, int nsteps);
  * kernel shape was fixed, src/cuda_helper.cpp:183-188).  The shape is the
  * workgroup size (block: 64, 128, 256, 512, 1024 threads), the number of
  * contiguous particle regions interleaved over the 8 XCDs (groups: 1 =
- * plain order, 8, 16, 32, 64) and the particles per lane (pack: 1, 2); it
- * changes the speed only, never a result bit.  nbs_autotune times `reps`
- * whole steps from the current state for each of the 50 shapes, keeps the
+ * plain order, 8, 16, 32, 64) and the particles per lane (pack: 1, or 2
+ * sharing one traversal); it changes the speed only, never a result bit.
+ * nbs_autotune times `reps` whole steps from the current state for each
+ * of the 50 shapes, keeps the
  * fastest for this context, reports it (best_*; may be NULL) and the times
  * in ms per step (ms[(pack_index * 5 + block_index) * 5 + groups_index],
  * may be NULL; -1 = not timed), and restores the state, accelerations, time
@@ -137,6 +138,16 @@ int nbs_step_graph(nbs_ctx* ctx, int nsteps);
 int nbs_autotune(nbs_ctx* ctx, int reps, int* best_block, int* best_groups,
                  int* best_pack, double* ms);
 int nbs_set_walk_shape(nbs_ctx* ctx, int block, int groups, int pack);
+
+/* Longest-first dispatch (default off; no counterpart in the reference):
+ * each walk records the node visits of every workgroup's tile and the next
+ * walk dispatches the tiles of each XCD region most expensive first, so a
+ * launch does not end with a few long workgroups running alone.  Changes
+ * the speed only, never a result bit.  On the uniform-sphere bench it
+ * shortens a repeated walk of one tree by ~4% but slows whole steps by
+ * ~5%: the cost order scatters Morton-neighbour tiles in time, and a
+ * freshly built tree then misses more in L2. */
+int nbs_set_walk_order(nbs_ctx* ctx, int on);
 
 /* Blocks until the context's stream has drained. */
 int nbs_synchronize(nbs_ctx* ctx);

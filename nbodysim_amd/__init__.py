@@ -166,6 +166,10 @@ class Simulation:
     def set_walk_shape(self, block, groups=1, pack=1):
         self._check(self._L.nbs_set_walk_shape(self._h, int(block), int(groups), int(pack)))
 
+    def set_walk_order(self, on=True):
+        """Longest-first workgroup dispatch of the walk (default off)."""
+        self._check(self._L.nbs_set_walk_order(self._h, int(bool(on))))
+
     def synchronize(self):
         self._check(self._L.nbs_synchronize(self._h))
 

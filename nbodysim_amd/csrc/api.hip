@@ -176,6 +176,12 @@ struct nbs_ctx {
   int walk_block = 256;
   int walk_groups = 1;
   int walk_pack = 1;
+  // longest-first workgroup order: per-tile visit counts of the last walk
+  // (recorded under cost_block / cost_pack) and the order derived from them
+  int walk_order = 0;
+  unsigned* tile_cost = nullptr;
+  int* tile_order = nullptr;
+  int cost_block = -1, cost_pack = -1, cost_tiles = -1;
 
   // Two captured steps (the state ping-pongs between its two buffers, so
   // a pair of steps returns to the same buffer); valid while the
@@ -357,6 +363,9 @@ int setup_systems(nbs_ctx* c) {
   s.n_waves2 = c->h_wave2_start[nsys];
   s.bbox = dalloc<double>(o, 6 * (size_t)std::max(s.n_waves, 1), &e);
   c->inter_total = dalloc<unsigned long long>(o, std::max(s.n_waves, 1), &e);
+  c->tile_cost = dalloc<unsigned>(o, std::max(s.n_waves, 1), &e);
+  c->tile_order = dalloc<int>(o, std::max(s.n_waves, 1), &e);
+  c->cost_tiles = -1;
   if (e != hipSuccess) return hip_fail(c, e, "hipMalloc(system buffers)");
   launch_fill_u64(c->inter_total, 0, std::max(s.n_waves, 1), c->stream);
   HIPCHK(c, hipMemcpyAsync(s.start, c->h_start.data(), (nsys + 1) * sizeof(int),
@@ -426,6 +435,19 @@ void enqueue_build(nbs_ctx* c) {
   prof_end(c, kPhaseBuild, e0);
 }
 
+// Tiles of the current walk shape; clears the recorded tile costs when they
+// were recorded under another shape (another tiling or traversal).
+int sync_tile_costs(nbs_ctx* c) {
+  const int tiles = walk_tiles(c->sys.n_waves, c->sys.n_waves2, c->walk_block, c->walk_pack);
+  if (tiles != c->cost_tiles || c->walk_block != c->cost_block || c->walk_pack != c->cost_pack) {
+    launch_fill_int((int*)c->tile_cost, 0, std::max(tiles, 1), c->stream);
+    c->cost_tiles = tiles;
+    c->cost_block = c->walk_block;
+    c->cost_pack = c->walk_pack;
+  }
+  return tiles;
+}
+
 void enqueue_walk(nbs_ctx* c, int fused) {
   hipEvent_t e0 = nullptr;
   prof_begin(c, kPhaseWalk, &e0);
@@ -460,6 +482,12 @@ void enqueue_walk(nbs_ctx* c, int fused) {
   a.inter_total = c->inter_total;
   a.fused_update = fused;
   a.groups = c->walk_groups;
+  if (c->walk_order) {
+    const int tiles = sync_tile_costs(c);
+    launch_tile_order(c->tile_cost, tiles, c->walk_groups, c->tile_order, c->stream);
+    a.order = c->tile_order;
+    a.tile_cost = c->tile_cost;
+  }
   launch_walk(a, c->cfg.force_precision == NBS_FORCE_F32, c->p.damping,
               c->walk_block, c->walk_pack, c->stream);
   if (fused) c->bbox_valid = true;
@@ -560,6 +588,7 @@ int run_graph_pairs(nbs_ctx* c, int pairs) {
     c->prof = false;  // no event records inside a captured graph
     const int cur0 = c->cur;
     const bool bbox0 = c->bbox_valid;
+    if (c->walk_order) sync_tile_costs(c);  // no clearing inside the graph
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < 2; k++) {
       enqueue_build(c);
@@ -836,6 +865,14 @@ int nbs_set_walk_shape(nbs_ctx* c, int block, int groups, int pack) {
   c->walk_block = block;
   c->walk_groups = groups;
   c->walk_pack = pack;
+  return NBS_OK;
+}
+
+int nbs_set_walk_order(nbs_ctx* c, int on) {
+  if (!c) return NBS_ERR_ARG;
+  drop_graph(c);
+  c->walk_order = on ? 1 : 0;
+  c->cost_tiles = -1;
   return NBS_OK;
 }
 

@@ -163,6 +163,8 @@ struct WalkArgs {
   unsigned long long* inter_total;  // running sum of interactions
   int fused_update;
   int groups;  // workgroup -> tile interleave over the XCDs (1 = plain)
+  const int* order;     // workgroup -> tile (longest first), or null
+  unsigned* tile_cost;  // per tile: node visits of this walk added, or null
   // frozen decisions (FP64 only; null otherwise): base tree + positions
   const WalkNode* mac_nodes;
   const double *mac_x, *mac_y, *mac_z;
@@ -170,9 +172,14 @@ struct WalkArgs {
 
 // Launch shape of the walk: block = workgroup size (64, 128, 256, 512 or
 // 1024 threads); a.groups = contiguous tile regions interleaved over the
-// XCDs; pack = particles per lane (1 or 2).
+// XCDs; pack = particles per lane (1, or 2 sharing one traversal).
 void launch_walk(const WalkArgs& a, int f32, int damping, int block, int pack,
                  hipStream_t s);
+// Workgroups (tiles) of a walk launch of this shape.
+int walk_tiles(int n_waves, int n_waves2, int block, int pack);
+// Longest-first tile order from the previous walk's tile costs (cleared).
+void launch_tile_order(unsigned* tile_cost, int n_tiles, int groups, int* order,
+                       hipStream_t s);
 constexpr int kWalkBlocks[] = {64, 128, 256, 512, 1024};
 constexpr int kWalkGroups[] = {1, 8, 16, 32, 64};
 constexpr int kWalkPacks[] = {1, 2};

@@ -123,6 +123,43 @@ __device__ __forceinline__ void kick_drift(double& x, double& v, double a,
   x = x + t2;
 }
 
+// One accepted body-node interaction of one lane: a += G m d / |d|^3
+// (src/main.cpp:306-333), d = com - x formed in FP64.
+template <bool F32, bool DAMP, typename AccT>
+__device__ __forceinline__ void interact(const NodeRegs& nd, double dx, double dy,
+                                         double dz, AccT& ax, AccT& ay, AccT& az,
+                                         int& cnt) {
+  if (F32) {
+    const float fdx = (float)dx, fdy = (float)dy, fdz = (float)dz;
+    const float fd2 = fmaf(fdx, fdx, fmaf(fdy, fdy, fdz * fdz));
+    const float r = __builtin_amdgcn_rsqf(fd2);
+    float sc;
+    if (DAMP) {
+      const float d = fd2 * r;
+      sc = nd.gm32 * r / (fd2 + __expf(-d));
+    } else {
+      sc = nd.gm32 * (r * r * r);
+    }
+    ax = fmaf(sc, fdx, (float)ax);
+    ay = fmaf(sc, fdy, (float)ay);
+    az = fmaf(sc, fdz, (float)az);
+  } else {
+    const double d2 = sq_sum_ref(dx, dy, dz);
+    const double r = rsqrt_f64(d2);
+    double sc;
+    if (DAMP) {
+      const double d = d2 * r;
+      sc = nd.gm * r / (d2 + exp(-d));
+    } else {
+      sc = nd.gm * (r * r * r);
+    }
+    ax = fma(sc, dx, (double)ax);
+    ay = fma(sc, dy, (double)ay);
+    az = fma(sc, dz, (double)az);
+  }
+  cnt++;
+}
+
 // Workgroup -> tile order.  Workgroups are handed to the 8 XCDs round-robin
 // by workgroup id, each XCD with its own L2.  With `groups` > 1 the tiles
 // (runs of Morton-consecutive waves) are cut into `groups` contiguous
@@ -151,7 +188,7 @@ template <bool F32, bool DAMP, bool FROZEN, int BLOCK, int P>
 __global__ void __launch_bounds__(BLOCK) walk_kernel(WalkArgs a) {
   if (*a.flags) return;
   const int lane = lane_id();
-  const int tile = tile_of_block(blockIdx.x, gridDim.x, a.groups);
+  const int tile = a.order ? a.order[blockIdx.x] : tile_of_block(blockIdx.x, gridDim.x, a.groups);
   const int hw = __builtin_amdgcn_readfirstlane(
       (int)((tile * BLOCK + threadIdx.x) >> 6));
   const int* hw_start = P == 1 ? a.wave_start : a.wave2_start;
@@ -208,7 +245,9 @@ __global__ void __launch_bounds__(BLOCK) walk_kernel(WalkArgs a) {
   // One scalar load of the node record per visit.  (Loading both possible
   // successors a visit ahead measured slower: the doubled scalar-memory
   // traffic costs more than the latency it hides.)
+  unsigned visits = 0;
   while (n < nend) {
+    visits++;
     const NodeRegs cur = load_node(nodes, n);
     const int leaf = cur.leaf;
     const int skip = cur.skip;
@@ -295,36 +334,9 @@ __global__ void __launch_bounds__(BLOCK) walk_kernel(WalkArgs a) {
       }
     }
 #pragma unroll
-    for (int p = 0; p < P; p++) {
-      if (inter[p]) {
-        if (F32) {
-          const float r = __builtin_amdgcn_rsqf(fd2[p]);
-          float sc;
-          if (DAMP) {
-            const float d = fd2[p] * r;
-            sc = cur.gm32 * r / (fd2[p] + __expf(-d));
-          } else {
-            sc = cur.gm32 * (r * r * r);
-          }
-          ax[p] = fmaf(sc, fdx[p], (float)ax[p]);
-          ay[p] = fmaf(sc, fdy[p], (float)ay[p]);
-          az[p] = fmaf(sc, fdz[p], (float)az[p]);
-        } else {
-          const double r = rsqrt_f64(d2[p]);
-          double sc;
-          if (DAMP) {
-            const double d = d2[p] * r;
-            sc = cur.gm * r / (d2[p] + exp(-d));
-          } else {
-            sc = cur.gm * (r * r * r);
-          }
-          ax[p] = fma(sc, dx[p], (double)ax[p]);
-          ay[p] = fma(sc, dy[p], (double)ay[p]);
-          az[p] = fma(sc, dz[p], (double)az[p]);
-        }
-        cnt[p]++;
-      }
-    }
+    for (int p = 0; p < P; p++)
+      if (inter[p])
+        interact<F32, DAMP>(cur, dx[p], dy[p], dz[p], ax[p], ay[p], az[p], cnt[p]);
     n = max(next, n + 1);  // always advance, even on a corrupt skip
     if (top > 0 && top_pos == n) {
 #pragma unroll
@@ -376,7 +388,94 @@ __global__ void __launch_bounds__(BLOCK) walk_kernel(WalkArgs a) {
   }
   // per-wave running interaction count (no contended atomics)
   if (lane == 0) a.inter_total[w0] += (unsigned long long)wtot;
+  if (a.tile_cost && lane == 0) atomicAdd(&a.tile_cost[tile], visits);
   if (hw == 0 && lane == 0) atomicAdd(a.steps_done, 1);
+}
+
+// ------------------------------------------------------- wave ordering --
+// Longest-first order of the walk's workgroups.  The walk adds each
+// wavefront's node visits to tile_cost[tile] (a tile is the run of
+// wavefronts one workgroup takes); the next walk dispatches the tiles of
+// every XCD region (see tile_of_block) in decreasing order of that cost, so
+// the expensive tiles start first and cheap ones fill the end of the
+// launch.  One workgroup per region: a stable counting sort on 32 cost
+// buckets (linear in cost up to the region's maximum; equal buckets keep
+// the plain order, so with no cost recorded the order is tile_of_block's).
+// The costs are cleared for the next walk to accumulate.
+constexpr int kOrderBuckets = 32;
+constexpr int kOrderThreads = 1024;
+constexpr int kOrderWaves = kOrderThreads / kWave;
+
+__global__ void __launch_bounds__(kOrderThreads) tile_order_kernel(
+    unsigned* __restrict__ cost, int n_tiles, int groups, int* __restrict__ order) {
+  const int G = groups <= 1 ? 1 : groups;
+  const int r = blockIdx.x;
+  const int q = n_tiles / G, rem = n_tiles % G;
+  const int beg = r * q + min(r, rem);
+  const int cnt = q + (r < rem ? 1 : 0);
+  const int lane = lane_id(), w = threadIdx.x >> 6;
+  __shared__ unsigned wmax[kOrderWaves];
+  __shared__ int base[kOrderBuckets];
+  __shared__ int wcount[kOrderWaves][kOrderBuckets];
+  __shared__ int chunk_total[kOrderBuckets];
+
+  // maximum cost of the region
+  unsigned mx = 0;
+  for (int k = threadIdx.x; k < cnt; k += kOrderThreads) mx = max(mx, cost[beg + k]);
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) mx = max(mx, (unsigned)__shfl_xor((int)mx, d, 64));
+  if (lane == 0) wmax[w] = mx;
+  if (threadIdx.x < kOrderBuckets) base[threadIdx.x] = 0;
+  __syncthreads();
+  mx = 0;
+  for (int k = 0; k < kOrderWaves; k++) mx = max(mx, wmax[k]);
+  const unsigned long long den = (unsigned long long)mx + 1;
+  auto bucket = [&](unsigned c) {  // 0 = most expensive
+    return kOrderBuckets - 1 - (int)(((unsigned long long)c * kOrderBuckets) / den);
+  };
+
+  // histogram, exclusive scan into base
+  for (int k = threadIdx.x; k < cnt; k += kOrderThreads) atomicAdd(&base[bucket(cost[beg + k])], 1);
+  __syncthreads();
+  if (w == 0) {
+    const int h = lane < kOrderBuckets ? base[lane] : 0;
+    const int incl = wave_inclusive_scan(h);
+    if (lane < kOrderBuckets) base[lane] = incl - h;
+  }
+  __syncthreads();
+
+  // stable scatter, one chunk of kOrderThreads tiles at a time
+  for (int k0 = 0; k0 < cnt; k0 += kOrderThreads) {
+    const int k = k0 + threadIdx.x;
+    const bool v = k < cnt;
+    const int b = v ? bucket(cost[beg + k]) : 0;
+    unsigned long long same = ballot(v);
+#pragma unroll
+    for (int bit = 0; bit < 5; bit++) {
+      const unsigned long long m = ballot(v && ((b >> bit) & 1));
+      same &= ((b >> bit) & 1) ? m : ~m;
+    }
+    const int rank = mbcnt(same);
+    for (int j = lane; j < kOrderBuckets; j += kWave) wcount[w][j] = 0;
+    __syncthreads();
+    if (v && rank == 0) wcount[w][b] = __popcll(same);
+    __syncthreads();
+    if (threadIdx.x < kOrderBuckets) {  // prefix over the waves, per bucket
+      int acc = 0;
+      for (int j = 0; j < kOrderWaves; j++) {
+        const int c = wcount[j][threadIdx.x];
+        wcount[j][threadIdx.x] = acc;
+        acc += c;
+      }
+      chunk_total[threadIdx.x] = acc;
+    }
+    __syncthreads();
+    if (v) order[(base[b] + wcount[w][b] + rank) * G + r] = beg + k;
+    __syncthreads();
+    if (threadIdx.x < kOrderBuckets) base[threadIdx.x] += chunk_total[threadIdx.x];
+    __syncthreads();
+  }
+  for (int k = threadIdx.x; k < cnt; k += kOrderThreads) cost[beg + k] = 0;
 }
 
 // ------------------------------------------------------------ direct sum --
@@ -499,6 +598,18 @@ void launch_walk(const WalkArgs& a, int f32, int damping, int block, int pack,
     launch_walk_pack<2>(a, f32, damping, block, s);
   else
     launch_walk_pack<1>(a, f32, damping, block, s);
+}
+
+void launch_tile_order(unsigned* tile_cost, int n_tiles, int groups, int* order,
+                       hipStream_t s) {
+  if (n_tiles <= 0) return;
+  const int G = groups <= 1 ? 1 : groups;
+  tile_order_kernel<<<G, kOrderThreads, 0, s>>>(tile_cost, n_tiles, G, order);
+}
+
+int walk_tiles(int n_waves, int n_waves2, int block, int pack) {
+  const int nhw = pack == 1 ? n_waves : n_waves2;
+  return (nhw * kWave + block - 1) / block;
 }
 
 size_t direct_tmp_doubles(size_t m) { return (size_t)kDirectChunks * m * 3; }

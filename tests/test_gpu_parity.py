@@ -285,12 +285,14 @@ def test_walk_shapes_give_identical_results(precision):
     wavefront is partly empty."""
     pos, vel, mass = O.generate(30001, "sphere", seed=7)
     ref = None
-    for shape in WALK_SHAPES:
+    for k, shape in enumerate(WALK_SHAPES):
         with sim(theta=0.5, precision=precision) as s:
             s.set_state(pos, vel, mass)
             s.set_walk_shape(*shape)
+            s.set_walk_order(k % 2 == 0)  # longest-first dispatch on / off
             acc = s.forces()
             s.step(3)
+            s.step(2, graph=True)
             out = (acc, s.interactions(per_particle=True)[1]) + s.state()
         if ref is None:
             ref = out
@@ -307,7 +309,7 @@ def test_walk_shapes_batched():
     off = np.cumsum([0] + [len(p[2]) for p in parts])
     ref = None
     for shape in [(256, 1, 1), (64, 64, 1), (128, 8, 1), (1024, 16, 1), (256, 1, 2),
-                  (64, 8, 2)]:
+                  (64, 8, 2), (512, 1, 2)]:
         with sim(theta=0.6, precision="f32", adaptive=True) as s:
             s.set_state(pos, vel, mass)
             s.set_systems(off)
