@@ -176,6 +176,10 @@ struct nbs_ctx {
   int walk_block = 256;
   int walk_groups = 1;
   int walk_pack = 1;
+  // key levels in the radix-sorted primary key (3 bits each).  All 21 that
+  // fit: with fewer, the fixup's per-run insertion sort met runs of
+  // thousands of equal keys while a cold sphere collapses (seconds per step).
+  int prim_levels = 21;
   // longest-first workgroup order: per-tile visit counts of the last walk
   // (recorded under cost_block / cost_pack) and the order derived from them
   int walk_order = 0;
@@ -361,6 +365,7 @@ int setup_systems(nbs_ctx* c) {
   }
   s.n_waves = c->h_wave_start[nsys];
   s.n_waves2 = c->h_wave2_start[nsys];
+  c->prim_levels = max_prim_levels(nsys);
   s.bbox = dalloc<double>(o, 6 * (size_t)std::max(s.n_waves, 1), &e);
   c->inter_total = dalloc<unsigned long long>(o, std::max(s.n_waves, 1), &e);
   c->tile_cost = dalloc<unsigned>(o, std::max(s.n_waves, 1), &e);
@@ -421,7 +426,7 @@ void enqueue_build(nbs_ctx* c) {
     c->bbox_valid = true;  // positions do not change during a build
   }
   launch_root(c->sys, c->p, c->flags, s);
-  const int nbits = launch_keys(cur, c->sys, c->bb, s);
+  const int nbits = launch_keys(cur, c->sys, c->bb, c->prim_levels, s);
   const int which = radix_sort_pairs(c->bb.prim0, c->bb.val0, c->bb.prim1,
                                      c->bb.val1, c->n, nbits, c->bb.sort_tmp, s);
   unsigned long long* prim = which ? c->bb.prim1 : c->bb.prim0;

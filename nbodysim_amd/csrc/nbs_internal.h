@@ -118,9 +118,14 @@ void launch_bbox(const State& st, Systems& sys, hipStream_t s);
 void launch_root(Systems& sys, const StepParams& p, const int* flags,
                  hipStream_t s);
 // Keys of the particles in `st` (current order) into prim0/val0 and
-// khi_u/klo_u.  Returns the number of significant primary-key bits.
-int launch_keys(const State& st, Systems& sys, BuildBuffers& b,
+// khi_u/klo_u; the primary (sort) key holds the system and the first
+// prim_levels key levels.  Returns the number of significant primary-key
+// bits.
+int launch_keys(const State& st, Systems& sys, BuildBuffers& b, int prim_levels,
                 hipStream_t s);
+// Deepest primary key for nsys systems (system bits + 3 bits per level
+// within 64 bits, at most 21 levels).
+int max_prim_levels(int nsys);
 // Gathers the full keys into sorted order and sorts runs of equal primary
 // keys by (full key, caller index).  perm is updated in place.
 void launch_fixup(BuildBuffers& b, const unsigned long long* prim,

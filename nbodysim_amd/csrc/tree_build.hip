@@ -162,10 +162,9 @@ __global__ void __launch_bounds__(kBlock)
   }
   khi[i] = hi;
   klo[i] = lo;
-  unsigned long long p = hi;
-  if (sys_shift > 0)
-    p = ((unsigned long long)s << sys_shift) | (hi >> (63 - 3 * prim_levels));
-  prim[i] = p;
+  // primary sort key: system, then the first prim_levels levels (runs of
+  // equal primary keys are finished by fixup_kernel on the full key)
+  prim[i] = ((unsigned long long)s << sys_shift) | (hi >> (63 - 3 * prim_levels));
   val[i] = (unsigned)i;
 }
 
@@ -493,19 +492,26 @@ void launch_root(Systems& sys, const StepParams& p, const int* flags,
       p.root_center[0], p.root_center[1], p.root_center[2], p.root_size, flags);
 }
 
-int launch_keys(const State& st, Systems& sys, BuildBuffers& b,
+int max_prim_levels(int nsys) {
+  int sys_bits = 0;
+  while ((1 << sys_bits) < nsys) sys_bits++;
+  const int levels = (64 - sys_bits) / 3;
+  return levels > 21 ? 21 : levels;
+}
+
+int launch_keys(const State& st, Systems& sys, BuildBuffers& b, int prim_levels,
                 hipStream_t s) {
   const int nw = sys.n_waves;
   int sys_bits = 0;
   while ((1 << sys_bits) < sys.nsys) sys_bits++;
-  int levels = sys_bits > 0 ? (64 - sys_bits) / 3 : 21;
-  if (levels > 21) levels = 21;
-  const int sys_shift = sys_bits > 0 ? 3 * levels : 0;
+  const int levels = prim_levels < 1 ? 1 : (prim_levels > max_prim_levels(sys.nsys)
+                                                ? max_prim_levels(sys.nsys)
+                                                : prim_levels);
   if (nw > 0)
     keys_kernel<<<blocks_for((size_t)nw * 64, kBlock), kBlock, 0, s>>>(
         st.x, st.y, st.z, sys.start, sys.wave_start, sys.nsys, nw, sys.root,
-        sys_shift, levels, b.prim0, b.val0, b.khi_u, b.klo_u);
-  return sys_bits > 0 ? sys_bits + 3 * levels : 63;
+        3 * levels, levels, b.prim0, b.val0, b.khi_u, b.klo_u);
+  return sys_bits + 3 * levels;
 }
 
 void launch_fixup(BuildBuffers& b, const unsigned long long* prim,

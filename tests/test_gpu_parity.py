@@ -359,3 +359,31 @@ def test_autotune_restores_state(precision):
     for a, b in zip(got[:2], want[:2]):
         np.testing.assert_array_equal(a, b)
     np.testing.assert_array_equal(got_t, want_t)
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_dense_cluster(graph):
+    """9000 particles in a 0.01-wide cube inside a 1000-wide cloud: 21 tree
+    levels below the 12 the cloud needs, the reference's tree and forces,
+    graph replays equal to eager steps."""
+    rng = np.random.default_rng(5)
+    far = rng.uniform(-500, 500, size=(11000, 3))
+    near = 3.0 + rng.uniform(0, 0.01, size=(9000, 3))
+    pos = np.concatenate([far, near])
+    mass = rng.uniform(1e9, 1e10, size=len(pos))
+    vel = np.zeros_like(pos)
+    with sim(theta=0.5) as s:
+        s.set_state(pos, vel, mass)
+        if graph:
+            s.step(2, graph=True)
+            p, v, _ = s.state()
+            with sim(theta=0.5) as e:
+                e.set_state(pos, vel, mass)
+                e.step(2)
+                for a, b in zip(e.state()[:2], (p, v)):
+                    np.testing.assert_array_equal(a, b)
+            return
+        acc = s.forces()
+        assert_tree_equal(s.tree_dump(), O.Tree(pos, mass).dump())
+    oacc, _ = O.Tree(pos, mass).forces(theta=0.5)
+    assert rel_err(acc, oacc).max() < 1e-12
