@@ -259,7 +259,7 @@ __global__ void __launch_bounds__(BLOCK) walk_kernel(WalkArgs a) {
       dx[p] = cur.cx - x[p];
       dy[p] = cur.cy - y[p];
       dz[p] = cur.cz - z[p];
-      in[p] = (A[p] >> lane) & 1ull;
+      in[p] = __builtin_amdgcn_inverse_ballot_w64(A[p]);  // the mask itself
       if (F32) {
         fdx[p] = (float)dx[p];
         fdy[p] = (float)dy[p];
