@@ -436,26 +436,69 @@ __global__ void __launch_bounds__(kBlock)
        q += gridDim.x * kBlock) {
     const int p = bucket[q];
     const int end = nodes[p].skip;
-    double M = 0;
-    for (int c = p + 1; c < end; c = max(nodes[c].skip, c + 1)) {
-      M = M + nodes[c].m;
-      parent[c] = p;
-    }
-    double cxv = 0, cyv = 0, czv = 0;
-    if (M != 0) {  // else COM (0,0,0), src/octree.cpp:259-262
-      double tx = 0, ty = 0, tz = 0;
-      for (int c = p + 1; c < end; c = max(nodes[c].skip, c + 1)) {
-        const double cm = nodes[c].m;
-        const double px = nodes[c].cx * cm;
-        const double py = nodes[c].cy * cm;
-        const double pz = nodes[c].cz * cm;
-        tx = tx + px;
-        ty = ty + py;
-        tz = tz + pz;
+    // One pass along the sibling chain (child c's record holds the next
+    // sibling, skip): up to 8 children's mass and COM are read with their
+    // link, so the chain of dependent loads is walked once.
+    double cm[8], ccx[8], ccy[8], ccz[8];
+    int nc = 0;
+    int c = p + 1;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      if (c < end) {
+        cm[k] = nodes[c].m;
+        ccx[k] = nodes[c].cx;
+        ccy[k] = nodes[c].cy;
+        ccz[k] = nodes[c].cz;
+        parent[c] = p;
+        c = max(nodes[c].skip, c + 1);
+        nc = k + 1;
       }
-      cxv = tx / M;
-      cyv = ty / M;
-      czv = tz / M;
+    }
+    double M = 0;
+    double cxv = 0, cyv = 0, czv = 0;
+    if (c >= end) {
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+        if (k < nc) M = M + cm[k];
+      if (M != 0) {  // else COM (0,0,0), src/octree.cpp:259-262
+        double tx = 0, ty = 0, tz = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+          if (k < nc) {
+            const double px = ccx[k] * cm[k];
+            const double py = ccy[k] * cm[k];
+            const double pz = ccz[k] * cm[k];
+            tx = tx + px;
+            ty = ty + py;
+            tz = tz + pz;
+          }
+        }
+        cxv = tx / M;
+        cyv = ty / M;
+        czv = tz / M;
+      }
+    } else {
+      // a depth-42 bucket with more than 8 particles: the same sums over
+      // the whole chain
+      for (c = p + 1; c < end; c = max(nodes[c].skip, c + 1)) {
+        M = M + nodes[c].m;
+        parent[c] = p;
+      }
+      if (M != 0) {
+        double tx = 0, ty = 0, tz = 0;
+        for (c = p + 1; c < end; c = max(nodes[c].skip, c + 1)) {
+          const double cmv = nodes[c].m;
+          const double px = nodes[c].cx * cmv;
+          const double py = nodes[c].cy * cmv;
+          const double pz = nodes[c].cz * cmv;
+          tx = tx + px;
+          ty = ty + py;
+          tz = tz + pz;
+        }
+        cxv = tx / M;
+        cyv = ty / M;
+        czv = tz / M;
+      }
     }
     nodes[p].cx = cxv;
     nodes[p].cy = cyv;
@@ -465,6 +508,11 @@ __global__ void __launch_bounds__(kBlock)
     nodes[p].gm32 = (float)(kG * M);
   }
 }
+
+// Grid of every level launch (grid-stride over the level's cells).  1024
+// blocks shortened the widest levels (35 -> 27 us at N = 2^20) but made
+// each of the ~30 near-empty levels cost 4.4 us instead of ~1.8 us.
+constexpr int kLevelBlocks = 256;
 
 inline int blocks_for(size_t n, int b) { return (int)((n + b - 1) / b); }
 
@@ -553,8 +601,8 @@ void launch_bottom_up(BuildBuffers& b, WalkNode* nodes, TreeAux& aux, int n,
   const int nb = blocks_for(n, kBlock);
   const int top = max_depth < kLevels ? max_depth : kLevels;
   for (int d = top; d >= 0; d--)
-    level_kernel<<<256, kBlock, 0, s>>>(d, nb, b.doff, b.bucket, nodes,
-                                        aux.parent, b.flags);
+    level_kernel<<<kLevelBlocks, kBlock, 0, s>>>(d, nb, b.doff, b.bucket, nodes,
+                                                 aux.parent, b.flags);
 }
 
 void launch_leaf_refresh(BuildBuffers& b, const State& st, WalkNode* nodes,
