@@ -140,6 +140,224 @@ def copy_calibration(mib, reps):
                                            "launches": reps}}), flush=True)
 
 
+# ----------------------------------------------------------------------------
+# The other workloads of the step, each printing its own JSON line
+# (`--config 3|4|5`); the default run above is the headline.
+
+SPHERE = dict(r_sphere=100.0, rotation_magnitude=0.1, rotation_vector=(0.0, 0.0, 1.0),
+              min_mass=5e10, max_mass=5e11)
+
+
+def timed_steps(sim, steps, warmup):
+    if warmup:
+        sim.step(warmup, graph=True)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.step(steps, graph=True)
+    sim.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps
+
+
+def config_shell(args):
+    """Shell of N = 4,194,304 at r = 100 with 1 % radial jitter, rotation
+    0.1 about z, FP32 forces, adaptive dt (min 0.00333, max position /
+    velocity change 3), theta 0.3 / 0.5 / 0.7, 50 steps each from the same
+    initial state."""
+    import nbodysim_amd
+    from nbodysim_amd import host
+    n = args.n3
+    pos, vel, mass = host.generate(n, gen_type="shell", seed=0xDEADBEEF, shell_jitter=0.01,
+                                   **SPHERE)
+    res = []
+    for theta in (0.3, 0.5, 0.7):
+        sim = nbodysim_amd.Simulation(theta=theta, dt=0.0333, precision="f32", adaptive=True,
+                                      min_adaptive_dt=0.00333, max_pos_change=3.0,
+                                      max_vel_change=3.0)
+        sim.set_state(pos, vel, mass)
+        sim.set_walk_shape(64, 8, 1)
+        ms = timed_steps(sim, args.steps3, 2)
+        t_sim = float(sim.time()[0])
+        sim.interactions_total(reset=True)
+        sim.profile(True)
+        sim.profile_read(reset=True)
+        sim.step(2)
+        prof = sim.profile_read(reset=True)
+        inter = sim.interactions_total(reset=True) / 2
+        walk_ms = prof["walk_ms"] / max(prof["walk_launches"], 1)
+        res.append({"theta": theta, "value": n * 1e3 / ms, "ms_per_step": ms,
+                    "simulated_time": t_sim, "steps": args.steps3 + 2,
+                    "interactions_per_particle": inter / n, "walk_ms": walk_ms,
+                    "build_ms": prof["build_ms"] / 2,
+                    "walk_tflops": FLOP_PER_INTERACTION * inter / (walk_ms * 1e-3) / 1e12})
+        log("shell theta", theta, res[-1])
+        sim.close()
+    print(json.dumps({
+        "metric": "particle-steps/s, shell N=%d, FP32 forces, adaptive dt, theta sweep" % n,
+        "unit": "particle-steps/s", "higher_is_better": True, "n_gpus": 1, "dtype": "f32",
+        "data": "synthetic (seeded gen_type shell, 1% radial jitter)",
+        "config": {"workload": "gen_type shell N=%d r=100 jitter 1%% rotation 0.1 about z, "
+                               "masses U[5e10,5e11], adaptive dt (dt 0.0333, min 0.00333, "
+                               "max_pos_change 3, max_vel_change 3), %d timed steps per theta"
+                               % (n, args.steps3)},
+        "results": res}), flush=True)
+
+
+def batched_systems(count, seed=0xDEADBEEF):
+    """`count` systems: N_s = round(exp(U(ln 1024, ln 65536))), gen_type
+    uniform over cube / sphere / shell (numpy default_rng(seed) draws N_s
+    and the type), system s generated with seed + s.  Cubes: side 512 at
+    rest; spheres and shells: r = 100, rotation 0.1 about z."""
+    from nbodysim_amd import host
+    rng = np.random.default_rng(seed)
+    sizes = np.rint(np.exp(rng.uniform(np.log(1024), np.log(65536), count))).astype(int)
+    kinds = rng.integers(0, 3, count)
+    parts = []
+    for s_, (n_s, k) in enumerate(zip(sizes, kinds)):
+        g = ("cube", "sphere", "shell")[k]
+        kw = dict(size=512.0, min_mass=5e10, max_mass=5e11) if g == "cube" else SPHERE
+        parts.append(host.generate(int(n_s), gen_type=g, seed=seed + s_, **kw))
+    return parts
+
+
+def config_batched(args):
+    """256 systems stepped as one batched context (systems keyed into one
+    sort, one walk launch per step, HIP-graph replay, autotuned walk shape)
+    against one context and one launch sequence per system."""
+    import nbodysim_amd
+    parts = batched_systems(args.systems)
+    pos = np.concatenate([p[0] for p in parts])
+    vel = np.concatenate([p[1] for p in parts])
+    mass = np.concatenate([p[2] for p in parts])
+    off = np.cumsum([0] + [len(p[2]) for p in parts])
+    n = int(off[-1])
+    sim = nbodysim_amd.Simulation(theta=0.5, dt=0.0333, precision="f32")
+    sim.set_state(pos, vel, mass)
+    sim.set_systems(off)
+    shape, _ = sim.autotune(reps=3)
+    ms_b = timed_steps(sim, args.steps4, 2)
+    sim.close()
+    log("batched", shape, ms_b)
+    # one context per system, each system's steps launched on its own
+    sims = []
+    for p in parts:
+        t = nbodysim_amd.Simulation(theta=0.5, dt=0.0333, precision="f32")
+        t.set_state(*p)
+        t.set_walk_shape(*shape)
+        t.step(2)
+        sims.append(t)
+    for t in sims:
+        t.synchronize()
+    steps_s = max(2, args.steps4 // 10)
+    t0 = time.perf_counter()
+    for _ in range(steps_s):
+        for t in sims:
+            t.step(1)
+    for t in sims:
+        t.synchronize()
+    ms_s = (time.perf_counter() - t0) * 1e3 / steps_s
+    for t in sims:
+        t.close()
+    log("per-system", ms_s)
+    print(json.dumps({
+        "metric": "particle-steps/s, %d batched systems (total N=%d)" % (len(parts), n),
+        "value": n * 1e3 / ms_b, "unit": "particle-steps/s", "higher_is_better": True,
+        "n_gpus": 1, "dtype": "f32", "ms_per_step": ms_b, "steps": args.steps4,
+        "data": "synthetic (seeded per-system cube/sphere/shell, the reference's generator)",
+        "config": {"workload": "%d systems, N_s log-uniform in [1024, 65536], theta 0.5, "
+                               "dt 0.0333, FP32 forces" % len(parts),
+                   "walk_shape": list(shape), "launch": "graph, one batched context"},
+        "per_system_launches": {"value": n * 1e3 / ms_s, "ms_per_step": ms_s,
+                                "steps": steps_s},
+        "speedup_batched": ms_s / ms_b}), flush=True)
+
+
+def config_adjoint(args):
+    """Reverse mode on the N = 65536 sphere: loss = mean |x_T - R_z(30) x_0|^2
+    over 200 steps with checkpoints every 20; gradients w.r.t. x_0, v_0, m.
+    Finite-difference checks (FP64, frozen tree topology and cell
+    decisions) on 64 seeded coordinates at the horizon where central
+    differences converge, plus the convergence study at 200 steps; and the
+    tree gradient against the direct-sum (theta = 0) gradient at N = 2048."""
+    import nbodysim_amd
+    from nbodysim_amd import host
+    c30, s30 = np.cos(np.pi / 6), np.sin(np.pi / 6)
+    rot = np.array([[c30, -s30, 0.0], [s30, c30, 0.0], [0.0, 0.0, 1.0]])
+    n = args.n5
+    pos, vel, mass = host.generate(n, gen_type="sphere", seed=0xDEADBEEF, **SPHERE)
+    tgt = pos @ rot.T
+    out = {"metric": "adjoint of %d steps, sphere N=%d, checkpoints every %d" %
+                     (args.steps5, n, args.every5),
+           "unit": "ms", "higher_is_better": False, "n_gpus": 1, "dtype": "f64",
+           "data": "synthetic (seeded gen_type sphere)"}
+    # timing: FP32 forces, FP64 state and adjoint
+    with nbodysim_amd.Simulation(theta=0.5, dt=0.0333, precision="f32") as sim:
+        sim.set_state(pos, vel, mass)
+        t0 = time.perf_counter()
+        sim.adjoint_forward(args.steps5, args.every5)
+        xT, _, _ = sim.state()
+        t1 = time.perf_counter()
+        gx, gv, gm = sim.adjoint_backward(2.0 * (xT - tgt) / n, None)
+        t2 = time.perf_counter()
+    out.update({"forward_ms": (t1 - t0) * 1e3, "backward_ms": (t2 - t1) * 1e3,
+                "loss": float(np.mean(np.sum((xT - tgt) ** 2, axis=1))),
+                "grad_norms": {"x0": float(np.linalg.norm(gx)), "v0": float(np.linalg.norm(gv)),
+                               "m": float(np.linalg.norm(gm))}})
+    log("adjoint timing", out)
+
+    def fd_check(steps, ncoord, hs):
+        with nbodysim_amd.Simulation(theta=0.5, dt=0.0333, precision="f64") as sim:
+            sim.set_state(pos, vel, mass)
+            sim.adjoint_forward(steps, args.every5, record_all=True)
+            xT, _, _ = sim.state()
+            gx, gv, gm = sim.adjoint_backward(2.0 * (xT - tgt) / n, None)
+            rng = np.random.default_rng(0xDEADBEEF)
+            rows = []
+            for k in range(ncoord):
+                kind = ("x", "v", "m")[k % 3]
+                i, c = int(rng.integers(n)), int(rng.integers(3))
+                adj = {"x": gx[i, c], "v": gv[i, c], "m": gm[i]}[kind]
+                fds = []
+                for h0 in hs:
+                    p, v, m = pos.copy(), vel.copy(), mass.copy()
+                    h = h0 * (mass[i] if kind == "m" else 1.0)
+                    arr, idx = {"x": (p, (i, c)), "v": (v, (i, c)), "m": (m, i)}[kind]
+                    arr[idx] += h
+                    xp = sim.frozen_forward(p, v, m)[0]
+                    arr[idx] -= 2 * h
+                    xm = sim.frozen_forward(p, v, m)[0]
+                    fds.append(float(np.mean(np.sum((xp - xm) * (xp + xm - 2 * tgt), axis=1)))
+                               / (2 * h))
+                rows.append((kind, float(adj), fds))
+            return rows
+
+    rows = fd_check(args.fd_steps5, 64, (1e-5,))
+    errs = [abs(a - f[0]) / (abs(f[0]) + 1e-6 * max(abs(r[2][0]) for r in rows))
+            for _, a, f in rows]
+    out["fd_check"] = {"steps": args.fd_steps5, "coords": 64, "h": 1e-5,
+                       "max_rel_err": float(max(errs)), "median_rel_err": float(np.median(errs))}
+    log("fd check", out["fd_check"])
+    conv = fd_check(args.steps5, 3, (1e-5, 2.5e-6, 6.25e-7))
+    out["fd_convergence_at_horizon"] = {
+        "steps": args.steps5, "h": [1e-5, 2.5e-6, 6.25e-7],
+        "coords": [{"kind": k, "adjoint": a, "fd": f} for k, a, f in conv]}
+    log("fd convergence", out["fd_convergence_at_horizon"])
+    # tree gradient vs direct-sum gradient at reduced N
+    p2, v2, m2 = host.generate(2048, gen_type="sphere", seed=0xDEADBEEF, **SPHERE)
+    t2g = p2 @ rot.T
+    g = []
+    for theta in (0.5, 0.0):
+        with nbodysim_amd.Simulation(theta=theta, dt=0.0333, precision="f64") as sim:
+            sim.set_state(p2, v2, m2)
+            sim.adjoint_forward(10, 5)
+            x2, _, _ = sim.state()
+            g.append(sim.adjoint_backward(2.0 * (x2 - t2g) / len(p2), None))
+    out["direct_sum_check"] = {"n": 2048, "steps": 10, "rel_diff": {
+        k: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+        for k, a, b in zip(("x0", "v0", "m"), g[0], g[1])}}
+    out["value"] = out["forward_ms"] + out["backward_ms"]
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -161,6 +379,17 @@ def main():
                     help="timed steps launched one kernel at a time, not by graph replay")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="only run the counter yardstick: copy MIB MiB 5 times")
+    ap.add_argument("--config", type=int, default=0, choices=(0, 3, 4, 5),
+                    help="run one of the other workloads instead: 3 shell theta sweep, "
+                         "4 batched systems, 5 adjoint")
+    ap.add_argument("--n3", type=int, default=1 << 22)
+    ap.add_argument("--steps3", type=int, default=50)
+    ap.add_argument("--systems", type=int, default=256)
+    ap.add_argument("--steps4", type=int, default=200)
+    ap.add_argument("--n5", type=int, default=65536)
+    ap.add_argument("--steps5", type=int, default=200)
+    ap.add_argument("--every5", type=int, default=20)
+    ap.add_argument("--fd-steps5", type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -168,6 +397,9 @@ def main():
         sys.exit(2)
     if args.copy_calib:
         copy_calibration(args.copy_calib, 5)
+        return
+    if args.config:
+        {3: config_shell, 4: config_batched, 5: config_adjoint}[args.config](args)
         return
 
     import nbodysim_amd

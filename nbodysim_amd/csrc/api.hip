@@ -299,6 +299,7 @@ int alloc_particles(nbs_ctx* c, int n) {
   b.counts = dalloc<int>(o, cap, &e);
   b.offsets = dalloc<int>(o, (size_t)cap + 1, &e);
   b.sort_tmp = dalloc<int>(o, radix_tmp_ints(cap), &e);
+  b.runs = dalloc<int>(o, 2 + 2 * ((size_t)cap / 2 + 1), &e);
   b.dhist = dalloc<int>(o, depth_hist_ints(cap), &e);
   b.doff = dalloc<int>(o, depth_hist_ints(cap), &e);
   b.scan_tmp = dalloc<int>(
@@ -431,7 +432,7 @@ void enqueue_build(nbs_ctx* c) {
                                      c->bb.val1, c->n, nbits, c->bb.sort_tmp, s);
   unsigned long long* prim = which ? c->bb.prim1 : c->bb.prim0;
   unsigned* perm = which ? c->bb.val1 : c->bb.val0;
-  launch_fixup(c->bb, prim, perm, cur, c->n, s);
+  launch_fixup(c->bb, prim, perm, which ? c->bb.val0 : c->bb.val1, cur, c->n, s);
   launch_permute(cur, nxt, perm, c->n, s);
   c->cur ^= 1;
   launch_emit(c->bb, c->sys, c->p, c->st[c->cur], c->nodes, c->aux, c->n,

@@ -110,6 +110,7 @@ struct BuildBuffers {
   int* bucket;                        // [node cap] internal cells by depth
   int* sort_tmp;
   int* scan_tmp;
+  int* runs;                          // long runs of equal primary keys
   int* flags;                         // device flags word
   int* stats;                         // [4]: nodes, internal, maxdepth, buckets
 };
@@ -127,9 +128,11 @@ int launch_keys(const State& st, Systems& sys, BuildBuffers& b, int prim_levels,
 // within 64 bits, at most 21 levels).
 int max_prim_levels(int nsys);
 // Gathers the full keys into sorted order and sorts runs of equal primary
-// keys by (full key, caller index).  perm is updated in place.
+// keys by (full key, caller index).  perm is updated in place; free_perm
+// (the sort's other value array) is scratch.
 void launch_fixup(BuildBuffers& b, const unsigned long long* prim,
-                  unsigned* perm, const State& st, int n, hipStream_t s);
+                  unsigned* perm, unsigned* free_perm, const State& st, int n,
+                  hipStream_t s);
 void launch_permute(const State& src, State& dst, const unsigned* perm,
                     int n, hipStream_t s);
 // Emits the pre-order node array (capacity `cap`), leaves filled from `st`.

@@ -181,23 +181,51 @@ __global__ void gather_keys_kernel(const unsigned* __restrict__ perm, int n,
   l[j] = lu[p];
 }
 
-// Runs of equal primary keys (rare: particles closer than the primary
-// key's resolution) are sorted by (full key, caller index), one thread per
-// run, insertion sort in place.
+// Runs of equal primary keys (particles closer than the primary key's
+// resolution, common only while a cold system collapses) are sorted by
+// (full key, caller index).  One thread per run start: a run of up to
+// kShortRun entries is insertion-sorted in place; a longer one is listed
+// for long_run_kernel.  runs[0] counts listed runs, runs[1] is
+// long_run_kernel's work counter, runs[2 + 2r], runs[3 + 2r] = start,
+// length of run r.
+constexpr int kShortRun = 32;
+
 __global__ void fixup_kernel(const unsigned long long* __restrict__ prim,
                              unsigned* __restrict__ perm,
                              unsigned long long* __restrict__ khi,
                              unsigned long long* __restrict__ klo,
                              const unsigned* __restrict__ id, int n,
-                             int* __restrict__ flags) {
+                             int* __restrict__ runs) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j + 1 >= n) return;
-  if (prim[j] != prim[j + 1]) return;
-  if (j > 0 && prim[j - 1] == prim[j]) return;
-  int e = j + 1;
-  while (e + 1 < n && prim[e + 1] == prim[j]) e++;
-  if (e - j + 1 > 8192) {
-    atomicOr(flags, kFlagRunOverflow);
+  const unsigned long long pk = prim[j];
+  if (pk != prim[j + 1]) return;
+  if (j > 0 && prim[j - 1] == pk) return;
+  // last index of the run: galloping, then bisection (prim is sorted)
+  int good = j + 1, bad = n, step = 1;
+  for (;;) {
+    const int probe = good + step;
+    if (probe >= n) break;
+    if (prim[probe] == pk) {
+      good = probe;
+      step <<= 1;
+    } else {
+      bad = probe;
+      break;
+    }
+  }
+  while (bad - good > 1) {
+    const int mid = good + ((bad - good) >> 1);
+    if (prim[mid] == pk)
+      good = mid;
+    else
+      bad = mid;
+  }
+  const int e = good;
+  if (e - j + 1 > kShortRun) {
+    const int r = atomicAdd(&runs[0], 1);
+    runs[2 + 2 * r] = j;
+    runs[3 + 2 * r] = e - j + 1;
     return;
   }
   for (int a = j + 1; a <= e; a++) {
@@ -218,6 +246,118 @@ __global__ void fixup_kernel(const unsigned long long* __restrict__ prim,
     khi[b + 1] = h;
     klo[b + 1] = l;
     perm[b + 1] = p;
+  }
+}
+
+// A listed run sorted by (key_hi, key_lo, caller index) by one workgroup:
+// stable LSD radix sort, 8-bit digits, least significant first (caller
+// index bytes 0-3, key_lo bytes 0-7, key_hi bytes 0-7), skipping digits
+// that are equal over the whole run; entries ping-pong between the run's
+// range of (khi, klo, perm) and the same range of three free arrays.
+// Each chunk of 1024 entries is ranked stably with ballots (equal digits
+// within a wave) and per-wave digit counts.
+constexpr int kRunBlock = 1024;
+constexpr int kRunWaves = kRunBlock / 64;
+constexpr int kRunGrid = 64;
+
+__global__ void __launch_bounds__(kRunBlock)
+    long_run_kernel(int* __restrict__ runs, unsigned long long* __restrict__ khi,
+                    unsigned long long* __restrict__ klo, unsigned* __restrict__ perm,
+                    unsigned long long* __restrict__ shi, unsigned long long* __restrict__ slo,
+                    unsigned* __restrict__ sperm, const unsigned* __restrict__ id) {
+  __shared__ int hist[256];
+  __shared__ int wcnt[kRunWaves][256];
+  __shared__ int total[256];
+  __shared__ int run_sh;
+  __shared__ int skip_sh;
+  const int t = threadIdx.x, lane = lane_id(), w = t >> 6;
+  for (;;) {
+    if (t == 0) run_sh = atomicAdd(&runs[1], 1);
+    __syncthreads();
+    const int r = run_sh;
+    __syncthreads();
+    if (r >= runs[0]) return;
+    const int beg = runs[2 + 2 * r], len = runs[3 + 2 * r];
+    unsigned long long *h0 = khi + beg, *l0 = klo + beg, *h1 = shi + beg, *l1 = slo + beg;
+    unsigned *p0 = perm + beg, *p1 = sperm + beg;
+    for (int pass = 0; pass < 20; pass++) {
+      // digit of entry e of the current source arrays
+      auto digit = [&](int e) -> int {
+        if (pass < 4) return (id[p0[e]] >> (8 * pass)) & 255;
+        if (pass < 12) return (int)((l0[e] >> (8 * (pass - 4))) & 255);
+        return (int)((h0[e] >> (8 * (pass - 12))) & 255);
+      };
+      if (t < 256) hist[t] = 0;
+      if (t == 0) skip_sh = 0;
+      __syncthreads();
+      for (int e = t; e < len; e += kRunBlock) atomicAdd(&hist[digit(e)], 1);
+      __syncthreads();
+      if (t < 256 && hist[t] == len) skip_sh = 1;  // one digit value: order unchanged
+      __syncthreads();
+      if (skip_sh) continue;
+      if (w == 0) {  // exclusive scan of the 256 bins, 4 per lane
+        int v[4], s4 = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          v[k] = hist[lane * 4 + k];
+          s4 += v[k];
+        }
+        int run = wave_inclusive_scan(s4) - s4;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          hist[lane * 4 + k] = run;
+          run += v[k];
+        }
+      }
+      __syncthreads();
+      for (int c0 = 0; c0 < len; c0 += kRunBlock) {
+        const int e = c0 + t;
+        const bool v = e < len;
+        const int d = v ? digit(e) : 0;
+        unsigned long long same = ballot(v);
+#pragma unroll
+        for (int bit = 0; bit < 8; bit++) {
+          const unsigned long long m = ballot(v && ((d >> bit) & 1));
+          same &= ((d >> bit) & 1) ? m : ~m;
+        }
+        const int rank = mbcnt(same);
+        for (int k = lane; k < 256; k += 64) wcnt[w][k] = 0;
+        __syncthreads();
+        if (v && rank == 0) wcnt[w][d] = __popcll(same);
+        __syncthreads();
+        if (t < 256) {  // prefix over the waves, per digit
+          int acc = 0;
+          for (int k = 0; k < kRunWaves; k++) {
+            const int c = wcnt[k][t];
+            wcnt[k][t] = acc;
+            acc += c;
+          }
+          total[t] = acc;
+        }
+        __syncthreads();
+        if (v) {
+          const int dst = hist[d] + wcnt[w][d] + rank;
+          h1[dst] = h0[e];
+          l1[dst] = l0[e];
+          p1[dst] = p0[e];
+        }
+        __syncthreads();
+        if (t < 256) hist[t] += total[t];
+        __syncthreads();
+      }
+      unsigned long long* th = h0; h0 = h1; h1 = th;
+      unsigned long long* tl = l0; l0 = l1; l1 = tl;
+      unsigned* tp = p0; p0 = p1; p1 = tp;
+      __syncthreads();
+    }
+    if (h0 != khi + beg) {  // result in the free arrays: copy back
+      for (int e = t; e < len; e += kRunBlock) {
+        khi[beg + e] = h0[e];
+        klo[beg + e] = l0[e];
+        perm[beg + e] = p0[e];
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -563,13 +703,19 @@ int launch_keys(const State& st, Systems& sys, BuildBuffers& b, int prim_levels,
 }
 
 void launch_fixup(BuildBuffers& b, const unsigned long long* prim,
-                  unsigned* perm, const State& st, int n, hipStream_t s) {
+                  unsigned* perm, unsigned* free_perm, const State& st, int n,
+                  hipStream_t s) {
   if (n <= 0) return;
   gather_keys_kernel<<<blocks_for(n, kBlock), kBlock, 0, s>>>(
       perm, n, b.khi_u, b.klo_u, b.khi, b.klo);
+  launch_fill_int(b.runs, 0, 2, s);
   fixup_kernel<<<blocks_for(n, kBlock), kBlock, 0, s>>>(prim, perm, b.khi,
                                                         b.klo, st.id, n,
-                                                        b.flags);
+                                                        b.runs);
+  // khi_u / klo_u (already gathered) and the sort's other value array are
+  // free until the next build: scratch for the long runs
+  long_run_kernel<<<kRunGrid, kRunBlock, 0, s>>>(b.runs, b.khi, b.klo, perm, b.khi_u,
+                                                 b.klo_u, free_perm, st.id);
 }
 
 void launch_permute(const State& src, State& dst, const unsigned* perm, int n,

@@ -387,3 +387,32 @@ def test_dense_cluster(graph):
         assert_tree_equal(s.tree_dump(), O.Tree(pos, mass).dump())
     oacc, _ = O.Tree(pos, mass).forces(theta=0.5)
     assert rel_err(acc, oacc).max() < 1e-12
+
+
+def test_long_runs_of_equal_primary_keys():
+    """15000 particles inside one 21-level cell (a 1e-5 cube in a 1000-wide
+    cloud) and 40 inside another: runs of equal primary sort keys longer
+    than the in-place insertion sort takes, sorted on the full key by the
+    workgroup radix sort; tree and forces as the reference's."""
+    rng = np.random.default_rng(6)
+    far = rng.uniform(-500, 500, size=(5000, 3))
+    big = -7.0 + rng.uniform(0, 1e-5, size=(15000, 3))
+    small = 11.0 + rng.uniform(0, 1e-6, size=(40, 3))
+    pos = np.concatenate([far, big, small])
+    perm = rng.permutation(len(pos))
+    pos = pos[perm]
+    mass = rng.uniform(1e9, 1e10, size=len(pos))
+    vel = np.zeros_like(pos)
+    with sim(theta=0.5) as s:
+        s.set_state(pos, vel, mass)
+        acc = s.forces()
+        assert_tree_equal(s.tree_dump(), O.Tree(pos, mass).dump())
+        s.step(2, graph=True)
+        got = s.state()
+    oacc, _ = O.Tree(pos, mass).forces(theta=0.5)
+    assert rel_err(acc, oacc).max() < 1e-12
+    with sim(theta=0.5) as s:
+        s.set_state(pos, vel, mass)
+        s.step(2)
+        for a, b in zip(s.state()[:2], got[:2]):
+            np.testing.assert_array_equal(a, b)
