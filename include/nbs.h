@@ -149,6 +149,14 @@ int nbs_set_walk_shape(nbs_ctx* ctx, int block, int groups, int pack);
  * freshly built tree then misses more in L2. */
 int nbs_set_walk_order(nbs_ctx* ctx, int on);
 
+/* Which instruction stream runs the FP32 walk (no counterpart in the
+ * reference; for tests and diagnostics): 0 (default) = the hand-written
+ * gfx950 loop where it applies (FP32 forces, no damping, one particle per
+ * lane), 1 = the compiler-generated loop everywhere.  Both make the same
+ * decisions and the same arithmetic in the same order: changes the speed
+ * only, never a result bit. */
+int nbs_set_walk_code(nbs_ctx* ctx, int generic);
+
 /* Blocks until the context's stream has drained. */
 int nbs_synchronize(nbs_ctx* ctx);
 

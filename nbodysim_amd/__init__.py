@@ -170,6 +170,11 @@ class Simulation:
         """Longest-first workgroup dispatch of the walk (default off)."""
         self._check(self._L.nbs_set_walk_order(self._h, int(bool(on))))
 
+    def set_walk_code(self, generic=False):
+        """FP32 walk loop: hand-written gfx950 code (default) or the
+        compiler-generated loop (generic=True); results are bit-identical."""
+        self._check(self._L.nbs_set_walk_code(self._h, int(bool(generic))))
+
     def synchronize(self):
         self._check(self._L.nbs_synchronize(self._h))
 

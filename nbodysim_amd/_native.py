@@ -75,6 +75,7 @@ SIGNATURES = [
     ("nbs_set_walk_shape", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                           ctypes.c_int]),
     ("nbs_set_walk_order", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("nbs_set_walk_code", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("nbs_synchronize", ctypes.c_int, [ctypes.c_void_p]),
     ("nbs_direct", ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, c_i64p, c_dp]),
     ("nbs_interactions", ctypes.c_int, [ctypes.c_void_p, c_i64p, c_i64p]),

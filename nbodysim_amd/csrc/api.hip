@@ -183,6 +183,8 @@ struct nbs_ctx {
   // longest-first workgroup order: per-tile visit counts of the last walk
   // (recorded under cost_block / cost_pack) and the order derived from them
   int walk_order = 0;
+  // 1: the compiler-generated walk loop even where the hand-written one applies
+  int walk_generic = 0;
   unsigned* tile_cost = nullptr;
   int* tile_order = nullptr;
   int cost_block = -1, cost_pack = -1, cost_tiles = -1;
@@ -488,6 +490,8 @@ void enqueue_walk(nbs_ctx* c, int fused) {
   a.inter_total = c->inter_total;
   a.fused_update = fused;
   a.groups = c->walk_groups;
+  // the hand-written loop addresses node records with 32-bit byte offsets
+  a.asm_loop = !c->walk_generic && (size_t)c->node_cap * sizeof(WalkNode) < (1ull << 32);
   if (c->walk_order) {
     const int tiles = sync_tile_costs(c);
     launch_tile_order(c->tile_cost, tiles, c->walk_groups, c->tile_order, c->stream);
@@ -871,6 +875,13 @@ int nbs_set_walk_shape(nbs_ctx* c, int block, int groups, int pack) {
   c->walk_block = block;
   c->walk_groups = groups;
   c->walk_pack = pack;
+  return NBS_OK;
+}
+
+int nbs_set_walk_code(nbs_ctx* c, int generic) {
+  if (!c) return NBS_ERR_ARG;
+  drop_graph(c);
+  c->walk_generic = generic ? 1 : 0;
   return NBS_OK;
 }
 

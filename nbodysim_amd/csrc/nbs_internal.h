@@ -171,6 +171,7 @@ struct WalkArgs {
   unsigned long long* inter_total;  // running sum of interactions
   int fused_update;
   int groups;  // workgroup -> tile interleave over the XCDs (1 = plain)
+  int asm_loop;  // FP32 walk: the hand-written loop where it applies
   const int* order;     // workgroup -> tile (longest first), or null
   unsigned* tile_cost;  // per tile: node visits of this walk added, or null
   // frozen decisions (FP64 only; null otherwise): base tree + positions

@@ -54,26 +54,28 @@ struct NodeRegs {
 };
 
 typedef __attribute__((address_space(4))) const unsigned long long* ConstQwords;
+typedef __attribute__((address_space(4))) const unsigned* ConstDwords;
 
 // 64-bit words of the record: 0-2 cx cy cz, 3 (skip, leaf), 4 gm, 5 side,
-// 6 (gm32, r2open32).
+// 6 (gm32, r2open32).  skip and leaf are read as 32-bit words so their
+// tests stay on the scalar unit.
 __device__ __forceinline__ NodeRegs load_node(ConstNodes nodes, int k) {
-  static_assert(offsetof(WalkNode, skip) == 24 && offsetof(WalkNode, gm) == 32 &&
-                    offsetof(WalkNode, side) == 40 && offsetof(WalkNode, gm32) == 48,
+  static_assert(offsetof(WalkNode, skip) == 24 && offsetof(WalkNode, leaf) == 28 &&
+                    offsetof(WalkNode, gm) == 32 && offsetof(WalkNode, side) == 40 &&
+                    offsetof(WalkNode, gm32) == 48 && offsetof(WalkNode, r2open32) == 52,
                 "load_node follows the WalkNode layout");
   const ConstQwords q = (ConstQwords)(nodes + k);
+  const ConstDwords d = (ConstDwords)(nodes + k);
   NodeRegs r;
   r.cx = __builtin_bit_cast(double, q[0]);
   r.cy = __builtin_bit_cast(double, q[1]);
   r.cz = __builtin_bit_cast(double, q[2]);
-  const unsigned long long link = q[3];
-  r.skip = (int)(unsigned)link;
-  r.leaf = (int)(unsigned)(link >> 32);
+  r.skip = (int)d[6];
+  r.leaf = (int)d[7];
   r.gm = __builtin_bit_cast(double, q[4]);
   r.side = __builtin_bit_cast(double, q[5]);
-  const unsigned long long mac = q[6];
-  r.gm32 = __builtin_bit_cast(float, (unsigned)mac);
-  r.r2open32 = __builtin_bit_cast(float, (unsigned)(mac >> 32));
+  r.gm32 = __builtin_bit_cast(float, d[12]);
+  r.r2open32 = __builtin_bit_cast(float, d[13]);
   return r;
 }
 
@@ -124,14 +126,14 @@ __device__ __forceinline__ void kick_drift(double& x, double& v, double a,
 }
 
 // One accepted body-node interaction of one lane: a += G m d / |d|^3
-// (src/main.cpp:306-333), d = com - x formed in FP64.
+// (src/main.cpp:306-333), d = com - x formed in FP64 (and, for FP32 forces,
+// rounded to FP32 with its square already formed by the caller).
 template <bool F32, bool DAMP, typename AccT>
 __device__ __forceinline__ void interact(const NodeRegs& nd, double dx, double dy,
-                                         double dz, AccT& ax, AccT& ay, AccT& az,
+                                         double dz, float fdx, float fdy, float fdz,
+                                         float fd2, AccT& ax, AccT& ay, AccT& az,
                                          int& cnt) {
   if (F32) {
-    const float fdx = (float)dx, fdy = (float)dy, fdz = (float)dz;
-    const float fd2 = fmaf(fdx, fdx, fmaf(fdy, fdy, fdz * fdz));
     const float r = __builtin_amdgcn_rsqf(fd2);
     float sc;
     if (DAMP) {
@@ -158,6 +160,156 @@ __device__ __forceinline__ void interact(const NodeRegs& nd, double dx, double d
     az = fma(sc, dz, (double)az);
   }
   cnt++;
+}
+
+// The FP32 walk loop (no damping, one particle per lane) written out for
+// gfx950.  Same traversal, lane decisions and arithmetic, in the same
+// order, as the generic loop in walk_kernel; what differs is the
+// instruction stream.  Per visit: 10 VALU instructions for d = com - x
+// (FP64, rounded to FP32), d^2 and the opening test, 8 for the force on
+// the interacting lanes, and scalar code for everything else.  The next
+// record is requested before the force arithmetic so the scalar-load
+// latency overlaps it.
+//
+// Fixed SGPRs (clobbered): s[64:79] the 64-byte node record (cx s[64:65],
+// cy s[66:67], cz s[68:69], skip s70, leaf s71, gm32 s76, r2open32 s77);
+// s[62:63] A (lanes taking the node), s[60:61] I (lanes interacting),
+// s[58:59] O (lanes opening), s[56:57] saved exec, s[54:55] a popped
+// stack mask.  The stack of waiting lanes is entry `slot` in lane `slot`
+// of sp (position), slo/shi (mask); lane 63 of sp stays -1, the position
+// an empty stack reports.  Every scalar load is waited for inside the
+// block.  M0 carries the lane index of stack writes (a VALU instruction
+// reads one SGPR at most); the block restores it.  Record offsets are 32-bit: callers keep node index * 64 below 2^32.
+__device__ __forceinline__ void walk_loop_f32(ConstNodes nodes, int n, int nend, int pbeg,
+                                              unsigned long long A, double x, double y,
+                                              double z, float& ax, float& ay, float& az,
+                                              int& cnt, unsigned& visits, int& overflow) {
+  int nx, top, tpos, t, gm, m0save;
+  int sp, slo, shi;
+  double dx, dy, dz;
+  float fx, fy, fz, d2, tv;
+  __asm__ volatile(
+      "s_mov_b32 %[m0save], m0\n\t"
+      "v_mov_b32 %[sp], -1\n\t"
+      "v_mov_b32 %[slo], 0\n\t"
+      "v_mov_b32 %[shi], 0\n\t"
+      "s_mov_b64 s[62:63], %[A0]\n\t"
+      "s_mov_b32 %[top], 0\n\t"
+      "s_mov_b32 %[tpos], -1\n\t"
+      "s_cmp_lt_i32 %[n], %[nend]\n\t"
+      "s_cbranch_scc0 L_done%=\n\t"
+      "s_lshl_b32 %[t], %[n], 6\n\t"
+      "s_load_dwordx16 s[64:79], %[base], %[t]\n"
+      "L_loop%=:\n\t"
+      "s_waitcnt lgkmcnt(0)\n\t"
+      "v_add_f64 %[dx], s[64:65], -%[x]\n\t"
+      "v_add_f64 %[dy], s[66:67], -%[y]\n\t"
+      "v_add_f64 %[dz], s[68:69], -%[z]\n\t"
+      "s_add_u32 %[visits], %[visits], 1\n\t"
+      "s_mov_b32 %[gm], s76\n\t"
+      "v_cvt_f32_f64 %[fx], %[dx]\n\t"
+      "v_cvt_f32_f64 %[fy], %[dy]\n\t"
+      "v_cvt_f32_f64 %[fz], %[dz]\n\t"
+      "s_add_i32 %[nx], %[n], 1\n\t"
+      "v_mul_f32 %[d2], %[fz], %[fz]\n\t"
+      "v_fmac_f32 %[d2], %[fy], %[fy]\n\t"
+      "v_fmac_f32 %[d2], %[fx], %[fx]\n\t"
+      // leaf >= 0: a leaf; -2: a depth-42 bucket; -1: a cell
+      "s_cmp_gt_i32 s71, -1\n\t"
+      "s_cbranch_scc1 L_leaf%=\n\t"
+      "s_cmp_eq_u32 s71, -2\n\t"
+      "s_cbranch_scc1 L_bucket%=\n\t"
+      // cell: open where d^2 < (side/theta)^2
+      "v_cmp_gt_f32 vcc, s77, %[d2]\n\t"
+      "s_and_b64 s[58:59], vcc, s[62:63]\n\t"
+      "s_cbranch_scc1 L_opened%=\n\t"
+      // every lane that takes it accepts it: all interact, jump the subtree
+      "s_mov_b64 s[60:61], s[62:63]\n\t"
+      "s_max_i32 %[nx], %[nx], s70\n\t"
+      "s_branch L_pop%=\n"
+      "L_leaf%=:\n\t"
+      // every lane that takes it interacts, except its own particle's
+      "s_mov_b64 s[60:61], s[62:63]\n\t"
+      "s_sub_u32 %[t], s71, %[pbeg]\n\t"
+      "s_cmp_lt_u32 %[t], 64\n\t"
+      "s_cbranch_scc0 L_pop%=\n\t"
+      "s_bitset0_b64 s[60:61], %[t]\n\t"
+      "s_branch L_pop%=\n"
+      "L_bucket%=:\n\t"
+      "s_mov_b64 s[60:61], 0\n\t"
+      "s_branch L_pop%=\n"
+      "L_opened%=:\n\t"
+      // I = A & ~O interact and wait at skip; O descend (A = O)
+      "s_andn2_b64 s[60:61], s[62:63], s[58:59]\n\t"
+      "s_mov_b64 s[62:63], s[58:59]\n\t"
+      "s_cbranch_scc0 L_pop%=\n\t"
+      "s_cmp_eq_u32 %[tpos], s70\n\t"
+      "s_cbranch_scc1 L_merge%=\n\t"
+      "s_cmp_gt_u32 %[top], 62\n\t"
+      "s_cbranch_scc1 L_ovf%=\n\t"
+      "s_mov_b32 m0, %[top]\n\t"
+      "v_writelane_b32 %[sp], s70, m0\n\t"
+      "v_writelane_b32 %[slo], s60, m0\n\t"
+      "v_writelane_b32 %[shi], s61, m0\n\t"
+      "s_add_u32 %[top], %[top], 1\n\t"
+      "s_mov_b32 %[tpos], s70\n\t"
+      "s_branch L_pop%=\n"
+      "L_merge%=:\n\t"
+      "s_sub_u32 m0, %[top], 1\n\t"
+      "v_readlane_b32 s54, %[slo], m0\n\t"
+      "v_readlane_b32 s55, %[shi], m0\n\t"
+      "s_or_b64 s[54:55], s[54:55], s[60:61]\n\t"
+      "v_writelane_b32 %[slo], s54, m0\n\t"
+      "v_writelane_b32 %[shi], s55, m0\n\t"
+      "s_branch L_pop%=\n"
+      "L_ovf%=:\n\t"
+      "s_mov_b32 %[ovf], 1\n"
+      "L_pop%=:\n\t"
+      // lanes waiting at the next node rejoin
+      "s_cmp_eq_u32 %[tpos], %[nx]\n\t"
+      "s_cbranch_scc0 L_fetch%=\n\t"
+      "s_sub_u32 %[top], %[top], 1\n\t"
+      "v_readlane_b32 s54, %[slo], %[top]\n\t"
+      "v_readlane_b32 s55, %[shi], %[top]\n\t"
+      "s_add_u32 %[t], %[top], 63\n\t"
+      "s_and_b32 %[t], %[t], 63\n\t"
+      "v_readlane_b32 %[tpos], %[sp], %[t]\n\t"
+      "s_or_b64 s[62:63], s[62:63], s[54:55]\n"
+      "L_fetch%=:\n\t"
+      // request the next record (the current one if the walk ends)
+      "s_cmp_lt_i32 %[nx], %[nend]\n\t"
+      "s_cselect_b32 %[t], %[nx], %[n]\n\t"
+      "s_lshl_b32 %[t], %[t], 6\n\t"
+      "s_load_dwordx16 s[64:79], %[base], %[t]\n\t"
+      // force on the interacting lanes: a += G m d / |d|^3
+      "s_and_saveexec_b64 s[56:57], s[60:61]\n\t"
+      "s_cbranch_execz L_skip%=\n\t"
+      "v_rsq_f32 %[tv], %[d2]\n\t"
+      "v_add_u32 %[cnt], 1, %[cnt]\n\t"
+      "v_mul_f32 %[d2], %[tv], %[tv]\n\t"
+      "v_mul_f32 %[d2], %[d2], %[tv]\n\t"
+      "v_mul_f32 %[d2], %[gm], %[d2]\n\t"
+      "v_fmac_f32 %[ax], %[d2], %[fx]\n\t"
+      "v_fmac_f32 %[ay], %[d2], %[fy]\n\t"
+      "v_fmac_f32 %[az], %[d2], %[fz]\n"
+      "L_skip%=:\n\t"
+      "s_mov_b64 exec, s[56:57]\n\t"
+      "s_mov_b32 %[n], %[nx]\n\t"
+      "s_cmp_lt_i32 %[n], %[nend]\n\t"
+      "s_cbranch_scc1 L_loop%=\n\t"
+      "s_waitcnt lgkmcnt(0)\n"
+      "L_done%=:\n\t"
+      "s_mov_b32 m0, %[m0save]"
+      : [n] "+s"(n), [visits] "+s"(visits), [ovf] "+s"(overflow), [ax] "+v"(ax),
+        [ay] "+v"(ay), [az] "+v"(az), [cnt] "+v"(cnt), [nx] "=&s"(nx), [top] "=&s"(top),
+        [tpos] "=&s"(tpos), [t] "=&s"(t), [gm] "=&s"(gm), [m0save] "=&s"(m0save), [sp] "=&v"(sp), [slo] "=&v"(slo),
+        [shi] "=&v"(shi), [dx] "=&v"(dx), [dy] "=&v"(dy), [dz] "=&v"(dz), [fx] "=&v"(fx),
+        [fy] "=&v"(fy), [fz] "=&v"(fz), [d2] "=&v"(d2), [tv] "=&v"(tv)
+      : [base] "s"(nodes), [nend] "s"(nend), [pbeg] "s"(pbeg), [A0] "s"(A), [x] "v"(x),
+        [y] "v"(y), [z] "v"(z)
+      : "s54", "s55", "s56", "s57", "s58", "s59", "s60", "s61", "s62", "s63", "s64", "s65",
+        "s66", "s67", "s68", "s69", "s70", "s71", "s72", "s73", "s74", "s75", "s76", "s77",
+        "s78", "s79", "vcc", "scc", "memory");
 }
 
 // Workgroup -> tile order.  Workgroups are handed to the 8 XCDs round-robin
@@ -205,7 +357,9 @@ __global__ void __launch_bounds__(BLOCK) walk_kernel(WalkArgs a) {
   int i[P];
   bool valid[P];
   double x[P], y[P], z[P], mx[P], my[P], mz[P];
-  unsigned long long A[P];  // lanes of set p that take the current node
+  // Lane masks, wave-uniform (SGPRs): A[p] = lanes of set p that take the
+  // current node.
+  unsigned long long A[P];
 #pragma unroll
   for (int p = 0; p < P; p++) {
     i[p] = pbeg + p * kWave + lane;
@@ -225,13 +379,7 @@ __global__ void __launch_bounds__(BLOCK) walk_kernel(WalkArgs a) {
   const double theta = a.theta;
   const double theta2 = theta * theta;
 
-  // waiting lanes: stack entry `slot` lives in lane `slot` of these
-  int st_pos = 0;
-  int st_lo[P], st_hi[P];
-#pragma unroll
-  for (int p = 0; p < P; p++) st_lo[p] = st_hi[p] = 0;
-  int top = 0;
-  int top_pos = -1;  // st_pos of the top entry, kept in an SGPR
+  bool overflow = false;
 
   typedef typename std::conditional<F32, float, double>::type acc_t;
   acc_t ax[P], ay[P], az[P];
@@ -242,113 +390,134 @@ __global__ void __launch_bounds__(BLOCK) walk_kernel(WalkArgs a) {
     cnt[p] = 0;
   }
 
-  // One scalar load of the node record per visit.  (Loading both possible
-  // successors a visit ahead measured slower: the doubled scalar-memory
-  // traffic costs more than the latency it hides.)
   unsigned visits = 0;
-  while (n < nend) {
-    visits++;
-    const NodeRegs cur = load_node(nodes, n);
-    const int leaf = cur.leaf;
-    const int skip = cur.skip;
-    double dx[P], dy[P], dz[P], d2[P];
-    float fdx[P], fdy[P], fdz[P], fd2[P];
-    bool in[P], inter[P];
+  if (F32 && !DAMP && !FROZEN && P == 1 && a.asm_loop) {
+    int ovf = 0;
+    float fax = 0.f, fay = 0.f, faz = 0.f;
+    walk_loop_f32(nodes, n, nend, pbeg, A[0], x[0], y[0], z[0], fax, fay, faz, cnt[0],
+                  visits, ovf);
+    ax[0] = fax;
+    ay[0] = fay;
+    az[0] = faz;
+    overflow = ovf != 0;
+  } else {
+    // waiting lanes: stack entry `slot` lives in lane `slot` of these
+    int st_pos = 0;
+    int st_lo[P], st_hi[P];
 #pragma unroll
-    for (int p = 0; p < P; p++) {
-      dx[p] = cur.cx - x[p];
-      dy[p] = cur.cy - y[p];
-      dz[p] = cur.cz - z[p];
-      in[p] = __builtin_amdgcn_inverse_ballot_w64(A[p]);  // the mask itself
-      if (F32) {
-        fdx[p] = (float)dx[p];
-        fdy[p] = (float)dy[p];
-        fdz[p] = (float)dz[p];
-        fd2[p] = fmaf(fdx[p], fdx[p], fmaf(fdy[p], fdy[p], fdz[p] * fdz[p]));
-        d2[p] = 0;
-      } else {
-        fdx[p] = fdy[p] = fdz[p] = fd2[p] = 0;
-        d2[p] = sq_sum_ref(dx[p], dy[p], dz[p]);
-      }
-    }
-    int next;
-    if (leaf >= 0) {
-#pragma unroll
-      for (int p = 0; p < P; p++) inter[p] = in[p] && leaf != i[p];
-      next = n + 1;
-    } else {
-      unsigned long long O[P], any_open = 0;
+    for (int p = 0; p < P; p++) st_lo[p] = st_hi[p] = 0;
+    int top = 0;
+    int top_pos = -1;  // st_pos of the top entry, kept in an SGPR
+
+    // One scalar load of the node record per visit.  (Loading both possible
+    // successors a visit ahead measured slower: the doubled scalar-memory
+    // traffic costs more than the latency it hides.)
+    while (n < nend) {
+      visits++;
+      const NodeRegs cur = load_node(nodes, n);
+      const int leaf = cur.leaf;
+      const int skip = cur.skip;
+      double dx[P], dy[P], dz[P], d2[P];
+      float fdx[P], fdy[P], fdz[P], fd2[P];
+      bool in[P], inter[P];
 #pragma unroll
       for (int p = 0; p < P; p++) {
-        bool open;
-        if (leaf == -2)
-          open = true;
-        else if (F32)
-          open = fd2[p] < cur.r2open32;
-        else
-          open = mac_open_f64(
-              FROZEN ? sq_sum_ref(mac_nodes[n].cx - mx[p], mac_nodes[n].cy - my[p],
-                                  mac_nodes[n].cz - mz[p])
-                     : d2[p],
-              cur.side, theta, theta2);
-        open = open && in[p];
-        O[p] = ballot(open);
-        inter[p] = in[p] && !open;
-        any_open |= O[p];
+        dx[p] = cur.cx - x[p];
+        dy[p] = cur.cy - y[p];
+        dz[p] = cur.cz - z[p];
+        in[p] = __builtin_amdgcn_inverse_ballot_w64(A[p]);  // the mask itself
+        if (F32) {
+          fdx[p] = (float)dx[p];
+          fdy[p] = (float)dy[p];
+          fdz[p] = (float)dz[p];
+          fd2[p] = fmaf(fdx[p], fdx[p], fmaf(fdy[p], fdy[p], fdz[p] * fdz[p]));
+          d2[p] = 0;
+        } else {
+          fdx[p] = fdy[p] = fdz[p] = fd2[p] = 0;
+          d2[p] = sq_sum_ref(dx[p], dy[p], dz[p]);
+        }
       }
-      if (any_open) {
-        unsigned long long C[P], any_wait = 0;
+      int next;
+      if (leaf >= 0) {
 #pragma unroll
-        for (int p = 0; p < P; p++) {
-          C[p] = A[p] & ~O[p];
-          any_wait |= C[p];
-        }
-        if (any_wait) {
-          if (top > 0 && top_pos == skip) {
-            const int slot = top - 1;
-#pragma unroll
-            for (int p = 0; p < P; p++) {
-              st_lo[p] = lane == slot ? (st_lo[p] | (int)(unsigned)C[p]) : st_lo[p];
-              st_hi[p] = lane == slot ? (st_hi[p] | (int)(unsigned)(C[p] >> 32)) : st_hi[p];
-            }
-          } else {
-            const int slot = top;
-            st_pos = lane == slot ? skip : st_pos;
-#pragma unroll
-            for (int p = 0; p < P; p++) {
-              st_lo[p] = lane == slot ? (int)(unsigned)C[p] : st_lo[p];
-              st_hi[p] = lane == slot ? (int)(unsigned)(C[p] >> 32) : st_hi[p];
-            }
-            top_pos = skip;
-            if (top < kWave - 1)
-              top++;
-            else if (lane == 0)
-              atomicOr(a.flags_out, kFlagStack);
-          }
-        }
-#pragma unroll
-        for (int p = 0; p < P; p++) A[p] = O[p];
+        for (int p = 0; p < P; p++) inter[p] = in[p] && leaf != i[p];
         next = n + 1;
       } else {
-        next = skip;
-      }
-    }
+        unsigned long long O[P], any_open = 0;
 #pragma unroll
-    for (int p = 0; p < P; p++)
-      if (inter[p])
-        interact<F32, DAMP>(cur, dx[p], dy[p], dz[p], ax[p], ay[p], az[p], cnt[p]);
-    n = max(next, n + 1);  // always advance, even on a corrupt skip
-    if (top > 0 && top_pos == n) {
+        for (int p = 0; p < P; p++) {
+          bool open;
+          if (leaf == -2)
+            open = true;
+          else if (F32)
+            open = fd2[p] < cur.r2open32;
+          else
+            open = mac_open_f64(
+                FROZEN ? sq_sum_ref(mac_nodes[n].cx - mx[p], mac_nodes[n].cy - my[p],
+                                    mac_nodes[n].cz - mz[p])
+                       : d2[p],
+                cur.side, theta, theta2);
+          open = open && in[p];
+          O[p] = ballot(open);
+          inter[p] = in[p] && !open;
+          any_open |= O[p];
+        }
+        if (any_open) {
+          unsigned long long C[P], any_wait = 0;
 #pragma unroll
-      for (int p = 0; p < P; p++) {
-        const unsigned lo = (unsigned)__builtin_amdgcn_readlane(st_lo[p], top - 1);
-        const unsigned hi = (unsigned)__builtin_amdgcn_readlane(st_hi[p], top - 1);
-        A[p] |= ((unsigned long long)hi << 32) | lo;
+          for (int p = 0; p < P; p++) {
+            C[p] = A[p] & ~O[p];
+            any_wait |= C[p];
+          }
+          if (any_wait) {
+            if (top > 0 && top_pos == skip) {
+              const int slot = top - 1;
+#pragma unroll
+              for (int p = 0; p < P; p++) {
+                st_lo[p] = lane == slot ? (st_lo[p] | (int)(unsigned)C[p]) : st_lo[p];
+                st_hi[p] = lane == slot ? (st_hi[p] | (int)(unsigned)(C[p] >> 32)) : st_hi[p];
+              }
+            } else {
+              const int slot = top;
+              st_pos = lane == slot ? skip : st_pos;
+#pragma unroll
+              for (int p = 0; p < P; p++) {
+                st_lo[p] = lane == slot ? (int)(unsigned)C[p] : st_lo[p];
+                st_hi[p] = lane == slot ? (int)(unsigned)(C[p] >> 32) : st_hi[p];
+              }
+              top_pos = skip;
+              if (top < kWave - 1)
+                top++;
+              else
+                overflow = true;
+            }
+          }
+#pragma unroll
+          for (int p = 0; p < P; p++) A[p] = O[p];
+          next = n + 1;
+        } else {
+          next = skip;
+        }
       }
-      top--;
-      top_pos = top > 0 ? __builtin_amdgcn_readlane(st_pos, top - 1) : -1;
+#pragma unroll
+      for (int p = 0; p < P; p++)
+        if (inter[p])
+          interact<F32, DAMP>(cur, dx[p], dy[p], dz[p], fdx[p], fdy[p], fdz[p], fd2[p],
+                              ax[p], ay[p], az[p], cnt[p]);
+      n = max(next, n + 1);  // always advance, even on a corrupt skip
+      if (top > 0 && top_pos == n) {
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+          const unsigned lo = (unsigned)__builtin_amdgcn_readlane(st_lo[p], top - 1);
+          const unsigned hi = (unsigned)__builtin_amdgcn_readlane(st_hi[p], top - 1);
+          A[p] |= ((unsigned long long)hi << 32) | lo;
+        }
+        top--;
+        top_pos = top > 0 ? __builtin_amdgcn_readlane(st_pos, top - 1) : -1;
+      }
     }
   }
+  if (overflow && lane == 0) atomicOr(a.flags_out, kFlagStack);
 
   int wtot = 0;
 #pragma unroll
