@@ -320,7 +320,9 @@ int alloc_nodes(nbs_ctx* c, int cap) {
   free_all(c->node_bufs);
   hipError_t e = hipSuccess;
   auto& o = c->node_bufs;
-  c->nodes = dalloc<WalkNode>(o, cap, &e);
+  // one record beyond the capacity: the hand-written FP32 walk requests the
+  // record after a system's last node (unused) without a bounds test
+  c->nodes = dalloc<WalkNode>(o, (size_t)cap + 1, &e);
   c->aux.depth = dalloc<int>(o, cap, &e);
   c->aux.first = dalloc<int>(o, cap, &e);
   c->aux.count = dalloc<int>(o, cap, &e);
@@ -491,7 +493,7 @@ void enqueue_walk(nbs_ctx* c, int fused) {
   a.fused_update = fused;
   a.groups = c->walk_groups;
   // the hand-written loop addresses node records with 32-bit byte offsets
-  a.asm_loop = !c->walk_generic && (size_t)c->node_cap * sizeof(WalkNode) < (1ull << 32);
+  a.asm_loop = !c->walk_generic && ((size_t)c->node_cap + 1) * sizeof(WalkNode) < (1ull << 32);
   if (c->walk_order) {
     const int tiles = sync_tile_costs(c);
     launch_tile_order(c->tile_cost, tiles, c->walk_groups, c->tile_order, c->stream);

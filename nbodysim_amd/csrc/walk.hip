@@ -167,9 +167,10 @@ __device__ __forceinline__ void interact(const NodeRegs& nd, double dx, double d
 // order, as the generic loop in walk_kernel; what differs is the
 // instruction stream.  Per visit: 10 VALU instructions for d = com - x
 // (FP64, rounded to FP32), d^2 and the opening test, 8 for the force on
-// the interacting lanes, and scalar code for everything else.  The next
-// record is requested before the force arithmetic so the scalar-load
-// latency overlaps it.
+// the interacting lanes, and scalar code for everything else; the common
+// case (a cell every lane accepts) takes 11 scalar ALU instructions and 5
+// branches.  The next record is requested before the force arithmetic so
+// the scalar-load latency overlaps it.
 //
 // Fixed SGPRs (clobbered): s[64:79] the 64-byte node record (cx s[64:65],
 // cy s[66:67], cz s[68:69], skip s70, leaf s71, gm32 s76, r2open32 s77);
@@ -177,13 +178,39 @@ __device__ __forceinline__ void interact(const NodeRegs& nd, double dx, double d
 // s[58:59] O (lanes opening), s[56:57] saved exec, s[54:55] a popped
 // stack mask.  The stack of waiting lanes is entry `slot` in lane `slot`
 // of sp (position), slo/shi (mask); lane 63 of sp stays -1, the position
-// an empty stack reports.  Every scalar load is waited for inside the
-// block.  M0 carries the lane index of stack writes (a VALU instruction
-// reads one SGPR at most); the block restores it.  Record offsets are 32-bit: callers keep node index * 64 below 2^32.
+// an empty stack reports.  M0 carries the lane index of stack writes (a
+// VALU instruction reads one SGPR at most); the block restores it.  Every
+// scalar load is waited for inside the block.
+//
+// The record after the last node of a system is requested (and not used)
+// when the walk ends: the node array holds one record beyond its capacity.
+// Record offsets are 32-bit: callers keep (capacity + 1) * 64 below 2^32.
+#define NBS_WALK_TAIL(MASK, L)                                        \
+  "s_lshl_b32 %[t], %[nx], 6\n\t"                                     \
+  "s_load_dwordx16 s[64:79], %[base], %[t]\n\t"                       \
+  "s_and_saveexec_b64 s[56:57], " MASK "\n\t"                          \
+  "s_cbranch_execz L_skip" L "%=\n\t"                                 \
+  "v_rsq_f32 %[tv], %[d2]\n\t"                                        \
+  "v_add_u32 %[cnt], 1, %[cnt]\n\t"                                   \
+  "v_mul_f32 %[d2], %[tv], %[tv]\n\t"                                 \
+  "v_mul_f32 %[d2], %[d2], %[tv]\n\t"                                 \
+  "v_mul_f32 %[d2], %[gm], %[d2]\n\t"                                 \
+  "v_fmac_f32 %[ax], %[d2], %[fx]\n\t"                                \
+  "v_fmac_f32 %[ay], %[d2], %[fy]\n\t"                                \
+  "v_fmac_f32 %[az], %[d2], %[fz]\n"                                  \
+  "L_skip" L "%=:\n\t"                                                \
+  "s_mov_b64 exec, s[56:57]\n\t"                                      \
+  "s_cmp_eq_u32 %[tpos], %[nx]\n\t"                                   \
+  "s_cbranch_scc1 L_pop%=\n\t"                                        \
+  "s_mov_b32 %[n], %[nx]\n\t"                                         \
+  "s_cmp_lt_i32 %[n], %[nend]\n\t"                                    \
+  "s_cbranch_scc1 L_loop%=\n\t"                                       \
+  "s_branch L_end%=\n"
+
 __device__ __forceinline__ void walk_loop_f32(ConstNodes nodes, int n, int nend, int pbeg,
                                               unsigned long long A, double x, double y,
                                               double z, float& ax, float& ay, float& az,
-                                              int& cnt, unsigned& visits, int& overflow) {
+                                              int& cnt, int& overflow) {
   int nx, top, tpos, t, gm, m0save;
   int sp, slo, shi;
   double dx, dy, dz;
@@ -205,7 +232,6 @@ __device__ __forceinline__ void walk_loop_f32(ConstNodes nodes, int n, int nend,
       "v_add_f64 %[dx], s[64:65], -%[x]\n\t"
       "v_add_f64 %[dy], s[66:67], -%[y]\n\t"
       "v_add_f64 %[dz], s[68:69], -%[z]\n\t"
-      "s_add_u32 %[visits], %[visits], 1\n\t"
       "s_mov_b32 %[gm], s76\n\t"
       "v_cvt_f32_f64 %[fx], %[dx]\n\t"
       "v_cvt_f32_f64 %[fy], %[dy]\n\t"
@@ -214,35 +240,36 @@ __device__ __forceinline__ void walk_loop_f32(ConstNodes nodes, int n, int nend,
       "v_mul_f32 %[d2], %[fz], %[fz]\n\t"
       "v_fmac_f32 %[d2], %[fy], %[fy]\n\t"
       "v_fmac_f32 %[d2], %[fx], %[fx]\n\t"
-      // leaf >= 0: a leaf; -2: a depth-42 bucket; -1: a cell
-      "s_cmp_gt_i32 s71, -1\n\t"
-      "s_cbranch_scc1 L_leaf%=\n\t"
-      "s_cmp_eq_u32 s71, -2\n\t"
-      "s_cbranch_scc1 L_bucket%=\n\t"
+      // leaf field: -1 a cell, >= 0 a leaf, -2 a depth-42 bucket
+      "s_cmp_lg_u32 s71, -1\n\t"
+      "s_cbranch_scc1 L_notcell%=\n\t"
       // cell: open where d^2 < (side/theta)^2
       "v_cmp_gt_f32 vcc, s77, %[d2]\n\t"
       "s_and_b64 s[58:59], vcc, s[62:63]\n\t"
       "s_cbranch_scc1 L_opened%=\n\t"
-      // every lane that takes it accepts it: all interact, jump the subtree
-      "s_mov_b64 s[60:61], s[62:63]\n\t"
+      // every lane that takes it accepts it: all interact, jump the
+      // subtree (never backwards, even on a corrupt skip)
       "s_max_i32 %[nx], %[nx], s70\n\t"
-      "s_branch L_pop%=\n"
-      "L_leaf%=:\n\t"
-      // every lane that takes it interacts, except its own particle's
-      "s_mov_b64 s[60:61], s[62:63]\n\t"
+      NBS_WALK_TAIL("s[62:63]", "a")
+      "L_notcell%=:\n\t"
+      "s_cmp_gt_i32 s71, -1\n\t"
+      "s_cbranch_scc0 L_bucket%=\n\t"
+      // leaf: every lane that takes it interacts, except its own particle's
       "s_sub_u32 %[t], s71, %[pbeg]\n\t"
+      "s_mov_b64 s[60:61], s[62:63]\n\t"
       "s_cmp_lt_u32 %[t], 64\n\t"
-      "s_cbranch_scc0 L_pop%=\n\t"
+      "s_cbranch_scc0 L_int%=\n\t"
       "s_bitset0_b64 s[60:61], %[t]\n\t"
-      "s_branch L_pop%=\n"
+      "s_branch L_int%=\n"
       "L_bucket%=:\n\t"
+      // every lane opens a bucket; nobody interacts
       "s_mov_b64 s[60:61], 0\n\t"
-      "s_branch L_pop%=\n"
+      "s_branch L_int%=\n"
       "L_opened%=:\n\t"
       // I = A & ~O interact and wait at skip; O descend (A = O)
       "s_andn2_b64 s[60:61], s[62:63], s[58:59]\n\t"
       "s_mov_b64 s[62:63], s[58:59]\n\t"
-      "s_cbranch_scc0 L_pop%=\n\t"
+      "s_cbranch_scc0 L_int%=\n\t"
       "s_cmp_eq_u32 %[tpos], s70\n\t"
       "s_cbranch_scc1 L_merge%=\n\t"
       "s_cmp_gt_u32 %[top], 62\n\t"
@@ -253,7 +280,7 @@ __device__ __forceinline__ void walk_loop_f32(ConstNodes nodes, int n, int nend,
       "v_writelane_b32 %[shi], s61, m0\n\t"
       "s_add_u32 %[top], %[top], 1\n\t"
       "s_mov_b32 %[tpos], s70\n\t"
-      "s_branch L_pop%=\n"
+      "s_branch L_int%=\n"
       "L_merge%=:\n\t"
       "s_sub_u32 m0, %[top], 1\n\t"
       "v_readlane_b32 s54, %[slo], m0\n\t"
@@ -261,48 +288,30 @@ __device__ __forceinline__ void walk_loop_f32(ConstNodes nodes, int n, int nend,
       "s_or_b64 s[54:55], s[54:55], s[60:61]\n\t"
       "v_writelane_b32 %[slo], s54, m0\n\t"
       "v_writelane_b32 %[shi], s55, m0\n\t"
-      "s_branch L_pop%=\n"
+      "s_branch L_int%=\n"
       "L_ovf%=:\n\t"
       "s_mov_b32 %[ovf], 1\n"
+      "L_int%=:\n\t"
+      NBS_WALK_TAIL("s[60:61]", "i")
       "L_pop%=:\n\t"
       // lanes waiting at the next node rejoin
-      "s_cmp_eq_u32 %[tpos], %[nx]\n\t"
-      "s_cbranch_scc0 L_fetch%=\n\t"
       "s_sub_u32 %[top], %[top], 1\n\t"
       "v_readlane_b32 s54, %[slo], %[top]\n\t"
       "v_readlane_b32 s55, %[shi], %[top]\n\t"
       "s_add_u32 %[t], %[top], 63\n\t"
       "s_and_b32 %[t], %[t], 63\n\t"
       "v_readlane_b32 %[tpos], %[sp], %[t]\n\t"
-      "s_or_b64 s[62:63], s[62:63], s[54:55]\n"
-      "L_fetch%=:\n\t"
-      // request the next record (the current one if the walk ends)
-      "s_cmp_lt_i32 %[nx], %[nend]\n\t"
-      "s_cselect_b32 %[t], %[nx], %[n]\n\t"
-      "s_lshl_b32 %[t], %[t], 6\n\t"
-      "s_load_dwordx16 s[64:79], %[base], %[t]\n\t"
-      // force on the interacting lanes: a += G m d / |d|^3
-      "s_and_saveexec_b64 s[56:57], s[60:61]\n\t"
-      "s_cbranch_execz L_skip%=\n\t"
-      "v_rsq_f32 %[tv], %[d2]\n\t"
-      "v_add_u32 %[cnt], 1, %[cnt]\n\t"
-      "v_mul_f32 %[d2], %[tv], %[tv]\n\t"
-      "v_mul_f32 %[d2], %[d2], %[tv]\n\t"
-      "v_mul_f32 %[d2], %[gm], %[d2]\n\t"
-      "v_fmac_f32 %[ax], %[d2], %[fx]\n\t"
-      "v_fmac_f32 %[ay], %[d2], %[fy]\n\t"
-      "v_fmac_f32 %[az], %[d2], %[fz]\n"
-      "L_skip%=:\n\t"
-      "s_mov_b64 exec, s[56:57]\n\t"
+      "s_or_b64 s[62:63], s[62:63], s[54:55]\n\t"
       "s_mov_b32 %[n], %[nx]\n\t"
       "s_cmp_lt_i32 %[n], %[nend]\n\t"
-      "s_cbranch_scc1 L_loop%=\n\t"
+      "s_cbranch_scc1 L_loop%=\n"
+      "L_end%=:\n\t"
       "s_waitcnt lgkmcnt(0)\n"
       "L_done%=:\n\t"
       "s_mov_b32 m0, %[m0save]"
-      : [n] "+s"(n), [visits] "+s"(visits), [ovf] "+s"(overflow), [ax] "+v"(ax),
-        [ay] "+v"(ay), [az] "+v"(az), [cnt] "+v"(cnt), [nx] "=&s"(nx), [top] "=&s"(top),
-        [tpos] "=&s"(tpos), [t] "=&s"(t), [gm] "=&s"(gm), [m0save] "=&s"(m0save), [sp] "=&v"(sp), [slo] "=&v"(slo),
+      : [n] "+s"(n), [ovf] "+s"(overflow), [ax] "+v"(ax), [ay] "+v"(ay), [az] "+v"(az),
+        [cnt] "+v"(cnt), [nx] "=&s"(nx), [top] "=&s"(top), [tpos] "=&s"(tpos), [t] "=&s"(t),
+        [gm] "=&s"(gm), [m0save] "=&s"(m0save), [sp] "=&v"(sp), [slo] "=&v"(slo),
         [shi] "=&v"(shi), [dx] "=&v"(dx), [dy] "=&v"(dy), [dz] "=&v"(dz), [fx] "=&v"(fx),
         [fy] "=&v"(fy), [fz] "=&v"(fz), [d2] "=&v"(d2), [tv] "=&v"(tv)
       : [base] "s"(nodes), [nend] "s"(nend), [pbeg] "s"(pbeg), [A0] "s"(A), [x] "v"(x),
@@ -311,6 +320,7 @@ __device__ __forceinline__ void walk_loop_f32(ConstNodes nodes, int n, int nend,
         "s66", "s67", "s68", "s69", "s70", "s71", "s72", "s73", "s74", "s75", "s76", "s77",
         "s78", "s79", "vcc", "scc", "memory");
 }
+#undef NBS_WALK_TAIL
 
 // Workgroup -> tile order.  Workgroups are handed to the 8 XCDs round-robin
 // by workgroup id, each XCD with its own L2.  With `groups` > 1 the tiles
@@ -391,11 +401,12 @@ __global__ void __launch_bounds__(BLOCK) walk_kernel(WalkArgs a) {
   }
 
   unsigned visits = 0;
-  if (F32 && !DAMP && !FROZEN && P == 1 && a.asm_loop) {
+  // (the hand-written loop does not count visits: per-tile costs come from
+  // the generic loop)
+  if (F32 && !DAMP && !FROZEN && P == 1 && a.asm_loop && !a.tile_cost) {
     int ovf = 0;
     float fax = 0.f, fay = 0.f, faz = 0.f;
-    walk_loop_f32(nodes, n, nend, pbeg, A[0], x[0], y[0], z[0], fax, fay, faz, cnt[0],
-                  visits, ovf);
+    walk_loop_f32(nodes, n, nend, pbeg, A[0], x[0], y[0], z[0], fax, fay, faz, cnt[0], ovf);
     ax[0] = fax;
     ay[0] = fay;
     az[0] = faz;
