@@ -299,6 +299,8 @@ int alloc_particles(nbs_ctx* c, int n) {
   b.khi = dalloc<unsigned long long>(o, cap, &e);
   b.klo = dalloc<unsigned long long>(o, cap, &e);
   b.counts = dalloc<int>(o, cap, &e);
+  b.close_at = dalloc<int>(o, cap, &e);
+  b.blkstats = dalloc<int>(o, 2 * (size_t)emit_blocks(cap), &e);
   b.offsets = dalloc<int>(o, (size_t)cap + 1, &e);
   b.sort_tmp = dalloc<int>(o, radix_tmp_ints(cap), &e);
   b.runs = dalloc<int>(o, 2 + 2 * ((size_t)cap / 2 + 1), &e);
@@ -441,7 +443,7 @@ void enqueue_build(nbs_ctx* c) {
   c->cur ^= 1;
   launch_emit(c->bb, c->sys, c->p, c->st[c->cur], c->nodes, c->aux, c->n,
               c->node_cap, s);
-  launch_bottom_up(c->bb, c->nodes, c->aux, c->n, kLevels, s);
+  launch_bottom_up(c->bb, c->nodes, c->aux, c->n, c->sys.nsys, s);
   prof_end(c, kPhaseBuild, e0);
 }
 
@@ -665,7 +667,7 @@ int nbs_create(const nbs_config* cfg, nbs_ctx** out) {
   c->flags = dalloc<int>(c->misc_bufs, 1, &e);
   c->steps_done = dalloc<int>(c->misc_bufs, 1, &e);
   c->bb.flags = c->flags;
-  c->bb.stats = dalloc<int>(c->misc_bufs, 4, &e);
+  c->bb.stats = dalloc<int>(c->misc_bufs, 2, &e);
   c->stats = c->bb.stats;
   if (e != hipSuccess) {
     free_all(c->misc_bufs);
@@ -674,7 +676,7 @@ int nbs_create(const nbs_config* cfg, nbs_ctx** out) {
     return NBS_ERR_NOMEM;
   }
   launch_fill_int(c->flags, 0, 1, c->stream);
-  launch_fill_int(c->stats, 0, 4, c->stream);
+  launch_fill_int(c->stats, 0, 2, c->stream);
   *out = c;
   return NBS_OK;
 }
@@ -1137,14 +1139,23 @@ int nbs_time(nbs_ctx* c, double* t, int cap) {
 int nbs_tree_info_get(nbs_ctx* c, nbs_tree_info* info) {
   if (!c || !info) return NBS_ERR_ARG;
   if (!c->tree_valid) return fail(c, NBS_ERR_STATE, "no tree built");
-  int s[4];
+  int s[2];
+  const int nb = emit_blocks(c->n);
+  std::vector<int> bs(2 * (size_t)nb);
   HIPCHK(c, hipMemcpyAsync(s, c->stats, sizeof(s), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(bs.data(), c->bb.blkstats, bs.size() * sizeof(int),
+                           hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  int max_depth = 0, buckets = 0;
+  for (int k = 0; k < nb; k++) {
+    max_depth = std::max(max_depth, bs[2 * k]);
+    buckets += bs[2 * k + 1];
+  }
   info->n_particles = c->n;
   info->n_nodes = s[0];
   info->n_internal = s[1];
-  info->max_depth = s[2];
-  info->n_buckets = s[3];
+  info->max_depth = max_depth;
+  info->n_buckets = buckets;
   info->node_capacity = c->node_cap;
   return NBS_OK;
 }
@@ -1530,7 +1541,7 @@ int nbs_frozen_forward(nbs_ctx* c, const double* pos0, const double* vel0,
     gather3_kernel<<<b, 256, 0, s>>>(c->p_v, st.id, (int)n, st.vx, st.vy, st.vz);
     launch_gather_comp(c->p_m, st.m, st.id, (int)n, 1, s);
     launch_leaf_refresh(c->bb, st, c->nodes, (int)n, s);
-    launch_bottom_up(c->bb, c->nodes, c->aux, (int)n, kLevels, s);
+    launch_bottom_up(c->bb, c->nodes, c->aux, (int)n, c->sys.nsys, s);
     WalkArgs a{};
     a.nodes = c->nodes;
     a.node_start = c->sys.node_start;

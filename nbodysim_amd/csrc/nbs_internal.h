@@ -108,11 +108,13 @@ struct BuildBuffers {
   int* dhist;                         // [depth_hist_ints(n)] per-block depths
   int* doff;                          // same, exclusive-scanned
   int* bucket;                        // [node cap] internal cells by depth
+  int* close_at;                      // [n] last slot of each cell, by opening rank
   int* sort_tmp;
   int* scan_tmp;
   int* runs;                          // long runs of equal primary keys
   int* flags;                         // device flags word
-  int* stats;                         // [4]: nodes, internal, maxdepth, buckets
+  int* stats;                         // [2]: nodes, internal cells
+  int* blkstats;                      // [2 * emit_blocks(n)]: per emit block deepest leaf, buckets
 };
 
 void launch_bbox(const State& st, Systems& sys, hipStream_t s);
@@ -137,12 +139,15 @@ void launch_permute(const State& src, State& dst, const unsigned* perm,
                     int n, hipStream_t s);
 // Emits the pre-order node array (capacity `cap`), leaves filled from `st`.
 size_t depth_hist_ints(size_t n);
+// Blocks of the emission kernels (per-block tree statistics).
+int emit_blocks(size_t n);
 void launch_emit(BuildBuffers& b, Systems& sys, const StepParams& p,
                  const State& st, WalkNode* nodes, TreeAux& aux, int n,
                  int cap, hipStream_t s);
-// Mass/COM of every internal cell, one launch per level from max_depth up.
+// Mass/COM of every internal cell, level by level from the deepest up
+// (nsys systems: the top levels hold nsys * 8^d cells at most).
 void launch_bottom_up(BuildBuffers& b, WalkNode* nodes, TreeAux& aux, int n,
-                      int max_depth, hipStream_t s);
+                      int nsys, hipStream_t s);
 // Leaves of the current topology take new particle data.
 void launch_leaf_refresh(BuildBuffers& b, const State& st, WalkNode* nodes,
                          int n, hipStream_t s);

@@ -1,8 +1,10 @@
 // radix_sort.hip -- stable LSD radix sort of (u64 key, u32 value) pairs,
-// 8-bit digits.  Per pass: an upsweep writes each tile's digit histogram
-// (digit-major), a device scan turns them into global offsets, and a
-// downsweep ranks the tile's keys stably with wave64 ballot matching,
-// sorts the tile in LDS and writes digit runs out contiguously.
+// 8-bit digits.  Per pass, three kernels: an upsweep writes each tile's
+// digit histogram (digit-major); a digit scan (one workgroup per digit)
+// turns a digit's tile counts into tile offsets within the digit and the
+// digit's total; a downsweep scans the 256 digit totals itself, ranks the
+// tile's keys stably with wave64 ballot matching, sorts the tile in LDS and
+// writes digit runs out contiguously.
 #include "nbs_device_util.h"
 #include "nbs_internal.h"
 
@@ -31,12 +33,43 @@ __global__ void __launch_bounds__(kRsBlock)
   hist[threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
 }
 
+// Workgroup d: exclusive scan of hist[d * ntiles + t] over the tiles t
+// (in place) and total[d] = the digit's count.
+__global__ void __launch_bounds__(kRsBlock)
+    digit_scan_kernel(int* __restrict__ hist, int ntiles, int* __restrict__ total) {
+  __shared__ int wsum[kRsWaves];
+  __shared__ int carry_s;
+  const int t = threadIdx.x, lane = lane_id(), w = t >> 6;
+  int* h = hist + (size_t)blockIdx.x * ntiles;
+  if (t == 0) carry_s = 0;
+  __syncthreads();
+  for (int base = 0; base < ntiles; base += kRsBlock) {
+    const int i = base + t;
+    const int v = i < ntiles ? h[i] : 0;
+    const int inc = wave_inclusive_scan(v);
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    int wb = 0, tot = 0;
+    for (int k = 0; k < kRsWaves; k++) {
+      if (k < w) wb += wsum[k];
+      tot += wsum[k];
+    }
+    const int carry = carry_s;
+    if (i < ntiles) h[i] = carry + wb + inc - v;
+    __syncthreads();
+    if (t == 0) carry_s = carry + tot;
+    __syncthreads();
+  }
+  if (t == 0) total[blockIdx.x] = carry_s;
+}
+
 __global__ void __launch_bounds__(kRsBlock)
     downsweep_kernel(const unsigned long long* __restrict__ kin,
                      const unsigned* __restrict__ vin,
                      unsigned long long* __restrict__ kout,
                      unsigned* __restrict__ vout, int n, int shift,
-                     int ntiles, const int* __restrict__ goff) {
+                     int ntiles, const int* __restrict__ toff,
+                     const int* __restrict__ total) {
   __shared__ unsigned long long sk[kRsTile];
   __shared__ unsigned sv[kRsTile];
   __shared__ int tile_start[kRadix];
@@ -44,6 +77,7 @@ __global__ void __launch_bounds__(kRsBlock)
   __shared__ int run[kRadix];
   __shared__ int wcnt[kRsWaves][kRadix];
   __shared__ int wsum[kRsWaves];
+  __shared__ int gsum[kRsWaves];
 
   const int t = threadIdx.x;
   const int lane = lane_id();
@@ -72,16 +106,26 @@ __global__ void __launch_bounds__(kRsBlock)
     }
   }
   __syncthreads();
-  // Tile-local exclusive scan of the digit counts (one digit per thread).
+  // Tile-local exclusive scan of the digit counts, and the digit's global
+  // start: exclusive scan of the digit totals plus this tile's offset
+  // within the digit (one digit per thread).
   {
-    int c = run[t];
-    int inc = wave_inclusive_scan(c);
-    if (lane == 63) wsum[w] = inc;
+    const int c = run[t];
+    const int g = total[t];
+    const int inc = wave_inclusive_scan(c);
+    const int ginc = wave_inclusive_scan(g);
+    if (lane == 63) {
+      wsum[w] = inc;
+      gsum[w] = ginc;
+    }
     __syncthreads();
-    int wb = 0;
-    for (int k = 0; k < w; k++) wb += wsum[k];
+    int wb = 0, gb = 0;
+    for (int k = 0; k < w; k++) {
+      wb += wsum[k];
+      gb += gsum[k];
+    }
     tile_start[t] = wb + inc - c;
-    gbase[t] = goff[t * ntiles + blockIdx.x];
+    gbase[t] = gb + ginc - g + toff[t * ntiles + blockIdx.x];
     run[t] = 0;
   }
   __syncthreads();
@@ -133,8 +177,7 @@ __global__ void __launch_bounds__(kRsBlock)
 size_t radix_tmp_ints(size_t n) {
   size_t ntiles = (n + kRsTile - 1) / kRsTile;
   if (ntiles == 0) ntiles = 1;
-  size_t h = kRadix * ntiles;
-  return 2 * h + 1 + scan_tmp_ints(h) + 64;
+  return kRadix * ntiles + kRadix;
 }
 
 int radix_sort_pairs(unsigned long long* k0, unsigned* v0,
@@ -142,10 +185,8 @@ int radix_sort_pairs(unsigned long long* k0, unsigned* v0,
                      int* tmp, hipStream_t s) {
   if (n <= 1 || nbits <= 0) return 0;
   int ntiles = (n + kRsTile - 1) / kRsTile;
-  int h = kRadix * ntiles;
-  int* hist = tmp;
-  int* goff = tmp + h;
-  int* stmp = tmp + 2 * h + 1;
+  int* hist = tmp;                   // [kRadix][ntiles], scanned in place
+  int* total = tmp + kRadix * ntiles;  // [kRadix]
   unsigned long long* ka = k0;
   unsigned* va = v0;
   unsigned long long* kb = k1;
@@ -153,9 +194,9 @@ int radix_sort_pairs(unsigned long long* k0, unsigned* v0,
   int which = 0;
   for (int shift = 0; shift < nbits; shift += 8) {
     upsweep_kernel<<<ntiles, kRsBlock, 0, s>>>(ka, n, shift, ntiles, hist);
-    exclusive_scan(hist, goff, h, stmp, s);
+    digit_scan_kernel<<<kRadix, kRsBlock, 0, s>>>(hist, ntiles, total);
     downsweep_kernel<<<ntiles, kRsBlock, 0, s>>>(ka, va, kb, vb, n, shift,
-                                                 ntiles, goff);
+                                                 ntiles, hist, total);
     unsigned long long* tk = ka;
     ka = kb;
     kb = tk;

@@ -72,8 +72,10 @@ __global__ void __launch_bounds__(kBlock)
 
 // Root cell per system: src/main.cpp:908-920 in FP64 (midpoint of the
 // bbox; side = largest extent + 2.0, ties falling through to z).  One block
-// per system reduces the system's wave partials.
-__global__ void __launch_bounds__(kBlock)
+// per system reduces the system's wave partials (1024 threads, four
+// partials in flight per thread: the reduction is latency-bound).
+constexpr int kRootBlock = 1024;
+__global__ void __launch_bounds__(kRootBlock)
     root_kernel(const double* __restrict__ part, int nwaves,
                 const int* __restrict__ wstart, double* __restrict__ root,
                 int nsys, int mode, double cx, double cy, double cz,
@@ -81,10 +83,12 @@ __global__ void __launch_bounds__(kBlock)
 #pragma clang fp contract(off)
   const int s = blockIdx.x;
   if (s >= nsys || *flags) return;
-  __shared__ double red[6][kBlock / 64];
+  __shared__ double red[6][kRootBlock / 64];
   double b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
   if (mode != 1) {
-    for (int w = wstart[s] + threadIdx.x; w < wstart[s + 1]; w += kBlock)
+#pragma unroll 4
+    for (int w = wstart[s] + threadIdx.x; w < wstart[s + 1]; w += kRootBlock)
+#pragma unroll
       for (int k = 0; k < 6; k++) {
         const double v = part[(size_t)k * nwaves + w];
         b[k] = k < 3 ? fmin(b[k], v) : fmax(b[k], v);
@@ -105,7 +109,7 @@ __global__ void __launch_bounds__(kBlock)
   }
   for (int k = 0; k < 6; k++) {
     double r = red[k][0];
-    for (int q = 1; q < kBlock / 64; q++) r = k < 3 ? fmin(r, red[k][q]) : fmax(r, red[k][q]);
+    for (int q = 1; q < kRootBlock / 64; q++) r = k < 3 ? fmin(r, red[k][q]) : fmax(r, red[k][q]);
     b[k] = r;
   }
   const double min_x = b[0], min_y = b[1], min_z = b[2];
@@ -396,26 +400,92 @@ __device__ __forceinline__ Neigh neighbours(
   return r;
 }
 
+// Cell openings and closings.  With l_i the levels keys i and i+1 share,
+// the internal cells that start at slot i have depths (l_{i-1}, l_i] and
+// those that end at slot i depths (l_i, l_{i-1}].  Cells of one depth are
+// disjoint runs of slots, so in slot order they open and close
+// alternately: the k-th cell of depth d to open ends at the k-th closing
+// of depth d.  Every opening and closing gets its rank among the events of
+// its depth from per-block histograms (depth-major, scanned) and, inside a
+// block, from per-wave ballots; a cell's last slot is then one lookup, not
+// a search of the keys.  The opening ranks also order the depth buckets
+// the per-level reduction walks.
+constexpr int kEmitWaves = kBlock / 64;
+typedef unsigned long long DepthMasks[kEmitWaves][kDepthBins];
+
+// m[w][d] = lanes of wave w with an event at depth d (lo < d <= hi).
+__device__ __forceinline__ void depth_ballots(int lo, int hi, DepthMasks& m) {
+  const int w = threadIdx.x >> 6;
+  for (int d = 0; d < kDepthBins; d++) {
+    const unsigned long long b = ballot(lo < d && d <= hi);
+    if (lane_id() == 0) m[w][d] = b;
+  }
+}
+
+// Events at depth d in this block before this thread (after depth_ballots
+// and a barrier).
+__device__ __forceinline__ int block_rank(int d, const DepthMasks& m) {
+  const int w = threadIdx.x >> 6;
+  int r = mbcnt(m[w][d]);
+  for (int q = 0; q < w; q++) r += __popcll(m[q][d]);
+  return r;
+}
+
 // Entries (internal cells + the leaf) that start at slot i, and this
-// block's histogram of internal-cell depths (depth-major: the buckets the
-// per-level reduction walks).
+// block's histograms of openings (dhist[d * nb + blk]) and closings
+// (dhist[(kDepthBins + d) * nb + blk]) per depth.
 __global__ void __launch_bounds__(kBlock)
     emit_count_kernel(const int* __restrict__ start, int nsys, int n,
                       const unsigned long long* __restrict__ khi,
                       const unsigned long long* __restrict__ klo,
                       int* __restrict__ counts, int* __restrict__ dhist) {
-  __shared__ int h[kDepthBins];
-  for (int d = threadIdx.x; d < kDepthBins; d += kBlock) h[d] = 0;
-  __syncthreads();
+  __shared__ DepthMasks mo, mc;
   const int i = blockIdx.x * kBlock + threadIdx.x;
+  int lp = 0, ln = 0;
   if (i < n) {
     const Neigh g = neighbours(i, start, nsys, khi, klo);
-    counts[i] = max(0, g.ln - g.lp) + 1;
-    for (int d = g.lp + 1; d <= g.ln; d++) atomicAdd(&h[d], 1);
+    lp = g.lp;
+    ln = g.ln;
+    counts[i] = max(0, ln - lp) + 1;
   }
+  depth_ballots(lp, ln, mo);
+  depth_ballots(ln, lp, mc);
   __syncthreads();
-  for (int d = threadIdx.x; d < kDepthBins; d += kBlock)
-    dhist[d * gridDim.x + blockIdx.x] = h[d];
+  for (int d = threadIdx.x; d < kDepthBins; d += kBlock) {
+    int o = 0, c = 0;
+    for (int q = 0; q < kEmitWaves; q++) {
+      o += __popcll(mo[q][d]);
+      c += __popcll(mc[q][d]);
+    }
+    dhist[d * gridDim.x + blockIdx.x] = o;
+    dhist[(kDepthBins + d) * gridDim.x + blockIdx.x] = c;
+  }
+}
+
+// close_at[rank of a cell's opening] = the cell's last slot, written by the
+// slot that closes it.  doff is dhist exclusive-scanned; closings of depth
+// d are ranked from the total number of openings of shallower depths,
+// doff[(kDepthBins + d) * nb] - doff[kDepthBins * nb], which is where the
+// openings of depth d start.  Same grid as emit_count_kernel.
+__global__ void __launch_bounds__(kBlock)
+    close_kernel(const int* __restrict__ start, int nsys, int n,
+                 const unsigned long long* __restrict__ khi,
+                 const unsigned long long* __restrict__ klo,
+                 const int* __restrict__ doff, int* __restrict__ close_at) {
+  __shared__ DepthMasks mc;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  int lp = 0, ln = 0;
+  if (i < n) {
+    const Neigh g = neighbours(i, start, nsys, khi, klo);
+    lp = g.lp;
+    ln = g.ln;
+  }
+  depth_ballots(ln, lp, mc);
+  __syncthreads();
+  const int nb = gridDim.x;
+  const int total = doff[kDepthBins * nb];
+  for (int d = ln + 1; d <= lp; d++)
+    close_at[doff[(kDepthBins + d) * nb + blockIdx.x] - total + block_rank(d, mc)] = i;
 }
 
 __global__ void sys_nodes_kernel(const int* __restrict__ start,
@@ -423,33 +493,6 @@ __global__ void sys_nodes_kernel(const int* __restrict__ start,
                                  int* __restrict__ node_start) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s <= nsys) node_start[s] = offsets[start[s]];
-}
-
-// Last slot j in [i, e) whose key shares >= d levels with key i.
-__device__ __forceinline__ int range_end(int i, int d, int e,
-                                         const unsigned long long* khi,
-                                         const unsigned long long* klo) {
-  const unsigned long long hi = khi[i], lo = klo[i];
-  int good = i, bad = e, step = 1;
-  for (;;) {
-    int probe = good + step;
-    if (probe >= e) break;
-    if (cpl(hi, lo, khi[probe], klo[probe]) >= d) {
-      good = probe;
-      step <<= 1;
-    } else {
-      bad = probe;
-      break;
-    }
-  }
-  while (bad - good > 1) {
-    int mid = good + ((bad - good) >> 1);
-    if (cpl(hi, lo, khi[mid], klo[mid]) >= d)
-      good = mid;
-    else
-      bad = mid;
-  }
-  return good;
 }
 
 // Writes the cells and the leaf that start at slot i (pre-order), files
@@ -461,30 +504,36 @@ __global__ void __launch_bounds__(kBlock)
                 const unsigned long long* __restrict__ klo,
                 const int* __restrict__ offsets,
                 const int* __restrict__ doff,
+                const int* __restrict__ close_at,
                 const double* __restrict__ root, double theta, int cap,
                 const double* __restrict__ x, const double* __restrict__ y,
                 const double* __restrict__ z, const double* __restrict__ m,
                 WalkNode* __restrict__ nodes, TreeAux aux,
                 int* __restrict__ bucket, int* __restrict__ flags,
-                int* __restrict__ stats) {
-  __shared__ int cursor[kDepthBins];
-  for (int d = threadIdx.x; d < kDepthBins; d += kBlock)
-    cursor[d] = doff[d * gridDim.x + blockIdx.x];
-  __syncthreads();
+                int* __restrict__ stats, int* __restrict__ blkstats) {
+  __shared__ DepthMasks mo;
   const int i = blockIdx.x * kBlock + threadIdx.x;
+  Neigh g;
+  g.s = g.a = g.e = 0;
+  g.lp = g.ln = 0;
+  if (i < n) g = neighbours(i, start, nsys, khi, klo);
+  depth_ballots(g.lp, g.ln, mo);
+  __syncthreads();
+  const int nb = gridDim.x;
   const int total = offsets[n];
   const bool fits = total <= cap;
   if (!fits && i == 0) atomicOr(flags, kFlagNodeOverflow);
   int my_depth = 0, my_bucket = 0;
   if (i < n && fits) {
-    const Neigh g = neighbours(i, start, nsys, khi, klo);
     const double s0 = root[4 * g.s + 3];
     const int base = offsets[i];
     const int nint = max(0, g.ln - g.lp);
     for (int k = 0; k < nint; k++) {
       const int d = g.lp + 1 + k;
       const int node = base + k;
-      const int b = range_end(i, d, g.e, khi, klo);
+      const int r = doff[d * nb + blockIdx.x] + block_rank(d, mo);
+      // (clamped into this system's slots: a wrong rank could not fault)
+      const int b = min(max(close_at[r], i), g.e - 1);
       WalkNode w;
       w.cx = w.cy = w.cz = 0;
       w.m = w.gm = 0;
@@ -500,7 +549,7 @@ __global__ void __launch_bounds__(kBlock)
       aux.first[node] = i;
       aux.count[node] = b - i + 1;
       if (d == 0) aux.parent[node] = -1;
-      bucket[atomicAdd(&cursor[d], 1)] = node;
+      bucket[r] = node;
       if (bkt) my_bucket++;
     }
     const int leafn = base + nint;
@@ -524,13 +573,27 @@ __global__ void __launch_bounds__(kBlock)
     if (ld == 0) aux.parent[leafn] = -1;
     my_depth = ld;
   }
+  // Deepest leaf and number of buckets of this block: plain stores of
+  // per-block partials (the host reduces them when asked), no atomics on a
+  // shared word from every wave.
+  __shared__ int wmd[kEmitWaves], wnb[kEmitWaves];
   int md = my_depth;
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) md = max(md, __shfl_xor(md, d, 64));
-  const int nb = wave_sum(my_bucket);
+  const int nbk = wave_sum(my_bucket);
   if (lane_id() == 0) {
-    atomicMax(&stats[2], md);
-    if (nb) atomicAdd(&stats[3], nb);
+    wmd[threadIdx.x >> 6] = md;
+    wnb[threadIdx.x >> 6] = nbk;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int bmd = 0, bnb = 0;
+    for (int q = 0; q < kEmitWaves; q++) {
+      bmd = max(bmd, wmd[q]);
+      bnb += wnb[q];
+    }
+    blkstats[2 * blockIdx.x] = bmd;
+    blkstats[2 * blockIdx.x + 1] = bnb;
   }
   if (i == 0) {
     stats[0] = total;
@@ -560,92 +623,112 @@ __global__ void __launch_bounds__(kBlock)
   nodes[leafn].gm32 = (float)(kG * mi);
 }
 
-// One tree level: every internal cell of depth d sums its children (depth
-// d+1, already final) in child order 0..7 with the reference's operation
-// sequence, src/octree.cpp:203-221 (mass) and :252-278 (COM), and links
-// them to itself.  Launched from the deepest level up; kernel boundaries
-// order the levels, so no fences are needed.
-__global__ void __launch_bounds__(kBlock)
-    level_kernel(int d, int nblocks, const int* __restrict__ doff,
-                 const int* __restrict__ bucket, WalkNode* __restrict__ nodes,
-                 int* __restrict__ parent, const int* __restrict__ flags) {
+// One internal cell p sums its children (already final) in child order
+// 0..7 with the reference's operation sequence, src/octree.cpp:203-221
+// (mass) and :252-278 (COM), and links them to itself.
+__device__ __forceinline__ void reduce_cell(int p, WalkNode* __restrict__ nodes,
+                                            int* __restrict__ parent) {
 #pragma clang fp contract(off)
-  if (*flags) return;
-  const int b0 = doff[d * nblocks], b1 = doff[(d + 1) * nblocks];
-  for (int q = b0 + blockIdx.x * kBlock + threadIdx.x; q < b1;
-       q += gridDim.x * kBlock) {
-    const int p = bucket[q];
-    const int end = nodes[p].skip;
-    // One pass along the sibling chain (child c's record holds the next
-    // sibling, skip): up to 8 children's mass and COM are read with their
-    // link, so the chain of dependent loads is walked once.
-    double cm[8], ccx[8], ccy[8], ccz[8];
-    int nc = 0;
-    int c = p + 1;
+  const int end = nodes[p].skip;
+  // One pass along the sibling chain (child c's record holds the next
+  // sibling, skip): up to 8 children's mass and COM are read with their
+  // link, so the chain of dependent loads is walked once.
+  double cm[8], ccx[8], ccy[8], ccz[8];
+  int nc = 0;
+  int c = p + 1;
 #pragma unroll
-    for (int k = 0; k < 8; k++) {
-      if (c < end) {
-        cm[k] = nodes[c].m;
-        ccx[k] = nodes[c].cx;
-        ccy[k] = nodes[c].cy;
-        ccz[k] = nodes[c].cz;
-        parent[c] = p;
-        c = max(nodes[c].skip, c + 1);
-        nc = k + 1;
-      }
+  for (int k = 0; k < 8; k++) {
+    if (c < end) {
+      cm[k] = nodes[c].m;
+      ccx[k] = nodes[c].cx;
+      ccy[k] = nodes[c].cy;
+      ccz[k] = nodes[c].cz;
+      parent[c] = p;
+      c = max(nodes[c].skip, c + 1);
+      nc = k + 1;
     }
-    double M = 0;
-    double cxv = 0, cyv = 0, czv = 0;
-    if (c >= end) {
+  }
+  double M = 0;
+  double cxv = 0, cyv = 0, czv = 0;
+  if (c >= end) {
 #pragma unroll
-      for (int k = 0; k < 8; k++)
-        if (k < nc) M = M + cm[k];
-      if (M != 0) {  // else COM (0,0,0), src/octree.cpp:259-262
-        double tx = 0, ty = 0, tz = 0;
+    for (int k = 0; k < 8; k++)
+      if (k < nc) M = M + cm[k];
+    if (M != 0) {  // else COM (0,0,0), src/octree.cpp:259-262
+      double tx = 0, ty = 0, tz = 0;
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
-          if (k < nc) {
-            const double px = ccx[k] * cm[k];
-            const double py = ccy[k] * cm[k];
-            const double pz = ccz[k] * cm[k];
-            tx = tx + px;
-            ty = ty + py;
-            tz = tz + pz;
-          }
-        }
-        cxv = tx / M;
-        cyv = ty / M;
-        czv = tz / M;
-      }
-    } else {
-      // a depth-42 bucket with more than 8 particles: the same sums over
-      // the whole chain
-      for (c = p + 1; c < end; c = max(nodes[c].skip, c + 1)) {
-        M = M + nodes[c].m;
-        parent[c] = p;
-      }
-      if (M != 0) {
-        double tx = 0, ty = 0, tz = 0;
-        for (c = p + 1; c < end; c = max(nodes[c].skip, c + 1)) {
-          const double cmv = nodes[c].m;
-          const double px = nodes[c].cx * cmv;
-          const double py = nodes[c].cy * cmv;
-          const double pz = nodes[c].cz * cmv;
+      for (int k = 0; k < 8; k++) {
+        if (k < nc) {
+          const double px = ccx[k] * cm[k];
+          const double py = ccy[k] * cm[k];
+          const double pz = ccz[k] * cm[k];
           tx = tx + px;
           ty = ty + py;
           tz = tz + pz;
         }
-        cxv = tx / M;
-        cyv = ty / M;
-        czv = tz / M;
       }
+      cxv = tx / M;
+      cyv = ty / M;
+      czv = tz / M;
     }
-    nodes[p].cx = cxv;
-    nodes[p].cy = cyv;
-    nodes[p].cz = czv;
-    nodes[p].m = M;
-    nodes[p].gm = kG * M;
-    nodes[p].gm32 = (float)(kG * M);
+  } else {
+    // a depth-42 bucket with more than 8 particles: the same sums over
+    // the whole chain
+    for (c = p + 1; c < end; c = max(nodes[c].skip, c + 1)) {
+      M = M + nodes[c].m;
+      parent[c] = p;
+    }
+    if (M != 0) {
+      double tx = 0, ty = 0, tz = 0;
+      for (c = p + 1; c < end; c = max(nodes[c].skip, c + 1)) {
+        const double cmv = nodes[c].m;
+        const double px = nodes[c].cx * cmv;
+        const double py = nodes[c].cy * cmv;
+        const double pz = nodes[c].cz * cmv;
+        tx = tx + px;
+        ty = ty + py;
+        tz = tz + pz;
+      }
+      cxv = tx / M;
+      cyv = ty / M;
+      czv = tz / M;
+    }
+  }
+  nodes[p].cx = cxv;
+  nodes[p].cy = cyv;
+  nodes[p].cz = czv;
+  nodes[p].m = M;
+  nodes[p].gm = kG * M;
+  nodes[p].gm32 = (float)(kG * M);
+}
+
+// One tree level: every internal cell of depth d.  Launched from the
+// deepest level up; kernel boundaries order the levels, so no fences are
+// needed.
+__global__ void __launch_bounds__(kBlock)
+    level_kernel(int d, int nblocks, const int* __restrict__ doff,
+                 const int* __restrict__ bucket, WalkNode* __restrict__ nodes,
+                 int* __restrict__ parent, const int* __restrict__ flags) {
+  if (*flags) return;
+  const int b0 = doff[d * nblocks], b1 = doff[(d + 1) * nblocks];
+  for (int q = b0 + blockIdx.x * kBlock + threadIdx.x; q < b1; q += gridDim.x * kBlock)
+    reduce_cell(bucket[q], nodes, parent);
+}
+
+// Levels d1 down to d0 in one workgroup, a barrier between levels (which
+// makes each level's results visible to the workgroup): for the levels
+// that hold few cells -- the deepest ones and, per system, the top ones --
+// one launch instead of one per level.
+constexpr int kFewBlock = 1024;
+__global__ void __launch_bounds__(kFewBlock)
+    few_levels_kernel(int d1, int d0, int nblocks, const int* __restrict__ doff,
+                      const int* __restrict__ bucket, WalkNode* __restrict__ nodes,
+                      int* __restrict__ parent, const int* __restrict__ flags) {
+  if (*flags) return;
+  for (int d = d1; d >= d0; d--) {
+    const int b0 = doff[d * nblocks], b1 = doff[(d + 1) * nblocks];
+    for (int q = b0 + threadIdx.x; q < b1; q += kFewBlock) reduce_cell(bucket[q], nodes, parent);
+    __syncthreads();
   }
 }
 
@@ -653,6 +736,8 @@ __global__ void __launch_bounds__(kBlock)
 // blocks shortened the widest levels (35 -> 27 us at N = 2^20) but made
 // each of the ~30 near-empty levels cost 4.4 us instead of ~1.8 us.
 constexpr int kLevelBlocks = 256;
+// First of the levels reduced together by one workgroup.
+constexpr int kDeepLevels = 24;
 
 inline int blocks_for(size_t n, int b) { return (int)((n + b - 1) / b); }
 
@@ -675,7 +760,7 @@ void launch_bbox(const State& st, Systems& sys, hipStream_t s) {
 
 void launch_root(Systems& sys, const StepParams& p, const int* flags,
                  hipStream_t s) {
-  root_kernel<<<sys.nsys, kBlock, 0, s>>>(
+  root_kernel<<<sys.nsys, kRootBlock, 0, s>>>(
       sys.bbox, sys.n_waves, sys.wave_start, sys.root, sys.nsys, p.root_mode,
       p.root_center[0], p.root_center[1], p.root_center[2], p.root_size, flags);
 }
@@ -723,7 +808,8 @@ void launch_permute(const State& src, State& dst, const unsigned* perm, int n,
   if (n) permute_kernel<<<blocks_for(n, kBlock), kBlock, 0, s>>>(src, dst, perm, n);
 }
 
-size_t depth_hist_ints(size_t n) { return (size_t)kDepthBins * blocks_for(n, kBlock) + 1; }
+int emit_blocks(size_t n) { return blocks_for(n, kBlock); }
+size_t depth_hist_ints(size_t n) { return 2 * (size_t)kDepthBins * blocks_for(n, kBlock) + 1; }
 
 void launch_emit(BuildBuffers& b, Systems& sys, const StepParams& p,
                  const State& st, WalkNode* nodes, TreeAux& aux, int n,
@@ -732,23 +818,34 @@ void launch_emit(BuildBuffers& b, Systems& sys, const StepParams& p,
   emit_count_kernel<<<nb, kBlock, 0, s>>>(sys.start, sys.nsys, n, b.khi, b.klo,
                                           b.counts, b.dhist);
   exclusive_scan(b.counts, b.offsets, n, b.scan_tmp, s);
-  exclusive_scan(b.dhist, b.doff, kDepthBins * nb, b.scan_tmp, s);
+  exclusive_scan(b.dhist, b.doff, 2 * kDepthBins * nb, b.scan_tmp, s);
+  close_kernel<<<nb, kBlock, 0, s>>>(sys.start, sys.nsys, n, b.khi, b.klo, b.doff,
+                                     b.close_at);
   sys_nodes_kernel<<<blocks_for(sys.nsys + 1, 64), 64, 0, s>>>(
       sys.start, b.offsets, sys.nsys, sys.node_start);
-  launch_fill_int(b.stats, 0, 4, s);
   emit_kernel<<<nb, kBlock, 0, s>>>(
-      sys.start, sys.nsys, n, b.khi, b.klo, b.offsets, b.doff, sys.root,
+      sys.start, sys.nsys, n, b.khi, b.klo, b.offsets, b.doff, b.close_at, sys.root,
       p.theta, cap, st.x, st.y, st.z, st.m, nodes, aux, b.bucket, b.flags,
-      b.stats);
+      b.stats, b.blkstats);
 }
 
 void launch_bottom_up(BuildBuffers& b, WalkNode* nodes, TreeAux& aux, int n,
-                      int max_depth, hipStream_t s) {
+                      int nsys, hipStream_t s) {
   const int nb = blocks_for(n, kBlock);
-  const int top = max_depth < kLevels ? max_depth : kLevels;
-  for (int d = top; d >= 0; d--)
+  // one launch for the levels from kDeepLevels down (rarely populated: a
+  // cell there is under 2^-24 of its root's side), one per level above, and
+  // one for the top levels of all systems while they hold at most
+  // kFewBlock cells (nsys * 8^d)
+  int top = -1;
+  for (long long c = nsys; c <= kFewBlock; c *= 8) top++;
+  few_levels_kernel<<<1, kFewBlock, 0, s>>>(kLevels, kDeepLevels, nb, b.doff, b.bucket, nodes,
+                                            aux.parent, b.flags);
+  for (int d = kDeepLevels - 1; d > top; d--)
     level_kernel<<<kLevelBlocks, kBlock, 0, s>>>(d, nb, b.doff, b.bucket, nodes,
                                                  aux.parent, b.flags);
+  if (top >= 0)
+    few_levels_kernel<<<1, kFewBlock, 0, s>>>(top, 0, nb, b.doff, b.bucket, nodes, aux.parent,
+                                              b.flags);
 }
 
 void launch_leaf_refresh(BuildBuffers& b, const State& st, WalkNode* nodes,
