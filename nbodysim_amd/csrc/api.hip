@@ -355,8 +355,6 @@ int setup_systems(nbs_ctx* c) {
   s.root = dalloc<double>(o, 4 * (size_t)nsys, &e);
   s.dt = dalloc<double>(o, nsys, &e);
   s.time = dalloc<double>(o, nsys, &e);
-  s.amax = dalloc<unsigned long long>(o, nsys, &e);
-  s.vmax = dalloc<unsigned long long>(o, nsys, &e);
   if (e != hipSuccess) return hip_fail(c, e, "hipMalloc(system buffers)");
   c->h_start.assign(nsys + 1, 0);
   c->h_wave_start.assign(nsys + 1, 0);
@@ -374,6 +372,8 @@ int setup_systems(nbs_ctx* c) {
   s.n_waves2 = c->h_wave2_start[nsys];
   c->prim_levels = max_prim_levels(nsys);
   s.bbox = dalloc<double>(o, 6 * (size_t)std::max(s.n_waves, 1), &e);
+  s.amax = dalloc<double>(o, std::max(s.n_waves, 1), &e);
+  s.vmax = dalloc<double>(o, std::max(s.n_waves, 1), &e);
   c->inter_total = dalloc<unsigned long long>(o, std::max(s.n_waves, 1), &e);
   c->tile_cost = dalloc<unsigned>(o, std::max(s.n_waves, 1), &e);
   c->tile_order = dalloc<int>(o, std::max(s.n_waves, 1), &e);
@@ -393,8 +393,6 @@ int setup_systems(nbs_ctx* c) {
                            hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(s.dt, dts.data(), nsys * sizeof(double),
                            hipMemcpyHostToDevice, c->stream));
-  launch_fill_u64(s.amax, 0, nsys, c->stream);
-  launch_fill_u64(s.vmax, 0, nsys, c->stream);
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->bbox_valid = false;
   c->tree_valid = false;

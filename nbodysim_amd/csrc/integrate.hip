@@ -18,6 +18,8 @@ namespace {
 
 constexpr int kBlock = 256;
 
+// Per-wave maxima of |a| and |v| (plain stores of wave partials; the
+// per-system reduction follows in adaptive_dt_kernel).
 __global__ void __launch_bounds__(kBlock)
     adaptive_max_kernel(const double* __restrict__ vx,
                         const double* __restrict__ vy,
@@ -27,8 +29,7 @@ __global__ void __launch_bounds__(kBlock)
                         const double* __restrict__ az,
                         const int* __restrict__ start,
                         const int* __restrict__ wstart, int nsys, int nwaves,
-                        unsigned long long* __restrict__ amax,
-                        unsigned long long* __restrict__ vmax,
+                        double* __restrict__ amax, double* __restrict__ vmax,
                         const int* __restrict__ flags) {
   if (*flags) return;
   const int w = (blockIdx.x * kBlock + threadIdx.x) >> 6;
@@ -44,31 +45,48 @@ __global__ void __launch_bounds__(kBlock)
   a2 = wave_max(a2);
   v2 = wave_max(v2);
   if (lane_id() == 0) {
-    // non-negative doubles order like their bit patterns
-    atomicMax(amax + s, (unsigned long long)__double_as_longlong(a2));
-    atomicMax(vmax + s, (unsigned long long)__double_as_longlong(v2));
+    amax[w] = a2;
+    vmax[w] = v2;
   }
 }
 
-__global__ void adaptive_dt_kernel(unsigned long long* __restrict__ amax,
-                                   unsigned long long* __restrict__ vmax,
-                                   int nsys, double dt, double min_dt,
-                                   double max_dp, double max_dv,
-                                   double* __restrict__ sys_dt,
-                                   double* __restrict__ sys_time,
-                                   const int* __restrict__ flags) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+// One workgroup per system: the system's maxima over its wave partials,
+// then its step length.
+constexpr int kDtBlock = 1024;
+__global__ void __launch_bounds__(kDtBlock)
+    adaptive_dt_kernel(const double* __restrict__ amax, const double* __restrict__ vmax,
+                       const int* __restrict__ wstart, int nsys, double dt, double min_dt,
+                       double max_dp, double max_dv, double* __restrict__ sys_dt,
+                       double* __restrict__ sys_time, const int* __restrict__ flags) {
+  const int s = blockIdx.x;
   if (s >= nsys || *flags) return;
-  const double am = __longlong_as_double((long long)amax[s]);
-  const double vm = __longlong_as_double((long long)vmax[s]);
+  __shared__ double red[2][kDtBlock / 64];
+  double am = 0, vm = 0;
+#pragma unroll 4
+  for (int w = wstart[s] + threadIdx.x; w < wstart[s + 1]; w += kDtBlock) {
+    am = fmax(am, amax[w]);
+    vm = fmax(vm, vmax[w]);
+  }
+  am = wave_max(am);
+  vm = wave_max(vm);
+  if (lane_id() == 0) {
+    red[0][threadIdx.x >> 6] = am;
+    red[1][threadIdx.x >> 6] = vm;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int q = 1; q < kDtBlock / 64; q++) {
+    am = fmax(am, red[0][q]);
+    vm = fmax(vm, red[1][q]);
+  }
+  am = fmax(am, red[0][0]);
+  vm = fmax(vm, red[1][0]);
   double h = dt;
   if (am > 0) h = fmin(h, max_dv / am);
   if (vm > 0) h = fmin(h, max_dp / vm);
   h = fmax(h, min_dt);
   sys_dt[s] = h;
   sys_time[s] += h;
-  amax[s] = 0;
-  vmax[s] = 0;
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -119,9 +137,9 @@ void launch_adaptive_dt(const State& st, const double* ax, const double* ay,
     adaptive_max_kernel<<<(nw * 64 + kBlock - 1) / kBlock, kBlock, 0, s>>>(
         st.vx, st.vy, st.vz, ax, ay, az, sys.start, sys.wave_start, sys.nsys,
         nw, sys.amax, sys.vmax, flags);
-  adaptive_dt_kernel<<<(sys.nsys + 63) / 64, 64, 0, s>>>(
-      sys.amax, sys.vmax, sys.nsys, p.dt, p.min_adaptive_dt, p.max_pos_change,
-      p.max_vel_change, sys.dt, sys.time, flags);
+  adaptive_dt_kernel<<<sys.nsys, kDtBlock, 0, s>>>(
+      sys.amax, sys.vmax, sys.wave_start, sys.nsys, p.dt, p.min_adaptive_dt,
+      p.max_pos_change, p.max_vel_change, sys.dt, sys.time, flags);
 }
 
 void launch_update(State& st, const double* ax, const double* ay,

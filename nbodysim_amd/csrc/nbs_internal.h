@@ -58,8 +58,8 @@ struct Systems {
   double* root;      // [nsys*4] centre xyz, side
   double* dt;        // [nsys] step length used by the last update
   double* time;      // [nsys] simulated time
-  unsigned long long* amax;  // [nsys] adaptive: max |a| (ordered bits)
-  unsigned long long* vmax;  // [nsys] adaptive: max |v|
+  double* amax;      // [n_waves] adaptive: per-wave max |a|
+  double* vmax;      // [n_waves] adaptive: per-wave max |v|
 };
 
 struct State {

@@ -189,10 +189,12 @@ __global__ void gather_keys_kernel(const unsigned* __restrict__ perm, int n,
 // resolution, common only while a cold system collapses) are sorted by
 // (full key, caller index).  One thread per run start: a run of up to
 // kShortRun entries is insertion-sorted in place; a longer one is listed
-// for long_run_kernel.  runs[0] counts listed runs, runs[1] is
+// for long_run_kernel.  (Runs are many and short when they occur: a cold
+// 4M-particle shell passing through its centre puts ~90 % of its
+// particles into ~10^5 runs, the longest a few hundred entries.)  runs[0] counts listed runs, runs[1] is
 // long_run_kernel's work counter, runs[2 + 2r], runs[3 + 2r] = start,
 // length of run r.
-constexpr int kShortRun = 32;
+constexpr int kShortRun = 8;
 
 __global__ void fixup_kernel(const unsigned long long* __restrict__ prim,
                              unsigned* __restrict__ perm,
@@ -253,16 +255,19 @@ __global__ void fixup_kernel(const unsigned long long* __restrict__ prim,
   }
 }
 
-// A listed run sorted by (key_hi, key_lo, caller index) by one workgroup:
-// stable LSD radix sort, 8-bit digits, least significant first (caller
-// index bytes 0-3, key_lo bytes 0-7, key_hi bytes 0-7), skipping digits
-// that are equal over the whole run; entries ping-pong between the run's
-// range of (khi, klo, perm) and the same range of three free arrays.
-// Each chunk of 1024 entries is ranked stably with ballots (equal digits
-// within a wave) and per-wave digit counts.
+// A listed run sorted by (key_hi, key_lo, caller index) by one workgroup.
+// Up to kRunBitonic entries: a bitonic sort in LDS (the triple is unique,
+// so stability does not arise).  Longer runs: stable LSD radix sort, 8-bit
+// digits, least significant first (caller index bytes 0-3, key_lo bytes
+// 0-7, key_hi bytes 0-7), skipping digits that are equal over the whole
+// run; entries ping-pong between the run's range of (khi, klo, perm) and
+// the same range of three free arrays.  Each chunk of 1024 entries is
+// ranked stably with ballots (equal digits within a wave) and per-wave
+// digit counts.  Workgroups take runs from a queue (runs[1]).
 constexpr int kRunBlock = 1024;
 constexpr int kRunWaves = kRunBlock / 64;
-constexpr int kRunGrid = 64;
+constexpr int kRunGrid = 1024;
+constexpr int kRunBitonic = 1024;
 
 __global__ void __launch_bounds__(kRunBlock)
     long_run_kernel(int* __restrict__ runs, unsigned long long* __restrict__ khi,
@@ -274,6 +279,8 @@ __global__ void __launch_bounds__(kRunBlock)
   __shared__ int total[256];
   __shared__ int run_sh;
   __shared__ int skip_sh;
+  __shared__ unsigned long long bh[kRunBitonic], bl[kRunBitonic];
+  __shared__ unsigned bid[kRunBitonic], bp[kRunBitonic];
   const int t = threadIdx.x, lane = lane_id(), w = t >> 6;
   for (;;) {
     if (t == 0) run_sh = atomicAdd(&runs[1], 1);
@@ -284,6 +291,50 @@ __global__ void __launch_bounds__(kRunBlock)
     const int beg = runs[2 + 2 * r], len = runs[3 + 2 * r];
     unsigned long long *h0 = khi + beg, *l0 = klo + beg, *h1 = shi + beg, *l1 = slo + beg;
     unsigned *p0 = perm + beg, *p1 = sperm + beg;
+    if (len <= kRunBitonic) {
+      int P = 2;
+      while (P < len) P <<= 1;
+      for (int e = t; e < P; e += kRunBlock) {
+        const bool v = e < len;
+        const unsigned pe = v ? p0[e] : 0u;
+        bh[e] = v ? h0[e] : ~0ull;
+        bl[e] = v ? l0[e] : ~0ull;
+        bid[e] = v ? id[pe] : ~0u;
+        bp[e] = pe;
+      }
+      __syncthreads();
+      for (int k = 2; k <= P; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+          for (int i = t; i < P; i += kRunBlock) {
+            const int q = i ^ j;
+            if (q > i) {
+              const bool gt = bh[i] > bh[q] ||
+                              (bh[i] == bh[q] && (bl[i] > bl[q] || (bl[i] == bl[q] && bid[i] > bid[q])));
+              if (gt == ((i & k) == 0)) {  // ascending where bit k of i is clear
+                const unsigned long long th = bh[i], tl = bl[i];
+                const unsigned ti = bid[i], tp = bp[i];
+                bh[i] = bh[q];
+                bl[i] = bl[q];
+                bid[i] = bid[q];
+                bp[i] = bp[q];
+                bh[q] = th;
+                bl[q] = tl;
+                bid[q] = ti;
+                bp[q] = tp;
+              }
+            }
+          }
+          __syncthreads();
+        }
+      }
+      for (int e = t; e < len; e += kRunBlock) {
+        h0[e] = bh[e];
+        l0[e] = bl[e];
+        p0[e] = bp[e];
+      }
+      __syncthreads();
+      continue;
+    }
     for (int pass = 0; pass < 20; pass++) {
       // digit of entry e of the current source arrays
       auto digit = [&](int e) -> int {
@@ -716,9 +767,8 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // Levels d1 down to d0 in one workgroup, a barrier between levels (which
-// makes each level's results visible to the workgroup): for the levels
-// that hold few cells -- the deepest ones and, per system, the top ones --
-// one launch instead of one per level.
+// makes each level's results visible to the workgroup): the top levels,
+// which hold few cells, in one launch instead of one per level.
 constexpr int kFewBlock = 1024;
 __global__ void __launch_bounds__(kFewBlock)
     few_levels_kernel(int d1, int d0, int nblocks, const int* __restrict__ doff,
@@ -736,8 +786,6 @@ __global__ void __launch_bounds__(kFewBlock)
 // blocks shortened the widest levels (35 -> 27 us at N = 2^20) but made
 // each of the ~30 near-empty levels cost 4.4 us instead of ~1.8 us.
 constexpr int kLevelBlocks = 256;
-// First of the levels reduced together by one workgroup.
-constexpr int kDeepLevels = 24;
 
 inline int blocks_for(size_t n, int b) { return (int)((n + b - 1) / b); }
 
@@ -832,15 +880,15 @@ void launch_emit(BuildBuffers& b, Systems& sys, const StepParams& p,
 void launch_bottom_up(BuildBuffers& b, WalkNode* nodes, TreeAux& aux, int n,
                       int nsys, hipStream_t s) {
   const int nb = blocks_for(n, kBlock);
-  // one launch for the levels from kDeepLevels down (rarely populated: a
-  // cell there is under 2^-24 of its root's side), one per level above, and
-  // one for the top levels of all systems while they hold at most
-  // kFewBlock cells (nsys * 8^d)
+  // One launch per level, except the top levels of all systems while they
+  // hold at most kFewBlock cells (nsys * 8^d), which one workgroup reduces
+  // in one launch.  (The deep levels keep a launch each although they are
+  // usually empty: in a collapse -- a cold shell passing through its
+  // centre -- nearly the whole tree lies below 2^-24 of the root's side,
+  // and one workgroup would then reduce millions of cells alone.)
   int top = -1;
   for (long long c = nsys; c <= kFewBlock; c *= 8) top++;
-  few_levels_kernel<<<1, kFewBlock, 0, s>>>(kLevels, kDeepLevels, nb, b.doff, b.bucket, nodes,
-                                            aux.parent, b.flags);
-  for (int d = kDeepLevels - 1; d > top; d--)
+  for (int d = kLevels; d > top; d--)
     level_kernel<<<kLevelBlocks, kBlock, 0, s>>>(d, nb, b.doff, b.bucket, nodes,
                                                  aux.parent, b.flags);
   if (top >= 0)

@@ -477,3 +477,35 @@ def test_hand_written_walk_equals_generic(case):
         np.testing.assert_array_equal(a, b)
     if case == "theta0":
         assert (got[1] == len(pos) - 1).all()
+
+
+def test_many_runs_of_equal_primary_keys():
+    """Many moderate runs of equal primary sort keys, as a cold shell makes
+    while it passes through its centre: 150 clusters of 9-400 particles,
+    each inside a 2e-6 cube (below the 21-level primary key's resolution of
+    a 1000-wide root), 120 pairs and triples (the in-place path), and a
+    background cloud (~34000 particles, 37 levels).  Tree (order, membership, mass/COM) as the
+    reference's, and graph replays equal eager steps."""
+    rng = np.random.default_rng(12)
+    parts = [rng.uniform(-500, 500, size=(4000, 3))]
+    sizes = list(rng.integers(9, 400, size=150)) + list(rng.integers(2, 4, size=120))
+    for k, m in enumerate(sizes):
+        c = rng.uniform(-400, 400, size=3)
+        parts.append(c + rng.uniform(0, 2e-6, size=(int(m), 3)))
+    pos = np.concatenate(parts)
+    pos = pos[rng.permutation(len(pos))]
+    mass = rng.uniform(1e9, 1e10, size=len(pos))
+    vel = np.zeros_like(pos)
+    with sim(theta=0.5) as s:
+        s.set_state(pos, vel, mass)
+        acc = s.forces()
+        assert_tree_equal(s.tree_dump(), O.Tree(pos, mass).dump())
+        s.step(2, graph=True)
+        got = s.state()
+    oacc, _ = O.Tree(pos, mass).forces(theta=0.5)
+    assert rel_err(acc, oacc).max() < 1e-12
+    with sim(theta=0.5) as s:
+        s.set_state(pos, vel, mass)
+        s.step(2)
+        for a, b in zip(s.state()[:2], got[:2]):
+            np.testing.assert_array_equal(a, b)
