@@ -56,7 +56,8 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def time_path(sim, pos, vel, mass, precision, steps, warmup, prof_steps, shape, graph):
+def time_path(sim, pos, vel, mass, precision, steps, warmup, prof_steps, shape, graph,
+              order=False):
     """Timed region: `steps` graph-replayed steps (nbs_step_graph; eager
     launches with graph=False) after `warmup` steps.  Before it (untimed)
     the walk's launch shape is tuned on the initial state (nbs_autotune,
@@ -66,6 +67,7 @@ def time_path(sim, pos, vel, mass, precision, steps, warmup, prof_steps, shape, 
     those walks counted."""
     sim.configure(precision=precision)
     sim.set_state(pos, vel, mass)
+    sim.set_walk_order(order)
     if shape is None:
         shape, _ = sim.autotune(reps=3)
     else:
@@ -414,6 +416,8 @@ def main():
                          "workload, so the committed counter profiles apply)")
     ap.add_argument("--autotune", action="store_true",
                     help="let nbs_autotune pick the walk shape before the timed steps")
+    ap.add_argument("--walk-order", action="store_true",
+                    help="longest-first dispatch of the walk's workgroups (nbs_set_walk_order)")
     ap.add_argument("--eager", action="store_true",
                     help="timed steps launched one kernel at a time, not by graph replay")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
@@ -471,10 +475,10 @@ def main():
     shape = None if args.autotune else tuple(int(v) for v in args.walk_shape.split(","))
     graph = not args.eager
     r32 = time_path(sim, pos, vel, mass, "f32", args.steps, args.warmup, args.prof_steps,
-                    shape, graph)
+                    shape, graph, args.walk_order)
     log("f32", {k: v for k, v in r32.items() if k != "end_state"})
     r64 = time_path(sim, pos, vel, mass, "f64", args.fp64_steps, args.warmup, args.prof_steps,
-                    shape, graph)
+                    shape, graph, args.walk_order)
     log("f64", {k: v for k, v in r64.items() if k != "end_state"})
     sim.close()
 
@@ -506,7 +510,8 @@ def main():
                                "U[5e10,5e11], theta=0.5, dt=0.0333, bbox+2 root; FP32 walk "
                                "forces, FP64 state/keys/COM" % n,
                    "n": n, "theta": theta, "dt": dt, "parallelism": "single GPU",
-                   "walk_shape": r32["walk_shape"], "launch": "graph" if graph else "eager"},
+                   "walk_shape": r32["walk_shape"], "launch": "graph" if graph else "eager",
+                   "walk_order": "longest-first" if args.walk_order else "plain"},
         "interactions_per_s": inter_per_launch / (r32["ms_per_step"] * 1e-3),
         "interactions_per_particle": inter_per_launch / n,
         "fp64": {"value": value64, "ms_per_step": r64["ms_per_step"],

@@ -168,9 +168,12 @@ __device__ __forceinline__ void interact(const NodeRegs& nd, double dx, double d
 // instruction stream.  Per visit: 10 VALU instructions for d = com - x
 // (FP64, rounded to FP32), d^2 and the opening test, 8 for the force on
 // the interacting lanes, and scalar code for everything else; the common
-// case (a cell every lane accepts) takes 11 scalar ALU instructions and 5
-// branches.  The next record is requested before the force arithmetic so
-// the scalar-load latency overlaps it.
+// case (a cell every lane accepts) takes 10 scalar ALU instructions and 3
+// branches.  The walk waits on each next record (the scalar cache misses
+// ~90 % of them), so the next record is requested as early as it is known:
+// for a cell after the opening test, before the force arithmetic; for a
+// leaf (next = this + 1) as soon as the position differences have read the
+// current record.
 //
 // Fixed SGPRs (clobbered): s[64:79] the 64-byte node record (cx s[64:65],
 // cy s[66:67], cz s[68:69], skip s70, leaf s71, gm32 s76, r2open32 s77);
@@ -185,11 +188,11 @@ __device__ __forceinline__ void interact(const NodeRegs& nd, double dx, double d
 // The record after the last node of a system is requested (and not used)
 // when the walk ends: the node array holds one record beyond its capacity.
 // Record offsets are 32-bit: callers keep (capacity + 1) * 64 below 2^32.
-#define NBS_WALK_TAIL(MASK, L)                                        \
-  "s_lshl_b32 %[t], %[nx], 6\n\t"                                     \
-  "s_load_dwordx16 s[64:79], %[base], %[t]\n\t"                       \
+
+// The force on the lanes of MASK and the loop test (the next record has
+// been requested); NBS_WALK_TAIL requests it first.
+#define NBS_WALK_FORCE(MASK)                                          \
   "s_and_saveexec_b64 s[56:57], " MASK "\n\t"                          \
-  "s_cbranch_execz L_skip" L "%=\n\t"                                 \
   "v_rsq_f32 %[tv], %[d2]\n\t"                                        \
   "v_add_u32 %[cnt], 1, %[cnt]\n\t"                                   \
   "v_mul_f32 %[d2], %[tv], %[tv]\n\t"                                 \
@@ -197,21 +200,22 @@ __device__ __forceinline__ void interact(const NodeRegs& nd, double dx, double d
   "v_mul_f32 %[d2], %[gm], %[d2]\n\t"                                 \
   "v_fmac_f32 %[ax], %[d2], %[fx]\n\t"                                \
   "v_fmac_f32 %[ay], %[d2], %[fy]\n\t"                                \
-  "v_fmac_f32 %[az], %[d2], %[fz]\n"                                  \
-  "L_skip" L "%=:\n\t"                                                \
+  "v_fmac_f32 %[az], %[d2], %[fz]\n\t"                                \
   "s_mov_b64 exec, s[56:57]\n\t"                                      \
-  "s_cmp_eq_u32 %[tpos], %[nx]\n\t"                                   \
-  "s_cbranch_scc1 L_pop%=\n\t"                                        \
-  "s_mov_b32 %[n], %[nx]\n\t"                                         \
-  "s_cmp_lt_i32 %[n], %[nend]\n\t"                                    \
+  "s_cmp_lt_u32 %[n], %[lim]\n\t"                                     \
   "s_cbranch_scc1 L_loop%=\n\t"                                       \
-  "s_branch L_end%=\n"
+  "s_branch L_slow%=\n"
+
+#define NBS_WALK_TAIL(MASK)                                           \
+  "s_lshl_b32 %[t], %[n], 6\n\t"                                      \
+  "s_load_dwordx16 s[64:79], %[base], %[t]\n\t"                       \
+  NBS_WALK_FORCE(MASK)
 
 __device__ __forceinline__ void walk_loop_f32(ConstNodes nodes, int n, int nend, int pbeg,
                                               unsigned long long A, double x, double y,
                                               double z, float& ax, float& ay, float& az,
                                               int& cnt, int& overflow) {
-  int nx, top, tpos, t, gm, m0save;
+  int top, tpos, lim, t, gm, m0save;
   int sp, slo, shi;
   double dx, dy, dz;
   float fx, fy, fz, d2, tv;
@@ -223,12 +227,16 @@ __device__ __forceinline__ void walk_loop_f32(ConstNodes nodes, int n, int nend,
       "s_mov_b64 s[62:63], %[A0]\n\t"
       "s_mov_b32 %[top], 0\n\t"
       "s_mov_b32 %[tpos], -1\n\t"
+      "s_mov_b32 %[lim], %[nend]\n\t"
       "s_cmp_lt_i32 %[n], %[nend]\n\t"
       "s_cbranch_scc0 L_done%=\n\t"
       "s_lshl_b32 %[t], %[n], 6\n\t"
       "s_load_dwordx16 s[64:79], %[base], %[t]\n"
       "L_loop%=:\n\t"
       "s_waitcnt lgkmcnt(0)\n\t"
+      // leaf field: >= 0 a leaf, -1 a cell, -2 a depth-42 bucket
+      "s_cmp_gt_i32 s71, -1\n\t"
+      "s_cbranch_scc1 L_leaf%=\n\t"
       "v_add_f64 %[dx], s[64:65], -%[x]\n\t"
       "v_add_f64 %[dy], s[66:67], -%[y]\n\t"
       "v_add_f64 %[dz], s[68:69], -%[z]\n\t"
@@ -236,37 +244,23 @@ __device__ __forceinline__ void walk_loop_f32(ConstNodes nodes, int n, int nend,
       "v_cvt_f32_f64 %[fx], %[dx]\n\t"
       "v_cvt_f32_f64 %[fy], %[dy]\n\t"
       "v_cvt_f32_f64 %[fz], %[dz]\n\t"
-      "s_add_i32 %[nx], %[n], 1\n\t"
       "v_mul_f32 %[d2], %[fz], %[fz]\n\t"
       "v_fmac_f32 %[d2], %[fy], %[fy]\n\t"
       "v_fmac_f32 %[d2], %[fx], %[fx]\n\t"
-      // leaf field: -1 a cell, >= 0 a leaf, -2 a depth-42 bucket
-      "s_cmp_lg_u32 s71, -1\n\t"
-      "s_cbranch_scc1 L_notcell%=\n\t"
+      "s_cmp_eq_u32 s71, -2\n\t"
+      "s_cbranch_scc1 L_bucket%=\n\t"
       // cell: open where d^2 < (side/theta)^2
       "v_cmp_gt_f32 vcc, s77, %[d2]\n\t"
       "s_and_b64 s[58:59], vcc, s[62:63]\n\t"
       "s_cbranch_scc1 L_opened%=\n\t"
       // every lane that takes it accepts it: all interact, jump the
       // subtree (never backwards, even on a corrupt skip)
-      "s_max_i32 %[nx], %[nx], s70\n\t"
-      NBS_WALK_TAIL("s[62:63]", "a")
-      "L_notcell%=:\n\t"
-      "s_cmp_gt_i32 s71, -1\n\t"
-      "s_cbranch_scc0 L_bucket%=\n\t"
-      // leaf: every lane that takes it interacts, except its own particle's
-      "s_sub_u32 %[t], s71, %[pbeg]\n\t"
-      "s_mov_b64 s[60:61], s[62:63]\n\t"
-      "s_cmp_lt_u32 %[t], 64\n\t"
-      "s_cbranch_scc0 L_int%=\n\t"
-      "s_bitset0_b64 s[60:61], %[t]\n\t"
-      "s_branch L_int%=\n"
-      "L_bucket%=:\n\t"
-      // every lane opens a bucket; nobody interacts
-      "s_mov_b64 s[60:61], 0\n\t"
-      "s_branch L_int%=\n"
+      "s_add_i32 %[n], %[n], 1\n\t"
+      "s_max_i32 %[n], %[n], s70\n\t"
+      NBS_WALK_TAIL("s[62:63]")
       "L_opened%=:\n\t"
       // I = A & ~O interact and wait at skip; O descend (A = O)
+      "s_add_i32 %[n], %[n], 1\n\t"
       "s_andn2_b64 s[60:61], s[62:63], s[58:59]\n\t"
       "s_mov_b64 s[62:63], s[58:59]\n\t"
       "s_cbranch_scc0 L_int%=\n\t"
@@ -280,6 +274,7 @@ __device__ __forceinline__ void walk_loop_f32(ConstNodes nodes, int n, int nend,
       "v_writelane_b32 %[shi], s61, m0\n\t"
       "s_add_u32 %[top], %[top], 1\n\t"
       "s_mov_b32 %[tpos], s70\n\t"
+      "s_min_u32 %[lim], %[tpos], %[nend]\n\t"
       "s_branch L_int%=\n"
       "L_merge%=:\n\t"
       "s_sub_u32 m0, %[top], 1\n\t"
@@ -290,11 +285,43 @@ __device__ __forceinline__ void walk_loop_f32(ConstNodes nodes, int n, int nend,
       "v_writelane_b32 %[shi], s55, m0\n\t"
       "s_branch L_int%=\n"
       "L_ovf%=:\n\t"
-      "s_mov_b32 %[ovf], 1\n"
+      "s_mov_b32 %[ovf], 1\n\t"
+      "s_branch L_int%=\n"
+      "L_bucket%=:\n\t"
+      // every lane opens a bucket; nobody interacts
+      "s_add_i32 %[n], %[n], 1\n\t"
+      "s_mov_b64 s[60:61], 0\n"
       "L_int%=:\n\t"
-      NBS_WALK_TAIL("s[60:61]", "i")
-      "L_pop%=:\n\t"
-      // lanes waiting at the next node rejoin
+      NBS_WALK_TAIL("s[60:61]")
+      "L_leaf%=:\n\t"
+      // leaf: the next entry is known at once, so its record is requested
+      // as soon as the three differences have read this one's position;
+      // every lane that takes the leaf interacts, except its own
+      // particle's (bit leaf - pbeg when that is below 64)
+      "v_add_f64 %[dx], s[64:65], -%[x]\n\t"
+      "v_add_f64 %[dy], s[66:67], -%[y]\n\t"
+      "v_add_f64 %[dz], s[68:69], -%[z]\n\t"
+      "s_mov_b32 %[gm], s76\n\t"
+      "s_sub_u32 %[t], s71, %[pbeg]\n\t"
+      "s_lshl_b64 s[54:55], 1, %[t]\n\t"
+      "s_cmp_lt_u32 %[t], 64\n\t"
+      "s_cselect_b64 s[54:55], s[54:55], 0\n\t"
+      "s_andn2_b64 s[60:61], s[62:63], s[54:55]\n\t"
+      "s_add_i32 %[n], %[n], 1\n\t"
+      "s_lshl_b32 %[t], %[n], 6\n\t"
+      "s_load_dwordx16 s[64:79], %[base], %[t]\n\t"
+      "v_cvt_f32_f64 %[fx], %[dx]\n\t"
+      "v_cvt_f32_f64 %[fy], %[dy]\n\t"
+      "v_cvt_f32_f64 %[fz], %[dz]\n\t"
+      "v_mul_f32 %[d2], %[fz], %[fz]\n\t"
+      "v_fmac_f32 %[d2], %[fy], %[fy]\n\t"
+      "v_fmac_f32 %[d2], %[fx], %[fx]\n\t"
+      NBS_WALK_FORCE("s[60:61]")
+      "L_slow%=:\n\t"
+      // n reached lim = min(stack top, nend): the lanes waiting at n
+      // rejoin (the stack top is never passed), or the walk is done
+      "s_cmp_eq_u32 %[n], %[tpos]\n\t"
+      "s_cbranch_scc0 L_end%=\n\t"
       "s_sub_u32 %[top], %[top], 1\n\t"
       "v_readlane_b32 s54, %[slo], %[top]\n\t"
       "v_readlane_b32 s55, %[shi], %[top]\n\t"
@@ -302,15 +329,15 @@ __device__ __forceinline__ void walk_loop_f32(ConstNodes nodes, int n, int nend,
       "s_and_b32 %[t], %[t], 63\n\t"
       "v_readlane_b32 %[tpos], %[sp], %[t]\n\t"
       "s_or_b64 s[62:63], s[62:63], s[54:55]\n\t"
-      "s_mov_b32 %[n], %[nx]\n\t"
-      "s_cmp_lt_i32 %[n], %[nend]\n\t"
+      "s_min_u32 %[lim], %[tpos], %[nend]\n\t"
+      "s_cmp_lt_u32 %[n], %[lim]\n\t"
       "s_cbranch_scc1 L_loop%=\n"
       "L_end%=:\n\t"
       "s_waitcnt lgkmcnt(0)\n"
       "L_done%=:\n\t"
       "s_mov_b32 m0, %[m0save]"
       : [n] "+s"(n), [ovf] "+s"(overflow), [ax] "+v"(ax), [ay] "+v"(ay), [az] "+v"(az),
-        [cnt] "+v"(cnt), [nx] "=&s"(nx), [top] "=&s"(top), [tpos] "=&s"(tpos), [t] "=&s"(t),
+        [cnt] "+v"(cnt), [top] "=&s"(top), [tpos] "=&s"(tpos), [lim] "=&s"(lim), [t] "=&s"(t),
         [gm] "=&s"(gm), [m0save] "=&s"(m0save), [sp] "=&v"(sp), [slo] "=&v"(slo),
         [shi] "=&v"(shi), [dx] "=&v"(dx), [dy] "=&v"(dy), [dz] "=&v"(dz), [fx] "=&v"(fx),
         [fy] "=&v"(fy), [fz] "=&v"(fz), [d2] "=&v"(d2), [tv] "=&v"(tv)
@@ -321,6 +348,7 @@ __device__ __forceinline__ void walk_loop_f32(ConstNodes nodes, int n, int nend,
         "s78", "s79", "vcc", "scc", "memory");
 }
 #undef NBS_WALK_TAIL
+#undef NBS_WALK_FORCE
 
 // Workgroup -> tile order.  Workgroups are handed to the 8 XCDs round-robin
 // by workgroup id, each XCD with its own L2.  With `groups` > 1 the tiles
