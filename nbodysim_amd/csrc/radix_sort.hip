@@ -63,7 +63,14 @@ __global__ void __launch_bounds__(kRsBlock)
   if (t == 0) total[blockIdx.x] = carry_s;
 }
 
-__global__ void __launch_bounds__(kRsBlock)
+// Downsweep: 1024 threads, 4 keys each, rank a 4096-key tile in 4 rounds
+// (each: per-wave ballot matching, per-digit prefix over the 16 waves, a
+// stable scatter into LDS), then write the tile's digit runs out.
+constexpr int kDsBlock = 1024;
+constexpr int kDsItems = kRsTile / kDsBlock;
+constexpr int kDsWaves = kDsBlock / 64;
+
+__global__ void __launch_bounds__(kDsBlock)
     downsweep_kernel(const unsigned long long* __restrict__ kin,
                      const unsigned* __restrict__ vin,
                      unsigned long long* __restrict__ kout,
@@ -75,9 +82,10 @@ __global__ void __launch_bounds__(kRsBlock)
   __shared__ int tile_start[kRadix];
   __shared__ int gbase[kRadix];
   __shared__ int run[kRadix];
-  __shared__ int wcnt[kRsWaves][kRadix];
-  __shared__ int wsum[kRsWaves];
-  __shared__ int gsum[kRsWaves];
+  __shared__ int round_total[kRadix];
+  __shared__ int wcnt[kDsWaves][kRadix];
+  __shared__ int wsum[kRadix / 64];
+  __shared__ int gsum[kRadix / 64];
 
   const int t = threadIdx.x;
   const int lane = lane_id();
@@ -85,15 +93,15 @@ __global__ void __launch_bounds__(kRsBlock)
   const int base = blockIdx.x * kRsTile;
   const int tile_n = min(kRsTile, n - base);
 
-  unsigned long long key[kRsItems];
-  unsigned val[kRsItems];
-  int dig[kRsItems];
+  unsigned long long key[kDsItems];
+  unsigned val[kDsItems];
+  int dig[kDsItems];
 
-  run[t] = 0;
+  if (t < kRadix) run[t] = 0;
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < kRsItems; k++) {
-    int j = k * kRsBlock + t;
+  for (int k = 0; k < kDsItems; k++) {
+    const int j = k * kDsBlock + t;
     if (j < tile_n) {
       key[k] = kin[base + j];
       val[k] = vin[base + j];
@@ -108,17 +116,20 @@ __global__ void __launch_bounds__(kRsBlock)
   __syncthreads();
   // Tile-local exclusive scan of the digit counts, and the digit's global
   // start: exclusive scan of the digit totals plus this tile's offset
-  // within the digit (one digit per thread).
-  {
-    const int c = run[t];
-    const int g = total[t];
-    const int inc = wave_inclusive_scan(c);
-    const int ginc = wave_inclusive_scan(g);
+  // within the digit (one digit per thread of the first four waves).
+  int c = 0, g = 0, inc = 0, ginc = 0;
+  if (t < kRadix) {
+    c = run[t];
+    g = total[t];
+    inc = wave_inclusive_scan(c);
+    ginc = wave_inclusive_scan(g);
     if (lane == 63) {
       wsum[w] = inc;
       gsum[w] = ginc;
     }
-    __syncthreads();
+  }
+  __syncthreads();
+  if (t < kRadix) {
     int wb = 0, gb = 0;
     for (int k = 0; k < w; k++) {
       wb += wsum[k];
@@ -130,40 +141,43 @@ __global__ void __launch_bounds__(kRsBlock)
   }
   __syncthreads();
 
-  for (int k = 0; k < kRsItems; k++) {
-    wcnt[w][lane] = 0;
-    wcnt[w][lane + 64] = 0;
-    wcnt[w][lane + 128] = 0;
-    wcnt[w][lane + 192] = 0;
+  for (int k = 0; k < kDsItems; k++) {
+    for (int q = lane; q < kRadix; q += 64) wcnt[w][q] = 0;
     const int d = dig[k];
     const bool valid = d < kRadix;
     unsigned long long peers = ballot(valid);
 #pragma unroll
     for (int b = 0; b < 8; b++) {
-      bool bit = (d >> b) & 1;
-      unsigned long long bb = ballot(bit);
+      const bool bit = (d >> b) & 1;
+      const unsigned long long bb = ballot(bit);
       peers &= bit ? bb : ~bb;
     }
     const int rank = mbcnt(peers);
     if (valid && rank == 0) wcnt[w][d] = __popcll(peers);
     __syncthreads();
+    if (t < kRadix) {  // exclusive prefix over the waves, per digit
+      int acc = 0;
+      for (int q = 0; q < kDsWaves; q++) {
+        const int c = wcnt[q][t];
+        wcnt[q][t] = acc;
+        acc += c;
+      }
+      round_total[t] = acc;
+    }
+    __syncthreads();
     if (valid) {
-      int pos = tile_start[d] + run[d] + rank;
-      for (int q = 0; q < w; q++) pos += wcnt[q][d];
+      const int pos = tile_start[d] + run[d] + wcnt[w][d] + rank;
       sk[pos] = key[k];
       sv[pos] = val[k];
     }
     __syncthreads();
-    int add = 0;
-#pragma unroll
-    for (int q = 0; q < kRsWaves; q++) add += wcnt[q][t];
-    run[t] += add;
-    __syncthreads();
+    if (t < kRadix) run[t] += round_total[t];
   }
+  __syncthreads();
 
-  for (int j = t; j < tile_n; j += kRsBlock) {
-    unsigned long long kk = sk[j];
-    int d = (int)((kk >> shift) & 255);
+  for (int j = t; j < tile_n; j += kDsBlock) {
+    const unsigned long long kk = sk[j];
+    const int d = (int)((kk >> shift) & 255);
     const int o = gbase[d] + (j - tile_start[d]);
     if (o >= 0 && o < n) {
       kout[o] = kk;
@@ -195,7 +209,7 @@ int radix_sort_pairs(unsigned long long* k0, unsigned* v0,
   for (int shift = 0; shift < nbits; shift += 8) {
     upsweep_kernel<<<ntiles, kRsBlock, 0, s>>>(ka, n, shift, ntiles, hist);
     digit_scan_kernel<<<kRadix, kRsBlock, 0, s>>>(hist, ntiles, total);
-    downsweep_kernel<<<ntiles, kRsBlock, 0, s>>>(ka, va, kb, vb, n, shift,
+    downsweep_kernel<<<ntiles, kDsBlock, 0, s>>>(ka, va, kb, vb, n, shift,
                                                  ntiles, hist, total);
     unsigned long long* tk = ka;
     ka = kb;
