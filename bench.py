@@ -399,6 +399,99 @@ def config_adjoint(args):
     print(json.dumps(out), flush=True)
 
 
+def config_frames(args):
+    """Frame output at N = 1,048,576 (sphere, FP32 forces), the reference's
+    frame loop (src/main.cpp:1420-1492): per frame one step, then the state
+    downloaded and written as a binary frame (80 B per particle), the
+    previous frame's file removed once the new one is in place.  Variants:
+    steps only; download only, synchronous (pageable get_state) or through
+    the asynchronous pinned snapshot; download + write in the stepping
+    thread; synchronous download + background writer (dump_threaded);
+    asynchronous snapshot + background writer (what nbs_run_frames does)."""
+    import shutil
+    import tempfile
+    import threading
+    import nbodysim_amd
+    from nbodysim_amd import host
+    n, frames = args.n, args.frames
+    pos, vel, mass = host.generate(n, gen_type="sphere", seed=0xDEADBEEF, **SPHERE)
+    d = tempfile.mkdtemp(prefix="nbs_frames_", dir="/tmp")
+    sim = nbodysim_amd.Simulation(theta=0.5, dt=0.0333, precision="f32")
+    sim.set_state(pos, vel, mass)
+    sim.set_walk_shape(64, 8, 1)
+    sim.step(2)
+
+    def write(f, p, v, a):
+        host.write_binary(host.frame_filename(d, f, True), p, v, a, mass)
+        prev = host.frame_filename(d, f - 1, True)
+        if f > 0 and os.path.exists(prev):
+            os.remove(prev)
+
+    def run(mode):
+        writer, pending = None, None
+        t0 = time.perf_counter()
+        for f in range(frames):
+            sim.step(1)
+            if mode == "steps":
+                continue
+            if mode in ("download_sync", "sync_write"):
+                p, v, a = sim.state()
+                if mode == "sync_write":
+                    write(f, p, v, a)
+            elif mode == "threaded_writer":
+                if writer:
+                    writer.join()
+                p, v, a = sim.state()
+                writer = threading.Thread(target=write, args=(f, p, v, a))
+                writer.start()
+            else:  # download_async, async_snapshot_writer
+                if pending is not None:
+                    p, v, a = sim.snapshot_end(copy=False)
+                    if writer:
+                        writer.join()
+                        writer = None
+                    if mode == "async_snapshot_writer":
+                        writer = threading.Thread(target=write, args=(pending, p, v, a))
+                        writer.start()
+                sim.snapshot_begin()
+                pending = f
+        if pending is not None:
+            p, v, a = sim.snapshot_end(copy=False)
+            if writer:
+                writer.join()
+                writer = None
+            if mode == "async_snapshot_writer":
+                write(pending, p, v, a)
+        if writer:
+            writer.join()
+        sim.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / frames
+
+    res = {}
+    try:
+        for mode in ("steps", "download_sync", "download_async", "sync_write",
+                     "threaded_writer", "async_snapshot_writer"):
+            res[mode] = run(mode)
+            log("frames", mode, res[mode])
+        p, v, a = sim.state()
+        t0 = time.perf_counter()
+        for k in range(3):
+            write(frames + k, p, v, a)
+        write_ms = (time.perf_counter() - t0) * 1e3 / 3
+    finally:
+        sim.close()
+        shutil.rmtree(d, ignore_errors=True)
+    print(json.dumps({
+        "metric": "ms per frame (one step + a binary frame of N=%d particles)" % n,
+        "unit": "ms", "higher_is_better": False, "n_gpus": 1, "dtype": "f32",
+        "data": "synthetic (seeded gen_type sphere, the reference's generator)",
+        "config": {"workload": "gen_type sphere N=%d r=100 rotation 0.1 about z, theta 0.5, "
+                               "dt 0.0333, FP32 forces, eager steps, %d frames per variant, "
+                               "frames written to /tmp" % (n, frames)},
+        "ms_per_frame": res, "write_one_frame_ms": write_ms,
+        "frame_bytes": 80 * n}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -425,6 +518,9 @@ def main():
     ap.add_argument("--config", type=int, default=0, choices=(0, 3, 4, 5),
                     help="run one of the other workloads instead: 3 shell theta sweep, "
                          "4 batched systems, 5 adjoint")
+    ap.add_argument("--frames-study", action="store_true",
+                    help="frame output variants (download, snapshot, writer thread)")
+    ap.add_argument("--frames", type=int, default=20)
     ap.add_argument("--n3", type=int, default=1 << 22)
     ap.add_argument("--steps3", type=int, default=50)
     ap.add_argument("--systems", type=int, default=256)
@@ -443,6 +539,9 @@ def main():
         return
     if args.config:
         {3: config_shell, 4: config_batched, 5: config_adjoint}[args.config](args)
+        return
+    if args.frames_study:
+        config_frames(args)
         return
 
     import nbodysim_amd

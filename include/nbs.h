@@ -105,6 +105,23 @@ int nbs_set_systems(nbs_ctx* ctx, int nsys, const int64_t* offsets);
  * in particle::acceleration, src/particle.h:35). */
 int nbs_get_state(nbs_ctx* ctx, double* pos, double* vel, double* acc);
 
+/* Asynchronous snapshot for frame dumps: the reference copies every
+ * particle into its dump thread's set before writing
+ * (src/main.cpp:1462-1489, dump_threaded :768-788); here
+ * nbs_snapshot_begin enqueues, behind the work already submitted, a gather
+ * of pos, vel and acc into caller order on the device and a copy into a
+ * pinned host buffer on a second stream, and returns at once, so the copy
+ * overlaps the steps submitted next.  nbs_snapshot_end waits for that copy
+ * and returns [n][3] pos, vel, acc (acc zero before the first force
+ * evaluation) and n.  The context owns two pinned buffers and alternates
+ * between them: the arrays stay valid until the second nbs_snapshot_begin
+ * after the one they came from (or until a snapshot of more particles than
+ * any before, or nbs_destroy).  One snapshot may be pending at a time.  No
+ * counterpart in the reference's GPU boundary. */
+int nbs_snapshot_begin(nbs_ctx* ctx);
+int nbs_snapshot_end(nbs_ctx* ctx, const double** pos, const double** vel,
+                     const double** acc, size_t* n);
+
 /* Accelerations for the current positions: barnes_hut_cuda's role
  * (src/cuda_helper.cpp:218-246) including the tree build it was handed.
  * acc (host, may be NULL to keep the result on the device). */

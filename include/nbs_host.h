@@ -90,6 +90,15 @@ int64_t nbs_read_binary(const char* path, int prec32, size_t cap, double* pos,
  * with a binary file in dir; returns that frame + 1, or 0 if none. */
 int nbs_resume_frame(const char* dir, unsigned num_frames);
 
+/* The reference's main (src/main.cpp:1359-1504) as a call, used by the
+ * nbodysim command: reads settings from cfgfile (NULL: "settings.cfg"),
+ * resumes from ./data or generates, then runs the frame loop on the GPU in
+ * the current directory, frames written under ./data.  Frame f's state is
+ * snapshotted asynchronously (nbs_snapshot_begin) and written by a
+ * background thread while later frames are computed.  Returns 0 on
+ * success, 1 on any failure (where the reference exits). */
+int nbs_run_frames(const char* cfgfile);
+
 #ifdef __cplusplus
 }
 #endif

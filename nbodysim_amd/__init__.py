@@ -132,6 +132,26 @@ class Simulation:
         self._check(self._L.nbs_get_state(self._h, _dp(pos), _dp(vel), _dp(acc)))
         return pos, vel, acc
 
+    def snapshot_begin(self):
+        """Starts an asynchronous copy of pos, vel, acc (caller order) to a
+        pinned host buffer behind the work already submitted; the steps
+        submitted next overlap it."""
+        self._check(self._L.nbs_snapshot_begin(self._h))
+
+    def snapshot_end(self, copy=True):
+        """Waits for the pending snapshot; returns [n][3] pos, vel, acc.
+        With copy=False the arrays view the context's pinned buffer, valid
+        until the second following snapshot_begin."""
+        p, v, a = _native.c_dp(), _native.c_dp(), _native.c_dp()
+        n = ctypes.c_size_t()
+        self._check(self._L.nbs_snapshot_end(self._h, ctypes.byref(p), ctypes.byref(v),
+                                             ctypes.byref(a), ctypes.byref(n)))
+        n = n.value
+        if n == 0:
+            return np.zeros((0, 3)), np.zeros((0, 3)), np.zeros((0, 3))
+        out = tuple(np.ctypeslib.as_array(x, shape=(n, 3)) for x in (p, v, a))
+        return tuple(x.copy() for x in out) if copy else out
+
     # ----------------------------------------------------------------- step
     def forces(self, download=True):
         if not download:

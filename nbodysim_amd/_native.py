@@ -67,6 +67,10 @@ SIGNATURES = [
     ("nbs_set_state", ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, c_dp, c_dp, c_dp]),
     ("nbs_set_systems", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_i64p]),
     ("nbs_get_state", ctypes.c_int, [ctypes.c_void_p, c_dp, c_dp, c_dp]),
+    ("nbs_snapshot_begin", ctypes.c_int, [ctypes.c_void_p]),
+    ("nbs_snapshot_end", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(c_dp),
+                                        ctypes.POINTER(c_dp), ctypes.POINTER(c_dp),
+                                        ctypes.POINTER(ctypes.c_size_t)]),
     ("nbs_forces", ctypes.c_int, [ctypes.c_void_p, c_dp]),
     ("nbs_step", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("nbs_step_graph", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
@@ -108,6 +112,7 @@ SIGNATURES = [
     ("nbs_read_binary", ctypes.c_int64, [ctypes.c_char_p, ctypes.c_int, ctypes.c_size_t, c_dp,
                                          c_dp, c_dp, c_dp]),
     ("nbs_resume_frame", ctypes.c_int, [ctypes.c_char_p, ctypes.c_uint]),
+    ("nbs_run_frames", ctypes.c_int, [ctypes.c_char_p]),
 ]
 
 _lib = None

@@ -31,7 +31,7 @@ CXX_FLAGS = ["-std=c++17", "-O2", "-ffp-contract=off", "-fPIC", "-Wall",
 
 HIP_SOURCES = ["scan.hip", "radix_sort.hip", "tree_build.hip", "walk.hip",
                "integrate.hip", "adjoint.hip", "api.hip"]
-HOST_SOURCES = ["host/settings.cpp", "host/ic.cpp", "host/dump.cpp"]
+HOST_SOURCES = ["host/settings.cpp", "host/ic.cpp", "host/dump.cpp", "host/frames.cpp"]
 CLI_SOURCES = ["host/nbodysim_main.cpp"]
 
 
@@ -87,7 +87,7 @@ def build(lazy=False, jobs=None):
         cli_objs = list(ex.map(lambda s: _compile(s, lazy), CLI_SOURCES))
     if not lazy or _stale(LIBNBS, objs):
         _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC"] + objs +
-             ["-o", LIBNBS + ".tmp"])
+             ["-pthread", "-o", LIBNBS + ".tmp"])
         os.replace(LIBNBS + ".tmp", LIBNBS)
     if not lazy or _stale(CLI, cli_objs + [LIBNBS]):
         _run([CXX, "-o", CLI + ".tmp"] + cli_objs +

@@ -214,6 +214,19 @@ struct nbs_ctx {
   int mac_nodes_cap = 0;
   double* mac_x = nullptr;
   double *p_x = nullptr, *p_v = nullptr, *p_m = nullptr;
+
+  // Snapshots for frame dumps (nbs_snapshot_begin / _end): pos, vel, acc
+  // gathered into caller order [n][3] on the device, then copied on a
+  // second stream into one of two pinned host buffers while the following
+  // steps run on `stream`.
+  hipStream_t copy_stream = nullptr;
+  hipEvent_t snap_ready = nullptr, snap_done = nullptr;
+  double* snap_dev = nullptr;                  // 9 * snap_cap doubles
+  double* snap_host[2] = {nullptr, nullptr};   // pinned, 9 * snap_cap each
+  size_t snap_cap = 0;
+  int snap_next = 0;     // host buffer the next snapshot fills
+  int snap_pending = -1; // host buffer being filled, -1 when none
+  int snap_n = 0;
 };
 
 namespace {
@@ -625,6 +638,22 @@ int run_graph_pairs(nbs_ctx* c, int pairs) {
 
 }  // namespace
 
+namespace {
+
+void free_snapshots(nbs_ctx* c) {
+  if (c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);
+  if (c->snap_dev) (void)hipFree(c->snap_dev);
+  for (double*& h : c->snap_host) {
+    if (h) (void)hipHostFree(h);
+    h = nullptr;
+  }
+  c->snap_dev = nullptr;
+  c->snap_cap = 0;
+  c->snap_pending = -1;
+}
+
+}  // namespace
+
 extern "C" {
 
 void nbs_config_default(nbs_config* cfg) {
@@ -694,6 +723,10 @@ void nbs_destroy(nbs_ctx* c) {
   free_all(c->node_bufs);
   free_all(c->sys_bufs);
   free_all(c->misc_bufs);
+  free_snapshots(c);
+  if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
+  if (c->snap_ready) (void)hipEventDestroy(c->snap_ready);
+  if (c->snap_done) (void)hipEventDestroy(c->snap_done);
   (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -792,6 +825,71 @@ int nbs_get_state(nbs_ctx* c, double* pos, double* vel, double* acc) {
     }
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  return NBS_OK;
+}
+
+int nbs_snapshot_begin(nbs_ctx* c) {
+  if (!c) return NBS_ERR_ARG;
+  if (!c->has_state) return fail(c, NBS_ERR_STATE, "no particle state set");
+  if (c->snap_pending >= 0)
+    return fail(c, NBS_ERR_STATE, "end the pending snapshot first");
+  HIPCHK(c, hipSetDevice(c->device));
+  const size_t n = (size_t)c->n;
+  if (!c->copy_stream)
+    HIPCHK(c, hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
+  if (!c->snap_ready) HIPCHK(c, hipEventCreateWithFlags(&c->snap_ready, hipEventDisableTiming));
+  if (!c->snap_done) HIPCHK(c, hipEventCreateWithFlags(&c->snap_done, hipEventDisableTiming));
+  if (n > c->snap_cap) {
+    free_snapshots(c);
+    hipError_t e = hipMalloc(&c->snap_dev, 9 * n * sizeof(double));
+    for (double*& h : c->snap_host)
+      if (e == hipSuccess) e = hipHostMalloc(&h, 9 * n * sizeof(double), hipHostMallocDefault);
+    if (e != hipSuccess) {
+      free_snapshots(c);
+      return hip_fail(c, e, "snapshot buffers");
+    }
+    c->snap_cap = n;
+  }
+  const int b = c->snap_next;
+  if (n > 0) {
+    const State& st = c->st[c->cur];
+    const int blocks = (int)((n + 255) / 256);
+    double* d = c->snap_dev;
+    interleave_kernel<<<blocks, 256, 0, c->stream>>>(st.x, st.y, st.z, st.id, (int)n, d);
+    interleave_kernel<<<blocks, 256, 0, c->stream>>>(st.vx, st.vy, st.vz, st.id, (int)n,
+                                                     d + 3 * n);
+    if (c->acc_valid)
+      interleave_kernel<<<blocks, 256, 0, c->stream>>>(c->ax, c->ay, c->az, st.id, (int)n,
+                                                       d + 6 * n);
+    else
+      launch_fill_u64((unsigned long long*)(d + 6 * n), 0, 3 * n, c->stream);
+    HIPCHK(c, hipGetLastError());
+  }
+  HIPCHK(c, hipEventRecord(c->snap_ready, c->stream));
+  HIPCHK(c, hipStreamWaitEvent(c->copy_stream, c->snap_ready, 0));
+  if (n > 0)
+    HIPCHK(c, hipMemcpyAsync(c->snap_host[b], c->snap_dev, 9 * n * sizeof(double),
+                             hipMemcpyDeviceToHost, c->copy_stream));
+  HIPCHK(c, hipEventRecord(c->snap_done, c->copy_stream));
+  c->snap_pending = b;
+  c->snap_n = (int)n;
+  c->snap_next = b ^ 1;
+  return NBS_OK;
+}
+
+int nbs_snapshot_end(nbs_ctx* c, const double** pos, const double** vel,
+                     const double** acc, size_t* n) {
+  if (!c) return NBS_ERR_ARG;
+  if (c->snap_pending < 0) return fail(c, NBS_ERR_STATE, "no snapshot pending");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipEventSynchronize(c->snap_done));
+  const double* h = c->snap_host[c->snap_pending];
+  const size_t m = (size_t)c->snap_n;
+  c->snap_pending = -1;
+  if (pos) *pos = h;
+  if (vel) *vel = h ? h + 3 * m : nullptr;
+  if (acc) *acc = h ? h + 6 * m : nullptr;
+  if (n) *n = m;
   return NBS_OK;
 }
 

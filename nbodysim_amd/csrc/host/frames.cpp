@@ -1,0 +1,199 @@
+// frames.cpp -- the reference's frame loop (src/main.cpp:1359-1504) as a
+// library call: read settings, seed, resume from the newest ./data/NNNN.dat
+// or generate, then per frame run one step on the GPU and write the frame
+// files.  Frame f's state is copied to pinned host memory on a second
+// stream while frame f+1 is computed, and its files are written by a
+// background thread while frame f+2 is computed (the reference's
+// dump_threaded); the previous frame's files are removed only after the
+// new ones are in place.
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <cstdio>
+#include <cstring>
+#include <ctime>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../../include/nbs.h"
+#include "../../../include/nbs_host.h"
+
+namespace {
+
+bool exists(const char* p) {
+  struct stat st;
+  return stat(p, &st) == 0;
+}
+
+// One frame's arrays: pos, vel, acc point into the context's pinned
+// snapshot buffer ([n][3], caller order), mass is the host copy (masses do
+// not change).
+struct Frame {
+  unsigned frame = 0;
+  size_t n = 0;
+  const double *pos = nullptr, *vel = nullptr, *acc = nullptr, *mass = nullptr;
+};
+
+int write_frame(const nbs_settings& s, const Frame& f) {
+  char bn[4096], tn[4096], pbn[4096], ptn[4096];
+  nbs_frame_filename("./data", f.frame, 1, bn, sizeof(bn));
+  nbs_frame_filename("./data", f.frame, 0, tn, sizeof(tn));
+  for (const char* p : {bn, tn}) {
+    if (!s.overwrite_data && exists(p)) {
+      fprintf(stderr, "Overwrite disabled, cannot overwrite %s. Aborting.\n", p);
+      return NBS_ERR_IO;
+    }
+  }
+  if (s.dump_binary &&
+      nbs_write_binary(bn, f.n, f.pos, f.vel, f.acc, f.mass, 0) != NBS_OK) {
+    fprintf(stderr, "Cannot write %s\n", bn);
+    return NBS_ERR_IO;
+  }
+  if (s.dump_text && nbs_write_text(tn, f.n, f.pos) != NBS_OK) {
+    fprintf(stderr, "Cannot write %s\n", tn);
+    return NBS_ERR_IO;
+  }
+  if (f.frame > 0) {
+    nbs_frame_filename("./data", f.frame - 1, 1, pbn, sizeof(pbn));
+    nbs_frame_filename("./data", f.frame - 1, 0, ptn, sizeof(ptn));
+    if (!s.keep_previous_binary && exists(pbn)) unlink(pbn);
+    if (!s.keep_previous_text && exists(ptn)) unlink(ptn);
+  }
+  return NBS_OK;
+}
+
+}  // namespace
+
+extern "C" int nbs_run_frames(const char* cfgfile) {
+  nbs_settings s;
+  nbs_settings_default(&s);
+  if (!cfgfile) cfgfile = "settings.cfg";
+  printf("Reading config file %s\n", cfgfile);
+  char msg[8192];
+  int r = nbs_settings_read(&s, cfgfile, msg, sizeof(msg));
+  if (msg[0]) fputs(msg, stderr);
+  if (r != NBS_OK) return 1;
+  if (s.collide)
+    printf("WARNING: collision is not supported by the GPU step; ignored\n");
+  if (s.min_node_size > 0)
+    printf("WARNING: min_node_size is not supported by the GPU step; ignored\n");
+  if (s.dump_image)
+    printf("WARNING: image output is not built in; dump_image ignored\n");
+
+  const unsigned long seed = s.use_seed ? s.seed : (unsigned long)time(NULL);
+  std::vector<double> pos, vel, acc, mass;
+  unsigned start_frame = 0;
+  if (s.read_existing) {
+    start_frame = (unsigned)nbs_resume_frame("./data", s.num_frames);
+    if (start_frame == 0) {
+      printf("No data to resume from.\n");
+    } else {
+      char bn[4096];
+      nbs_frame_filename("./data", start_frame - 1, 1, bn, sizeof(bn));
+      const int64_t n = nbs_binary_count(bn, 0);
+      pos.resize(3 * n);
+      vel.resize(3 * n);
+      acc.resize(3 * n);
+      mass.resize(n);
+      if (nbs_read_binary(bn, 0, n, pos.data(), vel.data(), acc.data(),
+                          mass.data()) != n) {
+        fprintf(stderr, "Cannot read %s\n", bn);
+        return 1;
+      }
+      printf("Resuming from %u\n", start_frame);
+    }
+  }
+  if (!s.read_existing || start_frame == 0) {
+    printf("Seed: %lu\n", seed);
+    printf("Generating particles...\n");
+    const size_t n = s.num_particles;
+    pos.assign(3 * n, 0);
+    vel.assign(3 * n, 0);
+    acc.assign(3 * n, 0);
+    mass.assign(n, 0);
+    if (nbs_generate(&s, seed, n, 0, pos.data(), vel.data(), mass.data())) {
+      fprintf(stderr, "Invalid generation settings.\n");
+      return 1;
+    }
+  }
+
+  nbs_config cfg;
+  nbs_config_default(&cfg);
+  cfg.theta = s.theta;
+  cfg.dt = s.dt;
+  cfg.damping = s.damping;
+  cfg.force_precision = s.force_precision;
+  cfg.adaptive = s.adaptive;
+  cfg.min_adaptive_dt = s.min_adaptive_dt;
+  cfg.max_pos_change = s.max_pos_change;
+  cfg.max_vel_change = s.max_vel_change;
+  nbs_ctx* ctx = nullptr;
+  if (nbs_create(&cfg, &ctx) != NBS_OK) {
+    fprintf(stderr, "nbs_create failed\n");
+    return 1;
+  }
+  const size_t n = mass.size();
+  if (nbs_set_state(ctx, n, pos.data(), vel.data(), mass.data()) != NBS_OK) {
+    fprintf(stderr, "%s\n", nbs_last_error(ctx));
+    nbs_destroy(ctx);
+    return 1;
+  }
+  if (s.dump_binary || s.dump_text) mkdir("./data", 0755);
+
+  unsigned frame = start_frame;
+  printf("Frame %u/%u\n", frame, s.num_frames);
+  const bool dumping = s.dump_binary || s.dump_text;
+  std::thread writer;
+  int write_status = NBS_OK;
+  int status = 0;
+  Frame out;
+  bool pending = false;  // a snapshot of frame `pending_frame` is in flight
+  unsigned pending_frame = 0;
+  // Hands the finished snapshot to the writer thread.  The writer of the
+  // frame before reads the other pinned buffer; it is joined first, and
+  // must be before the next nbs_snapshot_begin reuses its buffer.
+  auto hand_over = [&]() -> bool {
+    Frame f;
+    f.frame = pending_frame;
+    f.mass = mass.data();
+    if (nbs_snapshot_end(ctx, &f.pos, &f.vel, &f.acc, &f.n) != NBS_OK) {
+      fprintf(stderr, "%s\n", nbs_last_error(ctx));
+      return false;
+    }
+    pending = false;
+    if (writer.joinable()) writer.join();
+    if (write_status != NBS_OK) return false;
+    if (s.verbose) printf("Dumping data...\n");
+    out = f;
+    writer = std::thread([&s, &out, &write_status] { write_status = write_frame(s, out); });
+    return true;
+  };
+  while (frame < s.num_frames) {
+    if (s.verbose) printf("Stepping...\n");
+    if (nbs_step(ctx, 1) != NBS_OK) {
+      fprintf(stderr, "step failed: %s\n", nbs_last_error(ctx));
+      status = 1;
+      break;
+    }
+    if (dumping) {
+      if (pending && !hand_over()) {
+        status = 1;
+        break;
+      }
+      if (nbs_snapshot_begin(ctx) != NBS_OK) {
+        fprintf(stderr, "%s\n", nbs_last_error(ctx));
+        status = 1;
+        break;
+      }
+      pending = true;
+      pending_frame = frame;
+    }
+    frame++;
+    printf("Frame %u/%u\n", frame, s.num_frames);
+  }
+  if (pending && !hand_over()) status = 1;
+  if (writer.joinable()) writer.join();
+  nbs_destroy(ctx);
+  return status == 0 && write_status == NBS_OK ? 0 : 1;
+}

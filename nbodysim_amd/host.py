@@ -132,3 +132,9 @@ def read_binary(path, prec="f64"):
 
 def resume_frame(directory, num_frames):
     return lib().nbs_resume_frame(directory.encode(), num_frames)
+
+
+def run_frames(cfgfile="settings.cfg"):
+    """The nbodysim frame loop in the current directory (nbs_run_frames);
+    returns its status (0 = success)."""
+    return lib().nbs_run_frames(cfgfile.encode())

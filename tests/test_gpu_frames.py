@@ -1,0 +1,141 @@
+"""Frame output: asynchronous snapshots (nbs_snapshot_begin / _end) against
+nbs_get_state, and the frame loop of the nbodysim command (nbs_run_frames,
+src/main.cpp:1359-1504) against the same steps taken through the library.
+Bar: bit-identical arrays (the same kernels produce both)."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from nbodysim_amd import NbsError, host
+
+pytestmark = pytest.mark.gpu
+
+
+def sim(**kw):
+    import nbodysim_amd
+    return nbodysim_amd.Simulation(**kw)
+
+
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+def test_snapshot_equals_get_state(precision):
+    pos, vel, mass = O.generate(5000, "sphere", seed=4)
+    with sim(theta=0.5, dt=0.0333, precision=precision) as s:
+        s.set_state(pos, vel, mass)
+        s.snapshot_begin()
+        p, v, a = s.snapshot_end()
+        np.testing.assert_array_equal(p, pos)
+        np.testing.assert_array_equal(v, vel)
+        assert not a.any()  # no force evaluation yet
+        s.step(3)
+        want = s.state()
+        s.snapshot_begin()
+        s.step(2)  # submitted while the copy of step 3's state is pending
+        got = s.snapshot_end()
+        for g, w in zip(got, want):
+            np.testing.assert_array_equal(g, w)
+        s.step(4, graph=True)
+        want = s.state()
+        s.snapshot_begin()
+        got = s.snapshot_end()
+        for g, w in zip(got, want):
+            np.testing.assert_array_equal(g, w)
+
+
+def test_snapshot_buffers_alternate():
+    pos, vel, mass = O.generate(3000, "cube", seed=2)
+    with sim(theta=0.5, dt=0.0333, precision="f64") as s:
+        s.set_state(pos, vel, mass)
+        s.step(1)
+        s.snapshot_begin()
+        first = s.snapshot_end(copy=False)
+        keep = [x.copy() for x in first]
+        s.step(1)
+        s.snapshot_begin()  # fills the other buffer: `first` stays intact
+        second = s.snapshot_end(copy=False)
+        for x, k in zip(first, keep):
+            np.testing.assert_array_equal(x, k)
+        assert not np.array_equal(second[0], keep[0])
+        np.testing.assert_array_equal(second[0], s.state()[0])
+        # one snapshot at a time; end without begin is an error
+        s.snapshot_begin()
+        with pytest.raises(NbsError):
+            s.snapshot_begin()
+        s.snapshot_end()
+        with pytest.raises(NbsError):
+            s.snapshot_end()
+
+
+def test_snapshot_batched_systems_caller_order():
+    parts = [O.generate(n, g, seed=10 + k) for k, (n, g) in
+             enumerate([(700, "cube"), (1500, "sphere"), (64, "shell")])]
+    pos = np.concatenate([p[0] for p in parts])
+    vel = np.concatenate([p[1] for p in parts])
+    mass = np.concatenate([p[2] for p in parts])
+    off = np.cumsum([0] + [len(p[2]) for p in parts])
+    with sim(theta=0.5, dt=0.0333, precision="f32") as s:
+        s.set_state(pos, vel, mass)
+        s.set_systems(off)
+        s.step(3)
+        want = s.state()
+        s.snapshot_begin()
+        got = s.snapshot_end()
+        for g, w in zip(got, want):
+            np.testing.assert_array_equal(g, w)
+
+
+SETTINGS = """use_seed true
+seed 3735928559
+num_particles 3000
+num_frames 4
+gen_type cube
+size 512
+min_mass 5e10
+max_mass 5e11
+min_vel 0
+max_vel 1
+theta 0.5
+dt 0.0333
+dump_binary true
+dump_text false
+dump_image false
+overwrite_data true
+force_precision {prec}
+keep_previous_binary {keep}
+"""
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_frame_loop_writes_the_library_steps(tmp_path, monkeypatch, prec):
+    monkeypatch.chdir(tmp_path)
+    with open("settings.cfg", "w") as f:
+        f.write(SETTINGS.format(prec=prec, keep="true"))
+    print("frame loop, force_precision", prec, flush=True)
+    assert host.run_frames("settings.cfg") == 0
+    settings = host.read_settings("settings.cfg")[0]
+    pos, vel, mass = host.generate(3000, gen_type="cube", seed=0xDEADBEEF, settings=settings)
+    with sim(theta=0.5, dt=0.0333, precision=prec) as s:
+        s.set_state(pos, vel, mass)
+        for frame in range(4):
+            s.step(1)
+            want = s.state()
+            got = host.read_binary(host.frame_filename("./data", frame, True))
+            for g, w in zip(got[:3], want):
+                np.testing.assert_array_equal(g, w)
+            np.testing.assert_array_equal(got[3], mass)
+    assert not [f for f in os.listdir("data") if f.endswith(".tmp")]
+
+
+def test_frame_loop_keeps_only_the_newest_frame(tmp_path, monkeypatch):
+    monkeypatch.chdir(tmp_path)
+    with open("settings.cfg", "w") as f:
+        f.write(SETTINGS.format(prec="f32", keep="false"))
+    assert host.run_frames("settings.cfg") == 0
+    assert sorted(os.listdir("data")) == ["0003.dat"]
+    # resume: the newest frame is read back and the loop continues from
+    # frame 4 up to num_frames (nothing left to do)
+    with open("settings.cfg", "a") as f:
+        f.write("read_existing true\n")
+    assert host.run_frames("settings.cfg") == 0
+    assert sorted(os.listdir("data")) == ["0003.dat"]
