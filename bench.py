@@ -226,12 +226,30 @@ def config_batched(args):
     sort, one walk launch per step, HIP-graph replay, autotuned walk shape)
     against one context and one launch sequence per system."""
     import nbodysim_amd
+    t0 = time.perf_counter()
     parts = batched_systems(args.systems)
+    ic_host_ms = (time.perf_counter() - t0) * 1e3
     pos = np.concatenate([p[0] for p in parts])
     vel = np.concatenate([p[1] for p in parts])
     mass = np.concatenate([p[2] for p in parts])
     off = np.cumsum([0] + [len(p[2]) for p in parts])
     n = int(off[-1])
+    # the same systems generated on the device (nbs_generate_state)
+    rng = np.random.default_rng(0xDEADBEEF)
+    sizes = np.rint(np.exp(rng.uniform(np.log(1024), np.log(65536), args.systems))).astype(int)
+    kinds = rng.integers(0, 3, args.systems)
+    systems = [(int(n_s), ("cube", "sphere", "shell")[k], 0xDEADBEEF + s_)
+               for s_, (n_s, k) in enumerate(zip(sizes, kinds))]
+    gen = nbodysim_amd.Simulation(theta=0.5, dt=0.0333, precision="f32")
+    gen.generate_state(systems=systems[:4], size=512.0, **SPHERE)  # first call: set-up
+    gen.synchronize()
+    t0 = time.perf_counter()
+    gen.generate_state(systems=systems, size=512.0, **SPHERE)
+    gen.synchronize()
+    ic_dev_ms = (time.perf_counter() - t0) * 1e3
+    ic_masses_equal = bool(np.array_equal(gen.masses(), mass))
+    gen.close()
+    log("ICs host %.1f ms, device %.1f ms" % (ic_host_ms, ic_dev_ms))
     sim = nbodysim_amd.Simulation(theta=0.5, dt=0.0333, precision="f32")
     sim.set_state(pos, vel, mass)
     sim.set_systems(off)
@@ -270,7 +288,11 @@ def config_batched(args):
                    "walk_shape": list(shape), "launch": "graph, one batched context"},
         "per_system_launches": {"value": n * 1e3 / ms_s, "ms_per_step": ms_s,
                                 "steps": steps_s},
-        "speedup_batched": ms_s / ms_b}), flush=True)
+        "speedup_batched": ms_s / ms_b,
+        "initial_conditions": {"host_ms": ic_host_ms, "device_ms": ic_dev_ms,
+                               "device_includes": "upload of per-system seeds/offsets, "
+                                                  "generation, context set-up",
+                               "masses_bit_identical": ic_masses_equal}}), flush=True)
 
 
 def config_adjoint(args):

@@ -105,6 +105,10 @@ int nbs_set_systems(nbs_ctx* ctx, int nsys, const int64_t* offsets);
  * in particle::acceleration, src/particle.h:35). */
 int nbs_get_state(nbs_ctx* ctx, double* pos, double* vel, double* acc);
 
+/* Downloads the masses in the caller's order (the mass field of the
+ * reference's particle record, src/particle.h:36). */
+int nbs_get_mass(nbs_ctx* ctx, double* mass);
+
 /* Asynchronous snapshot for frame dumps: the reference copies every
  * particle into its dump thread's set before writing
  * (src/main.cpp:1462-1489, dump_threaded :768-788); here

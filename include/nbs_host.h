@@ -10,6 +10,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "nbs.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -62,6 +64,21 @@ int nbs_settings_read(nbs_settings* s, const char* path, char* msg,
  * values widened exactly).  Outputs are [n][3] / [n]. */
 int nbs_generate(const nbs_settings* s, unsigned long seed, size_t n,
                  int prec32, double* pos, double* vel, double* mass);
+
+/* nbs_generate on the device, straight into a context's state (replaces
+ * nbs_generate + nbs_set_state + nbs_set_systems): system k holds caller
+ * indices [offsets[k], offsets[k+1]) generated from an engine seeded with
+ * seeds[k], gen type gen_types[k] (NULL: s->gen_type), every other key
+ * from s; one thread per particle reaches its first draw by LCG
+ * skip-ahead.  FP64 (-DDOUBLE) generation; mass_dist (and, for cubes,
+ * vel_dist) must be linear, since exp / normal draw a variable number of
+ * words (NBS_ERR_ARG otherwise).  Masses and cubes are bit-identical to
+ * nbs_generate; sphere and shell coordinates go through the device's
+ * cbrt, acos, sin, cos and may differ from glibc's in the last bits.
+ * Extends src/main.cpp:184-275 to the device; no reference counterpart. */
+int nbs_generate_state(nbs_ctx* ctx, const nbs_settings* s, int nsys,
+                       const int64_t* offsets, const uint64_t* seeds,
+                       const uint32_t* gen_types);
 
 /* "./data/NNNN.dat" / ".txt" names of src/main.cpp:283-291 under dir. */
 int nbs_frame_filename(const char* dir, unsigned frame, int binary,

@@ -121,6 +121,31 @@ class Simulation:
         self._check(self._L.nbs_set_state(self._h, n, _dp(pos), _dp(vel), _dp(mass)))
         self.n = n
 
+    def generate_state(self, n=0, gen_type="cube", seed=0xDEADBEEF, settings=None,
+                       systems=None, **kw):
+        """Seeded particles generated on the device straight into the state
+        (nbs_generate_state): n particles of gen_type from `seed`, or with
+        systems = [(n_s, gen_type, seed), ...] one batched system each.
+        Other settings as in host.generate (mass_dist linear)."""
+        from . import host as _host
+        d = dict(settings or _host.default_settings())
+        d["gen_type"] = gen_type
+        d.update(kw)
+        st = _host._from_dict(d)
+        if systems is None:
+            systems = [(n, gen_type, seed)]
+        off = np.ascontiguousarray(np.cumsum([0] + [int(x[0]) for x in systems]),
+                                   dtype=np.int64)
+        seeds = np.ascontiguousarray([int(x[2]) for x in systems], dtype=np.uint64)
+        kinds = np.ascontiguousarray([_host._ENUMS["gen_type"][x[1]] for x in systems],
+                                     dtype=np.uint32)
+        self._check(self._L.nbs_generate_state(
+            self._h, ctypes.byref(st), len(systems), off.ctypes.data_as(_native.c_i64p),
+            seeds.ctypes.data_as(_native.c_u64p),
+            kinds.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))))
+        self.n = int(off[-1])
+        return off
+
     def set_systems(self, offsets):
         off = np.ascontiguousarray(offsets, dtype=np.int64)
         self._check(self._L.nbs_set_systems(self._h, len(off) - 1,
@@ -131,6 +156,12 @@ class Simulation:
         pos, vel, acc = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
         self._check(self._L.nbs_get_state(self._h, _dp(pos), _dp(vel), _dp(acc)))
         return pos, vel, acc
+
+    def masses(self):
+        """Masses in the caller's order."""
+        m = np.zeros(self.n)
+        self._check(self._L.nbs_get_mass(self._h, _dp(m)))
+        return m
 
     def snapshot_begin(self):
         """Starts an asynchronous copy of pos, vel, acc (caller order) to a

@@ -67,6 +67,7 @@ SIGNATURES = [
     ("nbs_set_state", ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, c_dp, c_dp, c_dp]),
     ("nbs_set_systems", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_i64p]),
     ("nbs_get_state", ctypes.c_int, [ctypes.c_void_p, c_dp, c_dp, c_dp]),
+    ("nbs_get_mass", ctypes.c_int, [ctypes.c_void_p, c_dp]),
     ("nbs_snapshot_begin", ctypes.c_int, [ctypes.c_void_p]),
     ("nbs_snapshot_end", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(c_dp),
                                         ctypes.POINTER(c_dp), ctypes.POINTER(c_dp),
@@ -103,6 +104,9 @@ SIGNATURES = [
                                          ctypes.c_char_p, ctypes.c_size_t]),
     ("nbs_generate", ctypes.c_int, [ctypes.POINTER(Settings), ctypes.c_ulong, ctypes.c_size_t,
                                     ctypes.c_int, c_dp, c_dp, c_dp]),
+    ("nbs_generate_state", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Settings),
+                                          ctypes.c_int, c_i64p, c_u64p,
+                                          ctypes.POINTER(ctypes.c_uint32)]),
     ("nbs_frame_filename", ctypes.c_int, [ctypes.c_char_p, ctypes.c_uint, ctypes.c_int,
                                           ctypes.c_char_p, ctypes.c_size_t]),
     ("nbs_write_binary", ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t, c_dp, c_dp, c_dp,

@@ -30,7 +30,11 @@ CXX_FLAGS = ["-std=c++17", "-O2", "-ffp-contract=off", "-fPIC", "-Wall",
              "-I" + os.path.join(ROOT, "include")]
 
 HIP_SOURCES = ["scan.hip", "radix_sort.hip", "tree_build.hip", "walk.hip",
-               "integrate.hip", "adjoint.hip", "api.hip"]
+               "integrate.hip", "adjoint.hip", "ic_device.hip", "api.hip"]
+# Device initial conditions repeat the host generator's roundings: no
+# contraction into FMAs anywhere in that file (the device default would fuse
+# a multiply with the add that follows it, even across inlined helpers).
+FILE_FLAGS = {"ic_device.hip": ["-ffp-contract=off"]}
 HOST_SOURCES = ["host/settings.cpp", "host/ic.cpp", "host/dump.cpp", "host/frames.cpp"]
 CLI_SOURCES = ["host/nbodysim_main.cpp"]
 
@@ -69,7 +73,7 @@ def _compile(src, lazy):
     if lazy and not _stale(obj, _deps(path)):
         return obj
     if src.endswith(".hip"):
-        cmd = [HIPCC] + HIP_FLAGS + ["-c", path, "-o", obj + ".tmp"]
+        cmd = [HIPCC] + HIP_FLAGS + FILE_FLAGS.get(src, []) + ["-c", path, "-o", obj + ".tmp"]
     else:
         cmd = [CXX] + CXX_FLAGS + ["-c", path, "-o", obj + ".tmp"]
     _run(cmd)

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/nbs.h"
+#include "../../include/nbs_host.h"
 #include "nbs_device_util.h"
 #include "nbs_internal.h"
 
@@ -53,6 +54,13 @@ __global__ void interleave_kernel(const double* __restrict__ a,
   out[3 * (size_t)k] = a[i];
   out[3 * (size_t)k + 1] = b[i];
   out[3 * (size_t)k + 2] = c[i];
+}
+
+// slot order -> caller order, one value per particle
+__global__ void scatter_kernel(const double* __restrict__ a, const unsigned* __restrict__ id,
+                               int n, double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[id[i]] = a[i];
 }
 
 __global__ void gather_slots_kernel(const long long* __restrict__ ids, int m,
@@ -795,6 +803,92 @@ int nbs_set_systems(nbs_ctx* c, int nsys, const int64_t* off) {
   return setup_systems(c);
 }
 
+int nbs_generate_state(nbs_ctx* c, const nbs_settings* s, int nsys, const int64_t* off,
+                       const uint64_t* seeds, const uint32_t* gen_types) {
+  if (!c) return NBS_ERR_ARG;
+  if (!s || nsys < 1 || !off || !seeds)
+    return fail(c, NBS_ERR_ARG, "settings, nsys >= 1, offsets and seeds are required");
+  if (off[0] != 0) return fail(c, NBS_ERR_ARG, "offsets must start at 0");
+  for (int k = 0; k < nsys; k++)
+    if (off[k + 1] < off[k]) return fail(c, NBS_ERR_ARG, "offsets must not decrease");
+  const int64_t n = off[nsys];
+  if (n > 0x3fffffff) return fail(c, NBS_ERR_ARG, "n too large");
+  if (s->mass_dist != NBS_DIST_LINEAR)
+    return fail(c, NBS_ERR_ARG,
+                "device generation needs mass_dist linear (exp and normal draw a "
+                "variable number of words; use nbs_generate)");
+  std::vector<IcSystem> sys(nsys);
+  for (int k = 0; k < nsys; k++) {
+    const uint32_t t = gen_types ? gen_types[k] : s->gen_type;
+    if (t > NBS_GEN_SHELL) return fail(c, NBS_ERR_ARG, "gen_type out of range");
+    if (t == NBS_GEN_CUBE && s->vel_dist != NBS_DIST_LINEAR)
+      return fail(c, NBS_ERR_ARG, "device generation of cubes needs vel_dist linear");
+    sys[k].gen_type = (int)t;
+    // two engine words per double draw: mass, then the gen type's draws
+    sys[k].draws = 2 + (t == NBS_GEN_CUBE ? 14 : t == NBS_GEN_SPHERE ? 6
+                                            : (s->shell_jitter != 0 ? 6 : 4));
+    // minstd_rand0::seed: seed mod 2^31 - 1, with 0 replaced by 1
+    const uint64_t x0 = seeds[k] % 2147483647ull;
+    sys[k].x0 = (unsigned)(x0 ? x0 : 1);
+    sys[k].pad = 0;
+  }
+  IcParams p;
+  p.min_mass = s->min_mass;
+  p.max_mass = s->max_mass;
+  p.size = s->size;
+  p.min_vel = s->min_vel;
+  p.max_vel = s->max_vel;
+  p.r_sphere = s->r_sphere;
+  p.shell_jitter = s->shell_jitter;
+  p.scale[0] = s->scale_x;
+  p.scale[1] = s->scale_y;
+  p.scale[2] = s->scale_z;
+  {  // the rotation vector as generate_particle's caller forms it
+    const double r0 = s->rotation_vector[0], r1 = s->rotation_vector[1],
+                 r2 = s->rotation_vector[2];
+    const double rm = std::sqrt(r0 * r0 + r1 * r1 + r2 * r2);
+    for (int k = 0; k < 3; k++) p.rot[k] = s->rotation_vector[k] / rm * s->rotation_magnitude;
+  }
+  HIPCHK(c, hipSetDevice(c->device));
+  drop_graph(c);
+  int r = alloc_particles(c, (int)n);
+  if (r) return r;
+  c->n = (int)n;
+  c->cur = 0;
+  if (n > 0) {
+    std::vector<long long> h_off(off, off + nsys + 1);
+    const size_t off_bytes = (nsys + 1) * sizeof(long long);
+    void* tmp = nullptr;
+    HIPCHK(c, hipMalloc(&tmp, off_bytes + nsys * sizeof(IcSystem)));
+    long long* d_off = (long long*)tmp;
+    IcSystem* d_sys = (IcSystem*)((char*)tmp + off_bytes);
+    hipError_t e = hipMemcpyAsync(d_off, h_off.data(), off_bytes, hipMemcpyHostToDevice,
+                                  c->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(d_sys, sys.data(), nsys * sizeof(IcSystem), hipMemcpyHostToDevice,
+                         c->stream);
+    if (e == hipSuccess) {
+      launch_generate(p, nsys, d_off, d_sys, n, c->st[0], c->stream);
+      launch_fill_int(c->inter, 0, n, c->stream);
+      e = hipGetLastError();
+    }
+    const hipError_t e2 = hipStreamSynchronize(c->stream);
+    (void)hipFree(tmp);
+    if (e == hipSuccess) e = e2;
+    if (e != hipSuccess) return hip_fail(c, e, "device generation");
+  }
+  c->nsys_req = nsys;
+  if (nsys > 1)
+    c->sys_off_req.assign(off, off + nsys + 1);
+  else
+    c->sys_off_req.clear();
+  r = setup_systems(c);
+  if (r) return r;
+  c->has_state = true;
+  c->acc_valid = false;
+  return NBS_OK;
+}
+
 int nbs_get_state(nbs_ctx* c, double* pos, double* vel, double* acc) {
   if (!c) return NBS_ERR_ARG;
   if (!c->has_state) return fail(c, NBS_ERR_STATE, "no particle state set");
@@ -824,6 +918,21 @@ int nbs_get_state(nbs_ctx* c, double* pos, double* vel, double* acc) {
                                hipMemcpyDeviceToHost, c->stream));
     }
   }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return NBS_OK;
+}
+
+int nbs_get_mass(nbs_ctx* c, double* mass) {
+  if (!c || !mass) return NBS_ERR_ARG;
+  if (!c->has_state) return fail(c, NBS_ERR_STATE, "no particle state set");
+  HIPCHK(c, hipSetDevice(c->device));
+  const int n = c->n;
+  if (n == 0) return NBS_OK;
+  const State& st = c->st[c->cur];
+  scatter_kernel<<<(n + 255) / 256, 256, 0, c->stream>>>(st.m, st.id, n, c->stage_m);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(mass, c->stage_m, (size_t)n * sizeof(double),
+                           hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return NBS_OK;
 }

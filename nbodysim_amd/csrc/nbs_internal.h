@@ -253,4 +253,23 @@ void launch_fill_int(int* p, int v, size_t n, hipStream_t s);
 void launch_fill_u64(unsigned long long* p, unsigned long long v, size_t n,
                      hipStream_t s);
 
+// Device initial conditions (ic_device.hip): settings shared by all
+// systems, and per system its gen type, seeded engine state and draws per
+// particle.
+enum { kIcCube = 0, kIcSphere = 1, kIcShell = 2 };
+struct IcParams {
+  double min_mass, max_mass, size, min_vel, max_vel, r_sphere, shell_jitter;
+  double scale[3];
+  double rot[3];  // rotation_vector normalised, times rotation_magnitude
+};
+struct IcSystem {
+  int gen_type;
+  int draws;     // engine draws per particle
+  unsigned x0;   // engine state after seeding: seed mod 2^31 - 1, 0 -> 1
+  int pad;
+};
+// Particles [off[s], off[s + 1]) of system s into st (caller order, id = i).
+void launch_generate(const IcParams& p, int nsys, const long long* off, const IcSystem* sys,
+                     long long n, State st, hipStream_t s);
+
 }  // namespace nbs
