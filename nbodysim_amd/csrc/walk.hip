@@ -103,15 +103,13 @@ __device__ __forceinline__ bool mac_open_f64(double d2, double side,
   return open;
 }
 
+// 1/sqrt(d2) in FP64: the hardware estimate refined by one Newton-Raphson
+// step (relative error ~1e-15 after it; the estimate's ~2^-26 squares).
 __device__ __forceinline__ double rsqrt_f64(double d2) {
   double r = __builtin_amdgcn_rsq(d2);
-  double h = d2 * r;
-  double e = fma(-h, r, 1.0);
-  r = fma(0.5 * r, e, r);
-  h = d2 * r;
-  e = fma(-h, r, 1.0);
-  r = fma(0.5 * r, e, r);
-  return r;
+  const double h = d2 * r;
+  const double e = fma(-h, r, 1.0);
+  return fma(0.5 * r, e, r);
 }
 
 // Kick-drift update of src/particle.cpp:66-74: t = a dt; v += t;
