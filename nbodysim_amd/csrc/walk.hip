@@ -173,10 +173,10 @@ __device__ __forceinline__ void interact(const NodeRegs& nd, double dx, double d
 // leaf (next = this + 1) as soon as the position differences have read the
 // current record.
 //
-// Fixed SGPRs (clobbered): s[64:79] the 64-byte node record (cx s[64:65],
-// cy s[66:67], cz s[68:69], skip s70, leaf s71, gm32 s76, r2open32 s77);
-// s[62:63] A (lanes taking the node), s[60:61] I (lanes interacting),
-// s[58:59] O (lanes opening), s[56:57] saved exec, s[54:55] a popped
+// Fixed SGPRs (clobbered): s[48:63] the 64-byte node record (cx s[48:49],
+// cy s[50:51], cz s[52:53], skip s54, leaf s55, gm32 s60, r2open32 s61);
+// s[46:47] A (lanes taking the node), s[44:45] I (lanes interacting),
+// s[42:43] O (lanes opening), s[40:41] saved exec, s[38:39] a popped
 // stack mask.  The stack of waiting lanes is entry `slot` in lane `slot`
 // of sp (position), slo/shi (mask); lane 63 of sp stays -1, the position
 // an empty stack reports.  M0 carries the lane index of stack writes (a
@@ -190,7 +190,7 @@ __device__ __forceinline__ void interact(const NodeRegs& nd, double dx, double d
 // The force on the lanes of MASK and the loop test (the next record has
 // been requested); NBS_WALK_TAIL requests it first.
 #define NBS_WALK_FORCE(MASK)                                          \
-  "s_and_saveexec_b64 s[56:57], " MASK "\n\t"                          \
+  "s_and_saveexec_b64 s[40:41], " MASK "\n\t"                          \
   "v_rsq_f32 %[tv], %[d2]\n\t"                                        \
   "v_add_u32 %[cnt], 1, %[cnt]\n\t"                                   \
   "v_mul_f32 %[d2], %[tv], %[tv]\n\t"                                 \
@@ -199,14 +199,14 @@ __device__ __forceinline__ void interact(const NodeRegs& nd, double dx, double d
   "v_fmac_f32 %[ax], %[d2], %[fx]\n\t"                                \
   "v_fmac_f32 %[ay], %[d2], %[fy]\n\t"                                \
   "v_fmac_f32 %[az], %[d2], %[fz]\n\t"                                \
-  "s_mov_b64 exec, s[56:57]\n\t"                                      \
+  "s_mov_b64 exec, s[40:41]\n\t"                                      \
   "s_cmp_lt_u32 %[n], %[lim]\n\t"                                     \
   "s_cbranch_scc1 L_loop%=\n\t"                                       \
   "s_branch L_slow%=\n"
 
 #define NBS_WALK_TAIL(MASK)                                           \
   "s_lshl_b32 %[t], %[n], 6\n\t"                                      \
-  "s_load_dwordx16 s[64:79], %[base], %[t]\n\t"                       \
+  "s_load_dwordx16 s[48:63], %[base], %[t]\n\t"                       \
   NBS_WALK_FORCE(MASK)
 
 __device__ __forceinline__ void walk_loop_f32(ConstNodes nodes, int n, int nend, int pbeg,
@@ -222,65 +222,65 @@ __device__ __forceinline__ void walk_loop_f32(ConstNodes nodes, int n, int nend,
       "v_mov_b32 %[sp], -1\n\t"
       "v_mov_b32 %[slo], 0\n\t"
       "v_mov_b32 %[shi], 0\n\t"
-      "s_mov_b64 s[62:63], %[A0]\n\t"
+      "s_mov_b64 s[46:47], %[A0]\n\t"
       "s_mov_b32 %[top], 0\n\t"
       "s_mov_b32 %[tpos], -1\n\t"
       "s_mov_b32 %[lim], %[nend]\n\t"
       "s_cmp_lt_i32 %[n], %[nend]\n\t"
       "s_cbranch_scc0 L_done%=\n\t"
       "s_lshl_b32 %[t], %[n], 6\n\t"
-      "s_load_dwordx16 s[64:79], %[base], %[t]\n"
+      "s_load_dwordx16 s[48:63], %[base], %[t]\n"
       "L_loop%=:\n\t"
       "s_waitcnt lgkmcnt(0)\n\t"
       // leaf field: >= 0 a leaf, -1 a cell, -2 a depth-42 bucket
-      "s_cmp_gt_i32 s71, -1\n\t"
+      "s_cmp_gt_i32 s55, -1\n\t"
       "s_cbranch_scc1 L_leaf%=\n\t"
-      "v_add_f64 %[dx], s[64:65], -%[x]\n\t"
-      "v_add_f64 %[dy], s[66:67], -%[y]\n\t"
-      "v_add_f64 %[dz], s[68:69], -%[z]\n\t"
-      "s_mov_b32 %[gm], s76\n\t"
+      "v_add_f64 %[dx], s[48:49], -%[x]\n\t"
+      "v_add_f64 %[dy], s[50:51], -%[y]\n\t"
+      "v_add_f64 %[dz], s[52:53], -%[z]\n\t"
+      "s_mov_b32 %[gm], s60\n\t"
       "v_cvt_f32_f64 %[fx], %[dx]\n\t"
       "v_cvt_f32_f64 %[fy], %[dy]\n\t"
       "v_cvt_f32_f64 %[fz], %[dz]\n\t"
       "v_mul_f32 %[d2], %[fz], %[fz]\n\t"
       "v_fmac_f32 %[d2], %[fy], %[fy]\n\t"
       "v_fmac_f32 %[d2], %[fx], %[fx]\n\t"
-      "s_cmp_eq_u32 s71, -2\n\t"
+      "s_cmp_eq_u32 s55, -2\n\t"
       "s_cbranch_scc1 L_bucket%=\n\t"
       // cell: open where d^2 < (side/theta)^2
-      "v_cmp_gt_f32 vcc, s77, %[d2]\n\t"
-      "s_and_b64 s[58:59], vcc, s[62:63]\n\t"
+      "v_cmp_gt_f32 vcc, s61, %[d2]\n\t"
+      "s_and_b64 s[42:43], vcc, s[46:47]\n\t"
       "s_cbranch_scc1 L_opened%=\n\t"
       // every lane that takes it accepts it: all interact, jump the
       // subtree (never backwards, even on a corrupt skip)
       "s_add_i32 %[n], %[n], 1\n\t"
-      "s_max_i32 %[n], %[n], s70\n\t"
-      NBS_WALK_TAIL("s[62:63]")
+      "s_max_i32 %[n], %[n], s54\n\t"
+      NBS_WALK_TAIL("s[46:47]")
       "L_opened%=:\n\t"
       // I = A & ~O interact and wait at skip; O descend (A = O)
       "s_add_i32 %[n], %[n], 1\n\t"
-      "s_andn2_b64 s[60:61], s[62:63], s[58:59]\n\t"
-      "s_mov_b64 s[62:63], s[58:59]\n\t"
+      "s_andn2_b64 s[44:45], s[46:47], s[42:43]\n\t"
+      "s_mov_b64 s[46:47], s[42:43]\n\t"
       "s_cbranch_scc0 L_int%=\n\t"
-      "s_cmp_eq_u32 %[tpos], s70\n\t"
+      "s_cmp_eq_u32 %[tpos], s54\n\t"
       "s_cbranch_scc1 L_merge%=\n\t"
       "s_cmp_gt_u32 %[top], 62\n\t"
       "s_cbranch_scc1 L_ovf%=\n\t"
       "s_mov_b32 m0, %[top]\n\t"
-      "v_writelane_b32 %[sp], s70, m0\n\t"
-      "v_writelane_b32 %[slo], s60, m0\n\t"
-      "v_writelane_b32 %[shi], s61, m0\n\t"
+      "v_writelane_b32 %[sp], s54, m0\n\t"
+      "v_writelane_b32 %[slo], s44, m0\n\t"
+      "v_writelane_b32 %[shi], s45, m0\n\t"
       "s_add_u32 %[top], %[top], 1\n\t"
-      "s_mov_b32 %[tpos], s70\n\t"
+      "s_mov_b32 %[tpos], s54\n\t"
       "s_min_u32 %[lim], %[tpos], %[nend]\n\t"
       "s_branch L_int%=\n"
       "L_merge%=:\n\t"
       "s_sub_u32 m0, %[top], 1\n\t"
-      "v_readlane_b32 s54, %[slo], m0\n\t"
-      "v_readlane_b32 s55, %[shi], m0\n\t"
-      "s_or_b64 s[54:55], s[54:55], s[60:61]\n\t"
-      "v_writelane_b32 %[slo], s54, m0\n\t"
-      "v_writelane_b32 %[shi], s55, m0\n\t"
+      "v_readlane_b32 s38, %[slo], m0\n\t"
+      "v_readlane_b32 s39, %[shi], m0\n\t"
+      "s_or_b64 s[38:39], s[38:39], s[44:45]\n\t"
+      "v_writelane_b32 %[slo], s38, m0\n\t"
+      "v_writelane_b32 %[shi], s39, m0\n\t"
       "s_branch L_int%=\n"
       "L_ovf%=:\n\t"
       "s_mov_b32 %[ovf], 1\n\t"
@@ -288,45 +288,45 @@ __device__ __forceinline__ void walk_loop_f32(ConstNodes nodes, int n, int nend,
       "L_bucket%=:\n\t"
       // every lane opens a bucket; nobody interacts
       "s_add_i32 %[n], %[n], 1\n\t"
-      "s_mov_b64 s[60:61], 0\n"
+      "s_mov_b64 s[44:45], 0\n"
       "L_int%=:\n\t"
-      NBS_WALK_TAIL("s[60:61]")
+      NBS_WALK_TAIL("s[44:45]")
       "L_leaf%=:\n\t"
       // leaf: the next entry is known at once, so its record is requested
       // as soon as the three differences have read this one's position;
       // every lane that takes the leaf interacts, except its own
       // particle's (bit leaf - pbeg when that is below 64)
-      "v_add_f64 %[dx], s[64:65], -%[x]\n\t"
-      "v_add_f64 %[dy], s[66:67], -%[y]\n\t"
-      "v_add_f64 %[dz], s[68:69], -%[z]\n\t"
-      "s_mov_b32 %[gm], s76\n\t"
-      "s_sub_u32 %[t], s71, %[pbeg]\n\t"
-      "s_lshl_b64 s[54:55], 1, %[t]\n\t"
+      "v_add_f64 %[dx], s[48:49], -%[x]\n\t"
+      "v_add_f64 %[dy], s[50:51], -%[y]\n\t"
+      "v_add_f64 %[dz], s[52:53], -%[z]\n\t"
+      "s_mov_b32 %[gm], s60\n\t"
+      "s_sub_u32 %[t], s55, %[pbeg]\n\t"
+      "s_lshl_b64 s[38:39], 1, %[t]\n\t"
       "s_cmp_lt_u32 %[t], 64\n\t"
-      "s_cselect_b64 s[54:55], s[54:55], 0\n\t"
-      "s_andn2_b64 s[60:61], s[62:63], s[54:55]\n\t"
+      "s_cselect_b64 s[38:39], s[38:39], 0\n\t"
+      "s_andn2_b64 s[44:45], s[46:47], s[38:39]\n\t"
       "s_add_i32 %[n], %[n], 1\n\t"
       "s_lshl_b32 %[t], %[n], 6\n\t"
-      "s_load_dwordx16 s[64:79], %[base], %[t]\n\t"
+      "s_load_dwordx16 s[48:63], %[base], %[t]\n\t"
       "v_cvt_f32_f64 %[fx], %[dx]\n\t"
       "v_cvt_f32_f64 %[fy], %[dy]\n\t"
       "v_cvt_f32_f64 %[fz], %[dz]\n\t"
       "v_mul_f32 %[d2], %[fz], %[fz]\n\t"
       "v_fmac_f32 %[d2], %[fy], %[fy]\n\t"
       "v_fmac_f32 %[d2], %[fx], %[fx]\n\t"
-      NBS_WALK_FORCE("s[60:61]")
+      NBS_WALK_FORCE("s[44:45]")
       "L_slow%=:\n\t"
       // n reached lim = min(stack top, nend): the lanes waiting at n
       // rejoin (the stack top is never passed), or the walk is done
       "s_cmp_eq_u32 %[n], %[tpos]\n\t"
       "s_cbranch_scc0 L_end%=\n\t"
       "s_sub_u32 %[top], %[top], 1\n\t"
-      "v_readlane_b32 s54, %[slo], %[top]\n\t"
-      "v_readlane_b32 s55, %[shi], %[top]\n\t"
+      "v_readlane_b32 s38, %[slo], %[top]\n\t"
+      "v_readlane_b32 s39, %[shi], %[top]\n\t"
       "s_add_u32 %[t], %[top], 63\n\t"
       "s_and_b32 %[t], %[t], 63\n\t"
       "v_readlane_b32 %[tpos], %[sp], %[t]\n\t"
-      "s_or_b64 s[62:63], s[62:63], s[54:55]\n\t"
+      "s_or_b64 s[46:47], s[46:47], s[38:39]\n\t"
       "s_min_u32 %[lim], %[tpos], %[nend]\n\t"
       "s_cmp_lt_u32 %[n], %[lim]\n\t"
       "s_cbranch_scc1 L_loop%=\n"
@@ -341,9 +341,9 @@ __device__ __forceinline__ void walk_loop_f32(ConstNodes nodes, int n, int nend,
         [fy] "=&v"(fy), [fz] "=&v"(fz), [d2] "=&v"(d2), [tv] "=&v"(tv)
       : [base] "s"(nodes), [nend] "s"(nend), [pbeg] "s"(pbeg), [A0] "s"(A), [x] "v"(x),
         [y] "v"(y), [z] "v"(z)
-      : "s54", "s55", "s56", "s57", "s58", "s59", "s60", "s61", "s62", "s63", "s64", "s65",
-        "s66", "s67", "s68", "s69", "s70", "s71", "s72", "s73", "s74", "s75", "s76", "s77",
-        "s78", "s79", "vcc", "scc", "memory");
+      : "s38", "s39", "s40", "s41", "s42", "s43", "s44", "s45", "s46", "s47", "s48", "s49",
+        "s50", "s51", "s52", "s53", "s54", "s55", "s56", "s57", "s58", "s59", "s60", "s61",
+        "s62", "s63", "vcc", "scc", "memory");
 }
 #undef NBS_WALK_TAIL
 #undef NBS_WALK_FORCE
@@ -372,8 +372,16 @@ __device__ __forceinline__ int tile_of_block(int b, int nblocks, int groups) {
 // trajectory -- while forces use this step's tree and positions.  That
 // makes the step a smooth function of the perturbed state, for finite
 // differences of the adjoint.
+//
+// Eight wavefronts per SIMD: gfx950 holds 640 SGPRs per SIMD, so eight
+// wavefronts need at most 80 each.  With the scalar loop's fixed registers
+// at s38-s63 and the compiler held to eight wavefronts (it moves the few
+// values it cannot keep into VGPR lanes outside the loop) the walk uses 78;
+// at the 95 it used before, only seven fitted and the FP32 walk took 10 %
+// longer.
 template <bool F32, bool DAMP, bool FROZEN, int BLOCK, int P>
-__global__ void __launch_bounds__(BLOCK) walk_kernel(WalkArgs a) {
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8)))
+walk_kernel(WalkArgs a) {
   if (*a.flags) return;
   const int lane = lane_id();
   const int tile = a.order ? a.order[blockIdx.x] : tile_of_block(blockIdx.x, gridDim.x, a.groups);
