@@ -679,74 +679,36 @@ __global__ void __launch_bounds__(kBlock)
 
 // One internal cell p sums its children (already final) in child order
 // 0..7 with the reference's operation sequence, src/octree.cpp:203-221
-// (mass) and :252-278 (COM), and links them to itself.
+// (mass) and :252-278 (COM), and links them to itself.  One pass along the
+// sibling chain (child c's record holds the next sibling, skip) keeps four
+// running sums: the mass sum and the three COM sums are independent
+// sequences, each added in child order exactly as the reference's two
+// loops add them; the COM sums are used only when the mass is not zero.
+// (Running sums instead of the children's values: 30 VGPRs instead of 106,
+// eight wavefronts per SIMD instead of four for this latency-bound chain.)
 __device__ __forceinline__ void reduce_cell(int p, WalkNode* __restrict__ nodes,
                                             int* __restrict__ parent) {
 #pragma clang fp contract(off)
   const int end = nodes[p].skip;
-  // One pass along the sibling chain (child c's record holds the next
-  // sibling, skip): up to 8 children's mass and COM are read with their
-  // link, so the chain of dependent loads is walked once.
-  double cm[8], ccx[8], ccy[8], ccz[8];
-  int nc = 0;
-  int c = p + 1;
-#pragma unroll
-  for (int k = 0; k < 8; k++) {
-    if (c < end) {
-      cm[k] = nodes[c].m;
-      ccx[k] = nodes[c].cx;
-      ccy[k] = nodes[c].cy;
-      ccz[k] = nodes[c].cz;
-      parent[c] = p;
-      c = max(nodes[c].skip, c + 1);
-      nc = k + 1;
-    }
+  double M = 0, tx = 0, ty = 0, tz = 0;
+  for (int c = p + 1; c < end;) {
+    const double cm = nodes[c].m;
+    const double px = nodes[c].cx * cm;
+    const double py = nodes[c].cy * cm;
+    const double pz = nodes[c].cz * cm;
+    const int next = nodes[c].skip;
+    parent[c] = p;
+    M = M + cm;
+    tx = tx + px;
+    ty = ty + py;
+    tz = tz + pz;
+    c = max(next, c + 1);
   }
-  double M = 0;
   double cxv = 0, cyv = 0, czv = 0;
-  if (c >= end) {
-#pragma unroll
-    for (int k = 0; k < 8; k++)
-      if (k < nc) M = M + cm[k];
-    if (M != 0) {  // else COM (0,0,0), src/octree.cpp:259-262
-      double tx = 0, ty = 0, tz = 0;
-#pragma unroll
-      for (int k = 0; k < 8; k++) {
-        if (k < nc) {
-          const double px = ccx[k] * cm[k];
-          const double py = ccy[k] * cm[k];
-          const double pz = ccz[k] * cm[k];
-          tx = tx + px;
-          ty = ty + py;
-          tz = tz + pz;
-        }
-      }
-      cxv = tx / M;
-      cyv = ty / M;
-      czv = tz / M;
-    }
-  } else {
-    // a depth-42 bucket with more than 8 particles: the same sums over
-    // the whole chain
-    for (c = p + 1; c < end; c = max(nodes[c].skip, c + 1)) {
-      M = M + nodes[c].m;
-      parent[c] = p;
-    }
-    if (M != 0) {
-      double tx = 0, ty = 0, tz = 0;
-      for (c = p + 1; c < end; c = max(nodes[c].skip, c + 1)) {
-        const double cmv = nodes[c].m;
-        const double px = nodes[c].cx * cmv;
-        const double py = nodes[c].cy * cmv;
-        const double pz = nodes[c].cz * cmv;
-        tx = tx + px;
-        ty = ty + py;
-        tz = tz + pz;
-      }
-      cxv = tx / M;
-      cyv = ty / M;
-      czv = tz / M;
-    }
+  if (M != 0) {  // else COM (0,0,0), src/octree.cpp:259-262
+    cxv = tx / M;
+    cyv = ty / M;
+    czv = tz / M;
   }
   nodes[p].cx = cxv;
   nodes[p].cy = cyv;
