@@ -45,8 +45,12 @@ __device__ __forceinline__ bool mac_open_f64_a(double d2, double side,
   return side / __builtin_sqrt(d2) > theta;
 }
 
+// Eight wavefronts per SIMD (at most 64 VGPRs and 80 SGPRs each on gfx950)
+// where that needs no spills: without damping (54-56 VGPRs; 74-76 before
+// the bound); the damped variants keep their registers.
 template <bool MAC32, bool DAMP>
-__global__ void __launch_bounds__(kAdjBlock) adj_walk_kernel(AdjArgs a) {
+__global__ void __launch_bounds__(kAdjBlock) __attribute__((amdgpu_waves_per_eu(DAMP ? 1 : 8, 8)))
+adj_walk_kernel(AdjArgs a) {
   if (*a.flags) return;
   const int lane = lane_id();
   const int wave = __builtin_amdgcn_readfirstlane(
