@@ -172,6 +172,8 @@ struct nbs_ctx {
   double* d_direct_out = nullptr;
 
   bool has_state = false, bbox_valid = false, tree_valid = false;
+  // builds record each node's parent (the adjoint's propagation needs them)
+  bool want_parent = false;
   bool acc_valid = false;
   int nsys_req = 1;
   std::vector<int64_t> sys_off_req;
@@ -346,9 +348,6 @@ int alloc_nodes(nbs_ctx* c, int cap) {
   // one record beyond the capacity: the hand-written FP32 walk requests the
   // record after a system's last node (unused) without a bounds test
   c->nodes = dalloc<WalkNode>(o, (size_t)cap + 1, &e);
-  c->aux.depth = dalloc<int>(o, cap, &e);
-  c->aux.first = dalloc<int>(o, cap, &e);
-  c->aux.count = dalloc<int>(o, cap, &e);
   c->aux.parent = dalloc<int>(o, cap, &e);
   c->bb.bucket = dalloc<int>(o, cap, &e);
   if (e != hipSuccess) {
@@ -460,9 +459,10 @@ void enqueue_build(nbs_ctx* c) {
   launch_fixup(c->bb, prim, perm, which ? c->bb.val0 : c->bb.val1, cur, c->n, s);
   launch_permute(cur, nxt, perm, c->n, s);
   c->cur ^= 1;
-  launch_emit(c->bb, c->sys, c->p, c->st[c->cur], c->nodes, c->aux, c->n,
-              c->node_cap, s);
-  launch_bottom_up(c->bb, c->nodes, c->aux, c->n, c->sys.nsys, s);
+  TreeAux aux = c->aux;
+  if (!c->want_parent) aux.parent = nullptr;
+  launch_emit(c->bb, c->sys, c->p, c->st[c->cur], c->nodes, aux, c->n, c->node_cap, s);
+  launch_bottom_up(c->bb, c->nodes, aux, c->n, c->sys.nsys, s);
   prof_end(c, kPhaseBuild, e0);
 }
 
@@ -1378,9 +1378,6 @@ int nbs_tree_dump(nbs_ctx* c, int32_t* depth, int32_t* leaf, int64_t* first,
   std::vector<unsigned long long> khi(n), klo(n);
   hipStream_t s = c->stream;
   HIPCHK(c, hipMemcpyAsync(nd.data(), c->nodes, T * sizeof(WalkNode), hipMemcpyDeviceToHost, s));
-  HIPCHK(c, hipMemcpyAsync(dep.data(), c->aux.depth, T * sizeof(int), hipMemcpyDeviceToHost, s));
-  HIPCHK(c, hipMemcpyAsync(fst.data(), c->aux.first, T * sizeof(int), hipMemcpyDeviceToHost, s));
-  HIPCHK(c, hipMemcpyAsync(cnt.data(), c->aux.count, T * sizeof(int), hipMemcpyDeviceToHost, s));
   HIPCHK(c, hipMemcpyAsync(id.data(), c->st[c->cur].id, n * sizeof(unsigned),
                            hipMemcpyDeviceToHost, s));
   HIPCHK(c, hipMemcpyAsync(khi.data(), c->bb.khi, n * sizeof(unsigned long long),
@@ -1388,6 +1385,27 @@ int nbs_tree_dump(nbs_ctx* c, int32_t* depth, int32_t* leaf, int64_t* first,
   HIPCHK(c, hipMemcpyAsync(klo.data(), c->bb.klo, n * sizeof(unsigned long long),
                            hipMemcpyDeviceToHost, s));
   HIPCHK(c, hipStreamSynchronize(s));
+  {
+    // first slot and particle count from the leaves (a leaf's `leaf` is
+    // its particle slot; a cell spans the slots up to its skip node's
+    // first), depth = the cells enclosing a node in pre-order
+    std::vector<int> nxt(T + 1);
+    nxt[T] = n;
+    for (int k = T - 1; k >= 0; k--) nxt[k] = nd[k].leaf >= 0 ? nd[k].leaf : nxt[k + 1];
+    std::vector<int> ends;
+    for (int k = 0; k < T; k++) {
+      while (!ends.empty() && ends.back() <= k) ends.pop_back();
+      dep[k] = (int)ends.size();
+      fst[k] = nxt[k];
+      if (nd[k].leaf >= 0) {
+        cnt[k] = 1;
+      } else {
+        const int e = std::min(std::max(nd[k].skip, k + 1), T);
+        cnt[k] = nxt[e] - nxt[k];
+        ends.push_back(e);
+      }
+    }
+  }
   for (int k = 0; k < T; k++) {
     const int d = dep[k];
     if (depth) depth[k] = d;
@@ -1603,6 +1621,7 @@ void to_aos(const std::vector<double>& soa, double* aos, size_t n) {
 extern "C" {
 
 int nbs_adjoint_forward(nbs_ctx* c, int nsteps, int every, int record_all) {
+  if (c) c->want_parent = true;  // the backward pass propagates along parents
   if (!c) return NBS_ERR_ARG;
   if (!c->has_state) return fail(c, NBS_ERR_STATE, "no particle state set");
   if (nsteps < 1 || every < 1) return fail(c, NBS_ERR_ARG, "nsteps and every must be >= 1");

@@ -37,12 +37,12 @@ struct alignas(64) WalkNode {
 };
 static_assert(sizeof(WalkNode) == 64, "WalkNode must be one 64 B line");
 
-// Per-build tree bookkeeping beside the WalkNode array.
+// Per-build tree bookkeeping beside the WalkNode array: each node's parent
+// (-1 for a root), written only when the adjoint needs it (null otherwise).
+// A node's depth, first particle and particle count follow from the
+// pre-order array and its skip links (nbs_tree_dump derives them).
 struct TreeAux {
-  int* depth;   // [cap]
-  int* first;   // [cap] first particle slot
-  int* count;   // [cap] particle count
-  int* parent;  // [cap]
+  int* parent;  // [cap] or null
 };
 
 // Device-side per-system data (batched mode; one system otherwise).

@@ -599,10 +599,7 @@ __global__ void __launch_bounds__(kBlock)
       w.skip = offsets[b + 1];
       w.leaf = bkt ? -2 : -1;
       nodes[node] = w;
-      aux.depth[node] = d;
-      aux.first[node] = i;
-      aux.count[node] = b - i + 1;
-      if (d == 0) aux.parent[node] = -1;
+      if (d == 0 && aux.parent) aux.parent[node] = -1;
       bucket[r] = node;
       if (bkt) my_bucket++;
     }
@@ -621,10 +618,7 @@ __global__ void __launch_bounds__(kBlock)
     w.skip = leafn + 1;
     w.leaf = i;
     nodes[leafn] = w;
-    aux.depth[leafn] = ld;
-    aux.first[leafn] = i;
-    aux.count[leafn] = 1;
-    if (ld == 0) aux.parent[leafn] = -1;
+    if (ld == 0 && aux.parent) aux.parent[leafn] = -1;
     my_depth = ld;
   }
   // Deepest leaf and number of buckets of this block: plain stores of
@@ -684,6 +678,7 @@ __global__ void __launch_bounds__(kBlock)
 // running sums: the mass sum and the three COM sums are independent
 // sequences, each added in child order exactly as the reference's two
 // loops add them; the COM sums are used only when the mass is not zero.
+// The parent links are written only when parent is not null (adjoint).
 // (Running sums instead of the children's values: 30 VGPRs instead of 106,
 // eight wavefronts per SIMD instead of four for this latency-bound chain.)
 __device__ __forceinline__ void reduce_cell(int p, WalkNode* __restrict__ nodes,
@@ -697,7 +692,7 @@ __device__ __forceinline__ void reduce_cell(int p, WalkNode* __restrict__ nodes,
     const double py = nodes[c].cy * cm;
     const double pz = nodes[c].cz * cm;
     const int next = nodes[c].skip;
-    parent[c] = p;
+    if (parent) parent[c] = p;
     M = M + cm;
     tx = tx + px;
     ty = ty + py;
