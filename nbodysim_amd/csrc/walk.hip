@@ -348,6 +348,240 @@ __device__ __forceinline__ void walk_loop_f32(ConstNodes nodes, int n, int nend,
 #undef NBS_WALK_TAIL
 #undef NBS_WALK_FORCE
 
+// The FP64 walk loop (no damping, one particle per lane) written out for
+// gfx950, the FP64 counterpart of walk_loop_f32 with the same registers,
+// traversal and stack.  Per visit: 3 FP64 subtracts and the unfused
+// d^2 = (dx^2 + dy^2) + dz^2 of the reference's rounding (5), the opening
+// test s^2 - theta^2 d^2 > 1e-9 s^2 (4 and two compares) and, for the
+// interacting lanes, 1/r from v_rsq_f64 with one Newton step and the force
+// (12), the operations and their order those of the generic loop
+// (interact<false, false>, mac_open_f64).  When a lane that takes a cell
+// is within the margin of the opening boundary the loop returns 1 before
+// touching the cell (n, A and the stack describe the walk at that cell);
+// the caller then visits it with the exact test (walk_visit_f64) and calls
+// the loop again.  It returns 0 when the walk is done.  The record's FP64
+// mass-times-G is copied to `gm` before the next record is requested.
+#define NBS_WALK64_FORCE(MASK)                                        \
+  "s_and_saveexec_b64 s[40:41], " MASK "\n\t"                          \
+  "v_rsq_f64 %[tv], %[d2]\n\t"                                        \
+  "v_add_u32 %[cnt], 1, %[cnt]\n\t"                                   \
+  "v_mul_f64 %[tmp], %[d2], %[tv]\n\t"                                \
+  "v_fma_f64 %[tmp], -%[tmp], %[tv], 1.0\n\t"                         \
+  "v_mul_f64 %[s2], 0.5, %[tv]\n\t"                                   \
+  "v_fma_f64 %[tv], %[s2], %[tmp], %[tv]\n\t"                         \
+  "v_mul_f64 %[tmp], %[tv], %[tv]\n\t"                                \
+  "v_mul_f64 %[tmp], %[tmp], %[tv]\n\t"                               \
+  "v_mul_f64 %[tmp], %[gm], %[tmp]\n\t"                               \
+  "v_fma_f64 %[ax], %[tmp], %[dx], %[ax]\n\t"                         \
+  "v_fma_f64 %[ay], %[tmp], %[dy], %[ay]\n\t"                         \
+  "v_fma_f64 %[az], %[tmp], %[dz], %[az]\n\t"                         \
+  "s_mov_b64 exec, s[40:41]\n\t"                                      \
+  "s_cmp_lt_u32 %[n], %[lim]\n\t"                                     \
+  "s_cbranch_scc1 L_loop%=\n\t"                                       \
+  "s_branch L_slow%=\n"
+
+#define NBS_WALK64_TAIL(MASK)                                         \
+  "s_lshl_b32 %[t], %[n], 6\n\t"                                      \
+  "s_load_dwordx16 s[48:63], %[base], %[t]\n\t"                       \
+  NBS_WALK64_FORCE(MASK)
+
+#define NBS_WALK64_D2                                                 \
+  "v_mul_f64 %[d2], %[dx], %[dx]\n\t"                                 \
+  "v_mul_f64 %[tmp], %[dy], %[dy]\n\t"                                \
+  "v_add_f64 %[d2], %[d2], %[tmp]\n\t"                                \
+  "v_mul_f64 %[tmp], %[dz], %[dz]\n\t"                                \
+  "v_add_f64 %[d2], %[d2], %[tmp]\n\t"
+
+__device__ __forceinline__ int walk_loop_f64(ConstNodes nodes, int& n, int nend, int pbeg,
+                                             unsigned long long& A, int& top, int& tpos,
+                                             int& sp, int& slo, int& shi, double x, double y,
+                                             double z, double theta2, double& ax, double& ay,
+                                             double& az, int& cnt, int& overflow) {
+  int lim, t, m0save, tie;
+  double gm, dx, dy, dz, d2, tmp, tv, s2;
+  __asm__ volatile(
+      "s_mov_b32 %[m0save], m0\n\t"
+      "s_mov_b32 %[tie], 0\n\t"
+      "s_mov_b64 s[46:47], %[A]\n\t"
+      "s_min_u32 %[lim], %[tpos], %[nend]\n\t"
+      "s_cmp_lt_i32 %[n], %[nend]\n\t"
+      "s_cbranch_scc0 L_done%=\n\t"
+      "s_lshl_b32 %[t], %[n], 6\n\t"
+      "s_load_dwordx16 s[48:63], %[base], %[t]\n"
+      "L_loop%=:\n\t"
+      "s_waitcnt lgkmcnt(0)\n\t"
+      "s_cmp_gt_i32 s55, -1\n\t"
+      "s_cbranch_scc1 L_leaf%=\n\t"
+      "v_add_f64 %[dx], s[48:49], -%[x]\n\t"
+      "v_add_f64 %[dy], s[50:51], -%[y]\n\t"
+      "v_add_f64 %[dz], s[52:53], -%[z]\n\t"
+      "s_mov_b64 %[gm], s[56:57]\n\t"
+      NBS_WALK64_D2
+      "s_cmp_eq_u32 s55, -2\n\t"
+      "s_cbranch_scc1 L_bucket%=\n\t"
+      // cell: t = side^2 - theta^2 d^2, open where t > 1e-9 side^2; a
+      // taking lane with |t| <= 1e-9 side^2 (or NaN) needs the exact test
+      "v_mul_f64 %[s2], s[58:59], s[58:59]\n\t"
+      "v_mul_f64 %[tmp], %[th2], %[d2]\n\t"
+      "v_add_f64 %[tmp], %[s2], -%[tmp]\n\t"
+      "v_mul_f64 %[s2], %[tolf], %[s2]\n\t"
+      "v_cmp_gt_f64 vcc, %[tmp], %[s2]\n\t"
+      "s_and_b64 s[42:43], vcc, s[46:47]\n\t"
+      "v_cmp_ngt_f64 s[38:39], %[tmp], %[s2]\n\t"
+      "v_cmp_nlt_f64 vcc, %[tmp], -%[s2]\n\t"
+      "s_and_b64 s[38:39], s[38:39], vcc\n\t"
+      "s_and_b64 s[38:39], s[38:39], s[46:47]\n\t"
+      "s_cbranch_scc1 L_tie%=\n\t"
+      "s_cmp_lg_u64 s[42:43], 0\n\t"
+      "s_cbranch_scc1 L_opened%=\n\t"
+      "s_add_i32 %[n], %[n], 1\n\t"
+      "s_max_i32 %[n], %[n], s54\n\t"
+      NBS_WALK64_TAIL("s[46:47]")
+      "L_opened%=:\n\t"
+      "s_add_i32 %[n], %[n], 1\n\t"
+      "s_andn2_b64 s[44:45], s[46:47], s[42:43]\n\t"
+      "s_mov_b64 s[46:47], s[42:43]\n\t"
+      "s_cbranch_scc0 L_int%=\n\t"
+      "s_cmp_eq_u32 %[tpos], s54\n\t"
+      "s_cbranch_scc1 L_merge%=\n\t"
+      "s_cmp_gt_u32 %[top], 62\n\t"
+      "s_cbranch_scc1 L_ovf%=\n\t"
+      "s_mov_b32 m0, %[top]\n\t"
+      "v_writelane_b32 %[sp], s54, m0\n\t"
+      "v_writelane_b32 %[slo], s44, m0\n\t"
+      "v_writelane_b32 %[shi], s45, m0\n\t"
+      "s_add_u32 %[top], %[top], 1\n\t"
+      "s_mov_b32 %[tpos], s54\n\t"
+      "s_min_u32 %[lim], %[tpos], %[nend]\n\t"
+      "s_branch L_int%=\n"
+      "L_merge%=:\n\t"
+      "s_sub_u32 m0, %[top], 1\n\t"
+      "v_readlane_b32 s38, %[slo], m0\n\t"
+      "v_readlane_b32 s39, %[shi], m0\n\t"
+      "s_or_b64 s[38:39], s[38:39], s[44:45]\n\t"
+      "v_writelane_b32 %[slo], s38, m0\n\t"
+      "v_writelane_b32 %[shi], s39, m0\n\t"
+      "s_branch L_int%=\n"
+      "L_ovf%=:\n\t"
+      "s_mov_b32 %[ovf], 1\n\t"
+      "s_branch L_int%=\n"
+      "L_bucket%=:\n\t"
+      "s_add_i32 %[n], %[n], 1\n\t"
+      "s_mov_b64 s[44:45], 0\n"
+      "L_int%=:\n\t"
+      NBS_WALK64_TAIL("s[44:45]")
+      "L_leaf%=:\n\t"
+      "v_add_f64 %[dx], s[48:49], -%[x]\n\t"
+      "v_add_f64 %[dy], s[50:51], -%[y]\n\t"
+      "v_add_f64 %[dz], s[52:53], -%[z]\n\t"
+      "s_mov_b64 %[gm], s[56:57]\n\t"
+      "s_sub_u32 %[t], s55, %[pbeg]\n\t"
+      "s_lshl_b64 s[38:39], 1, %[t]\n\t"
+      "s_cmp_lt_u32 %[t], 64\n\t"
+      "s_cselect_b64 s[38:39], s[38:39], 0\n\t"
+      "s_andn2_b64 s[44:45], s[46:47], s[38:39]\n\t"
+      "s_add_i32 %[n], %[n], 1\n\t"
+      "s_lshl_b32 %[t], %[n], 6\n\t"
+      "s_load_dwordx16 s[48:63], %[base], %[t]\n\t"
+      NBS_WALK64_D2
+      NBS_WALK64_FORCE("s[44:45]")
+      "L_tie%=:\n\t"
+      "s_mov_b32 %[tie], 1\n\t"
+      "s_branch L_end%=\n"
+      "L_slow%=:\n\t"
+      "s_cmp_eq_u32 %[n], %[tpos]\n\t"
+      "s_cbranch_scc0 L_end%=\n\t"
+      "s_sub_u32 %[top], %[top], 1\n\t"
+      "v_readlane_b32 s38, %[slo], %[top]\n\t"
+      "v_readlane_b32 s39, %[shi], %[top]\n\t"
+      "s_add_u32 %[t], %[top], 63\n\t"
+      "s_and_b32 %[t], %[t], 63\n\t"
+      "v_readlane_b32 %[tpos], %[sp], %[t]\n\t"
+      "s_or_b64 s[46:47], s[46:47], s[38:39]\n\t"
+      "s_min_u32 %[lim], %[tpos], %[nend]\n\t"
+      "s_cmp_lt_u32 %[n], %[lim]\n\t"
+      "s_cbranch_scc1 L_loop%=\n"
+      "L_end%=:\n\t"
+      "s_waitcnt lgkmcnt(0)\n"
+      "L_done%=:\n\t"
+      "s_mov_b64 %[A], s[46:47]\n\t"
+      "s_mov_b32 m0, %[m0save]"
+      : [n] "+s"(n), [ovf] "+s"(overflow), [A] "+s"(A), [top] "+s"(top), [tpos] "+s"(tpos),
+        [ax] "+v"(ax), [ay] "+v"(ay), [az] "+v"(az), [cnt] "+v"(cnt), [sp] "+v"(sp),
+        [slo] "+v"(slo), [shi] "+v"(shi), [lim] "=&s"(lim), [t] "=&s"(t), [tie] "=&s"(tie),
+        [gm] "=&s"(gm), [m0save] "=&s"(m0save), [dx] "=&v"(dx), [dy] "=&v"(dy),
+        [dz] "=&v"(dz), [d2] "=&v"(d2), [tmp] "=&v"(tmp), [tv] "=&v"(tv), [s2] "=&v"(s2)
+      : [base] "s"(nodes), [nend] "s"(nend), [pbeg] "s"(pbeg), [x] "v"(x), [y] "v"(y),
+        [z] "v"(z), [th2] "s"(theta2), [tolf] "s"(1e-9)
+      : "s38", "s39", "s40", "s41", "s42", "s43", "s44", "s45", "s46", "s47", "s48", "s49",
+        "s50", "s51", "s52", "s53", "s54", "s55", "s56", "s57", "s58", "s59", "s60", "s61",
+        "s62", "s63", "vcc", "scc", "memory");
+  return tie;
+}
+#undef NBS_WALK64_D2
+#undef NBS_WALK64_TAIL
+#undef NBS_WALK64_FORCE
+
+// One visit of the FP64 walk (no damping, one particle per lane) at node
+// n with the exact opening test: the generic loop's body for one node,
+// used by the hand-written loop for the rare cell where a lane is within
+// rounding of the opening boundary.
+__device__ __forceinline__ void walk_visit_f64(ConstNodes nodes, int& n, int i,
+                                               unsigned long long& A, int& top, int& top_pos,
+                                               int& st_pos, int& st_lo, int& st_hi,
+                                               int& overflow, double x, double y, double z,
+                                               double theta, double theta2, double& ax,
+                                               double& ay, double& az, int& cnt) {
+  const int lane = lane_id();
+  const NodeRegs cur = load_node(nodes, n);
+  const double dx = cur.cx - x, dy = cur.cy - y, dz = cur.cz - z;
+  const double d2 = sq_sum_ref(dx, dy, dz);
+  const bool in = __builtin_amdgcn_inverse_ballot_w64(A);
+  bool inter;
+  int next;
+  if (cur.leaf >= 0) {
+    inter = in && cur.leaf != i;
+    next = n + 1;
+  } else {
+    bool open = cur.leaf == -2 ? true : mac_open_f64(d2, cur.side, theta, theta2);
+    open = open && in;
+    const unsigned long long O = ballot(open);
+    inter = in && !open;
+    if (O) {
+      const unsigned long long C = A & ~O;
+      if (C) {
+        if (top > 0 && top_pos == cur.skip) {
+          const int slot = top - 1;
+          st_lo = lane == slot ? (st_lo | (int)(unsigned)C) : st_lo;
+          st_hi = lane == slot ? (st_hi | (int)(unsigned)(C >> 32)) : st_hi;
+        } else if (top < kWave - 1) {
+          const int slot = top;
+          st_pos = lane == slot ? cur.skip : st_pos;
+          st_lo = lane == slot ? (int)(unsigned)C : st_lo;
+          st_hi = lane == slot ? (int)(unsigned)(C >> 32) : st_hi;
+          top_pos = cur.skip;
+          top++;
+        } else {
+          overflow = 1;
+        }
+      }
+      A = O;
+      next = n + 1;
+    } else {
+      next = cur.skip;
+    }
+  }
+  if (inter) interact<false, false>(cur, dx, dy, dz, 0.f, 0.f, 0.f, 0.f, ax, ay, az, cnt);
+  n = max(next, n + 1);
+  if (top > 0 && top_pos == n) {
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane(st_lo, top - 1);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane(st_hi, top - 1);
+    A |= ((unsigned long long)hi << 32) | lo;
+    top--;
+    top_pos = top > 0 ? __builtin_amdgcn_readlane(st_pos, top - 1) : -1;
+  }
+}
+
 // Workgroup -> tile order.  Workgroups are handed to the 8 XCDs round-robin
 // by workgroup id, each XCD with its own L2.  With `groups` > 1 the tiles
 // (runs of Morton-consecutive waves) are cut into `groups` contiguous
@@ -435,12 +669,24 @@ walk_kernel(WalkArgs a) {
   }
 
   unsigned visits = 0;
-  // (the hand-written loop does not count visits: per-tile costs come from
+  // (the hand-written loops do not count visits: per-tile costs come from
   // the generic loop)
   if (F32 && !DAMP && !FROZEN && P == 1 && a.asm_loop && !a.tile_cost) {
     int ovf = 0;
     float fax = 0.f, fay = 0.f, faz = 0.f;
     walk_loop_f32(nodes, n, nend, pbeg, A[0], x[0], y[0], z[0], fax, fay, faz, cnt[0], ovf);
+    ax[0] = fax;
+    ay[0] = fay;
+    az[0] = faz;
+    overflow = ovf != 0;
+  } else if (!F32 && !DAMP && !FROZEN && P == 1 && a.asm_loop && !a.tile_cost) {
+    int ovf = 0, top = 0, top_pos = -1, st_pos = -1, st_lo = 0, st_hi = 0;
+    double fax = 0, fay = 0, faz = 0;
+    unsigned long long Acur = A[0];
+    while (walk_loop_f64(nodes, n, nend, pbeg, Acur, top, top_pos, st_pos, st_lo, st_hi, x[0],
+                         y[0], z[0], theta2, fax, fay, faz, cnt[0], ovf))
+      walk_visit_f64(nodes, n, i[0], Acur, top, top_pos, st_pos, st_lo, st_hi, ovf, x[0], y[0],
+                     z[0], theta, theta2, fax, fay, faz, cnt[0]);
     ax[0] = fax;
     ay[0] = fay;
     az[0] = faz;

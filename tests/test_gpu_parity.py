@@ -418,8 +418,9 @@ def test_long_runs_of_equal_primary_keys():
             np.testing.assert_array_equal(a, b)
 
 
-def _walk_code_outputs(pos, vel, mass, generic, offsets=None, shape=None, theta=0.5):
-    with sim(theta=theta, precision="f32") as s:
+def _walk_code_outputs(pos, vel, mass, generic, offsets=None, shape=None, theta=0.5,
+                       precision="f32"):
+    with sim(theta=theta, precision=precision) as s:
         s.set_state(pos, vel, mass)
         if offsets is not None:
             s.set_systems(offsets)
@@ -434,11 +435,13 @@ def _walk_code_outputs(pos, vel, mass, generic, offsets=None, shape=None, theta=
         return (acc, cnt) + s.state(), info
 
 
+@pytest.mark.parametrize("precision", ["f32", "f64"])
 @pytest.mark.parametrize("case", ["sphere", "cluster", "buckets", "batched", "theta0"])
-def test_hand_written_walk_equals_generic(case):
-    """The hand-written gfx950 FP32 walk loop makes the generic loop's lane
-    decisions and arithmetic in the same order: forces, interaction counts
-    and four steps (eager and graph-replayed) agree bit for bit.  The cases
+def test_hand_written_walk_equals_generic(case, precision):
+    """The hand-written gfx950 walk loops (FP32 and FP64) make the generic
+    loop's lane decisions and arithmetic in the same order: forces,
+    interaction counts and four steps (eager and graph-replayed) agree bit
+    for bit.  The cases
     reach every branch of the loop: a partly empty last wave (30001
     particles), deep stacks of waiting lanes (a dense cluster), depth-42
     buckets (near-coincident particles), several systems, and theta = 0
@@ -469,14 +472,36 @@ def test_hand_written_walk_equals_generic(case):
     else:
         pos, vel, mass = O.generate(3000, "sphere", seed=9)
         theta = 0.0
-    got, info = _walk_code_outputs(pos, vel, mass, False, offsets, shape, theta)
-    want, _ = _walk_code_outputs(pos, vel, mass, True, offsets, shape, theta)
+    got, info = _walk_code_outputs(pos, vel, mass, False, offsets, shape, theta, precision)
+    want, _ = _walk_code_outputs(pos, vel, mass, True, offsets, shape, theta, precision)
     if case == "buckets":
         assert info["n_buckets"] > 0
     for a, b in zip(got, want):
         np.testing.assert_array_equal(a, b)
     if case == "theta0":
         assert (got[1] == len(pos) - 1).all()
+
+
+def test_fp64_walk_exact_opening_tie():
+    """A cell whose opening test is an exact tie: the FP64 walk's margin test
+    cannot decide it, so the hand-written loop hands the cell to the exact
+    test (side / |d| > theta, the reference's expression).  Five particles,
+    bbox [0, 30]^3 so the root side is 32: the cell [-1, 7)^3 holds two equal
+    masses with COM (3, 3, 3), and the particle at (19, 3, 3) sits exactly
+    side / theta = 16 away from it (8 / 16 = 0.5, not > 0.5: accepted)."""
+    pos = np.array([[2.0, 3.0, 3.0], [4.0, 3.0, 3.0], [19.0, 3.0, 3.0],
+                    [0.0, 30.0, 0.0], [30.0, 0.0, 30.0]])
+    mass = np.full(len(pos), 1e10)
+    vel = np.zeros_like(pos)
+    got, _ = _walk_code_outputs(pos, vel, mass, False, None, None, 0.5, "f64")
+    want, _ = _walk_code_outputs(pos, vel, mass, True, None, None, 0.5, "f64")
+    for a, b in zip(got, want):
+        np.testing.assert_array_equal(a, b)
+    oacc, ocnt = O.Tree(pos, mass).forces(theta=0.5)
+    np.testing.assert_array_equal(got[1], ocnt)
+    assert rel_err(got[0], oacc).max() < 1e-12
+    # the particle at (19, 3, 3) takes the pair as one cell: 3 interactions
+    assert got[1][2] == 3
 
 
 def test_many_runs_of_equal_primary_keys():
