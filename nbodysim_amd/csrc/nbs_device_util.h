@@ -48,6 +48,17 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
+__device__ __forceinline__ int wave_min_int(int v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v = min(v, __shfl_xor(v, d, 64));
+  return v;
+}
+__device__ __forceinline__ int wave_max_int(int v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v = max(v, __shfl_xor(v, d, 64));
+  return v;
+}
+
 // Order-preserving map of doubles onto unsigned 64-bit integers, so
 // min/max can use integer atomics.
 __device__ __host__ __forceinline__ unsigned long long ord_enc(double v) {

@@ -467,12 +467,18 @@ __device__ __forceinline__ Neigh neighbours(
 constexpr int kEmitWaves = kBlock / 64;
 typedef unsigned long long DepthMasks[kEmitWaves][kDepthBins];
 
-// m[w][d] = lanes of wave w with an event at depth d (lo < d <= hi).
+// m[w][d] = lanes of wave w with an event at depth d (lo < d <= hi).  Only
+// the depths some lane of the wave has, (min lo, max hi], take a ballot
+// (a wave of Morton-consecutive slots spans ~10-20 of the 43); the others
+// are cleared one per lane first (same wave, so the writes stay in order).
 __device__ __forceinline__ void depth_ballots(int lo, int hi, DepthMasks& m) {
-  const int w = threadIdx.x >> 6;
-  for (int d = 0; d < kDepthBins; d++) {
+  const int w = threadIdx.x >> 6, lane = lane_id();
+  for (int d = lane; d < kDepthBins; d += kWave) m[w][d] = 0;
+  const int d0 = max(wave_min_int(lo) + 1, 0);
+  const int d1 = min(wave_max_int(hi), kDepthBins - 1);
+  for (int d = d0; d <= d1; d++) {
     const unsigned long long b = ballot(lo < d && d <= hi);
-    if (lane_id() == 0) m[w][d] = b;
+    if (lane == 0) m[w][d] = b;
   }
 }
 

@@ -504,6 +504,31 @@ def test_fp64_walk_exact_opening_tie():
     assert got[1][2] == 3
 
 
+def test_many_long_runs_sorted_concurrently():
+    """Many runs longer than the LDS bitonic sort takes (1025-4000 entries,
+    the workgroup radix sort), so many workgroups sort long runs at once,
+    adjacent in memory to shorter ones: 24 clusters inside 2e-6 cubes, 40
+    clusters of 500-1024 (the bitonic path) and a background cloud (~80000
+    particles).  Tree as the reference's, forces within 1e-12."""
+    rng = np.random.default_rng(21)
+    parts = [rng.uniform(-500, 500, size=(3000, 3))]
+    sizes = list(rng.integers(1025, 4000, size=24)) + list(rng.integers(500, 1025, size=40))
+    for m in sizes:
+        c = rng.uniform(-400, 400, size=3)
+        parts.append(c + rng.uniform(0, 2e-6, size=(int(m), 3)))
+    pos = np.concatenate(parts)
+    pos = pos[rng.permutation(len(pos))]
+    mass = rng.uniform(1e9, 1e10, size=len(pos))
+    vel = np.zeros_like(pos)
+    print("many long runs: n =", len(pos), flush=True)
+    with sim(theta=0.5) as s:
+        s.set_state(pos, vel, mass)
+        acc = s.forces()
+        assert_tree_equal(s.tree_dump(), O.Tree(pos, mass).dump())
+    oacc, _ = O.Tree(pos, mass).forces(theta=0.5)
+    assert rel_err(acc, oacc).max() < 1e-12
+
+
 def test_many_runs_of_equal_primary_keys():
     """Many moderate runs of equal primary sort keys, as a cold shell makes
     while it passes through its centre: 150 clusters of 9-400 particles,
