@@ -608,13 +608,16 @@ __device__ __forceinline__ int tile_of_block(int b, int nblocks, int groups) {
 // differences of the adjoint.
 //
 // Eight wavefronts per SIMD: gfx950 holds 640 SGPRs per SIMD, so eight
-// wavefronts need at most 80 each.  With the scalar loop's fixed registers
+// wavefronts need at most 80 each.  (The FP64 walks with damping or two
+// particles per lane would spill to scratch within that bound: they keep
+// their registers.)  With the scalar loop's fixed registers
 // at s38-s63 and the compiler held to eight wavefronts (it moves the few
 // values it cannot keep into VGPR lanes outside the loop) the walk uses 78;
 // at the 95 it used before, only seven fitted and the FP32 walk took 10 %
 // longer.
 template <bool F32, bool DAMP, bool FROZEN, int BLOCK, int P>
-__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8)))
+__global__ void __launch_bounds__(BLOCK)
+__attribute__((amdgpu_waves_per_eu(!F32 && (DAMP || P == 2) ? 1 : 8, 8)))
 walk_kernel(WalkArgs a) {
   if (*a.flags) return;
   const int lane = lane_id();
