@@ -97,7 +97,9 @@ int nbs_set_state(nbs_ctx* ctx, size_t n, const double* pos,
 
 /* Batched mode: splits the uploaded particles into nsys independent
  * systems, system s = caller indices [offsets[s], offsets[s+1]).
- * offsets has nsys + 1 entries, offsets[0] = 0, offsets[nsys] = n. */
+ * offsets has nsys + 1 entries, offsets[0] = 0, offsets[nsys] = n, and
+ * increases strictly (every system holds at least one particle;
+ * NBS_ERR_ARG otherwise). */
 int nbs_set_systems(nbs_ctx* ctx, int nsys, const int64_t* offsets);
 
 /* Downloads state in the caller's order; any pointer may be NULL.  acc is

@@ -794,7 +794,8 @@ int nbs_set_systems(nbs_ctx* c, int nsys, const int64_t* off) {
   if (off[0] != 0 || off[nsys] != c->n)
     return fail(c, NBS_ERR_ARG, "offsets must run from 0 to n");
   for (int k = 0; k < nsys; k++)
-    if (off[k + 1] < off[k]) return fail(c, NBS_ERR_ARG, "offsets must not decrease");
+    if (off[k + 1] <= off[k])
+      return fail(c, NBS_ERR_ARG, "offsets must increase (every system holds a particle)");
   // Particles are in caller order only until the first build.
   if (c->cur != 0 || c->tree_valid)
     return fail(c, NBS_ERR_STATE, "set systems right after set_state");
@@ -810,7 +811,8 @@ int nbs_generate_state(nbs_ctx* c, const nbs_settings* s, int nsys, const int64_
     return fail(c, NBS_ERR_ARG, "settings, nsys >= 1, offsets and seeds are required");
   if (off[0] != 0) return fail(c, NBS_ERR_ARG, "offsets must start at 0");
   for (int k = 0; k < nsys; k++)
-    if (off[k + 1] < off[k]) return fail(c, NBS_ERR_ARG, "offsets must not decrease");
+    if (off[k + 1] <= off[k] && !(nsys == 1 && off[1] == 0))
+      return fail(c, NBS_ERR_ARG, "offsets must increase (every system holds a particle)");
   const int64_t n = off[nsys];
   if (n > 0x3fffffff) return fail(c, NBS_ERR_ARG, "n too large");
   if (s->mass_dist != NBS_DIST_LINEAR)

@@ -139,3 +139,12 @@ def test_frame_loop_keeps_only_the_newest_frame(tmp_path, monkeypatch):
         f.write("read_existing true\n")
     assert host.run_frames("settings.cfg") == 0
     assert sorted(os.listdir("data")) == ["0003.dat"]
+
+
+def test_snapshot_of_an_empty_state():
+    with sim(theta=0.5, dt=0.0333, precision="f32") as s:
+        s.set_state(np.zeros((0, 3)), np.zeros((0, 3)), np.zeros(0))
+        s.snapshot_begin()
+        p, v, a = s.snapshot_end()
+        assert p.shape == v.shape == a.shape == (0, 3)
+        s.step(1)  # nothing to do, not an error

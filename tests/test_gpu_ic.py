@@ -95,3 +95,15 @@ def test_rejects_variable_draw_distributions():
             s.generate_state(100, gen_type="cube", seed=1, vel_dist="normal", **CUBE)
         # spheres draw no speed: vel_dist does not matter there
         s.generate_state(100, gen_type="sphere", seed=1, vel_dist="normal", **SPHERE)
+
+
+def test_systems_must_hold_particles():
+    # an empty system is rejected on the host, before any device work
+    with sim() as s:
+        with pytest.raises(NbsError):
+            s.generate_state(systems=[(100, "cube", 1), (0, "sphere", 2)], **CUBE)
+        s.generate_state(200, gen_type="cube", seed=3, **CUBE)
+        with pytest.raises(NbsError):
+            s.set_systems(np.array([0, 100, 100, 200]))
+        s.set_systems(np.array([0, 100, 200]))
+        s.step(1)
