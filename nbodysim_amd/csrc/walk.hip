@@ -160,8 +160,9 @@ __device__ __forceinline__ void interact(const NodeRegs& nd, double dx, double d
   cnt++;
 }
 
-// The FP32 walk loop (no damping, one particle per lane) written out for
-// gfx950.  Same traversal, lane decisions and arithmetic, in the same
+// The FP32 walk loop (one particle per lane) written out for gfx950, with
+// the plain force (walk_loop_f32) or the damped one (walk_loop_f32_damped,
+// which differs only in the force macro).  Same traversal, lane decisions and arithmetic, in the same
 // order, as the generic loop in walk_kernel; what differs is the
 // instruction stream.  Per visit: 10 VALU instructions for d = com - x
 // (FP64, rounded to FP32), d^2 and the opening test, 8 for the force on
@@ -189,7 +190,7 @@ __device__ __forceinline__ void interact(const NodeRegs& nd, double dx, double d
 
 // The force on the lanes of MASK and the loop test (the next record has
 // been requested); NBS_WALK_TAIL requests it first.
-#define NBS_WALK_FORCE(MASK)                                          \
+#define NBS_WALK_FORCE_PLAIN(MASK)                                    \
   "s_and_saveexec_b64 s[40:41], " MASK "\n\t"                          \
   "v_rsq_f32 %[tv], %[d2]\n\t"                                        \
   "v_add_u32 %[cnt], 1, %[cnt]\n\t"                                   \
@@ -204,10 +205,162 @@ __device__ __forceinline__ void interact(const NodeRegs& nd, double dx, double d
   "s_cbranch_scc1 L_loop%=\n\t"                                       \
   "s_branch L_slow%=\n"
 
-#define NBS_WALK_TAIL(MASK)                                           \
+#define NBS_WALK_TAIL(FORCE, MASK)                                    \
   "s_lshl_b32 %[t], %[n], 6\n\t"                                      \
   "s_load_dwordx16 s[48:63], %[base], %[t]\n\t"                       \
-  NBS_WALK_FORCE(MASK)
+  FORCE(MASK)
+
+// The damped force (src/main.cpp:318: |a| = G m / (r^2 + exp(-r))), in the
+// generic loop's operations: d = r^2 * (1/r) = r, e = exp2(-d log2 e) (the
+// hardware exponential, __expf), q = (G m * (1/r)) / (r^2 + e) correctly
+// rounded by the scale / reciprocal / Newton / fixup division sequence.
+// s[38:39] receives the first scale's flag (unused); vcc the second's,
+// which v_div_fmas reads six instructions later.
+#define NBS_WALK_FORCE_DAMPED(MASK)                                   \
+  "s_and_saveexec_b64 s[40:41], " MASK "\n\t"                          \
+  "v_rsq_f32 %[tv], %[d2]\n\t"                                        \
+  "v_add_u32 %[cnt], 1, %[cnt]\n\t"                                   \
+  "v_mul_f32 %[te], %[d2], %[tv]\n\t"                                 \
+  "v_mul_f32 %[te], 0xbfb8aa3b, %[te]\n\t"                            \
+  "v_exp_f32 %[te], %[te]\n\t"                                        \
+  "v_mul_f32 %[tn], %[gm], %[tv]\n\t"                                 \
+  "v_add_f32 %[td], %[d2], %[te]\n\t"                                 \
+  "v_div_scale_f32 %[q0], s[38:39], %[td], %[td], %[tn]\n\t"          \
+  "v_rcp_f32 %[q1], %[q0]\n\t"                                        \
+  "v_div_scale_f32 %[q2], vcc, %[tn], %[td], %[tn]\n\t"               \
+  "v_fma_f32 %[q3], -%[q0], %[q1], 1.0\n\t"                           \
+  "v_fmac_f32 %[q1], %[q3], %[q1]\n\t"                                \
+  "v_mul_f32 %[q3], %[q2], %[q1]\n\t"                                 \
+  "v_fma_f32 %[q4], -%[q0], %[q3], %[q2]\n\t"                         \
+  "v_fmac_f32 %[q3], %[q4], %[q1]\n\t"                                \
+  "v_fma_f32 %[q0], -%[q0], %[q3], %[q2]\n\t"                         \
+  "v_div_fmas_f32 %[q0], %[q0], %[q1], %[q3]\n\t"                     \
+  "v_div_fixup_f32 %[q0], %[q0], %[td], %[tn]\n\t"                    \
+  "v_fmac_f32 %[ax], %[q0], %[fx]\n\t"                                \
+  "v_fmac_f32 %[ay], %[q0], %[fy]\n\t"                                \
+  "v_fmac_f32 %[az], %[q0], %[fz]\n\t"                                \
+  "s_mov_b64 exec, s[40:41]\n\t"                                      \
+  "s_cmp_lt_u32 %[n], %[lim]\n\t"                                     \
+  "s_cbranch_scc1 L_loop%=\n\t"                                       \
+  "s_branch L_slow%=\n"
+
+#define NBS_WALK_F32_LOOP(FORCE) \
+      "s_mov_b32 %[m0save], m0\n\t" \
+      "v_mov_b32 %[sp], -1\n\t" \
+      "v_mov_b32 %[slo], 0\n\t" \
+      "v_mov_b32 %[shi], 0\n\t" \
+      "s_mov_b64 s[46:47], %[A0]\n\t" \
+      "s_mov_b32 %[top], 0\n\t" \
+      "s_mov_b32 %[tpos], -1\n\t" \
+      "s_mov_b32 %[lim], %[nend]\n\t" \
+      "s_cmp_lt_i32 %[n], %[nend]\n\t" \
+      "s_cbranch_scc0 L_done%=\n\t" \
+      "s_lshl_b32 %[t], %[n], 6\n\t" \
+      "s_load_dwordx16 s[48:63], %[base], %[t]\n" \
+      "L_loop%=:\n\t" \
+      "s_waitcnt lgkmcnt(0)\n\t" \
+      /* leaf field: >= 0 a leaf, -1 a cell, -2 a depth-42 bucket */ \
+      "s_cmp_gt_i32 s55, -1\n\t" \
+      "s_cbranch_scc1 L_leaf%=\n\t" \
+      "v_add_f64 %[dx], s[48:49], -%[x]\n\t" \
+      "v_add_f64 %[dy], s[50:51], -%[y]\n\t" \
+      "v_add_f64 %[dz], s[52:53], -%[z]\n\t" \
+      "s_mov_b32 %[gm], s60\n\t" \
+      "v_cvt_f32_f64 %[fx], %[dx]\n\t" \
+      "v_cvt_f32_f64 %[fy], %[dy]\n\t" \
+      "v_cvt_f32_f64 %[fz], %[dz]\n\t" \
+      "v_mul_f32 %[d2], %[fz], %[fz]\n\t" \
+      "v_fmac_f32 %[d2], %[fy], %[fy]\n\t" \
+      "v_fmac_f32 %[d2], %[fx], %[fx]\n\t" \
+      "s_cmp_eq_u32 s55, -2\n\t" \
+      "s_cbranch_scc1 L_bucket%=\n\t" \
+      /* cell: open where d^2 < (side/theta)^2 */ \
+      "v_cmp_gt_f32 vcc, s61, %[d2]\n\t" \
+      "s_and_b64 s[42:43], vcc, s[46:47]\n\t" \
+      "s_cbranch_scc1 L_opened%=\n\t" \
+      /* every lane that takes it accepts it: all interact, jump the */ \
+      /* subtree (never backwards, even on a corrupt skip) */ \
+      "s_add_i32 %[n], %[n], 1\n\t" \
+      "s_max_i32 %[n], %[n], s54\n\t" \
+      NBS_WALK_TAIL(FORCE, "s[46:47]") \
+      "L_opened%=:\n\t" \
+      /* I = A & ~O interact and wait at skip; O descend (A = O) */ \
+      "s_add_i32 %[n], %[n], 1\n\t" \
+      "s_andn2_b64 s[44:45], s[46:47], s[42:43]\n\t" \
+      "s_mov_b64 s[46:47], s[42:43]\n\t" \
+      "s_cbranch_scc0 L_int%=\n\t" \
+      "s_cmp_eq_u32 %[tpos], s54\n\t" \
+      "s_cbranch_scc1 L_merge%=\n\t" \
+      "s_cmp_gt_u32 %[top], 62\n\t" \
+      "s_cbranch_scc1 L_ovf%=\n\t" \
+      "s_mov_b32 m0, %[top]\n\t" \
+      "v_writelane_b32 %[sp], s54, m0\n\t" \
+      "v_writelane_b32 %[slo], s44, m0\n\t" \
+      "v_writelane_b32 %[shi], s45, m0\n\t" \
+      "s_add_u32 %[top], %[top], 1\n\t" \
+      "s_mov_b32 %[tpos], s54\n\t" \
+      "s_min_u32 %[lim], %[tpos], %[nend]\n\t" \
+      "s_branch L_int%=\n" \
+      "L_merge%=:\n\t" \
+      "s_sub_u32 m0, %[top], 1\n\t" \
+      "v_readlane_b32 s38, %[slo], m0\n\t" \
+      "v_readlane_b32 s39, %[shi], m0\n\t" \
+      "s_or_b64 s[38:39], s[38:39], s[44:45]\n\t" \
+      "v_writelane_b32 %[slo], s38, m0\n\t" \
+      "v_writelane_b32 %[shi], s39, m0\n\t" \
+      "s_branch L_int%=\n" \
+      "L_ovf%=:\n\t" \
+      "s_mov_b32 %[ovf], 1\n\t" \
+      "s_branch L_int%=\n" \
+      "L_bucket%=:\n\t" \
+      /* every lane opens a bucket; nobody interacts */ \
+      "s_add_i32 %[n], %[n], 1\n\t" \
+      "s_mov_b64 s[44:45], 0\n" \
+      "L_int%=:\n\t" \
+      NBS_WALK_TAIL(FORCE, "s[44:45]") \
+      "L_leaf%=:\n\t" \
+      /* leaf: the next entry is known at once, so its record is requested */ \
+      /* as soon as the three differences have read this one's position; */ \
+      /* every lane that takes the leaf interacts, except its own */ \
+      /* particle's (bit leaf - pbeg when that is below 64) */ \
+      "v_add_f64 %[dx], s[48:49], -%[x]\n\t" \
+      "v_add_f64 %[dy], s[50:51], -%[y]\n\t" \
+      "v_add_f64 %[dz], s[52:53], -%[z]\n\t" \
+      "s_mov_b32 %[gm], s60\n\t" \
+      "s_sub_u32 %[t], s55, %[pbeg]\n\t" \
+      "s_lshl_b64 s[38:39], 1, %[t]\n\t" \
+      "s_cmp_lt_u32 %[t], 64\n\t" \
+      "s_cselect_b64 s[38:39], s[38:39], 0\n\t" \
+      "s_andn2_b64 s[44:45], s[46:47], s[38:39]\n\t" \
+      "s_add_i32 %[n], %[n], 1\n\t" \
+      "s_lshl_b32 %[t], %[n], 6\n\t" \
+      "s_load_dwordx16 s[48:63], %[base], %[t]\n\t" \
+      "v_cvt_f32_f64 %[fx], %[dx]\n\t" \
+      "v_cvt_f32_f64 %[fy], %[dy]\n\t" \
+      "v_cvt_f32_f64 %[fz], %[dz]\n\t" \
+      "v_mul_f32 %[d2], %[fz], %[fz]\n\t" \
+      "v_fmac_f32 %[d2], %[fy], %[fy]\n\t" \
+      "v_fmac_f32 %[d2], %[fx], %[fx]\n\t" \
+      FORCE("s[44:45]") \
+      "L_slow%=:\n\t" \
+      /* n reached lim = min(stack top, nend): the lanes waiting at n */ \
+      /* rejoin (the stack top is never passed), or the walk is done */ \
+      "s_cmp_eq_u32 %[n], %[tpos]\n\t" \
+      "s_cbranch_scc0 L_end%=\n\t" \
+      "s_sub_u32 %[top], %[top], 1\n\t" \
+      "v_readlane_b32 s38, %[slo], %[top]\n\t" \
+      "v_readlane_b32 s39, %[shi], %[top]\n\t" \
+      "s_add_u32 %[t], %[top], 63\n\t" \
+      "s_and_b32 %[t], %[t], 63\n\t" \
+      "v_readlane_b32 %[tpos], %[sp], %[t]\n\t" \
+      "s_or_b64 s[46:47], s[46:47], s[38:39]\n\t" \
+      "s_min_u32 %[lim], %[tpos], %[nend]\n\t" \
+      "s_cmp_lt_u32 %[n], %[lim]\n\t" \
+      "s_cbranch_scc1 L_loop%=\n" \
+      "L_end%=:\n\t" \
+      "s_waitcnt lgkmcnt(0)\n" \
+      "L_done%=:\n\t" \
+      "s_mov_b32 m0, %[m0save]"
 
 __device__ __forceinline__ void walk_loop_f32(ConstNodes nodes, int n, int nend, int pbeg,
                                               unsigned long long A, double x, double y,
@@ -218,122 +371,7 @@ __device__ __forceinline__ void walk_loop_f32(ConstNodes nodes, int n, int nend,
   double dx, dy, dz;
   float fx, fy, fz, d2, tv;
   __asm__ volatile(
-      "s_mov_b32 %[m0save], m0\n\t"
-      "v_mov_b32 %[sp], -1\n\t"
-      "v_mov_b32 %[slo], 0\n\t"
-      "v_mov_b32 %[shi], 0\n\t"
-      "s_mov_b64 s[46:47], %[A0]\n\t"
-      "s_mov_b32 %[top], 0\n\t"
-      "s_mov_b32 %[tpos], -1\n\t"
-      "s_mov_b32 %[lim], %[nend]\n\t"
-      "s_cmp_lt_i32 %[n], %[nend]\n\t"
-      "s_cbranch_scc0 L_done%=\n\t"
-      "s_lshl_b32 %[t], %[n], 6\n\t"
-      "s_load_dwordx16 s[48:63], %[base], %[t]\n"
-      "L_loop%=:\n\t"
-      "s_waitcnt lgkmcnt(0)\n\t"
-      // leaf field: >= 0 a leaf, -1 a cell, -2 a depth-42 bucket
-      "s_cmp_gt_i32 s55, -1\n\t"
-      "s_cbranch_scc1 L_leaf%=\n\t"
-      "v_add_f64 %[dx], s[48:49], -%[x]\n\t"
-      "v_add_f64 %[dy], s[50:51], -%[y]\n\t"
-      "v_add_f64 %[dz], s[52:53], -%[z]\n\t"
-      "s_mov_b32 %[gm], s60\n\t"
-      "v_cvt_f32_f64 %[fx], %[dx]\n\t"
-      "v_cvt_f32_f64 %[fy], %[dy]\n\t"
-      "v_cvt_f32_f64 %[fz], %[dz]\n\t"
-      "v_mul_f32 %[d2], %[fz], %[fz]\n\t"
-      "v_fmac_f32 %[d2], %[fy], %[fy]\n\t"
-      "v_fmac_f32 %[d2], %[fx], %[fx]\n\t"
-      "s_cmp_eq_u32 s55, -2\n\t"
-      "s_cbranch_scc1 L_bucket%=\n\t"
-      // cell: open where d^2 < (side/theta)^2
-      "v_cmp_gt_f32 vcc, s61, %[d2]\n\t"
-      "s_and_b64 s[42:43], vcc, s[46:47]\n\t"
-      "s_cbranch_scc1 L_opened%=\n\t"
-      // every lane that takes it accepts it: all interact, jump the
-      // subtree (never backwards, even on a corrupt skip)
-      "s_add_i32 %[n], %[n], 1\n\t"
-      "s_max_i32 %[n], %[n], s54\n\t"
-      NBS_WALK_TAIL("s[46:47]")
-      "L_opened%=:\n\t"
-      // I = A & ~O interact and wait at skip; O descend (A = O)
-      "s_add_i32 %[n], %[n], 1\n\t"
-      "s_andn2_b64 s[44:45], s[46:47], s[42:43]\n\t"
-      "s_mov_b64 s[46:47], s[42:43]\n\t"
-      "s_cbranch_scc0 L_int%=\n\t"
-      "s_cmp_eq_u32 %[tpos], s54\n\t"
-      "s_cbranch_scc1 L_merge%=\n\t"
-      "s_cmp_gt_u32 %[top], 62\n\t"
-      "s_cbranch_scc1 L_ovf%=\n\t"
-      "s_mov_b32 m0, %[top]\n\t"
-      "v_writelane_b32 %[sp], s54, m0\n\t"
-      "v_writelane_b32 %[slo], s44, m0\n\t"
-      "v_writelane_b32 %[shi], s45, m0\n\t"
-      "s_add_u32 %[top], %[top], 1\n\t"
-      "s_mov_b32 %[tpos], s54\n\t"
-      "s_min_u32 %[lim], %[tpos], %[nend]\n\t"
-      "s_branch L_int%=\n"
-      "L_merge%=:\n\t"
-      "s_sub_u32 m0, %[top], 1\n\t"
-      "v_readlane_b32 s38, %[slo], m0\n\t"
-      "v_readlane_b32 s39, %[shi], m0\n\t"
-      "s_or_b64 s[38:39], s[38:39], s[44:45]\n\t"
-      "v_writelane_b32 %[slo], s38, m0\n\t"
-      "v_writelane_b32 %[shi], s39, m0\n\t"
-      "s_branch L_int%=\n"
-      "L_ovf%=:\n\t"
-      "s_mov_b32 %[ovf], 1\n\t"
-      "s_branch L_int%=\n"
-      "L_bucket%=:\n\t"
-      // every lane opens a bucket; nobody interacts
-      "s_add_i32 %[n], %[n], 1\n\t"
-      "s_mov_b64 s[44:45], 0\n"
-      "L_int%=:\n\t"
-      NBS_WALK_TAIL("s[44:45]")
-      "L_leaf%=:\n\t"
-      // leaf: the next entry is known at once, so its record is requested
-      // as soon as the three differences have read this one's position;
-      // every lane that takes the leaf interacts, except its own
-      // particle's (bit leaf - pbeg when that is below 64)
-      "v_add_f64 %[dx], s[48:49], -%[x]\n\t"
-      "v_add_f64 %[dy], s[50:51], -%[y]\n\t"
-      "v_add_f64 %[dz], s[52:53], -%[z]\n\t"
-      "s_mov_b32 %[gm], s60\n\t"
-      "s_sub_u32 %[t], s55, %[pbeg]\n\t"
-      "s_lshl_b64 s[38:39], 1, %[t]\n\t"
-      "s_cmp_lt_u32 %[t], 64\n\t"
-      "s_cselect_b64 s[38:39], s[38:39], 0\n\t"
-      "s_andn2_b64 s[44:45], s[46:47], s[38:39]\n\t"
-      "s_add_i32 %[n], %[n], 1\n\t"
-      "s_lshl_b32 %[t], %[n], 6\n\t"
-      "s_load_dwordx16 s[48:63], %[base], %[t]\n\t"
-      "v_cvt_f32_f64 %[fx], %[dx]\n\t"
-      "v_cvt_f32_f64 %[fy], %[dy]\n\t"
-      "v_cvt_f32_f64 %[fz], %[dz]\n\t"
-      "v_mul_f32 %[d2], %[fz], %[fz]\n\t"
-      "v_fmac_f32 %[d2], %[fy], %[fy]\n\t"
-      "v_fmac_f32 %[d2], %[fx], %[fx]\n\t"
-      NBS_WALK_FORCE("s[44:45]")
-      "L_slow%=:\n\t"
-      // n reached lim = min(stack top, nend): the lanes waiting at n
-      // rejoin (the stack top is never passed), or the walk is done
-      "s_cmp_eq_u32 %[n], %[tpos]\n\t"
-      "s_cbranch_scc0 L_end%=\n\t"
-      "s_sub_u32 %[top], %[top], 1\n\t"
-      "v_readlane_b32 s38, %[slo], %[top]\n\t"
-      "v_readlane_b32 s39, %[shi], %[top]\n\t"
-      "s_add_u32 %[t], %[top], 63\n\t"
-      "s_and_b32 %[t], %[t], 63\n\t"
-      "v_readlane_b32 %[tpos], %[sp], %[t]\n\t"
-      "s_or_b64 s[46:47], s[46:47], s[38:39]\n\t"
-      "s_min_u32 %[lim], %[tpos], %[nend]\n\t"
-      "s_cmp_lt_u32 %[n], %[lim]\n\t"
-      "s_cbranch_scc1 L_loop%=\n"
-      "L_end%=:\n\t"
-      "s_waitcnt lgkmcnt(0)\n"
-      "L_done%=:\n\t"
-      "s_mov_b32 m0, %[m0save]"
+      NBS_WALK_F32_LOOP(NBS_WALK_FORCE_PLAIN)
       : [n] "+s"(n), [ovf] "+s"(overflow), [ax] "+v"(ax), [ay] "+v"(ay), [az] "+v"(az),
         [cnt] "+v"(cnt), [top] "=&s"(top), [tpos] "=&s"(tpos), [lim] "=&s"(lim), [t] "=&s"(t),
         [gm] "=&s"(gm), [m0save] "=&s"(m0save), [sp] "=&v"(sp), [slo] "=&v"(slo),
@@ -345,8 +383,34 @@ __device__ __forceinline__ void walk_loop_f32(ConstNodes nodes, int n, int nend,
         "s50", "s51", "s52", "s53", "s54", "s55", "s56", "s57", "s58", "s59", "s60", "s61",
         "s62", "s63", "vcc", "scc", "memory");
 }
+
+__device__ __forceinline__ void walk_loop_f32_damped(ConstNodes nodes, int n, int nend,
+                                                     int pbeg, unsigned long long A, double x,
+                                                     double y, double z, float& ax, float& ay,
+                                                     float& az, int& cnt, int& overflow) {
+  int top, tpos, lim, t, gm, m0save;
+  int sp, slo, shi;
+  double dx, dy, dz;
+  float fx, fy, fz, d2, tv, te, tn, td, q0, q1, q2, q3, q4;
+  __asm__ volatile(
+      NBS_WALK_F32_LOOP(NBS_WALK_FORCE_DAMPED)
+      : [n] "+s"(n), [ovf] "+s"(overflow), [ax] "+v"(ax), [ay] "+v"(ay), [az] "+v"(az),
+        [cnt] "+v"(cnt), [top] "=&s"(top), [tpos] "=&s"(tpos), [lim] "=&s"(lim), [t] "=&s"(t),
+        [gm] "=&s"(gm), [m0save] "=&s"(m0save), [sp] "=&v"(sp), [slo] "=&v"(slo),
+        [shi] "=&v"(shi), [dx] "=&v"(dx), [dy] "=&v"(dy), [dz] "=&v"(dz), [fx] "=&v"(fx),
+        [fy] "=&v"(fy), [fz] "=&v"(fz), [d2] "=&v"(d2), [tv] "=&v"(tv),
+        [te] "=&v"(te), [tn] "=&v"(tn), [td] "=&v"(td), [q0] "=&v"(q0), [q1] "=&v"(q1),
+        [q2] "=&v"(q2), [q3] "=&v"(q3), [q4] "=&v"(q4)
+      : [base] "s"(nodes), [nend] "s"(nend), [pbeg] "s"(pbeg), [A0] "s"(A), [x] "v"(x),
+        [y] "v"(y), [z] "v"(z)
+      : "s38", "s39", "s40", "s41", "s42", "s43", "s44", "s45", "s46", "s47", "s48", "s49",
+        "s50", "s51", "s52", "s53", "s54", "s55", "s56", "s57", "s58", "s59", "s60", "s61",
+        "s62", "s63", "vcc", "scc", "memory");
+}
+#undef NBS_WALK_F32_LOOP
 #undef NBS_WALK_TAIL
-#undef NBS_WALK_FORCE
+#undef NBS_WALK_FORCE_DAMPED
+#undef NBS_WALK_FORCE_PLAIN
 
 // The FP64 walk loop (no damping, one particle per lane) written out for
 // gfx950, the FP64 counterpart of walk_loop_f32 with the same registers,
@@ -674,10 +738,14 @@ walk_kernel(WalkArgs a) {
   unsigned visits = 0;
   // (the hand-written loops do not count visits: per-tile costs come from
   // the generic loop)
-  if (F32 && !DAMP && !FROZEN && P == 1 && a.asm_loop && !a.tile_cost) {
+  if (F32 && !FROZEN && P == 1 && a.asm_loop && !a.tile_cost) {
     int ovf = 0;
     float fax = 0.f, fay = 0.f, faz = 0.f;
-    walk_loop_f32(nodes, n, nend, pbeg, A[0], x[0], y[0], z[0], fax, fay, faz, cnt[0], ovf);
+    if (DAMP)
+      walk_loop_f32_damped(nodes, n, nend, pbeg, A[0], x[0], y[0], z[0], fax, fay, faz, cnt[0],
+                           ovf);
+    else
+      walk_loop_f32(nodes, n, nend, pbeg, A[0], x[0], y[0], z[0], fax, fay, faz, cnt[0], ovf);
     ax[0] = fax;
     ay[0] = fay;
     az[0] = faz;

@@ -419,8 +419,8 @@ def test_long_runs_of_equal_primary_keys():
 
 
 def _walk_code_outputs(pos, vel, mass, generic, offsets=None, shape=None, theta=0.5,
-                       precision="f32"):
-    with sim(theta=theta, precision=precision) as s:
+                       precision="f32", damping=False):
+    with sim(theta=theta, precision=precision, damping=damping) as s:
         s.set_state(pos, vel, mass)
         if offsets is not None:
             s.set_systems(offsets)
@@ -435,11 +435,11 @@ def _walk_code_outputs(pos, vel, mass, generic, offsets=None, shape=None, theta=
         return (acc, cnt) + s.state(), info
 
 
-@pytest.mark.parametrize("precision", ["f32", "f64"])
+@pytest.mark.parametrize("precision", ["f32", "f64", "f32-damped"])
 @pytest.mark.parametrize("case", ["sphere", "cluster", "buckets", "batched", "theta0"])
 def test_hand_written_walk_equals_generic(case, precision):
-    """The hand-written gfx950 walk loops (FP32 and FP64) make the generic
-    loop's lane decisions and arithmetic in the same order: forces,
+    """The hand-written gfx950 walk loops (FP32, FP32 with damping and FP64)
+    make the generic loop's lane decisions and arithmetic in the same order: forces,
     interaction counts and four steps (eager and graph-replayed) agree bit
     for bit.  The cases
     reach every branch of the loop: a partly empty last wave (30001
@@ -472,8 +472,12 @@ def test_hand_written_walk_equals_generic(case, precision):
     else:
         pos, vel, mass = O.generate(3000, "sphere", seed=9)
         theta = 0.0
-    got, info = _walk_code_outputs(pos, vel, mass, False, offsets, shape, theta, precision)
-    want, _ = _walk_code_outputs(pos, vel, mass, True, offsets, shape, theta, precision)
+    damping = precision.endswith("-damped")
+    precision = precision.split("-")[0]
+    got, info = _walk_code_outputs(pos, vel, mass, False, offsets, shape, theta, precision,
+                                   damping)
+    want, _ = _walk_code_outputs(pos, vel, mass, True, offsets, shape, theta, precision,
+                                 damping)
     if case == "buckets":
         assert info["n_buckets"] > 0
     for a, b in zip(got, want):
