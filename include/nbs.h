@@ -144,6 +144,25 @@ int nbs_step(nbs_ctx* ctx, int nsteps);
  * size, systems, configuration or node capacity. */
 int nbs_step_graph(nbs_ctx* ctx, int nsteps);
 
+/* Collision merging (config.collide / collision_range, src/main.cpp:1442-1483;
+ * the reference's CUDA path ignores it, :1449-1452).  While enabled, each
+ * nbs_step step records the pairs (particle, accepted leaf) closer than
+ * `range` during the walk (src/main.cpp:386-396), merges every connected
+ * group of pairs into one particle -- collide() folded over the members in
+ * increasing caller index: mass sum, mass-weighted position and velocity
+ * (src/main.cpp:423-430, src/vector.cpp:129-132) -- with zero acceleration
+ * for the step's update, then updates and removes the absorbed particles.
+ * Survivors keep their caller order; the merged particle takes its
+ * lowest-index member's place.  The count shrinks (nbs_particle_count).
+ * Eager steps only: nbs_step_graph returns NBS_ERR_STATE while enabled;
+ * adaptive dt must be off.  nbs_forces never merges. */
+int nbs_set_collisions(nbs_ctx* ctx, int enabled, double range);
+/* Pairs detected and particles absorbed since collisions were enabled. */
+int nbs_collision_stats(nbs_ctx* ctx, int64_t* pairs, int64_t* removed);
+/* Current particle count and, if offsets is not NULL, the nsys + 1 system
+ * offsets (nsys_cap >= nsys + 1). */
+int nbs_particle_count(nbs_ctx* ctx, int64_t* n, int64_t* offsets, int nsys_cap);
+
 /* Launch-shape tuning of the walk (no counterpart in the reference, whose
  * kernel shape was fixed, src/cuda_helper.cpp:183-188).  The shape is the
  * workgroup size (block: 64, 128, 256, 512, 1024 threads), the number of

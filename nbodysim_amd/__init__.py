@@ -58,6 +58,7 @@ class Simulation:
             raise NbsError(rc, "nbs_create failed")
         self._h = h
         self.n = 0
+        self._collide = False
 
     def _fill_config(self, theta, dt, damping, precision, root_center, root_size, adaptive,
                      min_adaptive_dt, max_pos_change, max_vel_change, device, node_factor):
@@ -196,6 +197,30 @@ class Simulation:
         """nsteps frames; graph=True replays a captured HIP graph."""
         f = self._L.nbs_step_graph if graph else self._L.nbs_step
         self._check(f(self._h, int(nsteps)))
+        if self._collide:
+            n = ctypes.c_int64()
+            self._check(self._L.nbs_particle_count(self._h, ctypes.byref(n), None, 0))
+            self.n = n.value
+
+    def set_collisions(self, range=0.01, enabled=True):
+        """Collision merging in step() (nbs_set_collisions): pairs closer
+        than `range` found by the walk merge; the particle count shrinks."""
+        self._check(self._L.nbs_set_collisions(self._h, int(bool(enabled)), float(range)))
+        self._collide = bool(enabled)
+
+    def collision_stats(self):
+        """(pairs detected, particles absorbed) since set_collisions."""
+        p, r = ctypes.c_int64(), ctypes.c_int64()
+        self._check(self._L.nbs_collision_stats(self._h, ctypes.byref(p), ctypes.byref(r)))
+        return p.value, r.value
+
+    def offsets(self, nsys):
+        """Current system offsets (nsys + 1 entries)."""
+        off = np.zeros(nsys + 1, dtype=np.int64)
+        n = ctypes.c_int64()
+        self._check(self._L.nbs_particle_count(self._h, ctypes.byref(n),
+                                               off.ctypes.data_as(_native.c_i64p), nsys + 1))
+        return off
 
     WALK_BLOCKS = (64, 128, 256, 512, 1024)
     WALK_GROUPS = (1, 8, 16, 32, 64)

@@ -75,6 +75,9 @@ SIGNATURES = [
     ("nbs_forces", ctypes.c_int, [ctypes.c_void_p, c_dp]),
     ("nbs_step", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("nbs_step_graph", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("nbs_set_collisions", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]),
+    ("nbs_collision_stats", ctypes.c_int, [ctypes.c_void_p, c_i64p, c_i64p]),
+    ("nbs_particle_count", ctypes.c_int, [ctypes.c_void_p, c_i64p, c_i64p, ctypes.c_int]),
     ("nbs_autotune", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_i32p, c_i32p, c_i32p,
                                     c_dp]),
     ("nbs_set_walk_shape", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,

@@ -237,6 +237,19 @@ struct nbs_ctx {
   int snap_next = 0;     // host buffer the next snapshot fills
   int snap_pending = -1; // host buffer being filled, -1 when none
   int snap_n = 0;
+
+  // Collision merging (nbs_set_collisions): buffers sized for col_cap_n
+  // particles and cb.pair_cap pairs; col_active while a collide step's walk
+  // is enqueued; running totals of detections and absorbed particles.
+  bool collide = false;
+  double collide_range = 0;
+  bool col_active = false;
+  CollideBuffers cb{};
+  std::vector<void*> col_bufs, col_pair_bufs;
+  int col_cap_n = 0;
+  int col_pair_want = 0;
+  int col_info_cap = 0;
+  int64_t col_pairs = 0, col_removed = 0;
 };
 
 namespace {
@@ -521,6 +534,12 @@ void enqueue_walk(nbs_ctx* c, int fused) {
     a.order = c->tile_order;
     a.tile_cost = c->tile_cost;
   }
+  if (c->col_active) {
+    a.pairs = c->cb.pairs;
+    a.pair_count = c->cb.count;
+    a.pair_cap = c->cb.pair_cap;
+    a.collide_range = c->collide_range;
+  }
   launch_walk(a, c->cfg.force_precision == NBS_FORCE_F32, c->p.damping,
               c->walk_block, c->walk_pack, c->stream);
   if (fused) c->bbox_valid = true;
@@ -571,6 +590,163 @@ int read_flags(nbs_ctx* c, int* flags, int* done) {
 void reset_flags(nbs_ctx* c) {
   launch_fill_int(c->flags, 0, 1, c->stream);
   launch_fill_int(c->steps_done, 0, 1, c->stream);
+}
+
+// ------------------------------------------------------------ collide --
+// Buffers of collision merging: per particle (cap_n), per pair (the
+// wanted capacity) and the read-back info (nsys + 2 ints).
+int ensure_collide(nbs_ctx* c) {
+  hipError_t e = hipSuccess;
+  CollideBuffers& b = c->cb;
+  if (c->col_cap_n < c->cap_n) {
+    free_all(c->col_bufs);
+    c->col_info_cap = 0;
+    const int cap = c->cap_n;
+    auto& o = c->col_bufs;
+    b.parent = dalloc<int>(o, cap, &e);
+    b.mark = dalloc<int>(o, cap, &e);
+    b.alive = dalloc<int>(o, cap, &e);
+    b.id_alive = dalloc<int>(o, cap, &e);
+    b.newpos = dalloc<int>(o, (size_t)cap + 1, &e);
+    b.newid = dalloc<int>(o, (size_t)cap + 1, &e);
+    b.cax = dalloc<double>(o, cap, &e);
+    b.cay = dalloc<double>(o, cap, &e);
+    b.caz = dalloc<double>(o, cap, &e);
+    b.cinter = dalloc<int>(o, cap, &e);
+    b.count = dalloc<int>(o, 2, &e);
+    if (e != hipSuccess) {
+      free_all(c->col_bufs);
+      c->col_cap_n = 0;
+      return hip_fail(c, e, "hipMalloc(collision buffers)");
+    }
+    c->col_cap_n = cap;
+  }
+  if (c->col_info_cap < c->sys.nsys + 2) {
+    const int m = c->sys.nsys + 2;
+    b.info = dalloc<int>(c->col_bufs, m, &e);
+    if (e != hipSuccess) return hip_fail(c, e, "hipMalloc(collision info)");
+    c->col_info_cap = m;
+  }
+  const int want = std::max(c->col_pair_want, std::max(4096, c->n / 16));
+  if (b.pair_cap < want) {
+    free_all(c->col_pair_bufs);
+    b.pair_cap = 0;
+    auto& o = c->col_pair_bufs;
+    const size_t L = 2 * (size_t)want;
+    b.pairs = dalloc<int2>(o, want, &e);
+    b.key0 = dalloc<unsigned long long>(o, L, &e);
+    b.key1 = dalloc<unsigned long long>(o, L, &e);
+    b.val0 = dalloc<unsigned>(o, L, &e);
+    b.val1 = dalloc<unsigned>(o, L, &e);
+    b.sort_tmp = dalloc<int>(o, collide_sort_tmp_ints(want), &e);
+    if (e != hipSuccess) {
+      free_all(c->col_pair_bufs);
+      return hip_fail(c, e, "hipMalloc(collision pair buffers)");
+    }
+    b.pair_cap = want;
+  }
+  b.scan_tmp = c->bb.scan_tmp;
+  return NBS_OK;
+}
+
+// Host copies of the system layout after a compaction moved the starts
+// (the device starts were moved by the compaction itself).
+int shrink_systems(nbs_ctx* c, const std::vector<int>& info) {
+  const int nsys = c->sys.nsys;
+  for (int k = 0; k <= nsys; k++) c->h_start[k] = info[1 + k];
+  for (int k = 0; k < nsys; k++) {
+    const int cnt = c->h_start[k + 1] - c->h_start[k];
+    c->h_wave_start[k + 1] = c->h_wave_start[k] + (cnt + kWave - 1) / kWave;
+    c->h_wave2_start[k + 1] = c->h_wave2_start[k] + (cnt + 2 * kWave - 1) / (2 * kWave);
+  }
+  c->sys.n_waves = c->h_wave_start[nsys];
+  c->sys.n_waves2 = c->h_wave2_start[nsys];
+  if (!c->sys_off_req.empty())
+    for (int k = 0; k <= nsys; k++) c->sys_off_req[k] = c->h_start[k];
+  c->n = c->h_start[nsys];
+  HIPCHK(c, hipMemcpyAsync(c->sys.wave_start, c->h_wave_start.data(), (nsys + 1) * sizeof(int),
+                           hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->sys.wave2_start, c->h_wave2_start.data(),
+                           (nsys + 1) * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return NBS_OK;
+}
+
+// Steps with collision merging, one at a time (src/main.cpp:1420-1492 with
+// config.collide): build, walk recording pairs, merge, update, compact.
+// The particle count is read back after each step; a step whose pair or
+// node buffers overflowed changed nothing but the slot order, and is
+// redone with larger buffers.
+int run_collide(nbs_ctx* c, int nsteps) {
+  if (!c->has_state) return fail(c, NBS_ERR_STATE, "no particle state set");
+  if (c->p.adaptive)
+    return fail(c, NBS_ERR_STATE, "collision merging needs adaptive dt off");
+  for (int k = 0; k < nsteps && c->n > 0; k++) {
+    for (int attempt = 0;; attempt++) {
+      if (attempt == 8) return fail(c, NBS_ERR_OVERFLOW, "collision step overflow after retries");
+      int r = ensure_nodes(c);
+      if (r) return r;
+      r = ensure_collide(c);
+      if (r) return r;
+      const int n = c->n, nsys = c->sys.nsys;
+      reset_flags(c);
+      launch_fill_int(c->cb.count, 0, 2, c->stream);
+      enqueue_build(c);
+      c->col_active = true;
+      enqueue_walk(c, 0);
+      c->col_active = false;
+      hipEvent_t e0 = nullptr;
+      prof_begin(c, kPhaseUpdate, &e0);
+      State& cur = c->st[c->cur];
+      State& nxt = c->st[1 - c->cur];
+      launch_collide(c->cb, cur, nxt, c->ax, c->ay, c->az, c->inter, n, c->sys, c->p.dt,
+                     c->flags, c->stream);
+      prof_end(c, kPhaseUpdate, e0);
+      HIPCHK(c, hipGetLastError());
+      int flags = 0, done = 0;
+      r = read_flags(c, &flags, &done);
+      if (r) return r;
+      int cnt[2] = {0, 0};
+      std::vector<int> info(nsys + 2, 0);
+      HIPCHK(c, hipMemcpy(cnt, c->cb.count, sizeof(cnt), hipMemcpyDeviceToHost));
+      c->bbox_valid = false;
+      c->tree_valid = false;
+      if (flags) {
+        if (flags & kFlagRunOverflow)
+          return fail(c, NBS_ERR_OVERFLOW, "more than 8192 particles share one 21-level cell");
+        if (flags & kFlagPairs) {
+          c->col_pair_want = std::max(2 * c->cb.pair_cap, cnt[0] + cnt[0] / 4);
+        } else {
+          r = alloc_nodes(c, c->node_cap * 2);
+          if (r) return r;
+        }
+        continue;
+      }
+      HIPCHK(c, hipMemcpy(info.data(), c->cb.info, info.size() * sizeof(int),
+                          hipMemcpyDeviceToHost));
+      const size_t m = (size_t)n;
+      HIPCHK(c, hipMemcpyAsync(c->ax, c->cb.cax, m * sizeof(double), hipMemcpyDeviceToDevice,
+                               c->stream));
+      HIPCHK(c, hipMemcpyAsync(c->ay, c->cb.cay, m * sizeof(double), hipMemcpyDeviceToDevice,
+                               c->stream));
+      HIPCHK(c, hipMemcpyAsync(c->az, c->cb.caz, m * sizeof(double), hipMemcpyDeviceToDevice,
+                               c->stream));
+      HIPCHK(c, hipMemcpyAsync(c->inter, c->cb.cinter, m * sizeof(int), hipMemcpyDeviceToDevice,
+                               c->stream));
+      c->cur ^= 1;
+      c->col_pairs += cnt[0];
+      c->col_removed += n - info[0];
+      if (info[0] != n) {
+        drop_graph(c);
+        r = shrink_systems(c, info);
+        if (r) return r;
+      }
+      c->acc_valid = true;
+      break;
+    }
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return NBS_OK;
 }
 
 // Runs `nsteps` steps (or one force evaluation when nsteps == 0), growing
@@ -731,6 +907,8 @@ void nbs_destroy(nbs_ctx* c) {
   free_all(c->node_bufs);
   free_all(c->sys_bufs);
   free_all(c->misc_bufs);
+  free_all(c->col_bufs);
+  free_all(c->col_pair_bufs);
   free_snapshots(c);
   if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
   if (c->snap_ready) (void)hipEventDestroy(c->snap_ready);
@@ -1004,6 +1182,36 @@ int nbs_snapshot_end(nbs_ctx* c, const double** pos, const double** vel,
   return NBS_OK;
 }
 
+int nbs_set_collisions(nbs_ctx* c, int enabled, double range) {
+  if (!c) return NBS_ERR_ARG;
+  if (enabled && !(range >= 0)) return fail(c, NBS_ERR_ARG, "collision range must be >= 0");
+  c->collide = enabled != 0;
+  c->collide_range = enabled ? range : 0.0;
+  c->col_pairs = 0;
+  c->col_removed = 0;
+  drop_graph(c);
+  return NBS_OK;
+}
+
+int nbs_collision_stats(nbs_ctx* c, int64_t* pairs, int64_t* removed) {
+  if (!c) return NBS_ERR_ARG;
+  if (pairs) *pairs = c->col_pairs;
+  if (removed) *removed = c->col_removed;
+  return NBS_OK;
+}
+
+int nbs_particle_count(nbs_ctx* c, int64_t* n, int64_t* offsets, int nsys_cap) {
+  if (!c) return NBS_ERR_ARG;
+  if (n) *n = c->has_state ? c->n : 0;
+  if (offsets) {
+    if (!c->has_state) return fail(c, NBS_ERR_STATE, "no particle state set");
+    const int nsys = c->sys.nsys;
+    if (nsys_cap < nsys + 1) return fail(c, NBS_ERR_ARG, "offsets need nsys + 1 entries");
+    for (int k = 0; k <= nsys; k++) offsets[k] = c->h_start[k];
+  }
+  return NBS_OK;
+}
+
 int nbs_forces(nbs_ctx* c, double* acc) {
   if (!c) return NBS_ERR_ARG;
   HIPCHK(c, hipSetDevice(c->device));
@@ -1018,6 +1226,7 @@ int nbs_step(nbs_ctx* c, int nsteps) {
   if (nsteps < 0) return fail(c, NBS_ERR_ARG, "nsteps must be >= 0");
   if (nsteps == 0) return NBS_OK;
   HIPCHK(c, hipSetDevice(c->device));
+  if (c->collide) return run_collide(c, nsteps);
   return run(c, nsteps);
 }
 
@@ -1025,6 +1234,8 @@ int nbs_step_graph(nbs_ctx* c, int nsteps) {
   if (!c) return NBS_ERR_ARG;
   if (nsteps < 0) return fail(c, NBS_ERR_ARG, "nsteps must be >= 0");
   if (!c->has_state) return fail(c, NBS_ERR_STATE, "no particle state set");
+  if (c->collide)
+    return fail(c, NBS_ERR_STATE, "collision merging runs eager steps (nbs_step)");
   if (nsteps == 0 || c->n == 0) return NBS_OK;
   HIPCHK(c, hipSetDevice(c->device));
   int r = ensure_nodes(c);

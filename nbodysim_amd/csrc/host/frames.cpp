@@ -74,8 +74,6 @@ extern "C" int nbs_run_frames(const char* cfgfile) {
   int r = nbs_settings_read(&s, cfgfile, msg, sizeof(msg));
   if (msg[0]) fputs(msg, stderr);
   if (r != NBS_OK) return 1;
-  if (s.collide)
-    printf("WARNING: collision is not supported by the GPU step; ignored\n");
   if (s.min_node_size > 0)
     printf("WARNING: min_node_size is not supported by the GPU step; ignored\n");
   if (s.dump_image)
@@ -139,7 +137,15 @@ extern "C" int nbs_run_frames(const char* cfgfile) {
     nbs_destroy(ctx);
     return 1;
   }
+  if (s.collide && nbs_set_collisions(ctx, 1, s.collision_range) != NBS_OK) {
+    fprintf(stderr, "%s\n", nbs_last_error(ctx));
+    nbs_destroy(ctx);
+    return 1;
+  }
   if (s.dump_binary || s.dump_text) mkdir("./data", 0755);
+  // With collisions the masses change: each frame's are read back into one
+  // of two buffers (the writer of the frame before reads the other).
+  std::vector<double> mass_buf[2];
 
   unsigned frame = start_frame;
   printf("Frame %u/%u\n", frame, s.num_frames);
@@ -156,7 +162,7 @@ extern "C" int nbs_run_frames(const char* cfgfile) {
   auto hand_over = [&]() -> bool {
     Frame f;
     f.frame = pending_frame;
-    f.mass = mass.data();
+    f.mass = s.collide ? mass_buf[pending_frame & 1].data() : mass.data();
     if (nbs_snapshot_end(ctx, &f.pos, &f.vel, &f.acc, &f.n) != NBS_OK) {
       fprintf(stderr, "%s\n", nbs_last_error(ctx));
       return false;
@@ -180,6 +186,16 @@ extern "C" int nbs_run_frames(const char* cfgfile) {
       if (pending && !hand_over()) {
         status = 1;
         break;
+      }
+      if (s.collide) {
+        int64_t m = 0;
+        std::vector<double>& mb = mass_buf[frame & 1];
+        if (nbs_particle_count(ctx, &m, nullptr, 0) == NBS_OK) mb.resize((size_t)m);
+        if (nbs_get_mass(ctx, mb.data()) != NBS_OK) {
+          fprintf(stderr, "%s\n", nbs_last_error(ctx));
+          status = 1;
+          break;
+        }
       }
       if (nbs_snapshot_begin(ctx) != NBS_OK) {
         fprintf(stderr, "%s\n", nbs_last_error(ctx));

@@ -68,7 +68,7 @@ struct State {
 };
 
 // Flags word bits (device int).
-enum : int { kFlagNodeOverflow = 1, kFlagRunOverflow = 2, kFlagStack = 4 };
+enum : int { kFlagNodeOverflow = 1, kFlagRunOverflow = 2, kFlagStack = 4, kFlagPairs = 8 };
 
 struct StepParams {
   double theta;
@@ -182,6 +182,13 @@ struct WalkArgs {
   // frozen decisions (FP64 only; null otherwise): base tree + positions
   const WalkNode* mac_nodes;
   const double *mac_x, *mac_y, *mac_z;
+  // collision pairs (null: off): a leaf j accepted by particle i at
+  // distance < collide_range appends (min(i, j), max(i, j)); pair_count
+  // counts every detection, including those beyond pair_cap
+  int2* pairs;
+  int* pair_count;
+  int pair_cap;
+  double collide_range;
 };
 
 // Launch shape of the walk: block = workgroup size (64, 128, 256, 512 or
@@ -247,6 +254,32 @@ void launch_adaptive_dt(const State& st, const double* ax, const double* ay,
 void launch_update(State& st, const double* ax, const double* ay,
                    const double* az, int n, Systems& sys, const int* flags,
                    hipStream_t s);
+
+// ------------------------------------------------------------- collide --
+// Collision merging after a walk that recorded pairs (WalkArgs::pairs).
+// Buffers: parent, mark, alive, id_alive [cap_n]; newpos, newid [cap_n+1];
+// the compacted acceleration / interaction arrays [cap_n]; sort keys and
+// values [2][2 * pair_cap]; info = {members, total, new starts[nsys+1]}.
+struct CollideBuffers {
+  int2* pairs;
+  int* count;  // [2]: pairs detected, members listed
+  int pair_cap;
+  int *parent, *mark, *alive, *id_alive, *newpos, *newid;
+  double *cax, *cay, *caz;
+  int* cinter;
+  unsigned long long *key0, *key1;
+  unsigned *val0, *val1;
+  int* sort_tmp;
+  int* scan_tmp;
+  int* info;
+};
+size_t collide_sort_tmp_ints(int pair_cap);
+// Merges the groups of the recorded pairs, updates (kick-drift; merged
+// particles with zero acceleration) and compacts `cur` into `nxt`,
+// accelerations and interaction counts into cb.cax.. / cb.cinter.
+void launch_collide(CollideBuffers& cb, State& cur, State& nxt, double* ax, double* ay,
+                    double* az, const int* inter, int n, Systems& sys, double dt,
+                    int* flags, hipStream_t s);
 
 // Small fills (no hipMemsetAsync: these run inside captured graphs too).
 void launch_fill_int(int* p, int v, size_t n, hipStream_t s);

@@ -646,6 +646,25 @@ __device__ __forceinline__ void walk_visit_f64(ConstNodes nodes, int& n, int i,
   }
 }
 
+// Collision detection (src/main.cpp:386-396): a leaf accepted by a particle
+// (never its own) whose particle lies closer than the collision range --
+// distance(leaf particle, particle) with the reference's rounding -- is
+// recorded as the pair (min slot, max slot).  Both particles of a close
+// pair usually record it; duplicates are harmless to the merge.  Called
+// wave-uniformly; one atomic per wave.
+__device__ __forceinline__ void record_collision(const WalkArgs& a, bool inter, double dx,
+                                                 double dy, double dz, int i, int leaf) {
+  const bool hit = inter && __dsqrt_rn(sq_sum_ref(dx, dy, dz)) < a.collide_range;
+  const unsigned long long h = ballot(hit);
+  if (!h) return;
+  const int first = __builtin_ctzll(h);
+  int base = 0;
+  if (lane_id() == first) base = atomicAdd(a.pair_count, __popcll(h));
+  base = __builtin_amdgcn_readlane(base, first);
+  const int slot = base + mbcnt(h);
+  if (hit && slot < a.pair_cap) a.pairs[slot] = make_int2(min(i, leaf), max(i, leaf));
+}
+
 // Workgroup -> tile order.  Workgroups are handed to the 8 XCDs round-robin
 // by workgroup id, each XCD with its own L2.  With `groups` > 1 the tiles
 // (runs of Morton-consecutive waves) are cut into `groups` contiguous
@@ -736,9 +755,9 @@ walk_kernel(WalkArgs a) {
   }
 
   unsigned visits = 0;
-  // (the hand-written loops do not count visits: per-tile costs come from
-  // the generic loop)
-  if (F32 && !FROZEN && P == 1 && a.asm_loop && !a.tile_cost) {
+  // (the hand-written loops neither count visits -- per-tile costs come
+  // from the generic loop -- nor record collisions)
+  if (F32 && !FROZEN && P == 1 && a.asm_loop && !a.tile_cost && !a.pairs) {
     int ovf = 0;
     float fax = 0.f, fay = 0.f, faz = 0.f;
     if (DAMP)
@@ -750,7 +769,7 @@ walk_kernel(WalkArgs a) {
     ay[0] = fay;
     az[0] = faz;
     overflow = ovf != 0;
-  } else if (!F32 && !DAMP && !FROZEN && P == 1 && a.asm_loop && !a.tile_cost) {
+  } else if (!F32 && !DAMP && !FROZEN && P == 1 && a.asm_loop && !a.tile_cost && !a.pairs) {
     int ovf = 0, top = 0, top_pos = -1, st_pos = -1, st_lo = 0, st_hi = 0;
     double fax = 0, fay = 0, faz = 0;
     unsigned long long Acur = A[0];
@@ -804,6 +823,10 @@ walk_kernel(WalkArgs a) {
 #pragma unroll
         for (int p = 0; p < P; p++) inter[p] = in[p] && leaf != i[p];
         next = n + 1;
+        if (a.pairs) {
+#pragma unroll
+          for (int p = 0; p < P; p++) record_collision(a, inter[p], dx[p], dy[p], dz[p], i[p], leaf);
+        }
       } else {
         unsigned long long O[P], any_open = 0;
 #pragma unroll

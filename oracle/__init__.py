@@ -77,6 +77,10 @@ def lib():
         L.orc_run.argtypes = [ctypes.c_int, ctypes.c_size_t, _dp, _dp, _dp, _dp,
                               ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                               ctypes.c_int]
+        L.orc_run_collide.restype = ctypes.c_int64
+        L.orc_run_collide.argtypes = [ctypes.c_size_t, _dp, _dp, _dp, _dp, ctypes.c_double,
+                                      ctypes.c_double, ctypes.c_int, ctypes.c_double,
+                                      ctypes.c_int, ctypes.c_int, _i64p, _i64p]
         L.orc_direct.argtypes = [ctypes.c_size_t, _dp, _dp, ctypes.c_int, ctypes.c_size_t,
                                  _i64p, ctypes.c_int, _dp]
         _lib = L
@@ -207,6 +211,23 @@ def run(pos, vel, mass, theta=0.5, dt=0.0333, frames=1, damping=False, prec="f64
     if rc:
         raise ValueError("orc_run failed (coincident particles)")
     return pos, vel, acc
+
+
+def run_collide(pos, vel, mass, theta=0.5, dt=0.0333, frames=1, damping=False, range=0.01,
+                threads=1):
+    """FP64 frames with collision merging; returns (pos, vel, acc, mass,
+    pairs detected, particles absorbed), compacted to the survivors."""
+    pos = np.array(pos, dtype=np.float64, order="C")
+    vel = np.array(vel, dtype=np.float64, order="C")
+    mass = np.array(mass, dtype=np.float64, order="C")
+    acc = np.zeros_like(pos)
+    npairs, nrem = ctypes.c_int64(), ctypes.c_int64()
+    n = lib().orc_run_collide(len(pos), _p(pos), _p(vel), _p(acc), _p(mass), theta, dt,
+                              int(damping), range, frames, threads, ctypes.byref(npairs),
+                              ctypes.byref(nrem))
+    if n < 0:
+        raise ValueError("orc_run_collide failed (coincident particles)")
+    return pos[:n], vel[:n], acc[:n], mass[:n], npairs.value, nrem.value
 
 
 def direct(pos, mass, idx=None, damping=False, threads=8):

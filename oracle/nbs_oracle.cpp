@@ -42,6 +42,7 @@
 #include <cstring>
 #include <random>
 #include <thread>
+#include <utility>
 #include <vector>
 
 namespace {
@@ -280,7 +281,8 @@ struct Tree {
   // src/main.cpp:335-421 for one particle: BFS with a FIFO, MAC
   // side / distance(com, x) > theta on particle-free nodes, self skipped.
   V3<T> walk(int self, T theta, bool damping, std::vector<int>& q,
-             long* interactions) const {
+             long* interactions, double range = -1.0,
+             std::vector<std::pair<int, int>>* pairs = nullptr) const {
     V3<T> acc = {0, 0, 0};
     const V3<T>& x = pos[self];
     q.clear();
@@ -293,6 +295,13 @@ struct Tree {
         for (int i = 0; i < 8; i++)
           if (nd.child[i] >= 0) q.push_back(nd.child[i]);
       } else if (nd.p != self) {
+        // src/main.cpp:386-396: a leaf's particle closer than the
+        // collision range, distance(node particle, particle)
+        if (pairs && nd.p >= 0) {
+          const V3<T>& o = pos[nd.p];
+          if (ref_norm3(o.x - x.x, o.y - x.y, o.z - x.z) < range)
+            pairs->push_back({std::min(self, nd.p), std::max(self, nd.p)});
+        }
         V3<T> g = ref_gravity(nd.com, nd.mass, x, damping);
         acc.x += g.x;
         acc.y += g.y;
@@ -753,6 +762,132 @@ int orc_run(int prec, size_t n, double* pos, double* vel, double* acc,
     return 0;
   };
   return prec == 0 ? run(double()) : run(float());
+}
+
+// Frames with collision merging (src/main.cpp:1420-1492 with
+// config.collide, FP64): gen_root, calc_mass, calc_com, the walk recording
+// pairs closer than `range` (src/main.cpp:386-396), the merge of every
+// connected group of pairs into one particle, then update_all.  Each merge
+// is collide() (src/main.cpp:423-430): mass a.m + b.m, position and
+// velocity weighted_average(a, b, a.m, b.m) (src/vector.cpp:129-132),
+// acceleration zero.  The reference merges the pairs in the order of its
+// hash set, which is not reproducible; here each group folds collide over
+// its members in increasing index (for a group of two the order does not
+// matter: both products and the sum commute).  The merged particle takes
+// the place of its lowest-index member and the absorbed ones are removed,
+// the others keeping their order.  Arrays are compacted in place; returns
+// the new count (-1 on a tree overflow).  pairs_out / removed_out (may be
+// null) receive the detections and absorbed particles of all frames.
+int64_t orc_run_collide(size_t n, double* pos, double* vel, double* acc, double* mass,
+                        double theta, double dt, int damping, double range, int frames,
+                        int threads, int64_t* pairs_out, int64_t* removed_out) {
+  std::vector<V3<double>> p, v, a(n);
+  std::vector<double> m, dummy;
+  load_state<double>(n, pos, mass, p, m);
+  load_state<double>(n, vel, nullptr, v, dummy);
+  std::vector<std::vector<int>> qs(std::max(threads, 1));
+  int64_t npairs = 0, nremoved = 0;
+  for (int f = 0; f < frames && n > 0; f++) {
+    Tree<double> tree;
+    V3<double> o;
+    double side;
+    Tree<double>::root_of(p.data(), n, o, side);
+    tree.build(p.data(), m.data(), n, o, side);
+    if (tree.overflow) return -1;
+    tree.mass_com();
+    std::vector<std::vector<std::pair<int, int>>> tp(std::max(threads, 1));
+    parallel_for(n, threads, [&](size_t i, int w) {
+      a[i] = tree.walk((int)i, theta, damping != 0, qs[w], nullptr, range, &tp[w]);
+    });
+    std::vector<std::pair<int, int>> pairs;
+    for (auto& t : tp) pairs.insert(pairs.end(), t.begin(), t.end());
+    npairs += (int64_t)pairs.size();
+    // groups: union-find with the smaller index as root
+    std::vector<int> parent(n);
+    for (size_t i = 0; i < n; i++) parent[i] = (int)i;
+    auto find = [&](int x) {
+      while (parent[x] != x) x = parent[x] = parent[parent[x]];
+      return x;
+    };
+    for (auto& q : pairs) {
+      int r1 = find(q.first), r2 = find(q.second);
+      if (r1 == r2) continue;
+      if (r1 > r2) std::swap(r1, r2);
+      parent[r2] = r1;
+    }
+    std::vector<char> member(n, 0), alive(n, 1);
+    for (auto& q : pairs) member[q.first] = member[q.second] = 1;
+    std::vector<int> head(n, -1);  // group root -> merged particle's index
+    for (size_t i = 0; i < n; i++) {
+      if (!member[i]) continue;
+      const int r = find((int)i);
+      if (head[r] < 0) {
+        head[r] = (int)i;  // lowest index of the group: the fold starts here
+        continue;
+      }
+      // collide(acc, particle i), src/main.cpp:423-430
+      const int h = head[r];
+      const double am = m[h], bm = m[i];
+      V3<double>& hp = p[h];
+      V3<double>& hv = v[h];
+      hp = V3<double>{(hp.x * am + p[i].x * bm) / (am + bm), (hp.y * am + p[i].y * bm) / (am + bm),
+                      (hp.z * am + p[i].z * bm) / (am + bm)};
+      hv = V3<double>{(hv.x * am + v[i].x * bm) / (am + bm), (hv.y * am + v[i].y * bm) / (am + bm),
+                      (hv.z * am + v[i].z * bm) / (am + bm)};
+      m[h] = am + bm;
+      a[h] = V3<double>{0, 0, 0};
+      alive[i] = 0;
+    }
+    // update_all, src/particle.cpp:66-74
+    for (size_t i = 0; i < n; i++) {
+      V3<double> t1 = a[i];
+      t1.x *= dt;
+      t1.y *= dt;
+      t1.z *= dt;
+      v[i].x += t1.x;
+      v[i].y += t1.y;
+      v[i].z += t1.z;
+      V3<double> t2 = v[i];
+      t2.x *= dt;
+      t2.y *= dt;
+      t2.z *= dt;
+      p[i].x += t2.x;
+      p[i].y += t2.y;
+      p[i].z += t2.z;
+    }
+    size_t k = 0;
+    for (size_t i = 0; i < n; i++) {
+      if (!alive[i]) continue;
+      p[k] = p[i];
+      v[k] = v[i];
+      a[k] = a[i];
+      m[k] = m[i];
+      k++;
+    }
+    nremoved += (int64_t)(n - k);
+    n = k;
+    p.resize(n);
+    v.resize(n);
+    a.resize(n);
+    m.resize(n);
+  }
+  for (size_t i = 0; i < n; i++) {
+    pos[3 * i] = p[i].x;
+    pos[3 * i + 1] = p[i].y;
+    pos[3 * i + 2] = p[i].z;
+    vel[3 * i] = v[i].x;
+    vel[3 * i + 1] = v[i].y;
+    vel[3 * i + 2] = v[i].z;
+    if (acc) {
+      acc[3 * i] = a[i].x;
+      acc[3 * i + 1] = a[i].y;
+      acc[3 * i + 2] = a[i].z;
+    }
+    mass[i] = m[i];
+  }
+  if (pairs_out) *pairs_out = npairs;
+  if (removed_out) *removed_out = nremoved;
+  return (int64_t)n;
 }
 
 // FP64 direct sum (the accuracy reference), long-double accumulation,

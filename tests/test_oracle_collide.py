@@ -1,0 +1,137 @@
+"""Pins of the oracle's collision frames (orc_run_collide): what the
+reference's rules fix without its main.cpp, which does not compile here.
+
+- Range 0 detects nothing and reduces to the plain frames (orc_run).
+- Detection equals brute force: pairs closer than the range, found from
+  both ends when each reaches the other as a leaf (src/main.cpp:386-396).
+- Merging is transitive (the update table of src/main.cpp:472-490): a chain
+  a-b, b-c merges into one particle even though a and c are far apart.
+- A merge conserves mass exactly for two bodies, momentum and the centre of
+  mass to rounding, and leaves the merged velocity unchanged by the update
+  (its acceleration is zero).
+- A planted mistake in the weighting (masses swapped) breaks the centre of
+  mass, and one in the order of the survivors breaks their identities."""
+import numpy as np
+
+import oracle as O
+
+
+def planted(seed=5, n=400, rng_range=0.01):
+    """A cube of n particles plus close pairs, a triple and a chain."""
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(-50, 50, size=(n, 3))
+    extra = []
+    for k in range(6):  # pairs at 0.3 range
+        c = rng.uniform(-40, 40, size=3)
+        extra += [c, c + np.array([0.3 * rng_range, 0, 0])]
+    c = np.array([10.0, 20.0, 30.0])  # a triple, all within range
+    extra += [c, c + [0, 0.2 * rng_range, 0], c + [0, 0, 0.25 * rng_range]]
+    c = np.array([-25.0, 5.0, 12.0])  # a chain: ends 1.4 range apart
+    extra += [c, c + [0.7 * rng_range, 0, 0], c + [1.4 * rng_range, 0, 0]]
+    pos = np.concatenate([pos, np.array(extra)])
+    perm = rng.permutation(len(pos))
+    pos = pos[perm]
+    vel = rng.uniform(-1, 1, size=pos.shape)
+    mass = rng.uniform(1e6, 1e7, size=len(pos))
+    return pos, vel, mass
+
+
+def components(pos, rng_range):
+    """Brute force: groups of the graph of pairs closer than the range."""
+    n = len(pos)
+    d = np.sqrt(((pos[:, None, :] - pos[None, :, :]) ** 2).sum(-1))
+    adj = (d < rng_range) & ~np.eye(n, dtype=bool)
+    parent = list(range(n))
+
+    def find(x):
+        while parent[x] != x:
+            x = parent[x]
+        return x
+    for i, j in zip(*np.nonzero(adj)):
+        a, b = find(i), find(j)
+        if a != b:
+            parent[max(a, b)] = min(a, b)
+    groups = {}
+    for i in range(n):
+        groups.setdefault(find(i), []).append(i)
+    return [g for g in groups.values() if len(g) > 1], int(adj.sum())
+
+
+def test_range_zero_is_the_plain_frame():
+    pos, vel, mass = planted()
+    p, v, a, m, npairs, nrem = O.run_collide(pos, vel, mass, frames=3, range=0.0)
+    p2, v2, a2 = O.run(pos, vel, mass, frames=3)
+    assert npairs == 0 and nrem == 0
+    np.testing.assert_array_equal(p, p2)
+    np.testing.assert_array_equal(v, v2)
+    np.testing.assert_array_equal(a, a2)
+    np.testing.assert_array_equal(m, mass)
+
+
+def test_detection_matches_brute_force_and_chains_merge():
+    r = 0.01
+    pos, vel, mass = planted(rng_range=r)
+    groups, ndirected = components(pos, r)
+    sizes = sorted(len(g) for g in groups)
+    assert sizes == [2] * 6 + [3, 3]  # the chain merges whole
+    p, v, a, m, npairs, nrem = O.run_collide(pos, vel, mass, frames=1, range=r)
+    assert npairs == ndirected  # each close pair found from both ends
+    assert nrem == sum(len(g) - 1 for g in groups)
+    assert len(m) == len(mass) - nrem
+    # survivors in order; each merged particle in its lowest member's place
+    keep = np.ones(len(mass), dtype=bool)
+    for g in groups:
+        keep[sorted(g)[1:]] = False
+    idx = np.nonzero(keep)[0]
+    solo = np.array([all(i not in g for g in groups) for i in idx])
+    np.testing.assert_array_equal(m[solo], mass[idx[solo]])
+    for g in groups:
+        j = int(np.searchsorted(idx, min(g)))
+        gm = mass[g]
+        if len(g) == 2:
+            assert m[j] == gm[0] + gm[1]  # exact for two bodies
+        assert abs(m[j] - gm.sum()) <= 1e-15 * gm.sum()
+        mom = (gm[:, None] * vel[g]).sum(0)
+        np.testing.assert_allclose(m[j] * v[j], mom, rtol=1e-14, atol=1e-9)
+        # the merged particle drifts from the centre of mass with zero
+        # acceleration
+        com = (gm[:, None] * pos[g]).sum(0) / gm.sum()
+        np.testing.assert_allclose(p[j], com + v[j] * 0.0333, rtol=0, atol=1e-12)
+        np.testing.assert_array_equal(a[j], 0.0)
+
+
+def test_planted_mistakes_are_caught():
+    r = 0.01
+    pos, vel, mass = planted(rng_range=r)
+    groups, _ = components(pos, r)
+    g = sorted(groups[0])
+    p, v, a, m, _, _ = O.run_collide(pos, vel, mass, frames=1, range=r)
+    keep = np.ones(len(mass), dtype=bool)
+    for gg in groups:
+        keep[sorted(gg)[1:]] = False
+    idx = np.nonzero(keep)[0]
+    j = int(np.searchsorted(idx, g[0]))
+    gm = mass[g]
+    com = (gm[:, None] * pos[g]).sum(0) / gm.sum()
+    # weights swapped: the centre-of-mass check fails
+    wrong = (pos[g[0]] * gm[1] + pos[g[1]] * gm[0]) / gm.sum() + v[j] * 0.0333
+    assert np.abs(wrong - (com + v[j] * 0.0333)).max() > 1e-9
+    assert np.abs(p[j] - wrong).max() > 1e-9
+    # survivors out of order: identities (masses) no longer match
+    solo = [i for i in range(len(idx)) if all(idx[i] not in gg for gg in groups)]
+    assert not np.array_equal(m[solo][::-1], mass[idx[solo]])
+
+
+def test_whole_cluster_merges_into_one():
+    rng = np.random.default_rng(3)
+    pos = rng.uniform(0, 0.004, size=(60, 3))
+    vel = rng.uniform(-1, 1, size=(60, 3))
+    mass = rng.uniform(1e3, 1e4, size=60)
+    p, v, a, m, npairs, nrem = O.run_collide(pos, vel, mass, frames=1, range=0.01)
+    assert len(m) == 1 and nrem == 59
+    # pairs are detected only where a leaf is reached: cells of the blob
+    # far enough (side / distance <= theta) are accepted whole, so fewer
+    # than all 60 * 59 ordered pairs are seen, and transitivity merges all
+    assert 0 < npairs < 60 * 59
+    assert abs(m[0] - mass.sum()) <= 1e-15 * mass.sum()
+    np.testing.assert_allclose(m[0] * v[0], (mass[:, None] * vel).sum(0), rtol=1e-13)
