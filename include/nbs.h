@@ -145,7 +145,7 @@ int nbs_step(nbs_ctx* ctx, int nsteps);
 int nbs_step_graph(nbs_ctx* ctx, int nsteps);
 
 /* Collision merging (config.collide / collision_range, src/main.cpp:1442-1483;
- * the reference's CUDA path ignores it, :1449-1452).  While enabled, each
+ * the reference's CUDA path ignores it, :1445-1448).  While enabled, each
  * nbs_step step records the pairs (particle, accepted leaf) closer than
  * `range` during the walk (src/main.cpp:386-396), merges every connected
  * group of pairs into one particle -- collide() folded over the members in

@@ -319,3 +319,21 @@ def ref_run(pos, vel, mass, theta=0.5, dt=0.0333, frames=1, damping=False, prec=
     acc = np.zeros_like(pos)
     L.ref_run(len(pos), _p(pos), _p(vel), _p(acc), _p(mass), theta, dt, int(damping), frames)
     return pos, vel, acc
+
+
+def ref_run_collide(pos, vel, mass, theta=0.5, dt=0.0333, frames=1, damping=False, range=0.01):
+    """Collision frames over the reference's classes (FP64); returns (pos,
+    vel, acc, mass, distinct pairs recorded), compacted to the survivors."""
+    L = ref_lib("f64")
+    L.ref_run_collide.restype = ctypes.c_long
+    L.ref_run_collide.argtypes = [ctypes.c_size_t, _dp, _dp, _dp, _dp, ctypes.c_double,
+                                  ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                  _lp]
+    pos = np.array(pos, dtype=np.float64, order="C")
+    vel = np.array(vel, dtype=np.float64, order="C")
+    mass = np.array(mass, dtype=np.float64, order="C")
+    acc = np.zeros_like(pos)
+    npairs = ctypes.c_long()
+    n = L.ref_run_collide(len(pos), _p(pos), _p(vel), _p(acc), _p(mass), theta, dt,
+                          int(damping), range, frames, ctypes.byref(npairs))
+    return pos[:n], vel[:n], acc[:n], mass[:n], npairs.value

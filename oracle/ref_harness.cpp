@@ -21,6 +21,7 @@
 #include <cstring>
 #include <limits>
 #include <queue>
+#include <set>
 #include <unordered_map>
 #include <vector>
 
@@ -127,6 +128,46 @@ void walk(RefState& s, particle* curr, datatype theta, bool damping,
     }
   }
   if (cnt) *cnt = c;
+}
+
+// collide() of src/main.cpp:423-430 (main.cpp does not compile here):
+// weighted_average and the particle constructor are the reference's code.
+particle* ref_collide(particle* a, particle* b) {
+  vector pos = weighted_average(a->get_pos(), b->get_pos(), a->get_mass(), b->get_mass());
+  vector vel = weighted_average(a->get_vel(), b->get_vel(), a->get_mass(), b->get_mass());
+  vector null(0, 0, 0);
+  return new particle(&pos, &vel, &null, a->get_mass() + b->get_mass());
+}
+
+// The walk of src/main.cpp:368-400 with the collision test of :386-396:
+// an accepted leaf whose particle is closer than `range` (the reference's
+// distance()) records the pair, lower-x particle first, as indices.
+void walk_collide(RefState& s, particle* curr, datatype theta, bool damping, double range,
+                  const std::unordered_map<particle*, size_t>& index,
+                  std::set<std::pair<size_t, size_t>>& pairs) {
+  std::queue<octree*> nodes;
+  curr->set_acc_zero();
+  nodes.push(s.root);
+  while (!nodes.empty()) {
+    octree* node = nodes.front();
+    nodes.pop();
+    if (node->get_side() / distance(node->get_com(), curr->get_pos()) > theta &&
+        node->get_particle() == NULL) {
+      for (unsigned int i = 0; i < 8; i++)
+        if (node->get_child(i) != NULL) nodes.push(node->get_child(i));
+    } else if (curr != node->get_particle()) {
+      particle* o = node->get_particle();
+      if (o != NULL && distance(o->get_pos(), curr->get_pos()) < range) {
+        const size_t a = index.at(o), b = index.at(curr);
+        if (o->get_pos()->get_x() < curr->get_pos()->get_x())
+          pairs.insert({a, b});
+        else
+          pairs.insert({b, a});
+      }
+      vector g = node_gravity(curr, node, damping);
+      curr->set_acc_offset(&g);
+    }
+  }
 }
 
 }  // namespace
@@ -253,6 +294,82 @@ void ref_run(size_t n, double* pos, double* vel, double* acc,
     acc[3 * i + 1] = p->get_acc()->get_y();
     acc[3 * i + 2] = p->get_acc()->get_z();
   }
+}
+
+// Collision frames (src/main.cpp:1420-1492 with config.collide) over the
+// reference's classes: walk_collide, then the merge of src/main.cpp:472-498
+// (an update table redirects merged particles to their merger; a pair whose
+// ends already share one is skipped) with ref_collide, then update_collision
+// (:500-550) and particle::update.  The pairs are merged in increasing
+// (first, second) index order: the reference's unordered_set order is not
+// reproducible.  Output: survivors in index order, each merged particle in
+// its lowest-index member's place.  Returns the new count; pairs_out gets
+// the distinct pairs recorded over all frames.
+long ref_run_collide(size_t n, double* pos, double* vel, double* acc, double* mass,
+                     double theta, double dt, int damping, double range, int frames,
+                     long* pairs_out) {
+  RefState s;
+  load(s, n, pos, vel, mass);
+  long npairs = 0;
+  for (int f = 0; f < frames; f++) {
+    std::unordered_map<particle*, size_t> index;
+    for (size_t i = 0; i < s.ps.size(); i++) index[s.ps[i]] = i;
+    build(s);
+    std::set<std::pair<size_t, size_t>> pairs;
+    for (particle* p : s.ps)
+      walk_collide(s, p, (datatype)theta, damping != 0, range, index, pairs);
+    delete s.root;
+    s.root = nullptr;
+    npairs += (long)pairs.size();
+    std::unordered_map<particle*, particle*> table;
+    std::vector<particle*> made;
+    auto resolve = [&](particle* p) {
+      for (auto it = table.find(p); it != table.end(); it = table.find(p)) p = it->second;
+      return p;
+    };
+    for (const auto& q : pairs) {
+      particle* a = resolve(s.ps[q.first]);
+      particle* b = resolve(s.ps[q.second]);
+      if (a == b) continue;
+      particle* m = ref_collide(a, b);
+      made.push_back(m);
+      table[a] = m;
+      table[b] = m;
+    }
+    std::vector<particle*> next;
+    std::unordered_map<particle*, bool> placed;
+    for (particle* p : s.ps) {
+      if (table.find(p) == table.end()) {
+        next.push_back(p);
+        continue;
+      }
+      particle* r = resolve(p);
+      if (!placed[r]) {
+        placed[r] = true;
+        next.push_back(r);
+      }
+      delete p;
+    }
+    for (particle* m : made)
+      if (!placed[m]) delete m;
+    s.ps.swap(next);
+    for (particle* p : s.ps) p->update((datatype)dt);
+  }
+  for (size_t i = 0; i < s.ps.size(); i++) {
+    particle* p = s.ps[i];
+    pos[3 * i] = p->get_pos()->get_x();
+    pos[3 * i + 1] = p->get_pos()->get_y();
+    pos[3 * i + 2] = p->get_pos()->get_z();
+    vel[3 * i] = p->get_vel()->get_x();
+    vel[3 * i + 1] = p->get_vel()->get_y();
+    vel[3 * i + 2] = p->get_vel()->get_z();
+    acc[3 * i] = p->get_acc()->get_x();
+    acc[3 * i + 1] = p->get_acc()->get_y();
+    acc[3 * i + 2] = p->get_acc()->get_z();
+    mass[i] = p->get_mass();
+  }
+  if (pairs_out) *pairs_out = npairs;
+  return (long)s.ps.size();
 }
 
 }  // extern "C"

@@ -12,6 +12,7 @@ reference's rules fix without its main.cpp, which does not compile here.
 - A planted mistake in the weighting (masses swapped) breaks the centre of
   mass, and one in the order of the survivors breaks their identities."""
 import numpy as np
+import pytest
 
 import oracle as O
 
@@ -135,3 +136,34 @@ def test_whole_cluster_merges_into_one():
     assert 0 < npairs < 60 * 59
     assert abs(m[0] - mass.sum()) <= 1e-15 * mass.sum()
     np.testing.assert_allclose(m[0] * v[0], (mass[:, None] * vel).sum(0), rtol=1e-13)
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref not built (no reference tree)")
+@pytest.mark.parametrize("frames", [1, 3])
+def test_against_the_reference_classes(frames):
+    """The reference's own distance(), weighted_average(), particle and
+    octree code (oracle/_ref): the same particles absorbed, and everything
+    bit-identical outside groups of three or more (whose merge order the
+    reference leaves to its hash set), those within rounding."""
+    r = 0.01
+    pos, vel, mass = planted(rng_range=r)
+    groups, ndirected = components(pos, r)
+    p, v, a, m, npairs, nrem = O.run_collide(pos, vel, mass, frames=frames, range=r)
+    rp, rv, ra, rm, rpairs = O.ref_run_collide(pos, vel, mass, frames=frames, range=r)
+    assert len(m) == len(rm) == len(mass) - nrem
+    # the oracle counts detections from both ends; the reference's set keeps
+    # a close pair once, lower x first -- both orientations when the x are
+    # equal (the planted triple)
+    d = np.sqrt(((pos[:, None, :] - pos[None, :, :]) ** 2).sum(-1))
+    i, j = np.nonzero(np.triu(d < r, 1))
+    assert npairs == ndirected == 2 * len(i)
+    if frames == 1:
+        assert rpairs == len(i) + int((pos[i, 0] == pos[j, 0]).sum())
+    keep = np.ones(len(mass), dtype=bool)
+    for g in groups:
+        keep[sorted(g)[1:]] = False
+    idx = np.nonzero(keep)[0]
+    big = np.array([any(i == min(g) and len(g) > 2 for g in groups) for i in idx])
+    for x, y in ((p, rp), (v, rv), (a, ra), (m, rm)):
+        np.testing.assert_array_equal(x[~big], y[~big])
+        np.testing.assert_allclose(x[big], y[big], rtol=1e-14, atol=1e-12)
