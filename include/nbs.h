@@ -146,8 +146,8 @@ int nbs_step_graph(nbs_ctx* ctx, int nsteps);
 
 /* Collision merging (config.collide / collision_range, src/main.cpp:1442-1483;
  * the reference's CUDA path ignores it, :1445-1448).  While enabled, each
- * nbs_step step records the pairs (particle, accepted leaf) closer than
- * `range` during the walk (src/main.cpp:386-396), merges every connected
+ * nbs_step step records the pairs (particle, leaf its walk reaches under
+ * the FP64 MAC) closer than `range` (src/main.cpp:386-396), merges every connected
  * group of pairs into one particle -- collide() folded over the members in
  * increasing caller index: mass sum, mass-weighted position and velocity
  * (src/main.cpp:423-430, src/vector.cpp:129-132) -- with zero acceleration

@@ -239,11 +239,10 @@ struct nbs_ctx {
   int snap_n = 0;
 
   // Collision merging (nbs_set_collisions): buffers sized for col_cap_n
-  // particles and cb.pair_cap pairs; col_active while a collide step's walk
-  // is enqueued; running totals of detections and absorbed particles.
+  // particles and cb.pair_cap pairs; running totals of detections and
+  // absorbed particles.
   bool collide = false;
   double collide_range = 0;
-  bool col_active = false;
   CollideBuffers cb{};
   std::vector<void*> col_bufs, col_pair_bufs;
   int col_cap_n = 0;
@@ -534,12 +533,6 @@ void enqueue_walk(nbs_ctx* c, int fused) {
     a.order = c->tile_order;
     a.tile_cost = c->tile_cost;
   }
-  if (c->col_active) {
-    a.pairs = c->cb.pairs;
-    a.pair_count = c->cb.count;
-    a.pair_cap = c->cb.pair_cap;
-    a.collide_range = c->collide_range;
-  }
   launch_walk(a, c->cfg.force_precision == NBS_FORCE_F32, c->p.damping,
               c->walk_block, c->walk_pack, c->stream);
   if (fused) c->bbox_valid = true;
@@ -673,7 +666,8 @@ int shrink_systems(nbs_ctx* c, const std::vector<int>& info) {
 }
 
 // Steps with collision merging, one at a time (src/main.cpp:1420-1492 with
-// config.collide): build, walk recording pairs, merge, update, compact.
+// config.collide): build, walk (forces only), the pruned walk that records
+// the pairs, merge, update, compact.
 // The particle count is read back after each step; a step whose pair or
 // node buffers overflowed changed nothing but the slot order, and is
 // redone with larger buffers.
@@ -692,13 +686,31 @@ int run_collide(nbs_ctx* c, int nsteps) {
       reset_flags(c);
       launch_fill_int(c->cb.count, 0, 2, c->stream);
       enqueue_build(c);
-      c->col_active = true;
       enqueue_walk(c, 0);
-      c->col_active = false;
       hipEvent_t e0 = nullptr;
       prof_begin(c, kPhaseUpdate, &e0);
       State& cur = c->st[c->cur];
       State& nxt = c->st[1 - c->cur];
+      DetectArgs d{};
+      d.nodes = c->nodes;
+      d.node_start = c->sys.node_start;
+      d.sys_start = c->sys.start;
+      d.wave_start = c->sys.wave_start;
+      d.nsys = c->sys.nsys;
+      d.n_waves = c->sys.n_waves;
+      d.x = cur.x;
+      d.y = cur.y;
+      d.z = cur.z;
+      d.root = c->sys.root;
+      d.boxes = c->p.root_mode == NBS_ROOT_BBOX;
+      d.theta = c->p.theta;
+      d.range = c->collide_range;
+      d.pairs = c->cb.pairs;
+      d.pair_count = c->cb.count;
+      d.pair_cap = c->cb.pair_cap;
+      d.flags = c->flags;
+      d.flags_out = c->flags;
+      launch_detect(d, c->stream);
       launch_collide(c->cb, cur, nxt, c->ax, c->ay, c->az, c->inter, n, c->sys, c->p.dt,
                      c->flags, c->stream);
       prof_end(c, kPhaseUpdate, e0);

@@ -182,14 +182,29 @@ struct WalkArgs {
   // frozen decisions (FP64 only; null otherwise): base tree + positions
   const WalkNode* mac_nodes;
   const double *mac_x, *mac_y, *mac_z;
-  // collision pairs (null: off): a leaf j accepted by particle i at
-  // distance < collide_range appends (min(i, j), max(i, j)); pair_count
-  // counts every detection, including those beyond pair_cap
+};
+
+// Collision pairs: every (particle i, leaf j) that i's walk reaches -- the
+// FP64 MAC -- with distance < range appends (min(i, j), max(i, j));
+// pair_count counts every detection, including those beyond pair_cap.
+struct DetectArgs {
+  const WalkNode* nodes;
+  const int* node_start;
+  const int* sys_start;
+  const int* wave_start;
+  int nsys;
+  int n_waves;
+  const double *x, *y, *z;
+  const double* root;  // per system: centre xyz, side
+  int boxes;           // 1: every particle inside its root (bbox root)
+  double theta, range;
   int2* pairs;
   int* pair_count;
   int pair_cap;
-  double collide_range;
+  const int* flags;
+  int* flags_out;
 };
+void launch_detect(const DetectArgs& a, hipStream_t s);
 
 // Launch shape of the walk: block = workgroup size (64, 128, 256, 512 or
 // 1024 threads); a.groups = contiguous tile regions interleaved over the

@@ -646,25 +646,6 @@ __device__ __forceinline__ void walk_visit_f64(ConstNodes nodes, int& n, int i,
   }
 }
 
-// Collision detection (src/main.cpp:386-396): a leaf accepted by a particle
-// (never its own) whose particle lies closer than the collision range --
-// distance(leaf particle, particle) with the reference's rounding -- is
-// recorded as the pair (min slot, max slot).  Both particles of a close
-// pair usually record it; duplicates are harmless to the merge.  Called
-// wave-uniformly; one atomic per wave.
-__device__ __forceinline__ void record_collision(const WalkArgs& a, bool inter, double dx,
-                                                 double dy, double dz, int i, int leaf) {
-  const bool hit = inter && __dsqrt_rn(sq_sum_ref(dx, dy, dz)) < a.collide_range;
-  const unsigned long long h = ballot(hit);
-  if (!h) return;
-  const int first = __builtin_ctzll(h);
-  int base = 0;
-  if (lane_id() == first) base = atomicAdd(a.pair_count, __popcll(h));
-  base = __builtin_amdgcn_readlane(base, first);
-  const int slot = base + mbcnt(h);
-  if (hit && slot < a.pair_cap) a.pairs[slot] = make_int2(min(i, leaf), max(i, leaf));
-}
-
 // Workgroup -> tile order.  Workgroups are handed to the 8 XCDs round-robin
 // by workgroup id, each XCD with its own L2.  With `groups` > 1 the tiles
 // (runs of Morton-consecutive waves) are cut into `groups` contiguous
@@ -755,9 +736,9 @@ walk_kernel(WalkArgs a) {
   }
 
   unsigned visits = 0;
-  // (the hand-written loops neither count visits -- per-tile costs come
-  // from the generic loop -- nor record collisions)
-  if (F32 && !FROZEN && P == 1 && a.asm_loop && !a.tile_cost && !a.pairs) {
+  // (the hand-written loops do not count visits: per-tile costs come from
+  // the generic loop)
+  if (F32 && !FROZEN && P == 1 && a.asm_loop && !a.tile_cost) {
     int ovf = 0;
     float fax = 0.f, fay = 0.f, faz = 0.f;
     if (DAMP)
@@ -769,7 +750,7 @@ walk_kernel(WalkArgs a) {
     ay[0] = fay;
     az[0] = faz;
     overflow = ovf != 0;
-  } else if (!F32 && !DAMP && !FROZEN && P == 1 && a.asm_loop && !a.tile_cost && !a.pairs) {
+  } else if (!F32 && !DAMP && !FROZEN && P == 1 && a.asm_loop && !a.tile_cost) {
     int ovf = 0, top = 0, top_pos = -1, st_pos = -1, st_lo = 0, st_hi = 0;
     double fax = 0, fay = 0, faz = 0;
     unsigned long long Acur = A[0];
@@ -823,10 +804,6 @@ walk_kernel(WalkArgs a) {
 #pragma unroll
         for (int p = 0; p < P; p++) inter[p] = in[p] && leaf != i[p];
         next = n + 1;
-        if (a.pairs) {
-#pragma unroll
-          for (int p = 0; p < P; p++) record_collision(a, inter[p], dx[p], dy[p], dz[p], i[p], leaf);
-        }
       } else {
         unsigned long long O[P], any_open = 0;
 #pragma unroll
@@ -944,6 +921,120 @@ walk_kernel(WalkArgs a) {
   if (lane == 0) a.inter_total[w0] += (unsigned long long)wtot;
   if (a.tile_cost && lane == 0) atomicAdd(&a.tile_cost[tile], visits);
   if (hw == 0 && lane == 0) atomicAdd(a.steps_done, 1);
+}
+
+// ------------------------------------------------- collision detection --
+// The pairs the reference's walk records for collision merging
+// (src/main.cpp:386-396): particle i and a leaf j that i's walk reaches
+// (every cell above j opened by the MAC) with distance(j, i) < range.  A
+// pruned walk finds exactly those: it opens a cell when the MAC opens it --
+// the reference's FP64 test, whatever the precision of the forces -- and
+// the cell's box comes within range of the lane's particle.  The box is
+// the octree cell of side s that holds the centre of mass: per axis
+// [r0 + k s, r0 + (k + 1) s] with k = floor((com - r0) / s) from the root's
+// low corner r0.  Rounding: com - r0 is off by up to a few ulps of the
+// root side S, that is 1e-15 S / s cells, and the reference's chain of
+// rounded centres moves the boundaries by less than 1e-13 S.  So the box
+// is taken one cell wider on a side where the centre of mass lies within
+// 1e-6 + 1e-15 S / s cells of a boundary, and widened by 1e-6 s + 1e-13 S.
+// A lane follows about one cell per level.  A fixed root can leave
+// particles outside it (and so outside their cells' boxes): with a fixed
+// root only the MAC prunes.  Lanes that do not open a cell
+// wait at its skip, as in the walk.  A pair is recorded as (min slot, max
+// slot), usually once from each end; one atomic per wave.
+__device__ __forceinline__ double box_gap(double x, double com, double r0, double s,
+                                          double tol, double margin) {
+  const double u = (com - r0) / s;
+  const double lo = r0 + floor(u - tol) * s - margin;
+  const double hi = r0 + (floor(u + tol) + 1.0) * s + margin;
+  return x < lo ? lo - x : (x > hi ? x - hi : 0.0);
+}
+
+__global__ void __launch_bounds__(256) detect_kernel(DetectArgs a) {
+  if (*a.flags) return;
+  const int lane = lane_id();
+  const int hw = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * 256 + threadIdx.x) >> 6));
+  if (hw >= a.n_waves) return;
+  const int s = a.nsys > 1 ? __builtin_amdgcn_readfirstlane(sys_of_wave(hw, a.wave_start, a.nsys))
+                           : 0;
+  const int pbeg = a.sys_start[s] + (hw - a.wave_start[s]) * kWave;
+  const int i = pbeg + lane;
+  const bool valid = i < a.sys_start[s + 1];
+  const double x = valid ? a.x[i] : 0.0, y = valid ? a.y[i] : 0.0, z = valid ? a.z[i] : 0.0;
+  unsigned long long A = ballot(valid);
+  ConstNodes nodes = (ConstNodes)a.nodes;
+  int n = a.node_start[s];
+  const int nend = a.node_start[s + 1];
+  const double theta = a.theta, theta2 = theta * theta, range = a.range;
+  const double rside = a.root[4 * s + 3];
+  const double rx = a.root[4 * s] - 0.5 * rside, ry = a.root[4 * s + 1] - 0.5 * rside,
+               rz = a.root[4 * s + 2] - 0.5 * rside;
+  int st_pos = 0, st_lo = 0, st_hi = 0, top = 0, top_pos = -1;
+  bool overflow = false;
+  while (n < nend) {
+    const NodeRegs cur = load_node(nodes, n);
+    const double dx = cur.cx - x, dy = cur.cy - y, dz = cur.cz - z;
+    const double d2 = sq_sum_ref(dx, dy, dz);
+    const bool in = __builtin_amdgcn_inverse_ballot_w64(A);
+    int next;
+    if (cur.leaf >= 0) {
+      const bool hit = in && cur.leaf != i && __dsqrt_rn(d2) < range;
+      const unsigned long long h = ballot(hit);
+      if (h) {
+        const int first = __builtin_ctzll(h);
+        int base = 0;
+        if (lane == first) base = atomicAdd(a.pair_count, __popcll(h));
+        base = __builtin_amdgcn_readlane(base, first);
+        const int slot = base + mbcnt(h);
+        if (hit && slot < a.pair_cap)
+          a.pairs[slot] = make_int2(min(i, cur.leaf), max(i, cur.leaf));
+      }
+      next = n + 1;
+    } else {
+      const double tol = 1e-6 + 1e-15 * (rside / cur.side);
+      const double margin = 1e-6 * cur.side + 1e-13 * rside;
+      const double gx = box_gap(x, cur.cx, rx, cur.side, tol, margin);
+      const double gy = box_gap(y, cur.cy, ry, cur.side, tol, margin);
+      const double gz = box_gap(z, cur.cz, rz, cur.side, tol, margin);
+      const bool near =
+          !a.boxes || gx * gx + gy * gy + gz * gz < range * range * (1.0 + 1e-9);
+      const bool open =
+          (cur.leaf == -2 || mac_open_f64(d2, cur.side, theta, theta2)) && near && in;
+      const unsigned long long O = ballot(open);
+      if (O) {
+        const unsigned long long C = A & ~O;
+        if (C) {
+          if (top > 0 && top_pos == cur.skip) {
+            const int slot = top - 1;
+            st_lo = lane == slot ? (st_lo | (int)(unsigned)C) : st_lo;
+            st_hi = lane == slot ? (st_hi | (int)(unsigned)(C >> 32)) : st_hi;
+          } else if (top < kWave - 1) {
+            const int slot = top;
+            st_pos = lane == slot ? cur.skip : st_pos;
+            st_lo = lane == slot ? (int)(unsigned)C : st_lo;
+            st_hi = lane == slot ? (int)(unsigned)(C >> 32) : st_hi;
+            top_pos = cur.skip;
+            top++;
+          } else {
+            overflow = true;
+          }
+        }
+        A = O;
+        next = n + 1;
+      } else {
+        next = cur.skip;
+      }
+    }
+    n = max(next, n + 1);
+    if (top > 0 && top_pos == n) {
+      const unsigned lo = (unsigned)__builtin_amdgcn_readlane(st_lo, top - 1);
+      const unsigned hi = (unsigned)__builtin_amdgcn_readlane(st_hi, top - 1);
+      A |= ((unsigned long long)hi << 32) | lo;
+      top--;
+      top_pos = top > 0 ? __builtin_amdgcn_readlane(st_pos, top - 1) : -1;
+    }
+  }
+  if (overflow && lane == 0) atomicOr(a.flags_out, kFlagStack);
 }
 
 // ------------------------------------------------------- wave ordering --
@@ -1159,6 +1250,10 @@ void launch_tile_order(unsigned* tile_cost, int n_tiles, int groups, int* order,
   if (n_tiles <= 0) return;
   const int G = groups <= 1 ? 1 : groups;
   tile_order_kernel<<<G, kOrderThreads, 0, s>>>(tile_cost, n_tiles, G, order);
+}
+
+void launch_detect(const DetectArgs& a, hipStream_t s) {
+  if (a.n_waves > 0) detect_kernel<<<(a.n_waves + 3) / 4, 256, 0, s>>>(a);
 }
 
 int walk_tiles(int n_waves, int n_waves2, int block, int pack) {

@@ -129,3 +129,20 @@ def test_collisions_off_and_rejections():
     with nbodysim_amd.Simulation(theta=0.5, dt=0.0333) as s:
         with pytest.raises(NbsError):
             s.set_collisions(-1.0)
+
+
+def test_fixed_root_merges_the_same_groups():
+    # a fixed root prunes the pair search by the MAC alone; the planted
+    # groups are far from every other particle, so they merge as with the
+    # bbox root of the oracle
+    pos, vel, mass = planted(rng_range=R)
+    with nbodysim_amd.Simulation(theta=0.5, dt=0.0333, root_center=(0.0, 0.0, 0.0),
+                                 root_size=512.0) as s:
+        s.set_state(pos, vel, mass)
+        s.set_collisions(R)
+        s.step(1)
+        m = s.masses()
+        _, nrem = s.collision_stats()
+    _, _, _, om, _, onrem = O.run_collide(pos, vel, mass, frames=1, range=R)
+    assert nrem == onrem == 10
+    np.testing.assert_array_equal(m, om)

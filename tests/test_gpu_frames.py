@@ -148,3 +148,31 @@ def test_snapshot_of_an_empty_state():
         p, v, a = s.snapshot_end()
         assert p.shape == v.shape == a.shape == (0, 3)
         s.step(1)  # nothing to do, not an error
+
+
+def test_frame_loop_with_collisions(tmp_path, monkeypatch):
+    """collide true in the settings: each frame file holds the particles left
+    after that frame's merges, with their merged masses, as the library's
+    collision steps give them."""
+    monkeypatch.chdir(tmp_path)
+    text = SETTINGS.format(prec="f64", keep="true").replace("size 512", "size 4")
+    text += "collide true\ncollision_range 0.05\n"
+    with open("settings.cfg", "w") as f:
+        f.write(text)
+    assert host.run_frames("settings.cfg") == 0
+    settings = host.read_settings("settings.cfg")[0]
+    pos, vel, mass = host.generate(3000, gen_type="cube", seed=0xDEADBEEF, settings=settings)
+    with sim(theta=0.5, dt=0.0333, precision="f64") as s:
+        s.set_state(pos, vel, mass)
+        s.set_collisions(0.05)
+        counts = []
+        for frame in range(4):
+            s.step(1)
+            want = s.state()
+            got = host.read_binary(host.frame_filename("./data", frame, True))
+            counts.append(len(got[3]))
+            for g, w in zip(got[:3], want):
+                np.testing.assert_array_equal(g, w)
+            np.testing.assert_array_equal(got[3], s.masses())
+    print("particles per frame:", counts, flush=True)
+    assert counts[0] < 3000 and counts == sorted(counts, reverse=True)
