@@ -514,6 +514,52 @@ def config_frames(args):
         "frame_bytes": 80 * n}), flush=True)
 
 
+def config_collide(args):
+    """Collision merging (config.collide, src/main.cpp:1442-1483) at
+    N = 1,048,576 (sphere, theta 0.5, range 0.01): eager steps with and
+    without merging, FP32 and FP64 forces, per-phase times from HIP events
+    (the pair pass, merge, update and compaction count as the update
+    phase).  The sphere's close pairs merge during the first steps."""
+    import nbodysim_amd
+    from nbodysim_amd import host
+    n, steps = args.n, args.collide_steps
+    pos, vel, mass = host.generate(n, gen_type="sphere", seed=0xDEADBEEF, **SPHERE)
+    res = {}
+    for prec in ("f32", "f64"):
+        for coll in (False, True):
+            with nbodysim_amd.Simulation(theta=0.5, dt=0.0333, precision=prec) as sim:
+                sim.set_state(pos, vel, mass)
+                sim.set_walk_shape(64, 8, 1)
+                if coll:
+                    sim.set_collisions(args.collide_range)
+                sim.step(2)
+                sim.synchronize()
+                sim.profile(True)
+                sim.profile_read(reset=True)
+                t0 = time.perf_counter()
+                sim.step(steps)
+                sim.synchronize()
+                ms = (time.perf_counter() - t0) * 1e3 / steps
+                prof = sim.profile_read(reset=True)
+                key = "%s_%s" % (prec, "collide" if coll else "plain")
+                res[key] = {"ms_per_step": ms, "n_end": sim.n,
+                            "build_ms": prof["build_ms"] / steps,
+                            "walk_ms": prof["walk_ms"] / steps,
+                            "update_ms": prof["update_ms"] / steps}
+                if coll:
+                    pairs, removed = sim.collision_stats()
+                    res[key].update(pairs=pairs, absorbed=removed)
+                log("collide", key, res[key])
+    print(json.dumps({
+        "metric": "ms per step with collision merging", "unit": "ms",
+        "higher_is_better": False, "n_gpus": 1, "dtype": "f32/f64",
+        "data": "synthetic (seeded gen_type sphere, the reference's generator)",
+        "config": {"workload": "gen_type sphere N=%d r=100 rotation 0.1 about z, theta 0.5, "
+                               "dt 0.0333, collision_range %g, eager steps, %d timed after 2"
+                               % (n, args.collide_range, steps)},
+        "results": res}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -543,6 +589,10 @@ def main():
     ap.add_argument("--frames-study", action="store_true",
                     help="frame output variants (download, snapshot, writer thread)")
     ap.add_argument("--frames", type=int, default=20)
+    ap.add_argument("--collide-study", action="store_true",
+                    help="steps with collision merging against plain steps")
+    ap.add_argument("--collide-steps", type=int, default=10)
+    ap.add_argument("--collide-range", type=float, default=0.01)
     ap.add_argument("--n3", type=int, default=1 << 22)
     ap.add_argument("--steps3", type=int, default=50)
     ap.add_argument("--systems", type=int, default=256)
@@ -564,6 +614,9 @@ def main():
         return
     if args.frames_study:
         config_frames(args)
+        return
+    if args.collide_study:
+        config_collide(args)
         return
 
     import nbodysim_amd

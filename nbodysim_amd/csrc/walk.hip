@@ -932,19 +932,19 @@ walk_kernel(WalkArgs a) {
 // the cell's box comes within range of the lane's particle.  The box is
 // the octree cell of side s that holds the centre of mass: per axis
 // [r0 + k s, r0 + (k + 1) s] with k = floor((com - r0) / s) from the root's
-// low corner r0.  Rounding: com - r0 is off by up to a few ulps of the
-// root side S, that is 1e-15 S / s cells, and the reference's chain of
-// rounded centres moves the boundaries by less than 1e-13 S.  So the box
-// is taken one cell wider on a side where the centre of mass lies within
-// 1e-6 + 1e-15 S / s cells of a boundary, and widened by 1e-6 s + 1e-13 S.
-// A lane follows about one cell per level.  A fixed root can leave
-// particles outside it (and so outside their cells' boxes): with a fixed
-// root only the MAC prunes.  Lanes that do not open a cell
-// wait at its skip, as in the walk.  A pair is recorded as (min slot, max
-// slot), usually once from each end; one atomic per wave.
+// low corner r0.  Rounding: com - r0 and its product with 1 / s are off by
+// a few ulps of the root side S, under 1e-15 S / s cells, and the
+// reference's chain of rounded centres moves the boundaries by less than
+// 1e-13 S.  So the box is taken one cell wider on a side where the centre
+// of mass lies within 1e-6 + 1e-15 S / s cells of a boundary, and widened
+// by 1e-6 s + 1e-13 S.  A lane follows about one cell per level.  A fixed
+// root can leave particles outside it (and so outside their cells'
+// boxes): with a fixed root only the MAC prunes.  Lanes that do not open a
+// cell wait at its skip, as in the walk.  A pair is recorded as (min slot,
+// max slot), usually once from each end; one atomic per wave.
 __device__ __forceinline__ double box_gap(double x, double com, double r0, double s,
-                                          double tol, double margin) {
-  const double u = (com - r0) / s;
+                                          double inv_s, double tol, double margin) {
+  const double u = (com - r0) * inv_s;
   const double lo = r0 + floor(u - tol) * s - margin;
   const double hi = r0 + (floor(u + tol) + 1.0) * s + margin;
   return x < lo ? lo - x : (x > hi ? x - hi : 0.0);
@@ -978,7 +978,9 @@ __global__ void __launch_bounds__(256) detect_kernel(DetectArgs a) {
     const bool in = __builtin_amdgcn_inverse_ballot_w64(A);
     int next;
     if (cur.leaf >= 0) {
-      const bool hit = in && cur.leaf != i && __dsqrt_rn(d2) < range;
+      // the correctly rounded root only where d^2 is near range^2 or below
+      bool hit = in && cur.leaf != i && d2 < range * range * (1.0 + 1e-9);
+      if (hit) hit = __dsqrt_rn(d2) < range;
       const unsigned long long h = ballot(hit);
       if (h) {
         const int first = __builtin_ctzll(h);
@@ -991,11 +993,12 @@ __global__ void __launch_bounds__(256) detect_kernel(DetectArgs a) {
       }
       next = n + 1;
     } else {
-      const double tol = 1e-6 + 1e-15 * (rside / cur.side);
+      const double inv_s = 1.0 / cur.side;
+      const double tol = 1e-6 + 1e-15 * (rside * inv_s);
       const double margin = 1e-6 * cur.side + 1e-13 * rside;
-      const double gx = box_gap(x, cur.cx, rx, cur.side, tol, margin);
-      const double gy = box_gap(y, cur.cy, ry, cur.side, tol, margin);
-      const double gz = box_gap(z, cur.cz, rz, cur.side, tol, margin);
+      const double gx = box_gap(x, cur.cx, rx, cur.side, inv_s, tol, margin);
+      const double gy = box_gap(y, cur.cy, ry, cur.side, inv_s, tol, margin);
+      const double gz = box_gap(z, cur.cz, rz, cur.side, inv_s, tol, margin);
       const bool near =
           !a.boxes || gx * gx + gy * gy + gz * gz < range * range * (1.0 + 1e-9);
       const bool open =
