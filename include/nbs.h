@@ -163,6 +163,19 @@ int nbs_collision_stats(nbs_ctx* ctx, int64_t* pairs, int64_t* removed);
  * offsets (nsys_cap >= nsys + 1). */
 int nbs_particle_count(nbs_ctx* ctx, int64_t* n, int64_t* offsets, int nsys_cap);
 
+/* The frame image of write_image (src/main.cpp:556-678) for the current
+ * positions, rendered on the device: projection 0 front, 1 side, 2 top,
+ * 3 iso; color 0 bw, 1 heat; adaptive / nonlinear are the settings'
+ * adaptive_brightness / nonlinear_brightness.  rgb receives height rows of
+ * width RGB triples, top row first (the PNG order; the reference's pixel
+ * (x, y) is row height - 1 - y).  width and height must be at least 1. */
+typedef struct nbs_image_params {
+  uint32_t width, height, projection, color;
+  int32_t adaptive, nonlinear;
+  double scale, brightness;
+} nbs_image_params;
+int nbs_render_image(nbs_ctx* ctx, const nbs_image_params* p, uint8_t* rgb);
+
 /* Launch-shape tuning of the walk (no counterpart in the reference, whose
  * kernel shape was fixed, src/cuda_helper.cpp:183-188).  The shape is the
  * workgroup size (block: 64, 128, 256, 512, 1024 threads), the number of

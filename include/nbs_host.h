@@ -116,6 +116,11 @@ int nbs_resume_frame(const char* dir, unsigned num_frames);
  * success, 1 on any failure (where the reference exits). */
 int nbs_run_frames(const char* cfgfile);
 
+/* Writes an 8-bit RGB PNG of w x h pixels from rgb ([h][w][3], top row
+ * first, as nbs_render_image returns it): the file write_image saves
+ * through FreeImage (src/main.cpp:672).  Written to path.tmp, then renamed. */
+int nbs_write_png(const char* path, uint32_t w, uint32_t h, const uint8_t* rgb);
+
 #ifdef __cplusplus
 }
 #endif

@@ -214,6 +214,19 @@ class Simulation:
         self._check(self._L.nbs_collision_stats(self._h, ctypes.byref(p), ctypes.byref(r)))
         return p.value, r.value
 
+    def render_image(self, width=1920, height=1080, projection="iso", color="bw",
+                     adaptive=False, nonlinear=False, scale=1.0, brightness=255.0):
+        """The frame image of the current positions (nbs_render_image):
+        [height][width][3] uint8, top row first."""
+        ip = _native.ImageParams(width, height,
+                                 {"front": 0, "side": 1, "top": 2, "iso": 3}[projection],
+                                 {"bw": 0, "heat": 1}[color], int(bool(adaptive)),
+                                 int(bool(nonlinear)), float(scale), float(brightness))
+        out = np.zeros((height, width, 3), np.uint8)
+        self._check(self._L.nbs_render_image(self._h, ctypes.byref(ip),
+                                             out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
+        return out
+
     def offsets(self, nsys):
         """Current system offsets (nsys + 1 entries)."""
         off = np.zeros(nsys + 1, dtype=np.int64)

@@ -30,6 +30,13 @@ class Config(ctypes.Structure):
                 ("node_factor", ctypes.c_int), ("reserved", ctypes.c_int * 6)]
 
 
+class ImageParams(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
+                ("projection", ctypes.c_uint32), ("color", ctypes.c_uint32),
+                ("adaptive", ctypes.c_int32), ("nonlinear", ctypes.c_int32),
+                ("scale", ctypes.c_double), ("brightness", ctypes.c_double)]
+
+
 class TreeInfo(ctypes.Structure):
     _fields_ = [("n_particles", ctypes.c_int64), ("n_nodes", ctypes.c_int64),
                 ("n_internal", ctypes.c_int64), ("max_depth", ctypes.c_int32),
@@ -78,6 +85,8 @@ SIGNATURES = [
     ("nbs_set_collisions", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]),
     ("nbs_collision_stats", ctypes.c_int, [ctypes.c_void_p, c_i64p, c_i64p]),
     ("nbs_particle_count", ctypes.c_int, [ctypes.c_void_p, c_i64p, c_i64p, ctypes.c_int]),
+    ("nbs_render_image", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ImageParams),
+                                        ctypes.POINTER(ctypes.c_uint8)]),
     ("nbs_autotune", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_i32p, c_i32p, c_i32p,
                                     c_dp]),
     ("nbs_set_walk_shape", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
@@ -120,6 +129,8 @@ SIGNATURES = [
                                          c_dp, c_dp, c_dp]),
     ("nbs_resume_frame", ctypes.c_int, [ctypes.c_char_p, ctypes.c_uint]),
     ("nbs_run_frames", ctypes.c_int, [ctypes.c_char_p]),
+    ("nbs_write_png", ctypes.c_int, [ctypes.c_char_p, ctypes.c_uint32, ctypes.c_uint32,
+                                     ctypes.POINTER(ctypes.c_uint8)]),
 ]
 
 _lib = None

@@ -30,12 +30,15 @@ CXX_FLAGS = ["-std=c++17", "-O2", "-ffp-contract=off", "-fPIC", "-Wall",
              "-I" + os.path.join(ROOT, "include")]
 
 HIP_SOURCES = ["scan.hip", "radix_sort.hip", "tree_build.hip", "walk.hip",
-               "integrate.hip", "adjoint.hip", "ic_device.hip", "collide.hip", "api.hip"]
+               "integrate.hip", "adjoint.hip", "ic_device.hip", "collide.hip",
+               "image.hip", "api.hip"]
 # Device initial conditions repeat the host generator's roundings: no
 # contraction into FMAs anywhere in that file (the device default would fuse
 # a multiply with the add that follows it, even across inlined helpers).
-FILE_FLAGS = {"ic_device.hip": ["-ffp-contract=off"], "collide.hip": ["-ffp-contract=off"]}
-HOST_SOURCES = ["host/settings.cpp", "host/ic.cpp", "host/dump.cpp", "host/frames.cpp"]
+FILE_FLAGS = {"ic_device.hip": ["-ffp-contract=off"], "collide.hip": ["-ffp-contract=off"],
+              "image.hip": ["-ffp-contract=off"]}
+HOST_SOURCES = ["host/settings.cpp", "host/ic.cpp", "host/dump.cpp", "host/frames.cpp",
+                "host/png.cpp"]
 CLI_SOURCES = ["host/nbodysim_main.cpp"]
 
 
@@ -91,7 +94,7 @@ def build(lazy=False, jobs=None):
         cli_objs = list(ex.map(lambda s: _compile(s, lazy), CLI_SOURCES))
     if not lazy or _stale(LIBNBS, objs):
         _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC"] + objs +
-             ["-pthread", "-o", LIBNBS + ".tmp"])
+             ["-pthread", "-lz", "-o", LIBNBS + ".tmp"])
         os.replace(LIBNBS + ".tmp", LIBNBS)
     if not lazy or _stale(CLI, cli_objs + [LIBNBS]):
         _run([CXX, "-o", CLI + ".tmp"] + cli_objs +

@@ -249,6 +249,15 @@ struct nbs_ctx {
   int col_pair_want = 0;
   int col_info_cap = 0;
   int64_t col_pairs = 0, col_removed = 0;
+
+  // Frame images (nbs_render_image): per-pixel counts and values, the
+  // running maximum and the RGB rows, for img_pixels pixels.
+  std::vector<void*> img_bufs;
+  size_t img_pixels = 0;
+  unsigned* img_counts = nullptr;
+  double* img_temp = nullptr;
+  unsigned long long* img_max = nullptr;
+  unsigned char* img_rgb = nullptr;
 };
 
 namespace {
@@ -921,6 +930,7 @@ void nbs_destroy(nbs_ctx* c) {
   free_all(c->misc_bufs);
   free_all(c->col_bufs);
   free_all(c->col_pair_bufs);
+  free_all(c->img_bufs);
   free_snapshots(c);
   if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
   if (c->snap_ready) (void)hipEventDestroy(c->snap_ready);
@@ -1209,6 +1219,45 @@ int nbs_collision_stats(nbs_ctx* c, int64_t* pairs, int64_t* removed) {
   if (!c) return NBS_ERR_ARG;
   if (pairs) *pairs = c->col_pairs;
   if (removed) *removed = c->col_removed;
+  return NBS_OK;
+}
+
+int nbs_render_image(nbs_ctx* c, const nbs_image_params* ip, uint8_t* rgb) {
+  if (!c || !ip || !rgb) return NBS_ERR_ARG;
+  if (!c->has_state) return fail(c, NBS_ERR_STATE, "no particle state set");
+  if (ip->width < 1 || ip->height < 1 || ip->projection > 3 || ip->color > 1)
+    return fail(c, NBS_ERR_ARG, "image: width, height >= 1, projection 0-3, color 0-1");
+  const size_t npix = (size_t)ip->width * ip->height;
+  if (npix > ((size_t)1 << 31)) return fail(c, NBS_ERR_ARG, "image too large");
+  HIPCHK(c, hipSetDevice(c->device));
+  if (npix > c->img_pixels) {
+    free_all(c->img_bufs);
+    c->img_pixels = 0;
+    hipError_t e = hipSuccess;
+    c->img_counts = dalloc<unsigned>(c->img_bufs, npix, &e);
+    c->img_temp = dalloc<double>(c->img_bufs, npix, &e);
+    c->img_max = dalloc<unsigned long long>(c->img_bufs, 1, &e);
+    c->img_rgb = dalloc<unsigned char>(c->img_bufs, 3 * npix, &e);
+    if (e != hipSuccess) {
+      free_all(c->img_bufs);
+      return hip_fail(c, e, "hipMalloc(image buffers)");
+    }
+    c->img_pixels = npix;
+  }
+  ImageParams p;
+  p.width = ip->width;
+  p.height = ip->height;
+  p.projection = ip->projection;
+  p.color = ip->color;
+  p.adaptive = ip->adaptive != 0;
+  p.nonlinear = ip->nonlinear != 0;
+  p.scale = ip->scale;
+  p.brightness = ip->brightness;
+  launch_image(c->st[c->cur], c->n, p, c->img_counts, c->img_temp, c->img_max, c->img_rgb,
+               c->stream);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(rgb, c->img_rgb, 3 * npix, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return NBS_OK;
 }
 

@@ -33,6 +33,7 @@ struct Frame {
   unsigned frame = 0;
   size_t n = 0;
   const double *pos = nullptr, *vel = nullptr, *acc = nullptr, *mass = nullptr;
+  const uint8_t* rgb = nullptr;  // the frame image, when dump_image is set
 };
 
 int write_frame(const nbs_settings& s, const Frame& f) {
@@ -53,6 +54,18 @@ int write_frame(const nbs_settings& s, const Frame& f) {
   if (s.dump_text && nbs_write_text(tn, f.n, f.pos) != NBS_OK) {
     fprintf(stderr, "Cannot write %s\n", tn);
     return NBS_ERR_IO;
+  }
+  if (f.rgb) {  // gen_image (src/main.cpp:293-300): ./img/NNNN.png
+    char in[64];
+    snprintf(in, sizeof(in), "./img/%04u.png", f.frame);
+    if (!s.overwrite_data && exists(in)) {
+      fprintf(stderr, "Overwrite disabled, cannot overwrite %s. Aborting.\n", in);
+      return NBS_ERR_IO;
+    }
+    if (nbs_write_png(in, s.img_w, s.img_h, f.rgb) != NBS_OK) {
+      fprintf(stderr, "Cannot save %s. Exiting.\n", in);
+      return NBS_ERR_IO;
+    }
   }
   if (f.frame > 0) {
     nbs_frame_filename("./data", f.frame - 1, 1, pbn, sizeof(pbn));
@@ -76,8 +89,6 @@ extern "C" int nbs_run_frames(const char* cfgfile) {
   if (r != NBS_OK) return 1;
   if (s.min_node_size > 0)
     printf("WARNING: min_node_size is not supported by the GPU step; ignored\n");
-  if (s.dump_image)
-    printf("WARNING: image output is not built in; dump_image ignored\n");
 
   const unsigned long seed = s.use_seed ? s.seed : (unsigned long)time(NULL);
   std::vector<double> pos, vel, acc, mass;
@@ -143,13 +154,18 @@ extern "C" int nbs_run_frames(const char* cfgfile) {
     return 1;
   }
   if (s.dump_binary || s.dump_text) mkdir("./data", 0755);
+  if (s.dump_image) mkdir("./img", 0755);
+  const nbs_image_params image = {s.img_w, s.img_h, s.projection, s.color,
+                                  s.adaptive_brightness, s.nonlinear_brightness, s.scale,
+                                  s.brightness};
+  std::vector<uint8_t> img_buf[2];
   // With collisions the masses change: each frame's are read back into one
   // of two buffers (the writer of the frame before reads the other).
   std::vector<double> mass_buf[2];
 
   unsigned frame = start_frame;
   printf("Frame %u/%u\n", frame, s.num_frames);
-  const bool dumping = s.dump_binary || s.dump_text;
+  const bool dumping = s.dump_binary || s.dump_text || s.dump_image;
   std::thread writer;
   int write_status = NBS_OK;
   int status = 0;
@@ -163,6 +179,7 @@ extern "C" int nbs_run_frames(const char* cfgfile) {
     Frame f;
     f.frame = pending_frame;
     f.mass = s.collide ? mass_buf[pending_frame & 1].data() : mass.data();
+    f.rgb = s.dump_image ? img_buf[pending_frame & 1].data() : nullptr;
     if (nbs_snapshot_end(ctx, &f.pos, &f.vel, &f.acc, &f.n) != NBS_OK) {
       fprintf(stderr, "%s\n", nbs_last_error(ctx));
       return false;
@@ -186,6 +203,15 @@ extern "C" int nbs_run_frames(const char* cfgfile) {
       if (pending && !hand_over()) {
         status = 1;
         break;
+      }
+      if (s.dump_image) {
+        std::vector<uint8_t>& ib = img_buf[frame & 1];
+        ib.resize(3 * (size_t)s.img_w * s.img_h);
+        if (nbs_render_image(ctx, &image, ib.data()) != NBS_OK) {
+          fprintf(stderr, "%s\n", nbs_last_error(ctx));
+          status = 1;
+          break;
+        }
       }
       if (s.collide) {
         int64_t m = 0;

@@ -296,6 +296,19 @@ void launch_collide(CollideBuffers& cb, State& cur, State& nxt, double* ax, doub
                     double* az, const int* inter, int n, Systems& sys, double dt,
                     int* flags, hipStream_t s);
 
+// --------------------------------------------------------------- image --
+// write_image's settings (src/main.cpp:556): projection 0 front, 1 side,
+// 2 top, 3 iso; color 0 bw, 1 heat.
+struct ImageParams {
+  unsigned width, height, projection, color;
+  int adaptive, nonlinear;
+  double scale, brightness;
+};
+// counts [w*h] u32, temp [w*h] doubles, vmax one word; rgb [h][w][3]
+// bytes, PNG row order (top row first).
+void launch_image(const State& st, int n, const ImageParams& p, unsigned* counts, double* temp,
+                  unsigned long long* vmax, unsigned char* rgb, hipStream_t s);
+
 // Small fills (no hipMemsetAsync: these run inside captured graphs too).
 void launch_fill_int(int* p, int v, size_t n, hipStream_t s);
 void launch_fill_u64(unsigned long long* p, unsigned long long v, size_t n,

@@ -108,6 +108,39 @@ def write_binary(path, pos, vel, acc, mass, prec="f64"):
         raise OSError("cannot write %s" % path)
 
 
+def write_png(path, rgb):
+    """8-bit RGB PNG from [height][width][3] uint8, top row first."""
+    rgb = np.ascontiguousarray(rgb, dtype=np.uint8)
+    h, w = rgb.shape[:2]
+    if lib().nbs_write_png(path.encode(), w, h,
+                           rgb.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))):
+        raise OSError("cannot write %s" % path)
+
+
+def read_png(path):
+    """Pixels of an 8-bit RGB, non-interlaced PNG ([h][w][3] uint8)."""
+    import struct
+    import zlib
+    data = open(path, "rb").read()
+    assert data[:8] == b"\x89PNG\r\n\x1a\n", "not a PNG"
+    pos, idat, w, h = 8, b"", 0, 0
+    while pos < len(data):
+        ln, kind = struct.unpack(">I4s", data[pos:pos + 8])
+        body = data[pos + 8:pos + 8 + ln]
+        crc = struct.unpack(">I", data[pos + 8 + ln:pos + 12 + ln])[0]
+        if zlib.crc32(data[pos + 4:pos + 8 + ln]) != crc:
+            raise ValueError("bad CRC in chunk %r" % kind)
+        if kind == b"IHDR":
+            w, h, depth, ctype = struct.unpack(">IIBB", body[:10])
+            assert (depth, ctype) == (8, 2), "only 8-bit RGB"
+        elif kind == b"IDAT":
+            idat += body
+        pos += 12 + ln
+    raw = np.frombuffer(zlib.decompress(idat), np.uint8).reshape(h, 3 * w + 1)
+    assert (raw[:, 0] == 0).all(), "only filter type 0"
+    return raw[:, 1:].reshape(h, w, 3).copy()
+
+
 def write_text(path, pos):
     pos = np.ascontiguousarray(pos, dtype=np.float64)
     if lib().nbs_write_text(path.encode(), len(pos), _dp(pos)):

@@ -337,3 +337,22 @@ def ref_run_collide(pos, vel, mass, theta=0.5, dt=0.0333, frames=1, damping=Fals
     n = L.ref_run_collide(len(pos), _p(pos), _p(vel), _p(acc), _p(mass), theta, dt,
                           int(damping), range, frames, ctypes.byref(npairs))
     return pos[:n], vel[:n], acc[:n], mass[:n], npairs.value
+
+
+def render_image(pos, width=1920, height=1080, projection="iso", color="bw", adaptive=False,
+                 nonlinear=False, scale=1.0, brightness=255.0):
+    """write_image's pixels for FP64 positions: [height][width][3] uint8,
+    top row first."""
+    proj = {"front": 0, "side": 1, "top": 2, "iso": 3}[projection]
+    col = {"bw": 0, "heat": 1}[color]
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    out = np.zeros((height, width, 3), np.uint8)
+    L = lib()
+    L.orc_render_image.argtypes = [ctypes.c_size_t, _dp, ctypes.c_uint, ctypes.c_uint,
+                                   ctypes.c_uint, ctypes.c_uint, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_double, ctypes.c_double,
+                                   ctypes.POINTER(ctypes.c_uint8)]
+    L.orc_render_image(len(pos), _p(pos), width, height, proj, col, int(adaptive),
+                       int(nonlinear), scale, brightness,
+                       out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+    return out

@@ -890,6 +890,84 @@ int64_t orc_run_collide(size_t n, double* pos, double* vel, double* acc, double*
   return (int64_t)n;
 }
 
+// write_image (src/main.cpp:556-678) for FP64 positions, without
+// FreeImage: the same projection, bounds, per-particle accumulation
+// temp[x][y] += brightness with the running maximum, nonlinear root and
+// colour mapping.  rgb: [h][w][3], top row first (FreeImage's scanline 0
+// is the bottom one, so pixel (x, y) lands in row h - 1 - y).
+void orc_render_image(size_t n, const double* pos, unsigned img_w, unsigned img_h,
+                      unsigned projection, unsigned color, int adaptive, int nonlinear,
+                      double scale, double brightness, unsigned char* rgb) {
+  auto clamp = [](double a, double x, double b) {  // src/main.cpp:177-182
+    if (x < a) return a;
+    if (x > b) return b;
+    return x;
+  };
+  std::vector<std::vector<double>> temp(img_w, std::vector<double>(img_h, 0.0));
+  double max = 0;
+  if (projection == 3) scale *= 2.0;
+  for (size_t i = 0; i < n; i++) {
+    const double px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
+    double x, y;
+    if (projection == 0) {
+      x = px + (img_w / 2);
+      y = pz + (img_h / 2);
+    } else if (projection == 1) {
+      x = py + (img_w / 2);
+      y = pz + (img_h / 2);
+    } else if (projection == 2) {
+      x = px + (img_w / 2);
+      y = py + (img_h / 2);
+    } else {
+      x = ((std::sqrt(3) / 2.0) * (px - py) + img_w) / 2.0;
+      y = ((-0.5) * (px + py + 2 * pz) + img_h) / 2.0;
+    }
+    x += (x - (img_w / 2)) * (scale - 1);
+    y += (y - (img_h / 2)) * (scale - 1);
+    if (x < 0 || y < 0 || x > img_w - 1 || y > img_w - 1) continue;
+    if (x != x || y != y) continue;  // (int) of NaN: undefined in the reference
+    x = clamp(0, x, img_w - 1);
+    y = clamp(0, y, img_h - 1);
+    if (projection == 3) y = (img_h - 1) - y;
+    temp[(int)x][(int)y] += brightness;
+    if (temp[(int)x][(int)y] > max) max = temp[(int)x][(int)y];
+  }
+  if (nonlinear) {
+    max = std::sqrt(max);
+    for (unsigned i = 0; i < img_w; i++)
+      for (unsigned j = 0; j < img_h; j++) temp[i][j] = std::sqrt(temp[i][j]);
+  }
+  for (unsigned xi = 0; xi < img_w; xi++) {
+    for (unsigned yi = 0; yi < img_h; yi++) {
+      unsigned char r = 0, g = 0, b = 0;
+      double v = color == 0 ? (adaptive ? clamp(0, temp[xi][yi] * 255.0 / max, 255)
+                                        : clamp(0, temp[xi][yi], 255))
+                            : (adaptive ? clamp(0, temp[xi][yi] * 511.0 / max, 511)
+                                        : clamp(0, temp[xi][yi], 511));
+      // (int) NaN is INT_MIN on x86; the reference's byte stores and
+      // heat-map clamps make that black
+      if (v != v) {
+      } else if (color == 0) {
+        r = g = b = (unsigned char)(int)v;
+      } else {
+        int iv = (int)v;
+        if (iv < 256) {
+          b = (unsigned char)clamp(0, 255 - iv, 255);
+          g = (unsigned char)clamp(0, iv, 255);
+        } else {
+          iv -= 256;
+          g = (unsigned char)clamp(0, 255 - iv, 255);
+          r = (unsigned char)clamp(0, iv, 255);
+        }
+      }
+      unsigned char* o = rgb + ((size_t)(img_h - 1 - yi) * img_w + xi) * 3;
+      o[0] = r;
+      o[1] = g;
+      o[2] = b;
+    }
+  }
+}
+
 // FP64 direct sum (the accuracy reference), long-double accumulation,
 // for the particles listed in idx.  Skips j == i.  damping follows the
 // reference's damped law.
