@@ -157,6 +157,16 @@ int nbs_step_graph(nbs_ctx* ctx, int nsteps);
  * Eager steps only: nbs_step_graph returns NBS_ERR_STATE while enabled;
  * adaptive dt must be off.  nbs_forces never merges. */
 int nbs_set_collisions(nbs_ctx* ctx, int enabled, double range);
+/* apply_minimum_size (src/main.cpp:1001-1057; the reference runs it when
+ * min_node_size > 0 and verbose is set, :1425): while min_size > 0, each
+ * nbs_step step builds the tree, merges the particles of every largest
+ * cell of side < min_size into one -- collide() folded along the tree, in
+ * child order 0..7 at each level -- and walks the tree of the merged
+ * particles on the same root.  The merged particle takes its lowest-index
+ * member's place; the count shrinks.  Eager steps only, adaptive dt off.
+ * merged (may be NULL) receives the particles absorbed so far (before this
+ * call; the count restarts when the size changes). */
+int nbs_set_min_node_size(nbs_ctx* ctx, double min_size, int64_t* merged);
 /* Pairs detected and particles absorbed since collisions were enabled. */
 int nbs_collision_stats(nbs_ctx* ctx, int64_t* pairs, int64_t* removed);
 /* Current particle count and, if offsets is not NULL, the nsys + 1 system

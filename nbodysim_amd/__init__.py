@@ -208,6 +208,14 @@ class Simulation:
         self._check(self._L.nbs_set_collisions(self._h, int(bool(enabled)), float(range)))
         self._collide = bool(enabled)
 
+    def set_min_node_size(self, min_size):
+        """apply_minimum_size in step() (nbs_set_min_node_size); 0 turns it
+        off.  Returns the particles it absorbed before this call."""
+        m = ctypes.c_int64()
+        self._check(self._L.nbs_set_min_node_size(self._h, float(min_size), ctypes.byref(m)))
+        self._collide = self._collide or min_size > 0
+        return m.value
+
     def collision_stats(self):
         """(pairs detected, particles absorbed) since set_collisions."""
         p, r = ctypes.c_int64(), ctypes.c_int64()

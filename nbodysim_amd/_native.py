@@ -83,6 +83,7 @@ SIGNATURES = [
     ("nbs_step", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("nbs_step_graph", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("nbs_set_collisions", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]),
+    ("nbs_set_min_node_size", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, c_i64p]),
     ("nbs_collision_stats", ctypes.c_int, [ctypes.c_void_p, c_i64p, c_i64p]),
     ("nbs_particle_count", ctypes.c_int, [ctypes.c_void_p, c_i64p, c_i64p, ctypes.c_int]),
     ("nbs_render_image", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ImageParams),

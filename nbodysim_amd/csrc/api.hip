@@ -243,6 +243,8 @@ struct nbs_ctx {
   // absorbed particles.
   bool collide = false;
   double collide_range = 0;
+  double min_size = 0;  // nbs_set_min_node_size (0: off)
+  int64_t min_removed = 0;
   CollideBuffers cb{};
   std::vector<void*> col_bufs, col_pair_bufs;
   int col_cap_n = 0;
@@ -461,17 +463,19 @@ int ensure_nodes(nbs_ctx* c) {
 }
 
 // Enqueues one tree build of the current state.
-void enqueue_build(nbs_ctx* c) {
+void enqueue_build(nbs_ctx* c, bool keep_root = false) {
   hipStream_t s = c->stream;
   hipEvent_t e0 = nullptr;
   prof_begin(c, kPhaseBuild, &e0);
   State& cur = c->st[c->cur];
   State& nxt = c->st[1 - c->cur];
-  if (!c->bbox_valid) {
-    launch_bbox(cur, c->sys, s);
-    c->bbox_valid = true;  // positions do not change during a build
+  if (!keep_root) {  // keep_root: the roots of the last build (a rebuild)
+    if (!c->bbox_valid) {
+      launch_bbox(cur, c->sys, s);
+      c->bbox_valid = true;  // positions do not change during a build
+    }
+    launch_root(c->sys, c->p, c->flags, s);
   }
-  launch_root(c->sys, c->p, c->flags, s);
   const int nbits = launch_keys(cur, c->sys, c->bb, c->prim_levels, s);
   const int which = radix_sort_pairs(c->bb.prim0, c->bb.val0, c->bb.prim1,
                                      c->bb.val1, c->n, nbits, c->bb.sort_tmp, s);
@@ -683,7 +687,7 @@ int shrink_systems(nbs_ctx* c, const std::vector<int>& info) {
 int run_collide(nbs_ctx* c, int nsteps) {
   if (!c->has_state) return fail(c, NBS_ERR_STATE, "no particle state set");
   if (c->p.adaptive)
-    return fail(c, NBS_ERR_STATE, "collision merging needs adaptive dt off");
+    return fail(c, NBS_ERR_STATE, "merging particles needs adaptive dt off");
   for (int k = 0; k < nsteps && c->n > 0; k++) {
     for (int attempt = 0;; attempt++) {
       if (attempt == 8) return fail(c, NBS_ERR_OVERFLOW, "collision step overflow after retries");
@@ -691,10 +695,51 @@ int run_collide(nbs_ctx* c, int nsteps) {
       if (r) return r;
       r = ensure_collide(c);
       if (r) return r;
-      const int n = c->n, nsys = c->sys.nsys;
       reset_flags(c);
       launch_fill_int(c->cb.count, 0, 2, c->stream);
       enqueue_build(c);
+      if (c->min_size > 0) {
+        // apply_minimum_size on this build, then the tree of the merged
+        // particles on the same root (src/main.cpp:1421-1434)
+        const int n0 = c->n;
+        launch_min_size(c->cb, c->nodes, c->node_cap, c->min_size, c->st[c->cur],
+                        c->st[1 - c->cur], c->ax, c->ay, c->az, c->inter, n0, c->sys, c->flags,
+                        c->stream);
+        int flags = 0, done = 0;
+        r = read_flags(c, &flags, &done);
+        if (r) return r;
+        c->bbox_valid = false;
+        c->tree_valid = false;
+        if (flags) {
+          if (flags & kFlagRunOverflow)
+            return fail(c, NBS_ERR_OVERFLOW, "more than 8192 particles share one 21-level cell");
+          r = alloc_nodes(c, c->node_cap * 2);
+          if (r) return r;
+          continue;
+        }
+        std::vector<int> info(c->sys.nsys + 2, 0);
+        HIPCHK(c, hipMemcpy(info.data(), c->cb.info, info.size() * sizeof(int),
+                            hipMemcpyDeviceToHost));
+        c->cur ^= 1;
+        c->min_removed += n0 - info[0];
+        if (info[0] != n0) {
+          drop_graph(c);
+          r = shrink_systems(c, info);
+          if (r) return r;
+        }
+        enqueue_build(c, true);
+        if (!c->collide) {
+          enqueue_walk(c, 1);
+          r = read_flags(c, &flags, &done);
+          if (r) return r;
+          if (flags)
+            return fail(c, NBS_ERR_OVERFLOW, "tree overflow after the minimum-size merge");
+          c->bbox_valid = true;
+          c->acc_valid = true;
+          break;
+        }
+      }
+      const int n = c->n, nsys = c->sys.nsys;
       enqueue_walk(c, 0);
       hipEvent_t e0 = nullptr;
       prof_begin(c, kPhaseUpdate, &e0);
@@ -1215,6 +1260,16 @@ int nbs_set_collisions(nbs_ctx* c, int enabled, double range) {
   return NBS_OK;
 }
 
+int nbs_set_min_node_size(nbs_ctx* c, double min_size, int64_t* merged) {
+  if (!c) return NBS_ERR_ARG;
+  if (merged) *merged = c->min_removed;
+  if (!(min_size >= 0)) return fail(c, NBS_ERR_ARG, "min_node_size must be >= 0");
+  if (min_size != c->min_size) c->min_removed = 0;
+  c->min_size = min_size;
+  drop_graph(c);
+  return NBS_OK;
+}
+
 int nbs_collision_stats(nbs_ctx* c, int64_t* pairs, int64_t* removed) {
   if (!c) return NBS_ERR_ARG;
   if (pairs) *pairs = c->col_pairs;
@@ -1287,7 +1342,7 @@ int nbs_step(nbs_ctx* c, int nsteps) {
   if (nsteps < 0) return fail(c, NBS_ERR_ARG, "nsteps must be >= 0");
   if (nsteps == 0) return NBS_OK;
   HIPCHK(c, hipSetDevice(c->device));
-  if (c->collide) return run_collide(c, nsteps);
+  if (c->collide || c->min_size > 0) return run_collide(c, nsteps);
   return run(c, nsteps);
 }
 
@@ -1295,8 +1350,8 @@ int nbs_step_graph(nbs_ctx* c, int nsteps) {
   if (!c) return NBS_ERR_ARG;
   if (nsteps < 0) return fail(c, NBS_ERR_ARG, "nsteps must be >= 0");
   if (!c->has_state) return fail(c, NBS_ERR_STATE, "no particle state set");
-  if (c->collide)
-    return fail(c, NBS_ERR_STATE, "collision merging runs eager steps (nbs_step)");
+  if (c->collide || c->min_size > 0)
+    return fail(c, NBS_ERR_STATE, "merging particles runs eager steps (nbs_step)");
   if (nsteps == 0 || c->n == 0) return NBS_OK;
   HIPCHK(c, hipSetDevice(c->device));
   int r = ensure_nodes(c);

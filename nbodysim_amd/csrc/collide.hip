@@ -197,15 +197,121 @@ __global__ void compact_kernel(State src, State dst, const int* __restrict__ ali
 // nsys); the system starts and simulated times are updated in place.
 __global__ void finish_kernel(const int* __restrict__ newpos, int* __restrict__ start, int nsys,
                               const double* __restrict__ sys_dt, double* __restrict__ sys_time,
-                              int* __restrict__ info, const int* __restrict__ flags) {
+                              int advance, int* __restrict__ info, const int* __restrict__ flags) {
   if (*flags) return;
   for (int k = threadIdx.x; k <= nsys; k += blockDim.x) {
     const int ns = newpos[start[k]];
     start[k] = ns;
     info[1 + k] = ns;
     if (k == nsys) info[0] = ns;
-    if (k < nsys) sys_time[k] += sys_dt[k];
+    if (k < nsys && advance) sys_time[k] += sys_dt[k];
   }
+}
+
+// ------------------------------------------------------- minimum size --
+// apply_minimum_size (src/main.cpp:1001-1057): after the build, a cell of
+// side < min_size takes the particles of its leaf children -- merged by
+// collide() in child order 0..7 when there are several -- and, post-order,
+// that repeats up to the largest cells below min_size.  So every maximal
+// cell of side < min_size ends as one particle: the left fold of collide
+// over its children's results, each child's result the same fold over its
+// own children (a leaf's result is its particle).  One thread per such
+// cell (an internal node with side < min_size whose parent, of side 2
+// side, is not, or a system root) folds its pre-order subtree with a stack
+// of partial results, one entry per open cell.  The merged particle takes
+// the slot of the member with the lowest caller id; the others are marked
+// absorbed.  A depth-42 bucket's particles count as its children in slot
+// order (the reference would keep splitting).
+struct Part {
+  double m, x, y, z, vx, vy, vz;
+};
+
+__device__ __forceinline__ void collide_into(Part& a, const Part& b) {
+#pragma clang fp contract(off)
+  const double w = a.m + b.m;  // src/main.cpp:423-430
+  a.x = (a.x * a.m + b.x * b.m) / w;
+  a.y = (a.y * a.m + b.y * b.m) / w;
+  a.z = (a.z * a.m + b.z * b.m) / w;
+  a.vx = (a.vx * a.m + b.vx * b.m) / w;
+  a.vy = (a.vy * a.m + b.vy * b.m) / w;
+  a.vz = (a.vz * a.m + b.vz * b.m) / w;
+  a.m = w;
+}
+
+constexpr int kMinSizeStack = 48;  // open cells below a group root (depth <= 42)
+
+__global__ void min_size_kernel(const WalkNode* __restrict__ nodes,
+                                const int* __restrict__ node_start, int nsys, double min_size,
+                                State st, int* __restrict__ alive, int* __restrict__ flags) {
+  if (*flags) return;
+  const int total = node_start[nsys];
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= total) return;
+  const WalkNode nd = nodes[g];
+  if (nd.leaf >= 0 || !(nd.side < min_size)) return;
+  if (2.0 * nd.side < min_size) {
+    // the parent is below min_size too: only a system root (no parent)
+    // starts a group here
+    int lo = 0, hi = nsys;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (node_start[mid] < g) lo = mid + 1; else hi = mid;
+    }
+    if (lo >= nsys || node_start[lo] != g) return;
+  }
+  Part acc[kMinSizeStack];
+  int end[kMinSizeStack];
+  bool has[kMinSizeStack];
+  int top = 0;
+  end[0] = nd.skip;
+  has[0] = false;
+  unsigned best_id = 0xffffffffu;
+  int best = -1;
+  for (int k = g + 1; k < nd.skip; k++) {
+    while (top > 0 && end[top] <= k) {  // close finished cells into their parents
+      if (has[top]) {
+        if (has[top - 1]) collide_into(acc[top - 1], acc[top]); else acc[top - 1] = acc[top];
+        has[top - 1] = true;
+      }
+      top--;
+    }
+    const WalkNode c = nodes[k];
+    if (c.leaf >= 0) {
+      const int sl = c.leaf;
+      const Part q = {st.m[sl], st.x[sl], st.y[sl], st.z[sl], st.vx[sl], st.vy[sl], st.vz[sl]};
+      if (has[top]) collide_into(acc[top], q); else acc[top] = q;
+      has[top] = true;
+      alive[sl] = 0;
+      if (st.id[sl] < best_id) {
+        best_id = st.id[sl];
+        best = sl;
+      }
+    } else if (top + 1 < kMinSizeStack) {
+      top++;
+      end[top] = c.skip;
+      has[top] = false;
+    } else {
+      atomicOr(flags, kFlagStack);
+      return;
+    }
+  }
+  while (top > 0) {
+    if (has[top]) {
+      if (has[top - 1]) collide_into(acc[top - 1], acc[top]); else acc[top - 1] = acc[top];
+      has[top - 1] = true;
+    }
+    top--;
+  }
+  if (best < 0) return;
+  const Part r = acc[0];
+  st.m[best] = r.m;
+  st.x[best] = r.x;
+  st.y[best] = r.y;
+  st.z[best] = r.z;
+  st.vx[best] = r.vx;
+  st.vy[best] = r.vy;
+  st.vz[best] = r.vz;
+  alive[best] = 1;
 }
 
 }  // namespace
@@ -230,14 +336,30 @@ void launch_collide(CollideBuffers& cb, State& cur, State& nxt, double* ax, doub
   fold_kernel<<<(L + kBlock - 1) / kBlock, kBlock, 0, s>>>(key, val, cb.count, cur, ax, ay, az,
                                                           cb.alive, flags);
   launch_update(cur, ax, ay, az, n, sys, flags, s);
+  launch_compact(cb, cur, nxt, ax, ay, az, inter, n, sys, 1, flags, s);
+}
+
+void launch_compact(CollideBuffers& cb, State& cur, State& nxt, const double* ax,
+                    const double* ay, const double* az, const int* inter, int n, Systems& sys,
+                    int advance_time, int* flags, hipStream_t s) {
   exclusive_scan(cb.alive, cb.newpos, n, cb.scan_tmp, s);
   id_alive_kernel<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(cb.alive, cur.id, n, cb.id_alive);
   exclusive_scan(cb.id_alive, cb.newid, n, cb.scan_tmp, s);
   compact_kernel<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(
       cur, nxt, cb.alive, cb.newpos, cb.newid, ax, ay, az, inter, cb.cax, cb.cay, cb.caz,
       cb.cinter, n, flags);
-  finish_kernel<<<1, kBlock, 0, s>>>(cb.newpos, sys.start, sys.nsys, sys.dt, sys.time, cb.info,
-                                     flags);
+  finish_kernel<<<1, kBlock, 0, s>>>(cb.newpos, sys.start, sys.nsys, sys.dt, sys.time,
+                                     advance_time, cb.info, flags);
+}
+
+void launch_min_size(CollideBuffers& cb, const WalkNode* nodes, int node_cap, double min_size,
+                     State& cur, State& nxt, const double* ax, const double* ay,
+                     const double* az, const int* inter, int n, Systems& sys, int* flags,
+                     hipStream_t s) {
+  launch_fill_int(cb.alive, 1, n, s);
+  min_size_kernel<<<(node_cap + kBlock - 1) / kBlock, kBlock, 0, s>>>(
+      nodes, sys.node_start, sys.nsys, min_size, cur, cb.alive, flags);
+  launch_compact(cb, cur, nxt, ax, ay, az, inter, n, sys, 0, flags, s);
 }
 
 }  // namespace nbs

@@ -87,8 +87,6 @@ extern "C" int nbs_run_frames(const char* cfgfile) {
   int r = nbs_settings_read(&s, cfgfile, msg, sizeof(msg));
   if (msg[0]) fputs(msg, stderr);
   if (r != NBS_OK) return 1;
-  if (s.min_node_size > 0)
-    printf("WARNING: min_node_size is not supported by the GPU step; ignored\n");
 
   const unsigned long seed = s.use_seed ? s.seed : (unsigned long)time(NULL);
   std::vector<double> pos, vel, acc, mass;
@@ -148,6 +146,14 @@ extern "C" int nbs_run_frames(const char* cfgfile) {
     nbs_destroy(ctx);
     return 1;
   }
+  // the reference applies min_node_size only when verbose is set too
+  // (src/main.cpp:1425)
+  if (s.min_node_size > 0 && s.verbose &&
+      nbs_set_min_node_size(ctx, s.min_node_size, nullptr) != NBS_OK) {
+    fprintf(stderr, "%s\n", nbs_last_error(ctx));
+    nbs_destroy(ctx);
+    return 1;
+  }
   if (s.collide && nbs_set_collisions(ctx, 1, s.collision_range) != NBS_OK) {
     fprintf(stderr, "%s\n", nbs_last_error(ctx));
     nbs_destroy(ctx);
@@ -178,7 +184,8 @@ extern "C" int nbs_run_frames(const char* cfgfile) {
   auto hand_over = [&]() -> bool {
     Frame f;
     f.frame = pending_frame;
-    f.mass = s.collide ? mass_buf[pending_frame & 1].data() : mass.data();
+    f.mass = s.collide || (s.min_node_size > 0 && s.verbose) ? mass_buf[pending_frame & 1].data()
+                                                             : mass.data();
     f.rgb = s.dump_image ? img_buf[pending_frame & 1].data() : nullptr;
     if (nbs_snapshot_end(ctx, &f.pos, &f.vel, &f.acc, &f.n) != NBS_OK) {
       fprintf(stderr, "%s\n", nbs_last_error(ctx));
@@ -213,7 +220,7 @@ extern "C" int nbs_run_frames(const char* cfgfile) {
           break;
         }
       }
-      if (s.collide) {
+      if (s.collide || (s.min_node_size > 0 && s.verbose)) {
         int64_t m = 0;
         std::vector<double>& mb = mass_buf[frame & 1];
         if (nbs_particle_count(ctx, &m, nullptr, 0) == NBS_OK) mb.resize((size_t)m);

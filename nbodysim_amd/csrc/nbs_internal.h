@@ -309,6 +309,19 @@ struct ImageParams {
 void launch_image(const State& st, int n, const ImageParams& p, unsigned* counts, double* temp,
                   unsigned long long* vmax, unsigned char* rgb, hipStream_t s);
 
+// Stable compaction of `cur` into `nxt` by cb.alive (accelerations and
+// interaction counts into cb.cax.. / cb.cinter), the system starts moved;
+// advance_time adds each system's dt to its time.
+void launch_compact(CollideBuffers& cb, State& cur, State& nxt, const double* ax,
+                    const double* ay, const double* az, const int* inter, int n, Systems& sys,
+                    int advance_time, int* flags, hipStream_t s);
+// apply_minimum_size on the built tree: every largest cell of side <
+// min_size becomes one particle, then `cur` is compacted into `nxt`.
+void launch_min_size(CollideBuffers& cb, const WalkNode* nodes, int node_cap, double min_size,
+                     State& cur, State& nxt, const double* ax, const double* ay,
+                     const double* az, const int* inter, int n, Systems& sys, int* flags,
+                     hipStream_t s);
+
 // Small fills (no hipMemsetAsync: these run inside captured graphs too).
 void launch_fill_int(int* p, int v, size_t n, hipStream_t s);
 void launch_fill_u64(unsigned long long* p, unsigned long long v, size_t n,

@@ -80,7 +80,8 @@ def lib():
         L.orc_run_collide.restype = ctypes.c_int64
         L.orc_run_collide.argtypes = [ctypes.c_size_t, _dp, _dp, _dp, _dp, ctypes.c_double,
                                       ctypes.c_double, ctypes.c_int, ctypes.c_double,
-                                      ctypes.c_int, ctypes.c_int, _i64p, _i64p]
+                                      ctypes.c_int, ctypes.c_int, _i64p, _i64p,
+                                      ctypes.c_double]
         L.orc_direct.argtypes = [ctypes.c_size_t, _dp, _dp, ctypes.c_int, ctypes.c_size_t,
                                  _i64p, ctypes.c_int, _dp]
         _lib = L
@@ -214,9 +215,10 @@ def run(pos, vel, mass, theta=0.5, dt=0.0333, frames=1, damping=False, prec="f64
 
 
 def run_collide(pos, vel, mass, theta=0.5, dt=0.0333, frames=1, damping=False, range=0.01,
-                threads=1):
-    """FP64 frames with collision merging; returns (pos, vel, acc, mass,
-    pairs detected, particles absorbed), compacted to the survivors."""
+                threads=1, min_size=0.0):
+    """FP64 frames with collision merging (range < 0: none) and, for
+    min_size > 0, apply_minimum_size; returns (pos, vel, acc, mass, pairs
+    detected, particles absorbed), compacted to the survivors."""
     pos = np.array(pos, dtype=np.float64, order="C")
     vel = np.array(vel, dtype=np.float64, order="C")
     mass = np.array(mass, dtype=np.float64, order="C")
@@ -224,7 +226,7 @@ def run_collide(pos, vel, mass, theta=0.5, dt=0.0333, frames=1, damping=False, r
     npairs, nrem = ctypes.c_int64(), ctypes.c_int64()
     n = lib().orc_run_collide(len(pos), _p(pos), _p(vel), _p(acc), _p(mass), theta, dt,
                               int(damping), range, frames, threads, ctypes.byref(npairs),
-                              ctypes.byref(nrem))
+                              ctypes.byref(nrem), float(min_size))
     if n < 0:
         raise ValueError("orc_run_collide failed (coincident particles)")
     return pos[:n], vel[:n], acc[:n], mass[:n], npairs.value, nrem.value

@@ -40,6 +40,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <random>
 #include <thread>
 #include <utility>
@@ -778,9 +779,19 @@ int orc_run(int prec, size_t n, double* pos, double* vel, double* acc,
 // the others keeping their order.  Arrays are compacted in place; returns
 // the new count (-1 on a tree overflow).  pairs_out / removed_out (may be
 // null) receive the detections and absorbed particles of all frames.
+//
+// min_size > 0 adds apply_minimum_size (src/main.cpp:1001-1057) before the
+// mass and COM: every largest cell of side < min_size becomes one particle,
+// collide() folded along the tree (children 0..7 at each level, a leaf's
+// result its particle), in its lowest-index member's place; the tree is
+// then built again for the merged particles on the same root.  (The
+// reference keeps the collapsed tree instead.  It differs only where a
+// cell is left with a single particle below a chain of one-child cells,
+// whose centre of mass (x m) / m can differ from x in the last bit.)
 int64_t orc_run_collide(size_t n, double* pos, double* vel, double* acc, double* mass,
                         double theta, double dt, int damping, double range, int frames,
-                        int threads, int64_t* pairs_out, int64_t* removed_out) {
+                        int threads, int64_t* pairs_out, int64_t* removed_out,
+                        double min_size) {
   std::vector<V3<double>> p, v, a(n);
   std::vector<double> m, dummy;
   load_state<double>(n, pos, mass, p, m);
@@ -794,6 +805,78 @@ int64_t orc_run_collide(size_t n, double* pos, double* vel, double* acc, double*
     Tree<double>::root_of(p.data(), n, o, side);
     tree.build(p.data(), m.data(), n, o, side);
     if (tree.overflow) return -1;
+    if (min_size > 0) {
+      struct Part {
+        double m;
+        V3<double> x, v;
+      };
+      auto merge = [](Part& acc, const Part& b) {  // collide(), src/main.cpp:423-430
+        const double am = acc.m, bm = b.m;
+        acc.x = V3<double>{(acc.x.x * am + b.x.x * bm) / (am + bm),
+                           (acc.x.y * am + b.x.y * bm) / (am + bm),
+                           (acc.x.z * am + b.x.z * bm) / (am + bm)};
+        acc.v = V3<double>{(acc.v.x * am + b.v.x * bm) / (am + bm),
+                           (acc.v.y * am + b.v.y * bm) / (am + bm),
+                           (acc.v.z * am + b.v.z * bm) / (am + bm)};
+        acc.m = am + bm;
+      };
+      std::vector<char> alive(n, 1);
+      std::vector<int> members;
+      // the fold over a subtree; false when it holds no particle
+      std::function<bool(int, Part&)> fold = [&](int k, Part& r) {
+        const auto& nd = tree.nodes[k];
+        if (nd.p >= 0) {
+          r = Part{m[nd.p], p[nd.p], v[nd.p]};
+          members.push_back(nd.p);
+          return true;
+        }
+        bool has = false;
+        for (int i = 0; i < 8; i++) {
+          Part q;
+          if (nd.child[i] >= 0 && fold(nd.child[i], q)) {
+            if (has) merge(r, q); else r = q;
+            has = true;
+          }
+        }
+        return has;
+      };
+      std::function<void(int)> visit = [&](int k) {
+        const auto& nd = tree.nodes[k];
+        if (nd.p >= 0) return;
+        if (nd.side < min_size) {  // the largest cell below min_size on this path
+          members.clear();
+          Part r;
+          if (!fold(k, r)) return;
+          const int keep = *std::min_element(members.begin(), members.end());
+          for (int q : members) alive[q] = 0;
+          alive[keep] = 1;
+          m[keep] = r.m;
+          p[keep] = r.x;
+          v[keep] = r.v;
+          return;
+        }
+        for (int i = 0; i < 8; i++)
+          if (nd.child[i] >= 0) visit(nd.child[i]);
+      };
+      visit(0);
+      size_t k = 0;
+      for (size_t i = 0; i < n; i++) {
+        if (!alive[i]) continue;
+        p[k] = p[i];
+        v[k] = v[i];
+        m[k] = m[i];
+        k++;
+      }
+      nremoved += (int64_t)(n - k);
+      n = k;
+      p.resize(n);
+      v.resize(n);
+      a.resize(n);
+      m.resize(n);
+      tree = Tree<double>();
+      tree.build(p.data(), m.data(), n, o, side);  // the same root
+      if (tree.overflow) return -1;
+    }
     tree.mass_com();
     std::vector<std::vector<std::pair<int, int>>> tp(std::max(threads, 1));
     parallel_for(n, threads, [&](size_t i, int w) {

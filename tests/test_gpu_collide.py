@@ -146,3 +146,55 @@ def test_fixed_root_merges_the_same_groups():
     _, _, _, om, _, onrem = O.run_collide(pos, vel, mass, frames=1, range=R)
     assert nrem == onrem == 10
     np.testing.assert_array_equal(m, om)
+
+
+# ---------------------------------------------------------- minimum size --
+def gpu_min_size(pos, vel, mass, steps, min_size, dt=0.0333, r=None):
+    with nbodysim_amd.Simulation(theta=0.5, dt=dt) as s:
+        s.set_state(pos, vel, mass)
+        s.set_min_node_size(min_size)
+        if r is not None:
+            s.set_collisions(r)
+        s.step(steps)
+        p, v, a = s.state()
+        m = s.masses()
+        merged = s.set_min_node_size(min_size)
+    return p, v, m, merged
+
+
+def test_min_size_lattice_matches_oracle():
+    rng = np.random.default_rng(8)
+    pos = rng.integers(0, 32, size=(3000, 3)) * 0.25 + rng.uniform(0.02, 0.06, size=(3000, 3))
+    pos[0] = [0.0, 0.0, 0.0]
+    pos[1] = [8.0, 8.0, 8.0]
+    vel = rng.uniform(-0.01, 0.01, size=pos.shape)
+    mass = rng.uniform(1e3, 1e4, size=len(pos))
+    p, v, m, merged = gpu_min_size(pos, vel, mass, 2, 0.6)
+    op, ov, oa, om, _, onrem = O.run_collide(pos, vel, mass, frames=2, range=-1.0, min_size=0.6)
+    print("min size: %d absorbed, %d left" % (merged, len(m)), flush=True)
+    assert merged == onrem > 0
+    np.testing.assert_array_equal(m, om)
+    close(p, op, 10.0)
+    close(v, ov, 1.0)
+
+
+def test_min_size_folds_along_the_tree():
+    pos = np.array([[8.0, 8.0, 8.0], [7.9, 7.7, 7.3], [0.0, 0.0, 0.0]])
+    vel = np.array([[-0.967, 0.627, 0.826], [0.213, 0.459, 0.087], [0.87, 0.632, -0.995]])
+    mass = np.array([6.733, 3.428, 1.369])
+    p, v, m, merged = gpu_min_size(pos, vel, mass, 1, 100.0, dt=0.0)
+    op, ov, _, om, _, _ = O.run_collide(pos, vel, mass, frames=1, dt=0.0, range=-1.0,
+                                        min_size=100.0)
+    assert merged == 2
+    np.testing.assert_array_equal(m, om)
+    np.testing.assert_array_equal(p, op)
+    np.testing.assert_array_equal(v, ov)
+
+
+def test_min_size_with_collisions():
+    pos, vel, mass = planted(rng_range=R)
+    p, v, m, merged = gpu_min_size(pos, vel, mass, 2, 0.6, r=R)
+    op, ov, _, om, _, onrem = O.run_collide(pos, vel, mass, frames=2, range=R, min_size=0.6)
+    assert len(m) == len(om) == len(mass) - 10
+    np.testing.assert_array_equal(m, om)
+    close(p, op, 50.0)

@@ -167,3 +167,73 @@ def test_against_the_reference_classes(frames):
     for x, y in ((p, rp), (v, rv), (a, ra), (m, rm)):
         np.testing.assert_array_equal(x[~big], y[~big])
         np.testing.assert_allclose(x[big], y[big], rtol=1e-14, atol=1e-12)
+
+
+# ---------------------------------------------------------- minimum size --
+def collide_np(a, b):
+    """collide() of src/main.cpp:423-430 on (m, x[3], v[3]) tuples."""
+    (am, ax, av), (bm, bx, bv) = a, b
+    return (am + bm, (ax * am + bx * bm) / (am + bm), (av * am + bv * bm) / (am + bm))
+
+
+def test_min_size_below_every_cell_changes_nothing():
+    pos, vel, mass = planted()
+    p, v, a, m, _, nrem = O.run_collide(pos, vel, mass, frames=2, range=-1.0, min_size=1e-9)
+    p2, v2, a2 = O.run(pos, vel, mass, frames=2)
+    assert nrem == 0
+    np.testing.assert_array_equal(p, p2)
+    np.testing.assert_array_equal(v, v2)
+
+
+def test_min_size_groups_are_the_occupied_cells():
+    rng = np.random.default_rng(8)
+    # points on a lattice of spacing 0.25 (plus jitter) inside a cube whose
+    # bbox root has side 8 + 2 = 10: with min_size 0.6 the largest cells
+    # below it have side 10 / 32 = 0.3125
+    pos = rng.integers(0, 32, size=(3000, 3)) * 0.25 + rng.uniform(0.02, 0.06, size=(3000, 3))
+    pos[0] = [0.0, 0.0, 0.0]
+    pos[1] = [8.0, 8.0, 8.0]
+    vel = np.zeros_like(pos)
+    mass = rng.uniform(1e3, 1e4, size=len(pos))
+    _, _, _, m, _, nrem = O.run_collide(pos, vel, mass, frames=1, range=-1.0, min_size=0.6)
+    c, side = O.root(pos)
+    lo = np.asarray(c) - side / 2
+    cells = np.floor((pos - lo) / (side / 32)).astype(int)
+    occupied = len({tuple(r) for r in cells})
+    assert len(m) == occupied and nrem == len(pos) - occupied
+    assert abs(m.sum() - mass.sum()) <= 1e-12 * mass.sum()
+
+
+def test_min_size_above_the_root_merges_all():
+    rng = np.random.default_rng(9)
+    pos = rng.uniform(-1, 1, size=(50, 3))
+    vel = rng.uniform(-1, 1, size=(50, 3))
+    mass = rng.uniform(1.0, 2.0, size=50)
+    p, v, a, m, _, nrem = O.run_collide(pos, vel, mass, frames=1, range=-1.0, min_size=100.0)
+    assert len(m) == 1 and nrem == 49
+    assert abs(m[0] - mass.sum()) <= 1e-14 * mass.sum()
+    np.testing.assert_allclose(m[0] * v[0], (mass[:, None] * vel).sum(0), rtol=1e-13)
+    com = (mass[:, None] * pos).sum(0) / mass.sum()
+    np.testing.assert_allclose(p[0], com + v[0] * 0.0333, atol=1e-13)
+
+
+def test_min_size_folds_along_the_tree():
+    # root: bbox [0, 8]^3 + 2 -> side 10, centre 4.  C sits in octant 0 of
+    # the root; A and B in octant 7, where they part one level down (B in
+    # child 1, A in child 7).  With min_size above the root side everything
+    # merges as collide(C, collide(B, A)), not as the flat fold in tree
+    # order collide(collide(C, B), A) (these values tell the two apart)
+    A = (6.733, np.array([8.0, 8.0, 8.0]), np.array([-0.967, 0.627, 0.826]))
+    B = (3.428, np.array([7.9, 7.7, 7.3]), np.array([0.213, 0.459, 0.087]))
+    C = (1.369, np.array([0.0, 0.0, 0.0]), np.array([0.87, 0.632, -0.995]))
+    pos = np.array([A[1], B[1], C[1]])
+    vel = np.array([A[2], B[2], C[2]])
+    mass = np.array([A[0], B[0], C[0]])
+    p, v, a, m, _, nrem = O.run_collide(pos, vel, mass, frames=1, dt=0.0, range=-1.0,
+                                        min_size=100.0)
+    nested = collide_np(C, collide_np(B, A))
+    flat = collide_np(collide_np(C, B), A)
+    assert m[0] == nested[0]
+    np.testing.assert_array_equal(p[0], nested[1])
+    np.testing.assert_array_equal(v[0], nested[2])
+    assert not (np.array_equal(nested[1], flat[1]) and np.array_equal(nested[2], flat[2]))
