@@ -176,3 +176,31 @@ def test_frame_loop_with_collisions(tmp_path, monkeypatch):
             np.testing.assert_array_equal(got[3], s.masses())
     print("particles per frame:", counts, flush=True)
     assert counts[0] < 3000 and counts == sorted(counts, reverse=True)
+
+
+def test_frame_loop_with_min_node_size(tmp_path, monkeypatch):
+    """min_node_size applies only with verbose set (src/main.cpp:1425): with
+    both, each frame file holds the merged particles the library's
+    minimum-size steps give; without verbose nothing is merged."""
+    monkeypatch.chdir(tmp_path)
+    base = SETTINGS.format(prec="f64", keep="true") + "min_node_size 20\n"
+    with open("settings.cfg", "w") as f:
+        f.write(base + "verbose true\n")
+    assert host.run_frames("settings.cfg") == 0
+    settings = host.read_settings("settings.cfg")[0]
+    pos, vel, mass = host.generate(3000, gen_type="cube", seed=0xDEADBEEF, settings=settings)
+    with sim(theta=0.5, dt=0.0333, precision="f64") as s:
+        s.set_state(pos, vel, mass)
+        s.set_min_node_size(20.0)
+        for frame in range(4):
+            s.step(1)
+            want = s.state()
+            got = host.read_binary(host.frame_filename("./data", frame, True))
+            for g, w in zip(got[:3], want):
+                np.testing.assert_array_equal(g, w)
+            np.testing.assert_array_equal(got[3], s.masses())
+        assert s.n < 3000
+    with open("settings.cfg", "w") as f:
+        f.write(base + "verbose false\n")
+    assert host.run_frames("settings.cfg") == 0
+    assert len(host.read_binary(host.frame_filename("./data", 3, True))[3]) == 3000
