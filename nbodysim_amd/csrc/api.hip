@@ -689,6 +689,9 @@ int run_collide(nbs_ctx* c, int nsteps) {
   if (c->p.adaptive)
     return fail(c, NBS_ERR_STATE, "merging particles needs adaptive dt off");
   for (int k = 0; k < nsteps && c->n > 0; k++) {
+    // once this step's minimum-size merge is done, a redo (pair buffer
+    // grown) keeps it and its root
+    bool min_done = false;
     for (int attempt = 0;; attempt++) {
       if (attempt == 8) return fail(c, NBS_ERR_OVERFLOW, "collision step overflow after retries");
       int r = ensure_nodes(c);
@@ -697,8 +700,8 @@ int run_collide(nbs_ctx* c, int nsteps) {
       if (r) return r;
       reset_flags(c);
       launch_fill_int(c->cb.count, 0, 2, c->stream);
-      enqueue_build(c);
-      if (c->min_size > 0) {
+      enqueue_build(c, min_done);
+      if (c->min_size > 0 && !min_done) {
         // apply_minimum_size on this build, then the tree of the merged
         // particles on the same root (src/main.cpp:1421-1434)
         const int n0 = c->n;
@@ -721,6 +724,7 @@ int run_collide(nbs_ctx* c, int nsteps) {
         HIPCHK(c, hipMemcpy(info.data(), c->cb.info, info.size() * sizeof(int),
                             hipMemcpyDeviceToHost));
         c->cur ^= 1;
+        min_done = true;
         c->min_removed += n0 - info[0];
         if (info[0] != n0) {
           drop_graph(c);

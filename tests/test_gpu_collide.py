@@ -198,3 +198,19 @@ def test_min_size_with_collisions():
     assert len(m) == len(om) == len(mass) - 10
     np.testing.assert_array_equal(m, om)
     close(p, op, 50.0)
+
+
+def test_min_size_then_collisions_that_grow_the_pair_buffer():
+    # the dense cluster overflows the initial pair buffer: the step is redone
+    # keeping its minimum-size merge (and root) as they were
+    rng = np.random.default_rng(3)
+    blob = rng.uniform(0, 0.004, size=(700, 3))
+    far = rng.uniform(50, 100, size=(300, 3))
+    pos = np.concatenate([blob, far])
+    vel = rng.uniform(-1, 1, size=pos.shape)
+    mass = rng.uniform(1e3, 1e4, size=len(pos))
+    p, v, m, merged = gpu_min_size(pos, vel, mass, 1, 3e-4, r=R)
+    op, ov, _, om, _, onrem = O.run_collide(pos, vel, mass, frames=1, range=R, min_size=3e-4)
+    assert len(m) == len(om) == 301
+    np.testing.assert_array_equal(m, om)
+    close(p, op, 100.0)
