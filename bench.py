@@ -560,6 +560,45 @@ def config_collide(args):
         "results": res}), flush=True)
 
 
+def config_image(args):
+    """The frame image (write_image, src/main.cpp:556-678) of N = 1,048,576
+    sphere particles at the reference's defaults (1920 x 1080, iso, bw,
+    brightness 255): device render plus download of the RGB rows, against
+    the oracle's restatement of the reference's CPU loop on one core, and
+    the bytes compared."""
+    import nbodysim_amd
+    from nbodysim_amd import host
+    import oracle as O
+    n, reps = args.n, args.image_reps
+    pos, vel, mass = host.generate(n, gen_type="sphere", seed=0xDEADBEEF, **SPHERE)
+    res = {}
+    with nbodysim_amd.Simulation(theta=0.5, dt=0.0333, precision="f32") as sim:
+        sim.set_state(pos, vel, mass)
+        for kw in (dict(), dict(color="heat", adaptive=True, nonlinear=True)):
+            key = "bw" if not kw else "heat_adaptive_nonlinear"
+            img = sim.render_image(**kw)
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                img = sim.render_image(**kw)
+            dev_ms = (time.perf_counter() - t0) * 1e3 / reps
+            t0 = time.perf_counter()
+            ref = O.render_image(pos, **kw)
+            cpu_ms = (time.perf_counter() - t0) * 1e3
+            res[key] = {"device_ms": dev_ms, "cpu_oracle_ms": cpu_ms,
+                        "bytes_identical": bool(np.array_equal(img, ref)),
+                        "lit_pixels": int(img.any(axis=2).sum())}
+            log("image", key, res[key])
+    print(json.dumps({
+        "metric": "ms per frame image (render + download)", "unit": "ms",
+        "higher_is_better": False, "n_gpus": 1, "dtype": "f64",
+        "data": "synthetic (seeded gen_type sphere, the reference's generator)",
+        "config": {"workload": "gen_type sphere N=%d r=100, image 1920x1080 iso, "
+                               "brightness 255, %d renders timed" % (n, reps)},
+        "cpu_baseline": {"kind": "port", "cores": 1,
+                         "sample": "the oracle's write_image restatement, whole image"},
+        "results": res}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -593,6 +632,9 @@ def main():
                     help="steps with collision merging against plain steps")
     ap.add_argument("--collide-steps", type=int, default=10)
     ap.add_argument("--collide-range", type=float, default=0.01)
+    ap.add_argument("--image-study", action="store_true",
+                    help="frame image rendering on the device against the CPU oracle")
+    ap.add_argument("--image-reps", type=int, default=10)
     ap.add_argument("--n3", type=int, default=1 << 22)
     ap.add_argument("--steps3", type=int, default=50)
     ap.add_argument("--systems", type=int, default=256)
@@ -617,6 +659,9 @@ def main():
         return
     if args.collide_study:
         config_collide(args)
+        return
+    if args.image_study:
+        config_image(args)
         return
 
     import nbodysim_amd
