@@ -258,6 +258,7 @@ struct nbs_ctx {
   size_t img_pixels = 0;
   unsigned* img_counts = nullptr;
   double* img_temp = nullptr;
+  double* img_partial = nullptr;
   unsigned long long* img_max = nullptr;
   unsigned char* img_rgb = nullptr;
 };
@@ -1295,6 +1296,7 @@ int nbs_render_image(nbs_ctx* c, const nbs_image_params* ip, uint8_t* rgb) {
     hipError_t e = hipSuccess;
     c->img_counts = dalloc<unsigned>(c->img_bufs, npix, &e);
     c->img_temp = dalloc<double>(c->img_bufs, npix, &e);
+    c->img_partial = dalloc<double>(c->img_bufs, image_partials(npix), &e);
     c->img_max = dalloc<unsigned long long>(c->img_bufs, 1, &e);
     c->img_rgb = dalloc<unsigned char>(c->img_bufs, 3 * npix, &e);
     if (e != hipSuccess) {
@@ -1312,8 +1314,8 @@ int nbs_render_image(nbs_ctx* c, const nbs_image_params* ip, uint8_t* rgb) {
   p.nonlinear = ip->nonlinear != 0;
   p.scale = ip->scale;
   p.brightness = ip->brightness;
-  launch_image(c->st[c->cur], c->n, p, c->img_counts, c->img_temp, c->img_max, c->img_rgb,
-               c->stream);
+  launch_image(c->st[c->cur], c->n, p, c->img_counts, c->img_temp, c->img_partial, c->img_max,
+               c->img_rgb, c->stream);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipMemcpyAsync(rgb, c->img_rgb, 3 * npix, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));

@@ -304,10 +304,13 @@ struct ImageParams {
   int adaptive, nonlinear;
   double scale, brightness;
 };
-// counts [w*h] u32, temp [w*h] doubles, vmax one word; rgb [h][w][3]
-// bytes, PNG row order (top row first).
+// counts [w*h] u32, temp [w*h] doubles, partial [image_partials(w*h)]
+// doubles, vmax one word; rgb [h][w][3] bytes, PNG row order (top row
+// first).
+size_t image_partials(size_t npix);
 void launch_image(const State& st, int n, const ImageParams& p, unsigned* counts, double* temp,
-                  unsigned long long* vmax, unsigned char* rgb, hipStream_t s);
+                  double* partial, unsigned long long* vmax, unsigned char* rgb,
+                  hipStream_t s);
 
 // Stable compaction of `cur` into `nxt` by cb.alive (accelerations and
 // interaction counts into cb.cax.. / cb.cinter), the system starts moved;
