@@ -214,3 +214,33 @@ def test_min_size_then_collisions_that_grow_the_pair_buffer():
     assert len(m) == len(om) == 301
     np.testing.assert_array_equal(m, om)
     close(p, op, 100.0)
+
+
+def test_merging_edge_cases():
+    # two particles in range merge into one, which then steps on alone
+    pos = np.array([[0.0, 0.0, 0.0], [0.004, 0.0, 0.0]])
+    vel = np.array([[1.0, 0.0, 0.0], [-1.0, 0.5, 0.0]])
+    mass = np.array([2.0, 6.0])
+    with nbodysim_amd.Simulation(theta=0.5, dt=0.0333) as s:
+        s.set_state(pos, vel, mass)
+        s.set_collisions(R)
+        s.step(3)
+        p, v, a = s.state()
+        m = s.masses()
+        assert s.n == 1 and s.collision_stats()[1] == 1
+        img = s.render_image(64, 48, projection="front", brightness=10.0)
+    op, ov, oa, om, _, _ = O.run_collide(pos, vel, mass, frames=3, range=R)
+    np.testing.assert_array_equal(m, om)
+    np.testing.assert_array_equal(p, op)
+    np.testing.assert_array_equal(v, ov)
+    np.testing.assert_array_equal(a, 0.0)
+    np.testing.assert_array_equal(img, O.render_image(p, 64, 48, projection="front",
+                                                      brightness=10.0))
+    # an empty state: steps with merging on do nothing; its image is black
+    with nbodysim_amd.Simulation(theta=0.5, dt=0.0333) as s:
+        s.set_state(np.zeros((0, 3)), np.zeros((0, 3)), np.zeros(0))
+        s.set_collisions(R)
+        s.set_min_node_size(1.0)
+        s.step(2)
+        assert s.n == 0
+        assert not s.render_image(32, 16, adaptive=True, color="heat").any()
