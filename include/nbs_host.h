@@ -24,7 +24,9 @@ enum { NBS_COLOR_BW = 0, NBS_COLOR_HEAT = 1 };
 /* Every key of the reference's `settings` struct (src/main.cpp:115-160),
  * with the reference's types widened to double, plus the adaptive-dt keys
  * its README documents (README.md:54-60) and this build's extensions
- * (force_precision, shell_jitter). */
+ * (force_precision, shell_jitter, datatype).  `dump_plaintext`, the
+ * README's name for dump_text, is not a key: the reference's parser
+ * reports it as an unrecognized variable (src/main.cpp:1300-1302). */
 typedef struct nbs_settings {
   int read_existing, use_seed, display_progress, dump_binary, dump_text,
       dump_image, overwrite_data, keep_previous_binary, keep_previous_text,
@@ -44,6 +46,12 @@ typedef struct nbs_settings {
   /* extensions */
   int force_precision;   /* `force_precision f32|f64` (default f64) */
   double shell_jitter;   /* `shell_jitter x`: shell radius R(1 + x U(-1,1)) */
+  int datatype_float;    /* `datatype float|double` (default double): the
+                            reference's build macro (makefile:3 ships
+                            -DFLOAT).  float: particles generated as the
+                            FLOAT build draws them and .dat records of
+                            40 B, written and resumed from; the step's
+                            state stays FP64 either way. */
 } nbs_settings;
 
 /* Defaults of src/main.cpp:1312-1357 (plus adaptive defaults of the

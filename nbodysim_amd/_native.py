@@ -61,7 +61,7 @@ class Settings(ctypes.Structure):
         [("min_node_size", ctypes.c_double), ("adaptive", ctypes.c_int),
          ("min_adaptive_dt", ctypes.c_double), ("max_pos_change", ctypes.c_double),
          ("max_vel_change", ctypes.c_double), ("force_precision", ctypes.c_int),
-         ("shell_jitter", ctypes.c_double)])
+         ("shell_jitter", ctypes.c_double), ("datatype_float", ctypes.c_int)])
 
 
 # (name, restype, argtypes) of every exported function, in header order.

@@ -599,6 +599,18 @@ void reset_flags(nbs_ctx* c) {
   launch_fill_int(c->steps_done, 0, 1, c->stream);
 }
 
+// Device flags that larger buffers cannot cure.  kFlagStack: a walk's
+// stack of waiting lanes (64 entries, one per lane; a tree is at most 43
+// levels deep) or the minimum-size fold's stack ran past its end, which
+// only a corrupt tree can cause.  Node and pair overflows return NBS_OK
+// here: the caller grows that buffer and redoes the step.
+int fatal_flags(nbs_ctx* c, int flags) {
+  if (flags & kFlagStack)
+    return fail(c, NBS_ERR_OVERFLOW,
+                "tree walk stack overflow (a tree deeper than its 43 levels: corrupt build)");
+  return NBS_OK;
+}
+
 // ------------------------------------------------------------ collide --
 // Buffers of collision merging: per particle (cap_n), per pair (the
 // wanted capacity) and the read-back info (nsys + 2 ints).
@@ -715,8 +727,8 @@ int run_collide(nbs_ctx* c, int nsteps) {
         c->bbox_valid = false;
         c->tree_valid = false;
         if (flags) {
-          if (flags & kFlagRunOverflow)
-            return fail(c, NBS_ERR_OVERFLOW, "more than 8192 particles share one 21-level cell");
+          r = fatal_flags(c, flags);
+          if (r) return r;
           r = alloc_nodes(c, c->node_cap * 2);
           if (r) return r;
           continue;
@@ -783,11 +795,11 @@ int run_collide(nbs_ctx* c, int nsteps) {
       c->bbox_valid = false;
       c->tree_valid = false;
       if (flags) {
-        if (flags & kFlagRunOverflow)
-          return fail(c, NBS_ERR_OVERFLOW, "more than 8192 particles share one 21-level cell");
-        if (flags & kFlagPairs) {
+        r = fatal_flags(c, flags);
+        if (r) return r;
+        if (flags & kFlagPairs)
           c->col_pair_want = std::max(2 * c->cb.pair_cap, cnt[0] + cnt[0] / 4);
-        } else {
+        if (flags & kFlagNodeOverflow) {
           r = alloc_nodes(c, c->node_cap * 2);
           if (r) return r;
         }
@@ -848,9 +860,8 @@ int run(nbs_ctx* c, int nsteps) {
     }
     c->bbox_valid = false;
     c->tree_valid = false;
-    if (flags & kFlagRunOverflow)
-      return fail(c, NBS_ERR_OVERFLOW,
-                  "more than 8192 particles share one 21-level cell");
+    r = fatal_flags(c, flags);
+    if (r) return r;
     remaining -= done;
     r = alloc_nodes(c, c->node_cap * 2);
     if (r) return r;
@@ -1399,8 +1410,8 @@ int nbs_step_graph(nbs_ctx* c, int nsteps) {
   drop_graph(c);
   c->bbox_valid = false;
   c->tree_valid = false;
-  if (flags & kFlagRunOverflow)
-    return fail(c, NBS_ERR_OVERFLOW, "more than 8192 particles share one 21-level cell");
+  r = fatal_flags(c, flags);
+  if (r) return r;
   r = alloc_nodes(c, c->node_cap * 2);
   if (r) return r;
   return run(c, nsteps - done);
@@ -1887,8 +1898,8 @@ int build_checked(nbs_ctx* c) {
     int r = read_flags(c, &flags, &done);
     if (r) return r;
     if (!flags) return NBS_OK;
-    if (flags & kFlagRunOverflow)
-      return fail(c, NBS_ERR_OVERFLOW, "more than 8192 particles share one cell");
+    r = fatal_flags(c, flags);
+    if (r) return r;
     // state was permuted but not changed: rebuild with room to spare
     c->bbox_valid = false;
     r = alloc_nodes(c, c->node_cap * 2);

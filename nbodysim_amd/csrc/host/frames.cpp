@@ -47,7 +47,7 @@ int write_frame(const nbs_settings& s, const Frame& f) {
     }
   }
   if (s.dump_binary &&
-      nbs_write_binary(bn, f.n, f.pos, f.vel, f.acc, f.mass, 0) != NBS_OK) {
+      nbs_write_binary(bn, f.n, f.pos, f.vel, f.acc, f.mass, s.datatype_float) != NBS_OK) {
     fprintf(stderr, "Cannot write %s\n", bn);
     return NBS_ERR_IO;
   }
@@ -98,12 +98,16 @@ extern "C" int nbs_run_frames(const char* cfgfile) {
     } else {
       char bn[4096];
       nbs_frame_filename("./data", start_frame - 1, 1, bn, sizeof(bn));
-      const int64_t n = nbs_binary_count(bn, 0);
+      const int64_t n = nbs_binary_count(bn, s.datatype_float);
+      if (n < 0) {
+        fprintf(stderr, "Cannot read %s\n", bn);
+        return 1;
+      }
       pos.resize(3 * n);
       vel.resize(3 * n);
       acc.resize(3 * n);
       mass.resize(n);
-      if (nbs_read_binary(bn, 0, n, pos.data(), vel.data(), acc.data(),
+      if (nbs_read_binary(bn, s.datatype_float, n, pos.data(), vel.data(), acc.data(),
                           mass.data()) != n) {
         fprintf(stderr, "Cannot read %s\n", bn);
         return 1;
@@ -119,7 +123,7 @@ extern "C" int nbs_run_frames(const char* cfgfile) {
     vel.assign(3 * n, 0);
     acc.assign(3 * n, 0);
     mass.assign(n, 0);
-    if (nbs_generate(&s, seed, n, 0, pos.data(), vel.data(), mass.data())) {
+    if (nbs_generate(&s, seed, n, s.datatype_float, pos.data(), vel.data(), mass.data())) {
       fprintf(stderr, "Invalid generation settings.\n");
       return 1;
     }

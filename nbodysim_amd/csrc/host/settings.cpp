@@ -152,6 +152,7 @@ void nbs_settings_default(nbs_settings* s) {
   s->max_vel_change = 3;
   s->force_precision = NBS_FORCE_F64;
   s->shell_jitter = 0;
+  s->datatype_float = 0;
 }
 
 int nbs_settings_read(nbs_settings* s, const char* path, char* msg,
@@ -174,7 +175,6 @@ int nbs_settings_read(nbs_settings* s, const char* path, char* msg,
                {"display_progress", &s->display_progress},
                {"dump_binary", &s->dump_binary},
                {"dump_text", &s->dump_text},
-               {"dump_plaintext", &s->dump_text},  // README spelling
                {"dump_image", &s->dump_image},
                {"overwrite_data", &s->overwrite_data},
                {"keep_previous_binary", &s->keep_previous_binary},
@@ -283,6 +283,18 @@ int nbs_settings_read(nbs_settings* s, const char* path, char* msg,
         s->force_precision = NBS_FORCE_F64;
       else {
         note(msg, msg_len, "invalid value '" + v + "' for force_precision");
+        return NBS_ERR_ARG;
+      }
+    } else if (var == "datatype") {
+      // the reference's build macro (-DFLOAT / -DDOUBLE, makefile:3) as a key
+      std::string v;
+      tk.next(v);
+      if (v == "float")
+        s->datatype_float = 1;
+      else if (v == "double")
+        s->datatype_float = 0;
+      else {
+        note(msg, msg_len, "invalid value '" + v + "' for datatype");
         return NBS_ERR_ARG;
       }
     } else if (var == "true" || var == "false") {

@@ -87,7 +87,9 @@ __global__ void value_kernel(const unsigned* __restrict__ counts, size_t npix, I
       const unsigned long long mant = (bits & ((1ull << 52) - 1)) | (1ull << 52);
       odd = mant >> __builtin_ctzll(mant);
     }
-    if (odd != 0 && (unsigned long long)c * odd < (1ull << 53)) {
+    // c * odd < 2^53, tested without forming the product (it can pass
+    // 2^64 for a 53-bit odd mantissa such as 0.1's)
+    if (odd != 0 && (c == 0 || odd <= ((1ull << 53) - 1) / c)) {
       t = p.brightness * (double)c;
     } else {
       for (unsigned j = 0; j < c; j++) t += p.brightness;

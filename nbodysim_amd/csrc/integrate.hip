@@ -31,6 +31,10 @@ __global__ void __launch_bounds__(kBlock)
                         const int* __restrict__ wstart, int nsys, int nwaves,
                         double* __restrict__ amax, double* __restrict__ vmax,
                         const int* __restrict__ flags) {
+  // |a| and |v| as magnitude() forms them (src/vector.cpp:104-107): each
+  // square and sum rounded, summed left to right, so the step length can
+  // be restated exactly on the host
+#pragma clang fp contract(off)
   if (*flags) return;
   const int w = (blockIdx.x * kBlock + threadIdx.x) >> 6;
   if (w >= nwaves) return;

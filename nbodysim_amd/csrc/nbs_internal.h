@@ -68,7 +68,7 @@ struct State {
 };
 
 // Flags word bits (device int).
-enum : int { kFlagNodeOverflow = 1, kFlagRunOverflow = 2, kFlagStack = 4, kFlagPairs = 8 };
+enum : int { kFlagNodeOverflow = 1, kFlagStack = 4, kFlagPairs = 8 };
 
 struct StepParams {
   double theta;

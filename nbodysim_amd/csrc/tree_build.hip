@@ -352,7 +352,14 @@ __global__ void __launch_bounds__(kRunBlock)
       __syncthreads();
       if (t < 256 && hist[t] == len) skip_sh = 1;  // one digit value: order unchanged
       __syncthreads();
-      if (skip_sh) continue;
+      // Every wave reads the flag before any wave can leave for the next
+      // pass, whose first statements clear hist and skip_sh: without this
+      // barrier a wave that reads late sees the next pass's 0 and falls
+      // into this pass's scatter, out of step with the other waves'
+      // barriers.
+      const int skip = skip_sh;
+      __syncthreads();
+      if (skip) continue;
       if (w == 0) {  // exclusive scan of the 256 bins, 4 per lane
         int v[4], s4 = 0;
 #pragma unroll

@@ -33,7 +33,7 @@ def _to_dict(s):
     for name in ("read_existing", "use_seed", "display_progress", "dump_binary",
                  "dump_text", "dump_image", "overwrite_data", "keep_previous_binary",
                  "keep_previous_text", "verbose", "damping", "adaptive_brightness",
-                 "nonlinear_brightness", "collide", "cuda", "adaptive"):
+                 "nonlinear_brightness", "collide", "cuda", "adaptive", "datatype_float"):
         d[name] = bool(d[name])
     return d
 

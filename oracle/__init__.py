@@ -245,6 +245,35 @@ def direct(pos, mass, idx=None, damping=False, threads=8):
     return acc
 
 
+def magnitudes(v):
+    """|v| per row as vector::magnitude forms it (src/vector.cpp:104-107):
+    sqrt of (x*x + y*y) + z*z, every operation rounded (numpy does not
+    contract into FMAs)."""
+    v = np.asarray(v, dtype=np.float64)
+    return np.sqrt((v[:, 0] * v[:, 0] + v[:, 1] * v[:, 1]) + v[:, 2] * v[:, 2])
+
+
+def adaptive_dt(acc, vel, dt, min_dt, max_pos_change, max_vel_change):
+    """Step length of the adaptive-dt rule for one system.
+
+    The reference documents the keys (README.md:54-60: min_adaptive_dt,
+    max_pos_change, max_vel_change) but its loop never uses them; its only
+    hook is particle::update(dv, dp) (src/particle.cpp:76-80), which returns
+    |a| and |v|.  The rule, as this engine defines it from those: with
+    A = max |a| (forces of the step) and V = max |v| (before the kick),
+        h = dt;  h = min(h, max_vel_change / A) if A > 0;
+        h = min(h, max_pos_change / V) if V > 0;  h = max(h, min_dt).
+    """
+    a = magnitudes(acc).max(initial=0.0)
+    v = magnitudes(vel).max(initial=0.0)
+    h = dt
+    if a > 0:
+        h = min(h, max_vel_change / a)
+    if v > 0:
+        h = min(h, max_pos_change / v)
+    return max(h, min_dt)
+
+
 # ----------------------------------------------------------------- _ref --
 _ref = {}
 
@@ -266,6 +295,18 @@ def ref_lib(prec="f64"):
                               ctypes.c_double, ctypes.c_int, ctypes.c_int]
         _ref[prec] = L
     return _ref[prec]
+
+
+def ref_update_mags(vel, acc, prec="f64"):
+    """(|a|, |v|) per particle from the reference's particle::update(dv, dp)."""
+    L = ref_lib(prec)
+    L.ref_update_mags.argtypes = [ctypes.c_size_t, _dp, _dp, _dp, _dp]
+    vel = np.ascontiguousarray(vel, dtype=np.float64)
+    acc = np.ascontiguousarray(acc, dtype=np.float64)
+    dv = np.zeros(len(vel))
+    dp = np.zeros(len(vel))
+    L.ref_update_mags(len(vel), _p(vel), _p(acc), _p(dv), _p(dp))
+    return dv, dp
 
 
 def ref_particle_raw(pos, vel, acc, mass, prec="f64"):

@@ -190,6 +190,23 @@ void ref_particle_raw(size_t n, const double* pos, const double* vel,
   }
 }
 int ref_particle_bytes(void) { return (int)sizeof(particle); }
+
+// |a| and |v| of each particle from the reference's own adaptive-dt hook,
+// particle::update(dv, dp) (src/particle.cpp:76-80): dv = |acceleration|,
+// dp = |velocity|, both by vector::magnitude.
+void ref_update_mags(size_t n, const double* vel, const double* acc, double* dv,
+                     double* dp) {
+  vector p(0, 0, 0);
+  for (size_t i = 0; i < n; i++) {
+    vector v(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2]);
+    vector a(acc[3 * i], acc[3 * i + 1], acc[3 * i + 2]);
+    particle par(&p, &v, &a, 1);
+    datatype x = 0, y = 0;
+    par.update(x, y);
+    dv[i] = x;
+    dp[i] = y;
+  }
+}
 int ref_octree_bytes(void) { return (int)sizeof(octree); }
 
 // Builds the reference tree and dumps it in pre-order (children 0..7).

@@ -204,3 +204,103 @@ def test_frame_loop_with_min_node_size(tmp_path, monkeypatch):
         f.write(base + "verbose false\n")
     assert host.run_frames("settings.cfg") == 0
     assert len(host.read_binary(host.frame_filename("./data", 3, True))[3]) == 3000
+
+
+def test_frame_files_follow_the_oracle_trajectory(tmp_path, monkeypatch):
+    """The FP64 frame files of nbs_run_frames against the oracle's frames of
+    the reference loop from the same seeded particles: positions within
+    1e-9 of the coordinate scale, velocities within 1e-9 of theirs (only
+    the summation order of the accelerations differs)."""
+    monkeypatch.chdir(tmp_path)
+    with open("settings.cfg", "w") as f:
+        f.write(SETTINGS.format(prec="f64", keep="true"))
+    assert host.run_frames("settings.cfg") == 0
+    pos, vel, mass = O.generate(3000, "cube", seed=0xDEADBEEF, min_vel=0.0, max_vel=1.0)
+    for frame in range(4):
+        opos, ovel, _ = O.run(pos, vel, mass, theta=0.5, dt=0.0333, frames=frame + 1)
+        p, v, a, m = host.read_binary(host.frame_filename("./data", frame, True))
+        assert np.abs(p - opos).max() < 1e-9 * np.abs(opos).max(), frame
+        assert np.abs(v - ovel).max() < 1e-9 * np.abs(ovel).max(), frame
+        np.testing.assert_array_equal(m, mass)
+
+
+RESUME = """use_seed true
+seed 3735928559
+num_particles 3000
+num_frames {frames}
+gen_type sphere
+min_mass 5e10
+max_mass 5e11
+theta 0.5
+dt 0.0333
+dump_binary true
+dump_text false
+dump_image false
+overwrite_data true
+keep_previous_binary true
+datatype {datatype}
+read_existing {resume}
+"""
+
+
+def test_resume_continues_an_uninterrupted_run(tmp_path, monkeypatch):
+    """Frames 0-1, then a resumed run to num_frames 5 (src/main.cpp:790-811:
+    the newest frame is read back and the loop goes on from the frame after
+    it): frames 2-4 equal an uninterrupted 5-frame run's bit for bit (80-B
+    records hold the FP64 state exactly, and caller order is kept)."""
+    for d in ("a", "b"):
+        (tmp_path / d).mkdir()
+    monkeypatch.chdir(tmp_path / "a")
+    with open("settings.cfg", "w") as f:
+        f.write(RESUME.format(frames=5, datatype="double", resume="false"))
+    assert host.run_frames("settings.cfg") == 0
+    monkeypatch.chdir(tmp_path / "b")
+    with open("settings.cfg", "w") as f:
+        f.write(RESUME.format(frames=2, datatype="double", resume="false"))
+    assert host.run_frames("settings.cfg") == 0
+    assert sorted(os.listdir("data")) == ["0000.dat", "0001.dat"]
+    with open("settings.cfg", "w") as f:
+        f.write(RESUME.format(frames=5, datatype="double", resume="true"))
+    assert host.run_frames("settings.cfg") == 0
+    for frame in range(5):
+        want = open(tmp_path / "a" / "data" / ("%04d.dat" % frame), "rb").read()
+        got = open(tmp_path / "b" / "data" / ("%04d.dat" % frame), "rb").read()
+        assert got == want, frame
+
+
+def test_float_records_write_and_resume(tmp_path, monkeypatch):
+    """datatype float (the reference's shipped -DFLOAT build, makefile:3):
+    particles generated as the FLOAT build draws them, 40-byte records
+    holding the library's FP64 state rounded to float, and a resume that
+    reads them back: its frames equal library steps taken from the rounded
+    state of the frame it resumed from."""
+    monkeypatch.chdir(tmp_path)
+    with open("settings.cfg", "w") as f:
+        f.write(RESUME.format(frames=2, datatype="float", resume="false"))
+    assert host.run_frames("settings.cfg") == 0
+    settings = host.read_settings("settings.cfg")[0]
+    assert settings["datatype_float"]
+    pos, vel, mass = host.generate(3000, gen_type="sphere", seed=0xDEADBEEF, prec="f32",
+                                   settings=settings)
+    f32 = lambda a: a.astype(np.float32).astype(np.float64)
+    with sim(theta=0.5, dt=0.0333, precision="f64") as s:
+        s.set_state(pos, vel, mass)
+        for frame in range(2):
+            s.step(1)
+            path = host.frame_filename("./data", frame, True)
+            assert os.path.getsize(path) == 40 * 3000
+            got = host.read_binary(path, prec="f32")
+            for g, w in zip(got[:3], s.state()):
+                np.testing.assert_array_equal(g, f32(w))
+            np.testing.assert_array_equal(got[3], f32(mass))
+    with open("settings.cfg", "w") as f:
+        f.write(RESUME.format(frames=4, datatype="float", resume="true"))
+    assert host.run_frames("settings.cfg") == 0
+    p1, v1, _, m1 = host.read_binary(host.frame_filename("./data", 1, True), prec="f32")
+    with sim(theta=0.5, dt=0.0333, precision="f64") as s:
+        s.set_state(p1, v1, m1)
+        for frame in (2, 3):
+            s.step(1)
+            got = host.read_binary(host.frame_filename("./data", frame, True), prec="f32")
+            for g, w in zip(got[:3], s.state()):
+                np.testing.assert_array_equal(g, f32(w))

@@ -92,3 +92,26 @@ def test_frame_loop_writes_images(tmp_path, monkeypatch):
             s.step(1)
             want = s.render_image(400, 300, projection="top", color="heat", brightness=60.0)
             np.testing.assert_array_equal(host.read_png("img/%04d.png" % frame), want)
+
+
+@pytest.mark.parametrize("color,counts", [("bw", (5120, 3072, 4096, 1)),
+                                          ("heat", (10241, 4389, 5852, 2))])
+def test_dense_pixels_non_dyadic_brightness(color, counts):
+    """Pixels holding thousands of particles at brightness 0.1, adaptive:
+    0.1's odd mantissa (about 2^52) times such a count passes 2^64, so an
+    exactness test that forms the product wraps and takes the product where
+    the reference's repeated addition differs in the last bits.  With these
+    counts that last bit moves a byte of another pixel (152 against 153 for
+    3072 particles under a 5120-particle maximum; 218 against 219 in the heat
+    map), so the bytes equal the oracle's only with the repeated addition."""
+    cols = [10 + 7 * k for k in range(len(counts))]
+    pos = np.concatenate([np.tile([c - 160 + 0.5, 0.0, 0.25], (m, 1))
+                          for c, m in zip(cols, counts)])
+    mass = np.full(len(pos), 1e10)
+    kw = dict(projection="front", color=color, adaptive=True, brightness=0.1)
+    with nbodysim_amd.Simulation(theta=0.5, dt=0.0333) as s:
+        s.set_state(pos, None, mass)
+        got = s.render_image(320, 200, **kw)
+    want = O.render_image(pos, 320, 200, **kw)
+    np.testing.assert_array_equal(got, want)
+    assert want[200 - 1 - 100, cols[1]].any()

@@ -563,3 +563,55 @@ def test_many_runs_of_equal_primary_keys():
         s.step(2)
         for a, b in zip(s.state()[:2], got[:2]):
             np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("gen", ["cube", "sphere"])
+def test_fixed_root_512(gen):
+    """NBS_ROOT_FIXED: the root is the configured cell (centre 0, side 512,
+    the workloads' root size) instead of the bbox + 2 root; tree bit-exact
+    and forces against the oracle's tree on that root."""
+    pos, vel, mass = O.generate(6000, gen, seed=21)
+    tree = O.Tree(pos, mass, center=np.zeros(3), side=512.0)
+    oacc, ocnt = tree.forces(theta=0.5)
+    with sim(theta=0.5, root_center=(0.0, 0.0, 0.0), root_size=512.0) as s:
+        s.set_state(pos, vel, mass)
+        acc = s.forces()
+        _, cnt = s.interactions(per_particle=True)
+        c, side = s.roots()
+        assert side[0] == 512.0 and (c[0] == 0).all()
+        assert_tree_equal(s.tree_dump(), tree.dump())
+        s.configure(precision="f32")
+        acc32 = s.forces()
+    np.testing.assert_array_equal(cnt, ocnt)
+    assert rel_err(acc, oacc).max() < 1e-12
+    e = rel_err(acc32, oacc)
+    assert np.sqrt(np.mean(e ** 2)) < 1e-5 and e.max() < 1e-3
+
+
+def test_long_runs_with_skipped_digits():
+    """Runs of 1100-6000 entries (the workgroup radix sort of the run-sort
+    kernel) inside 1e-6 cubes of a 1000-wide cloud: every key_hi byte, the
+    upper three key_lo bytes and the upper two caller-index bytes are equal
+    over a run, so 13 of the twenty digit passes find a single digit value
+    and are skipped.  Each wave must read the skip decision before any wave
+    starts the next pass, which clears it.  Tree bit-exact, forces within
+    1e-12 of the oracle."""
+    rng = np.random.default_rng(33)
+    parts = [rng.uniform(-500, 500, size=(2000, 3))]
+    for m in [1100, 2048, 3000, 4500, 6000, 1500, 2500, 1025]:
+        c = rng.uniform(-400, 400, size=3)
+        parts.append(c + rng.uniform(0, 1e-6, size=(m, 3)))
+    pos = np.concatenate(parts)
+    pos = pos[rng.permutation(len(pos))]
+    mass = rng.uniform(1e9, 1e10, size=len(pos))
+    vel = np.zeros_like(pos)
+    print("long runs with skipped digits: n =", len(pos), flush=True)
+    tree = O.Tree(pos, mass)
+    with sim(theta=0.5) as s:
+        s.set_state(pos, vel, mass)
+        acc = s.forces()
+        assert s.tree_info()["n_buckets"] == 0
+        assert_tree_equal(s.tree_dump(), tree.dump())
+        s.step(2)
+    oacc, _ = tree.forces(theta=0.5)
+    assert rel_err(acc, oacc).max() < 1e-12

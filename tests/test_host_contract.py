@@ -176,3 +176,37 @@ def test_frame_names_and_resume(tmp_path):
                           np.ones(2))
     assert host.resume_frame(d, 300) == 10
     assert host.resume_frame(d, 5) == 4  # searches down from num_frames
+
+
+def test_dump_plaintext_is_not_a_key(tmp_path):
+    # The README's name for dump_text: the reference's parser does not know
+    # it (src/main.cpp:1300-1302), so it is reported, its value is skipped as
+    # a stray boolean, and dump_text keeps its default (true).
+    s, msg = host.read_settings(write(tmp_path, "dump_plaintext false\n"))
+    assert "Unrecognized variable dump_plaintext" in msg
+    assert s["dump_text"]
+
+
+def test_datatype_key(tmp_path):
+    s, msg = host.read_settings(write(tmp_path, "datatype float\n"))
+    assert msg == "" and s["datatype_float"]
+    s, msg = host.read_settings(write(tmp_path, "datatype double\n"))
+    assert not s["datatype_float"]
+    assert not host.default_settings()["datatype_float"]
+    with pytest.raises(ValueError):
+        host.read_settings(write(tmp_path, "datatype half\n"))
+
+
+def test_shell_jitter_draw_is_second():
+    # shell_jitter (an extension; 0 is the reference): the radius factor is
+    # drawn right after the mass, so a jittered shell uses 8 engine words
+    # per particle (FP64) and |p| = R (1 + j u) with u the second value
+    from test_oracle_pins import _canonical_double
+    n, R, j = 40, 100.0, 0.01
+    p, v, m = host.generate(n + 1, gen_type="shell", seed=99, shell_jitter=j, r_sphere=R)
+    raw = O.minstd_raw(8 * (n + 1), seed=99).astype(np.int64)
+    for k in range(n + 1):
+        w = raw[8 * k:8 * k + 8]
+        assert m[k] == _canonical_double(w[0], w[1]) * (5e11 - 5e10) + 5e10
+        u = _canonical_double(w[2], w[3]) * 2.0 - 1.0
+        np.testing.assert_allclose(np.linalg.norm(p[k]), R * (1 + j * u), rtol=1e-14)
