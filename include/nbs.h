@@ -133,6 +133,28 @@ int nbs_snapshot_end(nbs_ctx* ctx, const double** pos, const double** vel,
  * acc (host, may be NULL to keep the result on the device). */
 int nbs_forces(nbs_ctx* ctx, double* acc);
 
+/* nbs_forces for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  One force evaluation of
+ * the n particles in the caller's DEVICE arrays pos ([n][3] FP64) and mass
+ * ([n]) with the configuration cfg (device: the current one): the same
+ * root, tree, walk and results as nbs_set_state + nbs_forces on a context.
+ * acc (device, [n][3], caller order) receives the accelerations and counts
+ * (device, [n] int64, may be NULL) each particle's interactions.  It calls
+ * no hipMalloc / hipFree: every buffer comes from one fixed library-owned
+ * device array per GPU, sized for at most NBS_NOALLOC_MAX_N particles at
+ * node_factor <= 4 (about 40 MiB, held until the process ends).  Larger n
+ * (or a tree that outgrows the fixed node capacity) is rejected with
+ * NBS_ERR_ARG / NBS_ERR_OVERFLOW and a message in msg (may be NULL).  Work
+ * is enqueued on `stream` (hipStream_t, NULL = default stream) and the call
+ * returns after it has finished.  Every call on one GPU shares that GPU's
+ * array: calls that overlap on one GPU (other threads, other streams)
+ * race and must be serialised by the caller. */
+#define NBS_NOALLOC_MAX_N 65536
+int nbs_forces_noalloc(const nbs_config* cfg, size_t n, const double* pos,
+                       const double* mass, double* acc, int64_t* counts, void* stream,
+                       char* msg, size_t msg_len);
+
 /* nsteps frames of the reference loop body (src/main.cpp:1422-1480):
  * root, octree, mass, COM, walk, kick-drift update. */
 int nbs_step(nbs_ctx* ctx, int nsteps);

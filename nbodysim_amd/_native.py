@@ -80,6 +80,10 @@ SIGNATURES = [
                                         ctypes.POINTER(c_dp), ctypes.POINTER(c_dp),
                                         ctypes.POINTER(ctypes.c_size_t)]),
     ("nbs_forces", ctypes.c_int, [ctypes.c_void_p, c_dp]),
+    ("nbs_forces_noalloc", ctypes.c_int, [ctypes.POINTER(Config), ctypes.c_size_t,
+                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_char_p,
+                                          ctypes.c_size_t]),
     ("nbs_step", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("nbs_step_graph", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("nbs_set_collisions", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]),
@@ -133,6 +137,8 @@ SIGNATURES = [
     ("nbs_write_png", ctypes.c_int, [ctypes.c_char_p, ctypes.c_uint32, ctypes.c_uint32,
                                      ctypes.POINTER(ctypes.c_uint8)]),
 ]
+
+NBS_NOALLOC_MAX_N = 65536  # include/nbs.h
 
 _lib = None
 

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -59,6 +60,13 @@ __global__ void interleave_kernel(const double* __restrict__ a,
 // slot order -> caller order, one value per particle
 __global__ void scatter_kernel(const double* __restrict__ a, const unsigned* __restrict__ id,
                                int n, double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[id[i]] = a[i];
+}
+
+// slot order -> caller order, int widened to int64
+__global__ void scatter_count_kernel(const int* __restrict__ a, const unsigned* __restrict__ id,
+                                     int n, int64_t* __restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[id[i]] = a[i];
 }
@@ -133,6 +141,26 @@ void free_all(std::vector<void*>& v) {
   v.clear();
 }
 
+// A fixed block that buffers are carved from in order, 256-byte aligned,
+// instead of allocated (the no-allocation entry point's scratch).
+struct Arena {
+  char* base = nullptr;
+  size_t size = 0, used = 0;
+};
+
+template <class T>
+T* carve(Arena& a, size_t count, hipError_t* err) {
+  size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+  bytes = (bytes + 255) & ~size_t(255);
+  if (a.used + bytes > a.size) {
+    *err = hipErrorOutOfMemory;
+    return nullptr;
+  }
+  T* p = (T*)(a.base + a.used);
+  a.used += bytes;
+  return p;
+}
+
 }  // namespace
 
 struct nbs_ctx {
@@ -141,6 +169,9 @@ struct nbs_ctx {
   std::string err;
   int device = 0;
   hipStream_t stream = nullptr;
+  // When arena.base is set, every buffer of the step is carved from it
+  // (nothing is allocated or freed): the no-allocation entry point.
+  Arena arena;
 
   int n = 0, cap_n = 0;
   int node_cap = 0;
@@ -275,6 +306,13 @@ int hip_fail(nbs_ctx* c, hipError_t e, const char* what) {
               std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// A device buffer of the context: carved from its arena when it has one,
+// else allocated (and owned by `owner`, freed by free_all).
+template <class T>
+T* cbuf(nbs_ctx* c, std::vector<void*>& owner, size_t count, hipError_t* err) {
+  return c->arena.base ? carve<T>(c->arena, count, err) : dalloc<T>(owner, count, err);
+}
+
 #define HIPCHK(c, call)                                   \
   do {                                                    \
     hipError_t e_ = (call);                               \
@@ -319,42 +357,42 @@ int alloc_particles(nbs_ctx* c, int n) {
   auto& o = c->part_bufs;
   for (int b = 0; b < 2; b++) {
     State& s = c->st[b];
-    s.x = dalloc<double>(o, cap, &e);
-    s.y = dalloc<double>(o, cap, &e);
-    s.z = dalloc<double>(o, cap, &e);
-    s.vx = dalloc<double>(o, cap, &e);
-    s.vy = dalloc<double>(o, cap, &e);
-    s.vz = dalloc<double>(o, cap, &e);
-    s.m = dalloc<double>(o, cap, &e);
-    s.id = dalloc<unsigned>(o, cap, &e);
+    s.x = cbuf<double>(c, o, cap, &e);
+    s.y = cbuf<double>(c, o, cap, &e);
+    s.z = cbuf<double>(c, o, cap, &e);
+    s.vx = cbuf<double>(c, o, cap, &e);
+    s.vy = cbuf<double>(c, o, cap, &e);
+    s.vz = cbuf<double>(c, o, cap, &e);
+    s.m = cbuf<double>(c, o, cap, &e);
+    s.id = cbuf<unsigned>(c, o, cap, &e);
   }
-  c->ax = dalloc<double>(o, cap, &e);
-  c->ay = dalloc<double>(o, cap, &e);
-  c->az = dalloc<double>(o, cap, &e);
-  c->inter = dalloc<int>(o, cap, &e);
-  c->stage = dalloc<double>(o, 3 * (size_t)cap, &e);
-  c->stage_v = dalloc<double>(o, 3 * (size_t)cap, &e);
-  c->stage_m = dalloc<double>(o, cap, &e);
-  c->slot_of_id = dalloc<int>(o, cap, &e);
+  c->ax = cbuf<double>(c, o, cap, &e);
+  c->ay = cbuf<double>(c, o, cap, &e);
+  c->az = cbuf<double>(c, o, cap, &e);
+  c->inter = cbuf<int>(c, o, cap, &e);
+  c->stage = cbuf<double>(c, o, 3 * (size_t)cap, &e);
+  c->stage_v = cbuf<double>(c, o, 3 * (size_t)cap, &e);
+  c->stage_m = cbuf<double>(c, o, cap, &e);
+  c->slot_of_id = cbuf<int>(c, o, cap, &e);
   BuildBuffers& b = c->bb;
-  b.prim0 = dalloc<unsigned long long>(o, cap, &e);
-  b.prim1 = dalloc<unsigned long long>(o, cap, &e);
-  b.val0 = dalloc<unsigned>(o, cap, &e);
-  b.val1 = dalloc<unsigned>(o, cap, &e);
-  b.khi_u = dalloc<unsigned long long>(o, cap, &e);
-  b.klo_u = dalloc<unsigned long long>(o, cap, &e);
-  b.khi = dalloc<unsigned long long>(o, cap, &e);
-  b.klo = dalloc<unsigned long long>(o, cap, &e);
-  b.counts = dalloc<int>(o, cap, &e);
-  b.close_at = dalloc<int>(o, cap, &e);
-  b.blkstats = dalloc<int>(o, 2 * (size_t)emit_blocks(cap), &e);
-  b.offsets = dalloc<int>(o, (size_t)cap + 1, &e);
-  b.sort_tmp = dalloc<int>(o, radix_tmp_ints(cap), &e);
-  b.runs = dalloc<int>(o, 2 + 2 * ((size_t)cap / 2 + 1), &e);
-  b.dhist = dalloc<int>(o, depth_hist_ints(cap), &e);
-  b.doff = dalloc<int>(o, depth_hist_ints(cap), &e);
-  b.scan_tmp = dalloc<int>(
-      o, scan_tmp_ints(std::max((size_t)cap + 1, depth_hist_ints(cap))), &e);
+  b.prim0 = cbuf<unsigned long long>(c, o, cap, &e);
+  b.prim1 = cbuf<unsigned long long>(c, o, cap, &e);
+  b.val0 = cbuf<unsigned>(c, o, cap, &e);
+  b.val1 = cbuf<unsigned>(c, o, cap, &e);
+  b.khi_u = cbuf<unsigned long long>(c, o, cap, &e);
+  b.klo_u = cbuf<unsigned long long>(c, o, cap, &e);
+  b.khi = cbuf<unsigned long long>(c, o, cap, &e);
+  b.klo = cbuf<unsigned long long>(c, o, cap, &e);
+  b.counts = cbuf<int>(c, o, cap, &e);
+  b.close_at = cbuf<int>(c, o, cap, &e);
+  b.blkstats = cbuf<int>(c, o, 2 * (size_t)emit_blocks(cap), &e);
+  b.offsets = cbuf<int>(c, o, (size_t)cap + 1, &e);
+  b.sort_tmp = cbuf<int>(c, o, radix_tmp_ints(cap), &e);
+  b.runs = cbuf<int>(c, o, 2 + 2 * ((size_t)cap / 2 + 1), &e);
+  b.dhist = cbuf<int>(c, o, depth_hist_ints(cap), &e);
+  b.doff = cbuf<int>(c, o, depth_hist_ints(cap), &e);
+  b.scan_tmp = cbuf<int>(
+      c, o, scan_tmp_ints(std::max((size_t)cap + 1, depth_hist_ints(cap))), &e);
   if (e != hipSuccess) {
     free_all(c->part_bufs);
     c->cap_n = 0;
@@ -371,9 +409,9 @@ int alloc_nodes(nbs_ctx* c, int cap) {
   auto& o = c->node_bufs;
   // one record beyond the capacity: the hand-written FP32 walk requests the
   // record after a system's last node (unused) without a bounds test
-  c->nodes = dalloc<WalkNode>(o, (size_t)cap + 1, &e);
-  c->aux.parent = dalloc<int>(o, cap, &e);
-  c->bb.bucket = dalloc<int>(o, cap, &e);
+  c->nodes = cbuf<WalkNode>(c, o, (size_t)cap + 1, &e);
+  c->aux.parent = cbuf<int>(c, o, cap, &e);
+  c->bb.bucket = cbuf<int>(c, o, cap, &e);
   if (e != hipSuccess) {
     free_all(c->node_bufs);
     c->node_cap = 0;
@@ -392,13 +430,13 @@ int setup_systems(nbs_ctx* c) {
   auto& o = c->sys_bufs;
   Systems& s = c->sys;
   s.nsys = nsys;
-  s.start = dalloc<int>(o, nsys + 1, &e);
-  s.wave_start = dalloc<int>(o, nsys + 1, &e);
-  s.wave2_start = dalloc<int>(o, nsys + 1, &e);
-  s.node_start = dalloc<int>(o, nsys + 1, &e);
-  s.root = dalloc<double>(o, 4 * (size_t)nsys, &e);
-  s.dt = dalloc<double>(o, nsys, &e);
-  s.time = dalloc<double>(o, nsys, &e);
+  s.start = cbuf<int>(c, o, nsys + 1, &e);
+  s.wave_start = cbuf<int>(c, o, nsys + 1, &e);
+  s.wave2_start = cbuf<int>(c, o, nsys + 1, &e);
+  s.node_start = cbuf<int>(c, o, nsys + 1, &e);
+  s.root = cbuf<double>(c, o, 4 * (size_t)nsys, &e);
+  s.dt = cbuf<double>(c, o, nsys, &e);
+  s.time = cbuf<double>(c, o, nsys, &e);
   if (e != hipSuccess) return hip_fail(c, e, "hipMalloc(system buffers)");
   c->h_start.assign(nsys + 1, 0);
   c->h_wave_start.assign(nsys + 1, 0);
@@ -415,12 +453,12 @@ int setup_systems(nbs_ctx* c) {
   s.n_waves = c->h_wave_start[nsys];
   s.n_waves2 = c->h_wave2_start[nsys];
   c->prim_levels = max_prim_levels(nsys);
-  s.bbox = dalloc<double>(o, 6 * (size_t)std::max(s.n_waves, 1), &e);
-  s.amax = dalloc<double>(o, std::max(s.n_waves, 1), &e);
-  s.vmax = dalloc<double>(o, std::max(s.n_waves, 1), &e);
-  c->inter_total = dalloc<unsigned long long>(o, std::max(s.n_waves, 1), &e);
-  c->tile_cost = dalloc<unsigned>(o, std::max(s.n_waves, 1), &e);
-  c->tile_order = dalloc<int>(o, std::max(s.n_waves, 1), &e);
+  s.bbox = cbuf<double>(c, o, 6 * (size_t)std::max(s.n_waves, 1), &e);
+  s.amax = cbuf<double>(c, o, std::max(s.n_waves, 1), &e);
+  s.vmax = cbuf<double>(c, o, std::max(s.n_waves, 1), &e);
+  c->inter_total = cbuf<unsigned long long>(c, o, std::max(s.n_waves, 1), &e);
+  c->tile_cost = cbuf<unsigned>(c, o, std::max(s.n_waves, 1), &e);
+  c->tile_order = cbuf<int>(c, o, std::max(s.n_waves, 1), &e);
   c->cost_tiles = -1;
   if (e != hipSuccess) return hip_fail(c, e, "hipMalloc(system buffers)");
   launch_fill_u64(c->inter_total, 0, std::max(s.n_waves, 1), c->stream);
@@ -1351,6 +1389,136 @@ int nbs_forces(nbs_ctx* c, double* acc) {
   int r = run(c, 0);
   if (r) return r;
   if (acc) return nbs_get_state(c, nullptr, nullptr, acc);
+  return NBS_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------- no-allocation forces --
+// The device memory of nbs_forces_noalloc: every buffer of one force
+// evaluation (state, keys, sort, nodes, per-wave partials, flags) is carved
+// from this array, NBS_NOALLOC_MAX_N particles at node_factor 4 fit (about
+// 580 B per particle plus fixed parts).  Each device holds its own copy
+// until the process ends; all calls on one device share it.
+namespace {
+constexpr size_t kScratchBytes = 40ull << 20;
+constexpr int kMaxDevices = 64;
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices];  // the array's device address, per device
+
+// The address of `scratch` on the current device, looked up on the first
+// call there (the array's name in host code is only a host-side stand-in).
+hipError_t scratch_on_device(int dev, void** out) {
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!ptr) return hipErrorInvalidSymbol;
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return hipSuccess;
+}
+
+int noalloc_fail(char* msg, size_t msg_len, int code, const std::string& text) {
+  if (msg && msg_len) snprintf(msg, msg_len, "%s", text.c_str());
+  return code;
+}
+}  // namespace
+
+extern "C" {
+
+int nbs_forces_noalloc(const nbs_config* cfg, size_t n, const double* pos,
+                       const double* mass, double* acc, int64_t* counts, void* stream,
+                       char* msg, size_t msg_len) {
+  if (msg && msg_len) msg[0] = 0;
+  if (!cfg) return noalloc_fail(msg, msg_len, NBS_ERR_ARG, "cfg is required");
+  if (n > (size_t)NBS_NOALLOC_MAX_N)
+    return noalloc_fail(msg, msg_len, NBS_ERR_ARG,
+                        "n = " + std::to_string(n) + " exceeds the no-allocation maximum of " +
+                            std::to_string(NBS_NOALLOC_MAX_N) + " particles");
+  if (n == 0) return NBS_OK;
+  if (!pos || !mass || !acc)
+    return noalloc_fail(msg, msg_len, NBS_ERR_ARG, "pos, mass and acc are required");
+  nbs_ctx c;
+  int r = check_config(&c, cfg);
+  if (r) return noalloc_fail(msg, msg_len, r, c.err);
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  void* base = nullptr;
+  if (e == hipSuccess) e = scratch_on_device(dev, &base);
+  if (e != hipSuccess)
+    return noalloc_fail(msg, msg_len, NBS_ERR_HIP,
+                        std::string("scratch address: ") + hipGetErrorString(e));
+  c.cfg = *cfg;
+  c.cfg.device = dev;
+  c.p = params_of(c.cfg);
+  c.device = dev;
+  c.stream = (hipStream_t)stream;
+  c.arena.base = (char*)base;
+  c.arena.size = kScratchBytes;
+  // the flags and counters a previous call on this device left behind are
+  // cleared below (reset_flags, the stats fill, setup_systems' counters)
+  e = hipSuccess;
+  c.flags = carve<int>(c.arena, 1, &e);
+  c.steps_done = carve<int>(c.arena, 1, &e);
+  c.stats = carve<int>(c.arena, 2, &e);
+  c.bb.flags = c.flags;
+  c.bb.stats = c.stats;
+  if (e != hipSuccess) return noalloc_fail(msg, msg_len, NBS_ERR_NOMEM, "scratch exhausted");
+  launch_fill_int(c.stats, 0, 2, c.stream);
+  r = alloc_particles(&c, (int)n);
+  if (r == NBS_OK) {
+    c.n = (int)n;
+    c.cur = 0;
+    deinterleave_kernel<<<(int)((n + 255) / 256), 256, 0, c.stream>>>(
+        pos, nullptr, mass, (int)n, c.st[0], 0);
+    launch_fill_int(c.inter, 0, n, c.stream);
+    c.nsys_req = 1;
+    r = setup_systems(&c);
+  }
+  if (r == NBS_OK) r = ensure_nodes(&c);
+  if (r != NBS_OK) {
+    if (r == NBS_ERR_NOMEM)
+      return noalloc_fail(msg, msg_len, NBS_ERR_ARG,
+                          "the no-allocation scratch (" + std::to_string(kScratchBytes) +
+                              " bytes) cannot hold " + std::to_string(n) +
+                              " particles at node_factor " +
+                              std::to_string(cfg->node_factor > 0 ? cfg->node_factor : 4));
+    return noalloc_fail(msg, msg_len, r, c.err);
+  }
+  c.has_state = true;
+  reset_flags(&c);
+  enqueue_build(&c);
+  enqueue_walk(&c, 0);
+  int flags = 0, done = 0;
+  r = read_flags(&c, &flags, &done);
+  if (r) return noalloc_fail(msg, msg_len, r, c.err);
+  if (flags) {
+    r = fatal_flags(&c, flags);
+    if (r) return noalloc_fail(msg, msg_len, r, c.err);
+    return noalloc_fail(msg, msg_len, NBS_ERR_OVERFLOW,
+                        "tree needs more than the " + std::to_string(c.node_cap) +
+                            " nodes the no-allocation scratch holds");
+  }
+  const State& st = c.st[c.cur];
+  interleave_kernel<<<(int)((n + 255) / 256), 256, 0, c.stream>>>(c.ax, c.ay, c.az, st.id,
+                                                                   (int)n, acc);
+  if (counts)
+    scatter_count_kernel<<<(int)((n + 255) / 256), 256, 0, c.stream>>>(c.inter, st.id,
+                                                                       (int)n, counts);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+  if (e != hipSuccess)
+    return noalloc_fail(msg, msg_len, NBS_ERR_HIP,
+                        std::string("nbs_forces_noalloc: ") + hipGetErrorString(e));
   return NBS_OK;
 }
 

@@ -73,3 +73,19 @@ def test_missing_library_fails_loudly(monkeypatch):
     monkeypatch.setattr(_native, "LIB_PATH", "/nonexistent/libnbs.so")
     with pytest.raises(ImportError):
         _native.lib()
+
+
+def test_noalloc_rejects_sizes_beyond_its_maximum_without_touching_the_gpu():
+    # nbs_forces_noalloc checks the size before any HIP call
+    L = nbodysim_amd.lib()
+    cfg = _native.Config()
+    L.nbs_config_default(ctypes.byref(cfg))
+    msg = ctypes.create_string_buffer(256)
+    rc = L.nbs_forces_noalloc(ctypes.byref(cfg), _native.NBS_NOALLOC_MAX_N + 1, 8, 8, 8,
+                              None, None, msg, len(msg))
+    assert rc == _native.NBS_ERR_ARG
+    assert b"exceeds the no-allocation maximum of 65536" in msg.value
+    assert L.nbs_forces_noalloc(None, 10, 8, 8, 8, None, None, msg, len(msg)) == \
+        _native.NBS_ERR_ARG
+    assert L.nbs_forces_noalloc(ctypes.byref(cfg), 0, None, None, None, None, None, None,
+                                0) == _native.NBS_OK
