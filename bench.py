@@ -56,16 +56,22 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def time_path(sim, pos, vel, mass, precision, steps, warmup, prof_steps, shape, graph,
-              order=False):
+def time_path(sim, pos, vel, mass, precision, steps, warmup, shape, graph, order=False,
+              theta=None):
     """Timed region: `steps` graph-replayed steps (nbs_step_graph; eager
     launches with graph=False) after `warmup` steps.  Before it (untimed)
     the walk's launch shape is tuned on the initial state (nbs_autotune,
-    which restores the state) unless `shape` fixes it.  Kernel
-    timing for the roofline: `prof_steps` eager steps right after, with HIP
-    events around each phase on the context's stream, and the interactions
-    those walks counted."""
-    sim.configure(precision=precision)
+    which restores the state) unless `shape` fixes it.  Then the same
+    window again: the state at the window's start is set back and the same
+    `steps` steps run with eager launches and HIP events around each phase
+    on the context's stream (kernel times for the phases and the roofline,
+    and the interactions those walks counted).  The trajectory depends only
+    on positions and caller order, so both runs take the same steps; the
+    end states are compared bit for bit."""
+    kw = {"precision": precision}
+    if theta is not None:
+        kw["theta"] = theta
+    sim.configure(**kw)
     sim.set_state(pos, vel, mass)
     sim.set_walk_order(order)
     if shape is None:
@@ -74,6 +80,7 @@ def time_path(sim, pos, vel, mass, precision, steps, warmup, prof_steps, shape, 
         sim.set_walk_shape(*shape)
     if warmup:
         sim.step(warmup, graph=graph)
+    start_pos, start_vel, _ = sim.state()
     sim.synchronize()
     t0 = time.perf_counter()
     sim.step(steps, graph=graph)
@@ -81,30 +88,98 @@ def time_path(sim, pos, vel, mass, precision, steps, warmup, prof_steps, shape, 
     t1 = time.perf_counter()
     info = sim.tree_info()
     end_pos, end_vel, _ = sim.state()
+    # the same window, eager and profiled
+    sim.set_state(start_pos, start_vel, mass)
     sim.interactions_total(reset=True)
     sim.profile(True)
     sim.profile_read(reset=True)
+    sim.synchronize()
     t2 = time.perf_counter()
-    sim.step(prof_steps)
+    sim.step(steps)
     sim.synchronize()
     t3 = time.perf_counter()
     prof = sim.profile_read(reset=True)
     sim.profile(False)
     inter = sim.interactions_total(reset=True)
+    replay_pos, replay_vel, _ = sim.state()
     return {"ms_per_step": (t1 - t0) * 1e3 / steps, "prof": prof, "interactions": inter,
-            "prof_steps": prof_steps, "eager_ms_per_step": (t3 - t2) * 1e3 / prof_steps,
+            "prof_steps": steps, "eager_ms_per_step": (t3 - t2) * 1e3 / steps,
+            "same_window_bit_identical": bool(np.array_equal(replay_pos, end_pos) and
+                                              np.array_equal(replay_vel, end_vel)),
             "tree": info, "walk_shape": list(shape), "end_state": (end_pos, end_vel)}
 
 
-def cpu_baseline(pos, mass, theta, sample, threads):
+def accuracy_gated(sim, pos, vel, mass, ids, ref, rms, args, shape, graph):
+    """The largest theta whose FP32-force and FP64 walks both stay within
+    the rms error gate against the FP64 direct sum at the initial state
+    (a descending scan, then bisection to 0.0025), and both paths' speed
+    there: the reference's algorithm with the MAC tightened until it meets
+    the accuracy the metric is quoted at."""
+    def err(theta):
+        out = []
+        for prec in ("f32", "f64"):
+            sim.configure(theta=theta, precision=prec)
+            sim.set_state(pos, vel, mass)
+            out.append(rms(sim.forces()[ids], ref))
+        return out
+    scan = []
+    hi, lo = None, None  # hi fails the gate, lo meets it
+    for theta in (0.3, 0.2, 0.15, 0.12, 0.1, 0.08, 0.06, 0.04):
+        e = err(theta)
+        scan.append({"theta": theta, "e_total_f32": e[0], "e_total_f64": e[1]})
+        if max(e) <= args.gate:
+            lo = (theta, e)
+            break
+        hi = theta
+    if lo is None:
+        return {"gate": args.gate, "scan": scan, "theta": None}
+    while hi is not None and hi - lo[0] > 0.0025:
+        mid = round((hi + lo[0]) / 2, 5)
+        e = err(mid)
+        scan.append({"theta": mid, "e_total_f32": e[0], "e_total_f64": e[1]})
+        if max(e) <= args.gate:
+            lo = (mid, e)
+        else:
+            hi = mid
+    theta = lo[0]
+    res = {"gate": args.gate, "theta": theta, "e_total_f32": lo[1][0], "e_total_f64": lo[1][1],
+           "steps": args.gated_steps, "scan": scan}
+    n = len(mass)
+    for prec in ("f32", "f64"):
+        r = time_path(sim, pos, vel, mass, prec, args.gated_steps, args.warmup, shape, graph,
+                      theta=theta)
+        walk_ms = r["prof"]["walk_ms"] / max(r["prof"]["walk_launches"], 1)
+        res[prec] = {"value": n * 1e3 / r["ms_per_step"], "ms_per_step": r["ms_per_step"],
+                     "walk_ms": walk_ms,
+                     "interactions_per_particle": r["interactions"] / r["prof_steps"] / n,
+                     "walk_tflops": FLOP_PER_INTERACTION * r["interactions"] / r["prof_steps"] /
+                     (walk_ms * 1e-3) / 1e12}
+    res["speedup_fp32_over_fp64"] = res["f32"]["value"] / res["f64"]["value"]
+    sim.configure(theta=0.5)
+    return res
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(pos, mass, theta, sample, threads, prec="f64"):
     """The oracle's step (gen_root + octree + mass/COM, then the BFS walk)
     on the host cores, at the particle state the GPU's timed region ended
-    in, so both sides do the same interactions.  With sample < N the walk
-    covers a seeded subset and is extrapolated to N."""
+    in, so both sides do the same interactions.  prec "f32" is the
+    reference's FLOAT build (every value and operation in float).  With
+    sample < N the walk covers a seeded subset and is extrapolated to N."""
     import oracle as O
     n = len(pos)
     t0 = time.perf_counter()
-    tree = O.Tree(pos, mass)           # gen_root + add_particle + mass + COM
+    tree = O.Tree(pos, mass, prec=prec)  # gen_root + add_particle + mass + COM
     t1 = time.perf_counter()
     if sample >= n:
         _, cnt = tree.forces(theta=theta, threads=threads)
@@ -118,10 +193,11 @@ def cpu_baseline(pos, mass, theta, sample, threads):
     t_step = (t1 - t0) + (t2 - t1) * n / walked
     return {
         "value": n / t_step, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-        "sample": "oracle (C++ restatement of the reference step, FP64) at the state the "
+        "precision": prec, "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
+        "sample": "oracle (C++ restatement of the reference step, %s) at the state the "
                   "timed GPU steps ended in: tree build + mass/COM of all %d particles "
                   "(%.2f s, serial like gen_root) and the BFS walk of %d particles on %d "
-                  "threads (%.2f s)%s" % (n, t1 - t0, walked, threads, t2 - t1,
+                  "threads (%.2f s)%s" % (prec.upper(), n, t1 - t0, walked, threads, t2 - t1,
                                           "" if walked == n else ", extrapolated to N"),
         "interactions_per_particle": float(cnt.mean()),
         "seconds": t2 - t0,
@@ -605,12 +681,19 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=1 << 20)
-    ap.add_argument("--fp64-steps", type=int, default=20)
-    ap.add_argument("--prof-steps", type=int, default=10)
+    ap.add_argument("--fp64-steps", type=int, default=None,
+                    help="timed steps of the all-FP64 path (default: --steps)")
+    ap.add_argument("--gate", type=float, default=1e-4,
+                    help="rms relative error bound of the accuracy-gated line")
+    ap.add_argument("--gated-steps", type=int, default=10,
+                    help="timed steps of each path at the accuracy-gated theta")
+    ap.add_argument("--no-gated", action="store_true")
     ap.add_argument("--samples", type=int, default=4096)
     ap.add_argument("--cpu-sample", type=int, default=0,
                     help="particles the CPU baseline walks (0 = all)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-fp32", type=int, default=65536,
+                    help="particles the FP32 (FLOAT build) CPU baseline walks (0 = none)")
     ap.add_argument("--walk-shape", default="64,8,1", metavar="BLOCK,GROUPS,PACK",
                     help="walk launch shape (default: the one nbs_autotune picks on this "
                          "workload, so the committed counter profiles apply)")
@@ -644,6 +727,8 @@ def main():
     ap.add_argument("--every5", type=int, default=20)
     ap.add_argument("--fd-steps5", type=int, default=10)
     args = ap.parse_args()
+    if args.fp64_steps is None:
+        args.fp64_steps = args.steps
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
@@ -693,12 +778,16 @@ def main():
     # ---- timed paths
     shape = None if args.autotune else tuple(int(v) for v in args.walk_shape.split(","))
     graph = not args.eager
-    r32 = time_path(sim, pos, vel, mass, "f32", args.steps, args.warmup, args.prof_steps,
-                    shape, graph, args.walk_order)
+    r32 = time_path(sim, pos, vel, mass, "f32", args.steps, args.warmup, shape, graph,
+                    args.walk_order)
     log("f32", {k: v for k, v in r32.items() if k != "end_state"})
-    r64 = time_path(sim, pos, vel, mass, "f64", args.fp64_steps, args.warmup, args.prof_steps,
-                    shape, graph, args.walk_order)
+    r64 = time_path(sim, pos, vel, mass, "f64", args.fp64_steps, args.warmup, shape, graph,
+                    args.walk_order)
     log("f64", {k: v for k, v in r64.items() if k != "end_state"})
+    gated = None
+    if not args.no_gated:
+        gated = accuracy_gated(sim, pos, vel, mass, ids, ref, rms, args, shape, graph)
+        log("gated", gated)
     sim.close()
 
     value = n * 1e3 / r32["ms_per_step"]
@@ -741,7 +830,12 @@ def main():
         "accuracy": acc,
         "phases_ms_per_step": {"build": build_ms, "walk": walk_ms,
                                "update": r32["prof"]["update_ms"] / r32["prof_steps"],
-                               "eager_step": r32["eager_ms_per_step"]},
+                               "eager_step": r32["eager_ms_per_step"],
+                               "graph_step": r32["ms_per_step"],
+                               "window": "the %d timed steps, replayed from the same start "
+                                         "state with eager launches and HIP events per phase "
+                                         "(end state bit-identical: %s)"
+                                         % (args.steps, r32["same_window_bit_identical"])},
         "tree_build": {"ms": build_ms, "algorithmic_bytes": build_bytes,
                        "gbps": build_bytes / (build_ms * 1e-3) / 1e9,
                        "frac_hbm": build_bytes / (build_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
@@ -754,11 +848,19 @@ def main():
                      "flop_per_interaction": FLOP_PER_INTERACTION,
                      "avg_launch_ms": walk_ms},
     }
+    if gated is not None:
+        out["accuracy_gated"] = gated
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        # the host cores this process may use, at most 16: one GPU's share of
+        # the box's cores
+        threads = max(1, min(16, len(os.sched_getaffinity(0))))
         end_pos = r32["end_state"][0]
         out["cpu_baseline"] = cpu_baseline(end_pos, mass, theta, args.cpu_sample or n, threads)
         log("cpu", out["cpu_baseline"])
+        if args.cpu_fp32:
+            fp32 = cpu_baseline(end_pos, mass, theta, args.cpu_fp32, threads, prec="f32")
+            out["cpu_baseline"]["fp32"] = fp32
+            log("cpu fp32", fp32)
     print(json.dumps(out), flush=True)
 
 
