@@ -65,7 +65,16 @@ typedef struct nbs_config {
   double max_vel_change;
   int device;            /* HIP device ordinal */
   int node_factor;       /* node capacity = node_factor * n + 4096 (0: 4) */
-  int reserved[6];
+  int threaded_tree;     /* 1: the tree of the reference's threads > 1 path
+                            (gen_root_threaded, src/main.cpp:817-886): all
+                            eight root children exist (an empty one is a
+                            cell of mass 0 at the origin) and a particle on
+                            the root's centre planes has a leaf in every
+                            child whose closed cell holds it (inside(),
+                            src/octree.cpp:320-336).  One system, bbox
+                            root; no min_node_size, adjoint or graph
+                            capture (nbs_step_graph runs eager steps). */
+  int reserved[5];
 } nbs_config;
 
 /* Tree summary of the last build. */
@@ -267,7 +276,8 @@ int nbs_time(nbs_ctx* ctx, double* t, int nsys_cap);
 int nbs_tree_info_get(nbs_ctx* ctx, nbs_tree_info* info);
 
 /* Test hook: the last tree in pre-order, children 0..7, every system after
- * the other.  Arrays have n_nodes entries (order: n_particles):
+ * the other.  Arrays have n_nodes entries (order: one per leaf,
+ * n_nodes - n_internal; n_particles except with the threaded tree):
  *   depth, leaf (caller index of a leaf's particle, -1 internal,
  *   -2 bucket), first / count (position range in `order`), side, mass,
  *   com [n_nodes][3], key_hi / key_lo (node path, 3 bits per level,

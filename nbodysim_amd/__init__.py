@@ -45,13 +45,14 @@ class Simulation:
 
     def __init__(self, theta=0.5, dt=0.0333, damping=False, precision="f64",
                  root_center=None, root_size=None, adaptive=False, min_adaptive_dt=0.00333,
-                 max_pos_change=3.0, max_vel_change=3.0, device=0, node_factor=4):
+                 max_pos_change=3.0, max_vel_change=3.0, device=0, node_factor=4,
+                 threaded_tree=False):
         self._L = lib()
         self.cfg = Config()
         self._L.nbs_config_default(ctypes.byref(self.cfg))
         self._fill_config(theta, dt, damping, precision, root_center, root_size, adaptive,
                           min_adaptive_dt, max_pos_change, max_vel_change, device,
-                          node_factor)
+                          node_factor, threaded_tree)
         h = ctypes.c_void_p()
         rc = self._L.nbs_create(ctypes.byref(self.cfg), ctypes.byref(h))
         if rc != 0:
@@ -61,7 +62,8 @@ class Simulation:
         self._collide = False
 
     def _fill_config(self, theta, dt, damping, precision, root_center, root_size, adaptive,
-                     min_adaptive_dt, max_pos_change, max_vel_change, device, node_factor):
+                     min_adaptive_dt, max_pos_change, max_vel_change, device, node_factor,
+                     threaded_tree=False):
         c = self.cfg
         c.theta = theta
         c.dt = dt
@@ -80,6 +82,7 @@ class Simulation:
         c.max_vel_change = max_vel_change
         c.device = device
         c.node_factor = node_factor
+        c.threaded_tree = int(bool(threaded_tree))
 
     def configure(self, **kw):
         """Changes theta / dt / precision / ... of this context."""
@@ -90,7 +93,7 @@ class Simulation:
                     root_size=c.root_size, adaptive=bool(c.adaptive),
                     min_adaptive_dt=c.min_adaptive_dt, max_pos_change=c.max_pos_change,
                     max_vel_change=c.max_vel_change, device=c.device,
-                    node_factor=c.node_factor)
+                    node_factor=c.node_factor, threaded_tree=bool(c.threaded_tree))
         args.update(kw)
         self._fill_config(**args)
         self._check(self._L.nbs_configure(self._h, ctypes.byref(self.cfg)))
@@ -336,7 +339,7 @@ class Simulation:
 
     def tree_dump(self):
         info = self.tree_info()
-        m, n = info["n_nodes"], self.n
+        m, n = info["n_nodes"], info["n_nodes"] - info["n_internal"]  # n: leaves
         d = dict(depth=np.zeros(m, np.int32), leaf=np.zeros(m, np.int32),
                  first=np.zeros(m, np.int64), count=np.zeros(m, np.int64),
                  side=np.zeros(m), mass=np.zeros(m), com=np.zeros((m, 3)),

@@ -27,7 +27,8 @@ class Config(ctypes.Structure):
                 ("root_center", ctypes.c_double * 3), ("root_size", ctypes.c_double),
                 ("min_adaptive_dt", ctypes.c_double), ("max_pos_change", ctypes.c_double),
                 ("max_vel_change", ctypes.c_double), ("device", ctypes.c_int),
-                ("node_factor", ctypes.c_int), ("reserved", ctypes.c_int * 6)]
+                ("node_factor", ctypes.c_int), ("threaded_tree", ctypes.c_int),
+                ("reserved", ctypes.c_int * 5)]
 
 
 class ImageParams(ctypes.Structure):

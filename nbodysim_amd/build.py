@@ -29,13 +29,14 @@ CXX = os.environ.get("CXX", "g++")
 CXX_FLAGS = ["-std=c++17", "-O2", "-ffp-contract=off", "-fPIC", "-Wall",
              "-I" + os.path.join(ROOT, "include")]
 
-HIP_SOURCES = ["scan.hip", "radix_sort.hip", "tree_build.hip", "walk.hip",
+HIP_SOURCES = ["scan.hip", "radix_sort.hip", "tree_build.hip", "tree_threaded.hip", "walk.hip",
                "integrate.hip", "adjoint.hip", "ic_device.hip", "collide.hip",
                "image.hip", "api.hip"]
 # Device initial conditions repeat the host generator's roundings: no
 # contraction into FMAs anywhere in that file (the device default would fuse
 # a multiply with the add that follows it, even across inlined helpers).
 FILE_FLAGS = {"ic_device.hip": ["-ffp-contract=off"], "collide.hip": ["-ffp-contract=off"],
+              "tree_threaded.hip": ["-ffp-contract=off"],
               "image.hip": ["-ffp-contract=off"]}
 HOST_SOURCES = ["host/settings.cpp", "host/ic.cpp", "host/dump.cpp", "host/frames.cpp",
                 "host/png.cpp"]

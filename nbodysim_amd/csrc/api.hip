@@ -175,7 +175,8 @@ struct nbs_ctx {
 
   int n = 0, cap_n = 0;
   int node_cap = 0;
-  std::vector<void*> part_bufs, node_bufs, sys_bufs, misc_bufs;
+  int build_cap = 0;  // keys the build buffers hold
+  std::vector<void*> part_bufs, build_bufs, node_bufs, sys_bufs, misc_bufs;
 
   State st[2];
   int cur = 0;
@@ -283,6 +284,21 @@ struct nbs_ctx {
   int col_info_cap = 0;
   int64_t col_pairs = 0, col_removed = 0;
 
+  // The tree of the reference's threads > 1 path (cfg.threaded_tree):
+  // per-particle inside() masks and new slots (thr_cap_n particles),
+  // per-entry data (thr_cap_e entries), the last build's entry count, and
+  // an error of the build's host part (reported after the step).
+  std::vector<void*> thr_bufs, thr_ebufs;
+  int thr_cap_n = 0, thr_cap_e = 0, n_entries = 0;
+  int *t_mask = nullptr, *t_cnt = nullptr, *t_outside = nullptr, *t_eoff = nullptr;
+  int *t_orank = nullptr, *t_new_slot = nullptr, *t_occ = nullptr;
+  unsigned *t_gperm = nullptr, *t_eid = nullptr;
+  int *t_epart = nullptr, *t_sp = nullptr, *t_first = nullptr, *t_pslot = nullptr;
+  int* t_estart = nullptr;
+  double *t_ex = nullptr, *t_ey = nullptr, *t_ez = nullptr, *t_em = nullptr;
+  int thr_err = 0;
+  std::string thr_msg;
+
   // Frame images (nbs_render_image): per-pixel counts and values, the
   // running maximum and the RGB rows, for img_pixels pixels.
   std::vector<void*> img_bufs;
@@ -344,10 +360,22 @@ int check_config(nbs_ctx* c, const nbs_config* cfg) {
     return fail(c, NBS_ERR_ARG, "root_mode must be NBS_ROOT_BBOX/FIXED");
   if (cfg->root_mode == NBS_ROOT_FIXED && !(cfg->root_size > 0))
     return fail(c, NBS_ERR_ARG, "root_size must be > 0 for NBS_ROOT_FIXED");
+  if (cfg->threaded_tree != 0 && cfg->threaded_tree != 1)
+    return fail(c, NBS_ERR_ARG, "threaded_tree must be 0 or 1");
+  if (cfg->threaded_tree && cfg->root_mode != NBS_ROOT_BBOX)
+    return fail(c, NBS_ERR_ARG, "the threaded tree has the bbox + 2 root (NBS_ROOT_BBOX)");
   return NBS_OK;
 }
 
+// What the threaded-root tree does not support (one system; no
+// minimum-size merge, adjoint or frozen replay).
+int check_threaded(nbs_ctx* c, const char* what) {
+  if (!c->cfg.threaded_tree) return NBS_OK;
+  return fail(c, NBS_ERR_STATE, std::string(what) + " is not supported with the threaded tree");
+}
+
 void drop_graph(nbs_ctx* c);
+int alloc_build(nbs_ctx* c, int cap);
 
 int alloc_particles(nbs_ctx* c, int n) {
   if (n <= c->cap_n && c->cap_n > 0) return NBS_OK;
@@ -374,6 +402,22 @@ int alloc_particles(nbs_ctx* c, int n) {
   c->stage_v = cbuf<double>(c, o, 3 * (size_t)cap, &e);
   c->stage_m = cbuf<double>(c, o, cap, &e);
   c->slot_of_id = cbuf<int>(c, o, cap, &e);
+  if (e != hipSuccess) {
+    free_all(c->part_bufs);
+    c->cap_n = 0;
+    return hip_fail(c, e, "hipMalloc(particle buffers)");
+  }
+  c->cap_n = cap;
+  return alloc_build(c, cap);
+}
+
+// The build's buffers (keys, sort, emission) for `cap` keys: particles, or
+// tree entries with the threaded root.
+int alloc_build(nbs_ctx* c, int cap) {
+  free_all(c->build_bufs);
+  c->build_cap = 0;
+  hipError_t e = hipSuccess;
+  auto& o = c->build_bufs;
   BuildBuffers& b = c->bb;
   b.prim0 = cbuf<unsigned long long>(c, o, cap, &e);
   b.prim1 = cbuf<unsigned long long>(c, o, cap, &e);
@@ -394,11 +438,11 @@ int alloc_particles(nbs_ctx* c, int n) {
   b.scan_tmp = cbuf<int>(
       c, o, scan_tmp_ints(std::max((size_t)cap + 1, depth_hist_ints(cap))), &e);
   if (e != hipSuccess) {
-    free_all(c->part_bufs);
-    c->cap_n = 0;
-    return hip_fail(c, e, "hipMalloc(particle buffers)");
+    free_all(c->build_bufs);
+    return hip_fail(c, e, "hipMalloc(build buffers)");
   }
-  c->cap_n = cap;
+  c->build_cap = cap;
+  c->cb.scan_tmp = b.scan_tmp;  // the collision kernels borrow it
   return NBS_OK;
 }
 
@@ -501,8 +545,150 @@ int ensure_nodes(nbs_ctx* c) {
   return alloc_nodes(c, want);
 }
 
+// Buffers of the threaded-root build for the current particle count and
+// `ne` entries (entries grow with the particles on the root's centre
+// planes; the build buffers then hold entries, not particles).
+int ensure_threaded(nbs_ctx* c, int ne) {
+  hipError_t e = hipSuccess;
+  const int n = std::max(c->n, 1);
+  if (c->thr_cap_n < n) {  // per particle (filled before the entry count is known)
+    free_all(c->thr_bufs);
+    c->thr_cap_n = 0;
+    const int cn = std::max(n, c->cap_n);
+    auto& o = c->thr_bufs;
+    c->t_mask = dalloc<int>(o, cn, &e);
+    c->t_cnt = dalloc<int>(o, cn, &e);
+    c->t_outside = dalloc<int>(o, cn, &e);
+    c->t_eoff = dalloc<int>(o, (size_t)cn + 1, &e);
+    c->t_orank = dalloc<int>(o, (size_t)cn + 1, &e);
+    c->t_new_slot = dalloc<int>(o, cn, &e);
+    c->t_gperm = dalloc<unsigned>(o, cn, &e);
+    c->t_occ = dalloc<int>(o, member_blocks(cn), &e);
+    c->t_estart = dalloc<int>(o, 2, &e);
+    if (e != hipSuccess) {
+      free_all(c->thr_bufs);
+      return hip_fail(c, e, "hipMalloc(threaded-root buffers)");
+    }
+    c->thr_cap_n = cn;
+  }
+  if (c->thr_cap_e < ne) {  // per entry
+    free_all(c->thr_ebufs);
+    c->thr_cap_e = 0;
+    const int ce = std::max(ne + ne / 4, c->thr_cap_n + 64);
+    auto& o = c->thr_ebufs;
+    c->t_eid = dalloc<unsigned>(o, ce, &e);
+    c->t_epart = dalloc<int>(o, ce, &e);
+    c->t_sp = dalloc<int>(o, ce, &e);
+    c->t_first = dalloc<int>(o, ce, &e);
+    c->t_pslot = dalloc<int>(o, (size_t)ce + 1, &e);
+    c->t_ex = dalloc<double>(o, ce, &e);
+    c->t_ey = dalloc<double>(o, ce, &e);
+    c->t_ez = dalloc<double>(o, ce, &e);
+    c->t_em = dalloc<double>(o, ce, &e);
+    if (e != hipSuccess) {
+      free_all(c->thr_ebufs);
+      return hip_fail(c, e, "hipMalloc(threaded-root entry buffers)");
+    }
+    c->thr_cap_e = ce;
+  }
+  if (c->build_cap < ne) return alloc_build(c, c->thr_cap_e);
+  return NBS_OK;
+}
+
+// The tree gen_root_threaded builds (threads > 1, one system): eight
+// eager root children, each filled with the particles its inclusive
+// inside() accepts, built over (particle, child) entries (see
+// tree_threaded.hip).  The entry count is read back from the device
+// before the keys are formed.  A failure of the host part is kept in
+// thr_err and reported by the caller after the step; the step's kernels
+// then do nothing (the flags word is set).
+void enqueue_build_threaded(nbs_ctx* c, bool keep_root) {
+  hipStream_t s = c->stream;
+  State& cur = c->st[c->cur];
+  State& nxt = c->st[1 - c->cur];
+  const int n = c->n;
+  auto bail = [&](int code, const std::string& msg) {
+    if (!c->thr_err) {
+      c->thr_err = code;
+      c->thr_msg = msg;
+    }
+    launch_fill_int(c->flags, kFlagStack, 1, s);
+  };
+  if (!keep_root) {
+    if (!c->bbox_valid) {
+      launch_bbox(cur, c->sys, s);
+      c->bbox_valid = true;
+    }
+    launch_root(c->sys, c->p, c->flags, s);
+  }
+  int r = ensure_threaded(c, 0);
+  if (r) return bail(r, c->err);
+  launch_member(cur, n, c->sys.root, c->t_mask, c->t_cnt, c->t_outside, c->t_occ, s);
+  exclusive_scan(c->t_cnt, c->t_eoff, n, c->bb.scan_tmp, s);
+  int e_real = 0;
+  std::vector<int> occ(member_blocks(n));
+  if (hipMemcpyAsync(&e_real, c->t_eoff + n, sizeof(int), hipMemcpyDeviceToHost, s) !=
+          hipSuccess ||
+      hipMemcpyAsync(occ.data(), c->t_occ, occ.size() * sizeof(int), hipMemcpyDeviceToHost,
+                     s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return bail(NBS_ERR_HIP, "threaded root: entry count read-back failed");
+  int mask = 0;
+  for (int v : occ) mask |= v;
+  int empty[8] = {0, 0, 0, 0, 0, 0, 0, 0}, n_empty = 0;
+  for (int o = 0; o < 8; o++)
+    if (!((mask >> o) & 1)) empty[n_empty++] = o;
+  const int ne = e_real + n_empty;
+  r = ensure_threaded(c, ne);
+  if (r) return bail(r, c->err);
+  BuildBuffers& b = c->bb;
+  const int levels = std::min(c->prim_levels, max_prim_levels(1));
+  launch_entry_keys(cur, n, c->sys.root, c->t_mask, c->t_eoff, e_real, empty, n_empty,
+                    levels, b, c->t_epart, c->t_eid, s);
+  const int which = radix_sort_pairs(b.prim0, b.val0, b.prim1, b.val1, ne, 3 * levels,
+                                     b.sort_tmp, s);
+  unsigned long long* prim = which ? b.prim1 : b.prim0;
+  unsigned* perm = which ? b.val1 : b.val0;
+  State by_entry = cur;
+  by_entry.id = c->t_eid;  // runs of equal keys: caller index of each entry
+  launch_fixup(b, prim, perm, which ? b.val0 : b.val1, by_entry, ne, s);
+  launch_entry_gather(perm, c->t_epart, c->t_eoff, cur, ne, c->t_ex, c->t_ey, c->t_ez,
+                      c->t_em, c->t_sp, c->t_first, s);
+  exclusive_scan(c->t_first, c->t_pslot, ne, b.scan_tmp, s);
+  exclusive_scan(c->t_outside, c->t_orank, n, b.scan_tmp, s);
+  launch_entry_slots(c->t_sp, c->t_first, c->t_pslot, c->t_cnt, c->t_orank, ne, n,
+                     c->t_new_slot, c->t_gperm, s);
+  launch_permute(cur, nxt, c->t_gperm, n, s);
+  c->cur ^= 1;
+  launch_fill_int(c->t_estart, 0, 1, s);
+  launch_fill_int(c->t_estart + 1, ne, 1, s);
+  Systems es = c->sys;
+  es.nsys = 1;
+  es.start = c->t_estart;
+  State leaves{};
+  leaves.x = c->t_ex;
+  leaves.y = c->t_ey;
+  leaves.z = c->t_ez;
+  leaves.m = c->t_em;
+  TreeAux aux = c->aux;
+  if (!c->want_parent) aux.parent = nullptr;
+  launch_emit(b, es, c->p, leaves, c->nodes, aux, ne, c->node_cap, s);
+  launch_entry_patch(b.offsets, c->t_sp, c->t_new_slot, ne, c->p.theta, n_empty, c->nodes,
+                     b.stats, c->flags, s);
+  launch_bottom_up(b, c->nodes, aux, ne, 1, s);
+  c->n_entries = ne;
+}
+
 // Enqueues one tree build of the current state.
 void enqueue_build(nbs_ctx* c, bool keep_root = false) {
+  if (c->cfg.threaded_tree) {
+    hipEvent_t e0 = nullptr;
+    prof_begin(c, kPhaseBuild, &e0);
+    enqueue_build_threaded(c, keep_root);
+    prof_end(c, kPhaseBuild, e0);
+    return;
+  }
+  c->n_entries = c->n;
   hipStream_t s = c->stream;
   hipEvent_t e0 = nullptr;
   prof_begin(c, kPhaseBuild, &e0);
@@ -643,6 +829,11 @@ void reset_flags(nbs_ctx* c) {
 // only a corrupt tree can cause.  Node and pair overflows return NBS_OK
 // here: the caller grows that buffer and redoes the step.
 int fatal_flags(nbs_ctx* c, int flags) {
+  if (c->thr_err) {  // the threaded-root build's host part failed
+    const int e = c->thr_err;
+    c->thr_err = 0;
+    return fail(c, e, c->thr_msg);
+  }
   if (flags & kFlagStack)
     return fail(c, NBS_ERR_OVERFLOW,
                 "tree walk stack overflow (a tree deeper than its 43 levels: corrupt build)");
@@ -737,6 +928,10 @@ int shrink_systems(nbs_ctx* c, const std::vector<int>& info) {
 // redone with larger buffers.
 int run_collide(nbs_ctx* c, int nsteps) {
   if (!c->has_state) return fail(c, NBS_ERR_STATE, "no particle state set");
+  if (c->min_size > 0 || c->sys.nsys > 1) {
+    const int r = check_threaded(c, c->min_size > 0 ? "min_node_size" : "batched systems");
+    if (r) return r;
+  }
   if (c->p.adaptive)
     return fail(c, NBS_ERR_STATE, "merging particles needs adaptive dt off");
   for (int k = 0; k < nsteps && c->n > 0; k++) {
@@ -875,6 +1070,10 @@ int run_collide(nbs_ctx* c, int nsteps) {
 int run(nbs_ctx* c, int nsteps) {
   if (!c->has_state) return fail(c, NBS_ERR_STATE, "no particle state set");
   if (c->n == 0) return NBS_OK;
+  if (c->sys.nsys > 1) {
+    const int r = check_threaded(c, "batched systems");
+    if (r) return r;
+  }
   int r = ensure_nodes(c);
   if (r) return r;
   const bool forces_only = nsteps == 0;
@@ -1024,6 +1223,9 @@ void nbs_destroy(nbs_ctx* c) {
       (void)hipEventDestroy(p.second);
     }
   free_all(c->part_bufs);
+  free_all(c->build_bufs);
+  free_all(c->thr_bufs);
+  free_all(c->thr_ebufs);
   free_all(c->node_bufs);
   free_all(c->sys_bufs);
   free_all(c->misc_bufs);
@@ -1450,6 +1652,8 @@ int nbs_forces_noalloc(const nbs_config* cfg, size_t n, const double* pos,
   nbs_ctx c;
   int r = check_config(&c, cfg);
   if (r) return noalloc_fail(msg, msg_len, r, c.err);
+  if (cfg->threaded_tree)
+    return noalloc_fail(msg, msg_len, NBS_ERR_ARG, "the threaded tree allocates per build");
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
   void* base = nullptr;
@@ -1539,6 +1743,8 @@ int nbs_step_graph(nbs_ctx* c, int nsteps) {
     return fail(c, NBS_ERR_STATE, "merging particles runs eager steps (nbs_step)");
   if (nsteps == 0 || c->n == 0) return NBS_OK;
   HIPCHK(c, hipSetDevice(c->device));
+  // the threaded-root build reads its entry count back: eager steps
+  if (c->cfg.threaded_tree) return run(c, nsteps);
   int r = ensure_nodes(c);
   if (r) return r;
   reset_flags(c);
@@ -1859,7 +2065,7 @@ int nbs_tree_info_get(nbs_ctx* c, nbs_tree_info* info) {
   if (!c || !info) return NBS_ERR_ARG;
   if (!c->tree_valid) return fail(c, NBS_ERR_STATE, "no tree built");
   int s[2];
-  const int nb = emit_blocks(c->n);
+  const int nb = emit_blocks(c->n_entries);
   std::vector<int> bs(2 * (size_t)nb);
   HIPCHK(c, hipMemcpyAsync(s, c->stats, sizeof(s), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipMemcpyAsync(bs.data(), c->bb.blkstats, bs.size() * sizeof(int),
@@ -1885,47 +2091,43 @@ int nbs_tree_dump(nbs_ctx* c, int32_t* depth, int32_t* leaf, int64_t* first,
   nbs_tree_info info;
   int r = nbs_tree_info_get(c, &info);
   if (r) return r;
-  const int T = (int)info.n_nodes, n = c->n;
+  const int T = (int)info.n_nodes, n = c->n, ne = c->n_entries;
   std::vector<WalkNode> nd(T);
-  std::vector<int> dep(T), fst(T), cnt(T);
+  std::vector<int> dep(T), lpos(T + 1), offs(ne + 1);
   std::vector<unsigned> id(n);
-  std::vector<unsigned long long> khi(n), klo(n);
+  std::vector<unsigned long long> khi(ne), klo(ne);
   hipStream_t s = c->stream;
   HIPCHK(c, hipMemcpyAsync(nd.data(), c->nodes, T * sizeof(WalkNode), hipMemcpyDeviceToHost, s));
   HIPCHK(c, hipMemcpyAsync(id.data(), c->st[c->cur].id, n * sizeof(unsigned),
                            hipMemcpyDeviceToHost, s));
-  HIPCHK(c, hipMemcpyAsync(khi.data(), c->bb.khi, n * sizeof(unsigned long long),
+  HIPCHK(c, hipMemcpyAsync(khi.data(), c->bb.khi, ne * sizeof(unsigned long long),
                            hipMemcpyDeviceToHost, s));
-  HIPCHK(c, hipMemcpyAsync(klo.data(), c->bb.klo, n * sizeof(unsigned long long),
+  HIPCHK(c, hipMemcpyAsync(klo.data(), c->bb.klo, ne * sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipMemcpyAsync(offs.data(), c->bb.offsets, (ne + 1) * sizeof(int),
                            hipMemcpyDeviceToHost, s));
   HIPCHK(c, hipStreamSynchronize(s));
+  // leaves before each node in pre-order (a leaf's position in the DFS leaf
+  // order; with the threaded root a particle can own several leaves),
+  // depth = the cells enclosing a node in pre-order
+  lpos[0] = 0;
+  for (int k = 0; k < T; k++) lpos[k + 1] = lpos[k] + (nd[k].leaf >= 0 ? 1 : 0);
   {
-    // first slot and particle count from the leaves (a leaf's `leaf` is
-    // its particle slot; a cell spans the slots up to its skip node's
-    // first), depth = the cells enclosing a node in pre-order
-    std::vector<int> nxt(T + 1);
-    nxt[T] = n;
-    for (int k = T - 1; k >= 0; k--) nxt[k] = nd[k].leaf >= 0 ? nd[k].leaf : nxt[k + 1];
     std::vector<int> ends;
     for (int k = 0; k < T; k++) {
       while (!ends.empty() && ends.back() <= k) ends.pop_back();
       dep[k] = (int)ends.size();
-      fst[k] = nxt[k];
-      if (nd[k].leaf >= 0) {
-        cnt[k] = 1;
-      } else {
-        const int e = std::min(std::max(nd[k].skip, k + 1), T);
-        cnt[k] = nxt[e] - nxt[k];
-        ends.push_back(e);
-      }
+      if (nd[k].leaf < 0) ends.push_back(std::min(std::max(nd[k].skip, k + 1), T));
     }
   }
   for (int k = 0; k < T; k++) {
     const int d = dep[k];
+    const bool is_leaf = nd[k].leaf >= 0;
+    const int e = is_leaf ? k + 1 : std::min(std::max(nd[k].skip, k + 1), T);
     if (depth) depth[k] = d;
-    if (leaf) leaf[k] = nd[k].leaf >= 0 ? (int32_t)id[nd[k].leaf] : nd[k].leaf;
-    if (first) first[k] = fst[k];
-    if (count) count[k] = cnt[k];
+    if (leaf) leaf[k] = is_leaf ? (int32_t)id[nd[k].leaf] : nd[k].leaf;
+    if (first) first[k] = lpos[k];
+    if (count) count[k] = lpos[e] - lpos[k];
     if (side) side[k] = nd[k].side;
     if (mass) mass[k] = nd[k].m;
     if (com) {
@@ -1933,26 +2135,34 @@ int nbs_tree_dump(nbs_ctx* c, int32_t* depth, int32_t* leaf, int64_t* first,
       com[3 * k + 1] = nd[k].cy;
       com[3 * k + 2] = nd[k].cz;
     }
-    const int dd = std::min(d, kLevels);
-    const unsigned long long h = khi[fst[k]], l = klo[fst[k]];
-    unsigned long long mh, ml;
-    if (dd <= 21) {
-      mh = dd == 0 ? 0 : (h & (~0ull << (63 - 3 * dd)));
-      ml = 0;
-    } else {
-      mh = h;
-      ml = l & (~0ull << (63 - 3 * (dd - 21)));
+    if (key_hi || key_lo) {
+      // the sorted key whose nodes include this one: its path's prefix
+      const int j = (int)(std::upper_bound(offs.begin(), offs.end(), k) - offs.begin()) - 1;
+      const int dd = std::min(d, kLevels);
+      const unsigned long long h = khi[std::max(0, std::min(j, ne - 1))];
+      const unsigned long long l = klo[std::max(0, std::min(j, ne - 1))];
+      unsigned long long mh, ml;
+      if (dd <= 21) {
+        mh = dd == 0 ? 0 : (h & (~0ull << (63 - 3 * dd)));
+        ml = 0;
+      } else {
+        mh = h;
+        ml = l & (~0ull << (63 - 3 * (dd - 21)));
+      }
+      if (key_hi) key_hi[k] = mh;
+      if (key_lo) key_lo[k] = ml;
     }
-    if (key_hi) key_hi[k] = mh;
-    if (key_lo) key_lo[k] = ml;
+    if (order && is_leaf) order[lpos[k]] = (int32_t)id[nd[k].leaf];
   }
-  if (order)
-    for (int i = 0; i < n; i++) order[i] = (int32_t)id[i];
   return NBS_OK;
 }
 
 int nbs_keys(nbs_ctx* c, uint64_t* key_hi, uint64_t* key_lo) {
   if (!c) return NBS_ERR_ARG;
+  {
+    const int r = check_threaded(c, "per-particle keys (nbs_keys)");
+    if (r) return r;
+  }
   if (!c->tree_valid) return fail(c, NBS_ERR_STATE, "no tree built");
   const int n = c->n;
   std::vector<unsigned> id(n);
@@ -2137,6 +2347,10 @@ extern "C" {
 int nbs_adjoint_forward(nbs_ctx* c, int nsteps, int every, int record_all) {
   if (c) c->want_parent = true;  // the backward pass propagates along parents
   if (!c) return NBS_ERR_ARG;
+  {
+    const int r = check_threaded(c, "the adjoint");
+    if (r) return r;
+  }
   if (!c->has_state) return fail(c, NBS_ERR_STATE, "no particle state set");
   if (nsteps < 1 || every < 1) return fail(c, NBS_ERR_ARG, "nsteps and every must be >= 1");
   if (c->p.adaptive) return fail(c, NBS_ERR_ARG, "the adjoint needs a fixed dt");
@@ -2193,6 +2407,10 @@ int nbs_adjoint_forward(nbs_ctx* c, int nsteps, int every, int record_all) {
 int nbs_adjoint_backward(nbs_ctx* c, const double* gx_T, const double* gv_T,
                          double* gx0, double* gv0, double* gm) {
   if (!c) return NBS_ERR_ARG;
+  {
+    const int r = check_threaded(c, "the adjoint");
+    if (r) return r;
+  }
   if (!c->adj_steps || c->adj_n != c->n)
     return fail(c, NBS_ERR_STATE, "run nbs_adjoint_forward first");
   HIPCHK(c, hipSetDevice(c->device));
@@ -2240,6 +2458,10 @@ int nbs_adjoint_backward(nbs_ctx* c, const double* gx_T, const double* gv_T,
 int nbs_frozen_forward(nbs_ctx* c, const double* pos0, const double* vel0,
                        const double* mass, double* posT, double* velT) {
   if (!c || !pos0 || !mass) return NBS_ERR_ARG;
+  {
+    const int r = check_threaded(c, "the adjoint");
+    if (r) return r;
+  }
   if (!c->adj_steps || !c->adj_record || c->adj_n != c->n)
     return fail(c, NBS_ERR_STATE, "run nbs_adjoint_forward with record_all first");
   if (c->cfg.force_precision != NBS_FORCE_F64)

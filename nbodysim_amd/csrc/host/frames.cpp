@@ -139,6 +139,13 @@ extern "C" int nbs_run_frames(const char* cfgfile) {
   cfg.min_adaptive_dt = s.min_adaptive_dt;
   cfg.max_pos_change = s.max_pos_change;
   cfg.max_vel_change = s.max_vel_change;
+  // threads > 1 selects gen_root_threaded's tree (src/main.cpp:1423); its
+  // minimum-size merge (apply_minimum_size_threaded) is not reproduced, so
+  // with min_node_size in effect the serial tree is built
+  const bool min_size_on = s.min_node_size > 0 && s.verbose;
+  cfg.threaded_tree = s.threads > 1 && !min_size_on;
+  if (s.threads > 1 && min_size_on)
+    fprintf(stderr, "min_node_size with threads > 1: building the serial tree\n");
   nbs_ctx* ctx = nullptr;
   if (nbs_create(&cfg, &ctx) != NBS_OK) {
     fprintf(stderr, "nbs_create failed\n");

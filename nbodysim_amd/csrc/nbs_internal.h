@@ -152,6 +152,26 @@ void launch_bottom_up(BuildBuffers& b, WalkNode* nodes, TreeAux& aux, int n,
 void launch_leaf_refresh(BuildBuffers& b, const State& st, WalkNode* nodes,
                          int n, hipStream_t s);
 
+// -------------------------------------------------- threaded-root tree --
+// The tree of the reference's threads > 1 path (tree_threaded.hip): the
+// build over (particle, child) entries.
+int member_blocks(int n);
+void launch_member(const State& st, int n, const double* root, int* mask, int* cnt,
+                   int* outside, int* occ_part, hipStream_t s);
+void launch_entry_keys(const State& st, int n, const double* root, const int* mask,
+                       const int* eoff, int e_real, const int* empty, int n_empty,
+                       int prim_levels, BuildBuffers& b, int* epart, unsigned* eid,
+                       hipStream_t s);
+void launch_entry_gather(const unsigned* perm, const int* epart, const int* eoff,
+                         const State& cur, int ne, double* ex, double* ey, double* ez,
+                         double* em, int* sp, int* first, hipStream_t s);
+void launch_entry_slots(const int* sp, const int* first, const int* pslot, const int* cnt,
+                        const int* orank, int ne, int n, int* new_slot, unsigned* gperm,
+                        hipStream_t s);
+void launch_entry_patch(const int* offsets, const int* sp, const int* new_slot, int ne,
+                        double theta, int n_empty, WalkNode* nodes, int* stats,
+                        const int* flags, hipStream_t s);
+
 // ----------------------------------------------------------------- walk --
 struct WalkArgs {
   const WalkNode* nodes;

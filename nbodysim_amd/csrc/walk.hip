@@ -284,11 +284,12 @@ __device__ __forceinline__ void interact(const NodeRegs& nd, double dx, double d
       "s_max_i32 %[n], %[n], s54\n\t" \
       NBS_WALK_TAIL(FORCE, "s[46:47]") \
       "L_opened%=:\n\t" \
-      /* I = A & ~O interact and wait at skip; O descend (A = O) */ \
+      /* I = A & ~O interact and wait at skip; O descend (A = O); with */ \
+      /* no lane in I the force is skipped */ \
       "s_add_i32 %[n], %[n], 1\n\t" \
       "s_andn2_b64 s[44:45], s[46:47], s[42:43]\n\t" \
       "s_mov_b64 s[46:47], s[42:43]\n\t" \
-      "s_cbranch_scc0 L_int%=\n\t" \
+      "s_cbranch_scc0 L_noint%=\n\t" \
       "s_cmp_eq_u32 %[tpos], s54\n\t" \
       "s_cbranch_scc1 L_merge%=\n\t" \
       "s_cmp_gt_u32 %[top], 62\n\t" \
@@ -314,8 +315,14 @@ __device__ __forceinline__ void interact(const NodeRegs& nd, double dx, double d
       "s_branch L_int%=\n" \
       "L_bucket%=:\n\t" \
       /* every lane opens a bucket; nobody interacts */ \
-      "s_add_i32 %[n], %[n], 1\n\t" \
-      "s_mov_b64 s[44:45], 0\n" \
+      "s_add_i32 %[n], %[n], 1\n" \
+      "L_noint%=:\n\t" \
+      "s_lshl_b32 %[t], %[n], 6\n\t" \
+      "s_load_dwordx16 s[48:63], %[base], %[t]\n" \
+      "L_next%=:\n\t" \
+      "s_cmp_lt_u32 %[n], %[lim]\n\t" \
+      "s_cbranch_scc1 L_loop%=\n\t" \
+      "s_branch L_slow%=\n" \
       "L_int%=:\n\t" \
       NBS_WALK_TAIL(FORCE, "s[44:45]") \
       "L_leaf%=:\n\t" \
@@ -335,6 +342,9 @@ __device__ __forceinline__ void interact(const NodeRegs& nd, double dx, double d
       "s_add_i32 %[n], %[n], 1\n\t" \
       "s_lshl_b32 %[t], %[n], 6\n\t" \
       "s_load_dwordx16 s[48:63], %[base], %[t]\n\t" \
+      /* only its own particle's lane takes the leaf: no force */ \
+      "s_cmp_eq_u64 s[44:45], 0\n\t" \
+      "s_cbranch_scc1 L_next%=\n\t" \
       "v_cvt_f32_f64 %[fx], %[dx]\n\t" \
       "v_cvt_f32_f64 %[fy], %[dy]\n\t" \
       "v_cvt_f32_f64 %[fz], %[dz]\n\t" \
@@ -505,7 +515,7 @@ __device__ __forceinline__ int walk_loop_f64(ConstNodes nodes, int& n, int nend,
       "s_add_i32 %[n], %[n], 1\n\t"
       "s_andn2_b64 s[44:45], s[46:47], s[42:43]\n\t"
       "s_mov_b64 s[46:47], s[42:43]\n\t"
-      "s_cbranch_scc0 L_int%=\n\t"
+      "s_cbranch_scc0 L_noint%=\n\t"
       "s_cmp_eq_u32 %[tpos], s54\n\t"
       "s_cbranch_scc1 L_merge%=\n\t"
       "s_cmp_gt_u32 %[top], 62\n\t"
@@ -530,8 +540,15 @@ __device__ __forceinline__ int walk_loop_f64(ConstNodes nodes, int& n, int nend,
       "s_mov_b32 %[ovf], 1\n\t"
       "s_branch L_int%=\n"
       "L_bucket%=:\n\t"
-      "s_add_i32 %[n], %[n], 1\n\t"
-      "s_mov_b64 s[44:45], 0\n"
+      "s_add_i32 %[n], %[n], 1\n"
+      // no lane interacts: the next record, no force
+      "L_noint%=:\n\t"
+      "s_lshl_b32 %[t], %[n], 6\n\t"
+      "s_load_dwordx16 s[48:63], %[base], %[t]\n"
+      "L_next%=:\n\t"
+      "s_cmp_lt_u32 %[n], %[lim]\n\t"
+      "s_cbranch_scc1 L_loop%=\n\t"
+      "s_branch L_slow%=\n"
       "L_int%=:\n\t"
       NBS_WALK64_TAIL("s[44:45]")
       "L_leaf%=:\n\t"
@@ -547,6 +564,8 @@ __device__ __forceinline__ int walk_loop_f64(ConstNodes nodes, int& n, int nend,
       "s_add_i32 %[n], %[n], 1\n\t"
       "s_lshl_b32 %[t], %[n], 6\n\t"
       "s_load_dwordx16 s[48:63], %[base], %[t]\n\t"
+      "s_cmp_eq_u64 s[44:45], 0\n\t"
+      "s_cbranch_scc1 L_next%=\n\t"
       NBS_WALK64_D2
       NBS_WALK64_FORCE("s[44:45]")
       "L_tie%=:\n\t"

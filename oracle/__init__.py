@@ -76,7 +76,9 @@ def lib():
         L.orc_run.restype = ctypes.c_int
         L.orc_run.argtypes = [ctypes.c_int, ctypes.c_size_t, _dp, _dp, _dp, _dp,
                               ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
-                              ctypes.c_int]
+                              ctypes.c_int, ctypes.c_int]
+        L.orc_tree_leaves.restype = ctypes.c_size_t
+        L.orc_tree_leaves.argtypes = [ctypes.c_void_p]
         L.orc_run_collide.restype = ctypes.c_int64
         L.orc_run_collide.argtypes = [ctypes.c_size_t, _dp, _dp, _dp, _dp, ctypes.c_double,
                                       ctypes.c_double, ctypes.c_int, ctypes.c_double,
@@ -147,12 +149,21 @@ def keys(pos, center, side, prec="f64"):
 
 
 class Tree:
-    """The reference's one-particle-per-leaf octree with mass and COM."""
+    """The reference's one-particle-per-leaf octree with mass and COM.
 
-    def __init__(self, pos, mass, prec="f64", center=None, side=None):
+    threaded=True: the tree gen_root_threaded builds when threads > 1
+    (src/main.cpp:817-886): eight eager root children filled by the
+    inclusive inside() test (src/octree.cpp:320-336), so a particle on the
+    root's centre planes has a leaf in every child holding it."""
+
+    def __init__(self, pos, mass, prec="f64", center=None, side=None, threaded=False):
         self.pos = np.ascontiguousarray(pos, dtype=np.float64)
         self.mass = np.ascontiguousarray(mass, dtype=np.float64)
         mode = 0 if center is None else 1
+        if threaded:
+            if center is not None:
+                raise ValueError("the threaded tree has the bbox + 2 root")
+            mode = 2
         c = np.ascontiguousarray(center if center is not None else np.zeros(3),
                                  dtype=np.float64)
         h = lib().orc_tree_build(prec_code(prec), len(self.pos), _p(self.pos),
@@ -172,7 +183,7 @@ class Tree:
     def dump(self):
         """Pre-order node list (children 0..7) as a dict of arrays."""
         m = lib().orc_tree_size(self.h)
-        n = len(self.pos)
+        n = lib().orc_tree_leaves(self.h)  # leaves in DFS order
         d = dict(depth=np.zeros(m, np.int32), khi=np.zeros(m, np.uint64),
                  klo=np.zeros(m, np.uint64), leaf=np.zeros(m, np.int32),
                  first=np.zeros(m, np.int64), count=np.zeros(m, np.int64),
@@ -201,14 +212,15 @@ class Tree:
 
 
 def run(pos, vel, mass, theta=0.5, dt=0.0333, frames=1, damping=False, prec="f64",
-        threads=1):
-    """`frames` serial frames of the reference loop; returns (pos, vel, acc)."""
+        threads=1, threaded=False):
+    """`frames` frames of the reference loop; returns (pos, vel, acc).
+    threaded=True builds gen_root_threaded's tree (threads > 1)."""
     pos = np.array(pos, dtype=np.float64, order="C")
     vel = np.array(vel, dtype=np.float64, order="C")
     mass = np.ascontiguousarray(mass, dtype=np.float64)
     acc = np.zeros_like(pos)
     rc = lib().orc_run(prec_code(prec), len(pos), _p(pos), _p(vel), _p(acc), _p(mass),
-                       theta, dt, int(damping), frames, threads)
+                       theta, dt, int(damping), frames, threads, int(threaded))
     if rc:
         raise ValueError("orc_run failed (coincident particles)")
     return pos, vel, acc
@@ -324,7 +336,14 @@ def ref_particle_raw(pos, vel, acc, mass, prec="f64"):
     return out.tobytes()
 
 
-def ref_tree_dump(pos, mass, prec="f64"):
+def ref_set_threaded(on, prec="f64"):
+    """The _ref harness builds gen_root_threaded's tree (threads > 1)."""
+    L = ref_lib(prec)
+    L.ref_set_threaded.argtypes = [ctypes.c_int]
+    L.ref_set_threaded(int(on))
+
+
+def ref_tree_dump(pos, mass, prec="f64", threaded=False):
     L = ref_lib(prec)
     pos = np.ascontiguousarray(pos, dtype=np.float64)
     mass = np.ascontiguousarray(mass, dtype=np.float64)
@@ -333,18 +352,31 @@ def ref_tree_dump(pos, mass, prec="f64"):
     d = dict(depth=np.zeros(cap, np.int32), leaf=np.zeros(cap, np.int32),
              first=np.zeros(cap, np.int64), count=np.zeros(cap, np.int64),
              side=np.zeros(cap), mass=np.zeros(cap), com=np.zeros((cap, 3)),
-             order=np.zeros(n, np.int32))
-    m = L.ref_tree_dump(n, _p(pos), _p(mass), cap, _p(d["depth"], _ip), _p(d["leaf"], _ip),
-                        _p(d["first"], _lp), _p(d["count"], _lp), _p(d["side"]),
-                        _p(d["mass"]), _p(d["com"]), _p(d["order"], _ip))
+             order=np.zeros(cap, np.int32))
+    ref_set_threaded(threaded, prec)
+    try:
+        m = L.ref_tree_dump(n, _p(pos), _p(mass), cap, _p(d["depth"], _ip),
+                            _p(d["leaf"], _ip), _p(d["first"], _lp), _p(d["count"], _lp),
+                            _p(d["side"]), _p(d["mass"]), _p(d["com"]), _p(d["order"], _ip))
+    finally:
+        ref_set_threaded(False, prec)
     assert m <= cap
     for k in list(d):
         if k != "order":
             d[k] = d[k][:m].copy()
+    d["order"] = d["order"][:int((d["leaf"] >= 0).sum())].copy()
     return d
 
 
-def ref_forces(pos, mass, theta=0.5, damping=False, prec="f64"):
+def ref_forces(pos, mass, theta=0.5, damping=False, prec="f64", threaded=False):
+    ref_set_threaded(threaded, prec)
+    try:
+        return _ref_forces(pos, mass, theta, damping, prec)
+    finally:
+        ref_set_threaded(False, prec)
+
+
+def _ref_forces(pos, mass, theta=0.5, damping=False, prec="f64"):
     L = ref_lib(prec)
     pos = np.ascontiguousarray(pos, dtype=np.float64)
     mass = np.ascontiguousarray(mass, dtype=np.float64)
@@ -354,13 +386,19 @@ def ref_forces(pos, mass, theta=0.5, damping=False, prec="f64"):
     return acc, cnt
 
 
-def ref_run(pos, vel, mass, theta=0.5, dt=0.0333, frames=1, damping=False, prec="f64"):
+def ref_run(pos, vel, mass, theta=0.5, dt=0.0333, frames=1, damping=False, prec="f64",
+            threaded=False):
     L = ref_lib(prec)
     pos = np.array(pos, dtype=np.float64, order="C")
     vel = np.array(vel, dtype=np.float64, order="C")
     mass = np.ascontiguousarray(mass, dtype=np.float64)
     acc = np.zeros_like(pos)
-    L.ref_run(len(pos), _p(pos), _p(vel), _p(acc), _p(mass), theta, dt, int(damping), frames)
+    ref_set_threaded(threaded, prec)
+    try:
+        L.ref_run(len(pos), _p(pos), _p(vel), _p(acc), _p(mass), theta, dt, int(damping),
+                  frames)
+    finally:
+        ref_set_threaded(False, prec)
     return pos, vel, acc
 
 

@@ -12,6 +12,9 @@
 //   * root from bbox+2 ..... src/main.cpp:887-927 (gen_root)
 //   * one-per-leaf octree .. src/octree.cpp:54-85 (allocate_child),
 //                            :134-196 (add_particle)
+//   * threaded root ........ src/main.cpp:817-886 (gen_root_threaded),
+//                            src/octree.cpp:230-336 (calc_*_threaded,
+//                            inside)
 //   * mass / COM ........... src/octree.cpp:203-221, :252-278
 //   * BFS walk + gravity ... src/main.cpp:306-333, :335-421
 //   * kick-drift update .... src/particle.cpp:66-74
@@ -22,7 +25,9 @@
 // Parity pins (what checks each function; see tests/):
 //   * tree, mass, COM, update: bit-exact against oracle/_ref (the reference's
 //     own octree.cpp/particle.cpp/vector.cpp compiled here) and against the
-//     committed golden fixtures made from it.
+//     committed golden fixtures made from it; the threaded root's tree
+//     against the reference's own inside(), calc_mass_threaded() and
+//     calc_com_threaded() driven by the _ref harness.
 //   * gravity / walk: bit-exact against the _ref harness walk; theta = 0
 //     reduces to the textbook direct sum; two-body closed form; lattice
 //     mirror symmetry.
@@ -233,9 +238,73 @@ struct Tree {
     for (size_t i = 0; i < n && !overflow; i++) add(0, (int)i);
   }
 
+  // src/octree.cpp:320-336 (inside): inclusive bounds, the offset formed
+  // in the datatype and compared with side / 2.0 and (-1.0 * side) / 2.0
+  // in double.
+  bool inside(int n, const V3<T>& p) const {
+    const Node& nd = nodes[n];
+    const double hi = nd.side / 2.0, lo = (-1.0 * nd.side) / 2.0;
+    const T dx = p.x - nd.center.x, dy = p.y - nd.center.y, dz = p.z - nd.center.z;
+    if ((double)dx > hi || (double)dx < lo) return false;
+    if ((double)dy > hi || (double)dy < lo) return false;
+    if ((double)dz > hi || (double)dz < lo) return false;
+    return true;
+  }
+
+  // src/main.cpp:817-886 (gen_root_threaded / gen_root_thread): the root
+  // allocates all eight children first, then child i receives, in set
+  // order, every particle inside() it -- a particle on the root's centre
+  // planes goes into every child whose closed cell holds it, one outside
+  // every child into none.
+  void build_threaded(const V3<T>* p, const T* m, size_t n, const V3<T>& origin, T size) {
+    pos = p;
+    mass = m;
+    nodes.clear();
+    nodes.reserve(2 * n + 16);
+    make(origin, size, 0, 0, 0);
+    for (int i = 0; i < 8; i++) alloc_child(0, i);
+    for (int i = 0; i < 8 && !overflow; i++) {
+      const int c = nodes[0].child[i];
+      for (size_t j = 0; j < n && !overflow; j++)
+        if (inside(c, p[j])) add(c, (int)j);
+    }
+  }
+
+  // The root's COM by calc_com_threaded (src/octree.cpp:286-313): the
+  // children's COM times mass summed in child order, then divided by the
+  // root's mass -- with no test for a zero mass.
+  void root_com_threaded() {
+    V3<T> t = {0, 0, 0};
+    for (int i = 0; i < 8; i++) {
+      const int c = nodes[0].child[i];
+      if (c < 0) continue;
+      V3<T> t2 = nodes[c].com;
+      const T cm = nodes[c].mass;
+      t2.x *= cm;
+      t2.y *= cm;
+      t2.z *= cm;
+      t.x += t2.x;
+      t.y += t2.y;
+      t.z += t2.z;
+    }
+    const T M = nodes[0].mass;
+    t.x /= M;
+    t.y /= M;
+    t.z /= M;
+    nodes[0].com = t;
+  }
+
+  size_t leaves() const {
+    size_t k = 0;
+    for (const Node& nd : nodes) k += nd.p >= 0;
+    return k;
+  }
+
   // src/octree.cpp:203-221 (calc_mass) then :252-278 (calc_com), restated
   // as explicit post-order passes; child order 0..7 fixes the rounding.
-  void mass_com() {
+  // With the threaded tree the root's mass is the same child-order sum
+  // (calc_mass_threaded, :230-250) and its COM root_com_threaded's.
+  void mass_com(bool threaded = false) {
     // children always have larger indices than their parent (allocation
     // order), so a reverse sweep is a valid post-order for calc_mass.
     for (size_t k = nodes.size(); k-- > 0;) {
@@ -248,7 +317,13 @@ struct Tree {
           if (nd.child[i] >= 0) nd.mass += nodes[nd.child[i]].mass;
       }
     }
-    com_rec(0);
+    if (!threaded) {
+      com_rec(0);
+      return;
+    }
+    for (int i = 0; i < 8; i++)
+      if (nodes[0].child[i] >= 0) com_rec(nodes[0].child[i]);
+    root_com_threaded();
   }
   void com_rec(int k) {
     Node& nd = nodes[k];
@@ -566,7 +641,9 @@ struct orc_tree {
 };
 
 // Builds the reference tree.  root_mode 0: bbox + 2 (gen_root); 1: the
-// caller's centre/side.  Returns null on identical-position overflow.
+// caller's centre/side; 2: bbox + 2 with the tree gen_root_threaded builds
+// (threads > 1: eight eager root children, inclusive inside()).  Returns
+// null on identical-position overflow.
 orc_tree* orc_tree_build(int prec, size_t n, const double* pos,
                          const double* mass, int root_mode,
                          const double* center, double side) {
@@ -577,14 +654,17 @@ orc_tree* orc_tree_build(int prec, size_t n, const double* pos,
     load_state<T>(n, pos, mass, p, m);
     V3<T> o;
     T s;
-    if (root_mode == 0) {
+    if (root_mode != 1) {
       Tree<T>::root_of(p.data(), n, o, s);
     } else {
       o = V3<T>{(T)center[0], (T)center[1], (T)center[2]};
       s = (T)side;
     }
-    tree.build(p.data(), m.data(), n, o, s);
-    if (!tree.overflow) tree.mass_com();
+    if (root_mode == 2)
+      tree.build_threaded(p.data(), m.data(), n, o, s);
+    else
+      tree.build(p.data(), m.data(), n, o, s);
+    if (!tree.overflow) tree.mass_com(root_mode == 2);
     return !tree.overflow;
   };
   bool ok = prec == 0 ? run(t->td, t->pd, t->md) : run(t->tf, t->pf, t->mf);
@@ -599,6 +679,10 @@ void orc_tree_free(orc_tree* t) { delete t; }
 
 size_t orc_tree_size(orc_tree* t) {
   return t->prec == 0 ? t->td.nodes.size() : t->tf.nodes.size();
+}
+
+size_t orc_tree_leaves(orc_tree* t) {
+  return t->prec == 0 ? t->td.leaves() : t->tf.leaves();
 }
 
 int orc_tree_max_depth(orc_tree* t) {
@@ -709,7 +793,7 @@ void orc_tree_forces_subset(orc_tree* t, double theta, int damping,
 // last frame's accelerations, as the reference's particle records do.
 int orc_run(int prec, size_t n, double* pos, double* vel, double* acc,
             const double* mass, double theta, double dt, int damping,
-            int frames, int threads) {
+            int frames, int threads, int tree_threaded) {
   auto run = [&](auto tag) -> int {
     using T = decltype(tag);
     std::vector<V3<T>> p, v, a(n);
@@ -722,9 +806,12 @@ int orc_run(int prec, size_t n, double* pos, double* vel, double* acc,
       V3<T> o;
       T s;
       Tree<T>::root_of(p.data(), n, o, s);
-      tree.build(p.data(), m.data(), n, o, s);
+      if (tree_threaded)
+        tree.build_threaded(p.data(), m.data(), n, o, s);
+      else
+        tree.build(p.data(), m.data(), n, o, s);
       if (tree.overflow) return 1;
-      tree.mass_com();
+      tree.mass_com(tree_threaded != 0);
       parallel_for(n, threads, [&](size_t i, int w) {
         a[i] = tree.walk((int)i, (T)theta, damping != 0, qs[w], nullptr);
       });

@@ -99,8 +99,27 @@ void load(RefState& s, size_t n, const double* pos, const double* vel,
   }
 }
 
+// ref_set_threaded(1): build() makes the tree of the threads > 1 path.
+int g_threaded = 0;
+
+// gen_root_threaded (src/main.cpp:817-886) with one thread: the root
+// allocates its eight children (octree::allocate_child), then each child
+// takes, in set order, the particles its octree::inside() accepts
+// (octree::add_particle); mass and COM by the reference's
+// calc_mass_threaded / calc_com_threaded (src/octree.cpp:230-313).
 void build(RefState& s) {
   s.root = make_root(s.ps);
+  if (g_threaded) {
+    for (unsigned int i = 0; i < 8; i++) s.root->allocate_child(i);
+    for (unsigned int i = 0; i < 8; i++) {
+      octree* c = s.root->get_child(i);
+      for (particle* p : s.ps)
+        if (c->inside(p)) c->add_particle(p);
+    }
+    s.root->calc_mass_threaded();
+    s.root->calc_com_threaded();
+    return;
+  }
   for (particle* p : s.ps) s.root->add_particle(p);
   s.root->calc_mass();
   s.root->calc_com();
@@ -209,10 +228,13 @@ void ref_update_mags(size_t n, const double* vel, const double* acc, double* dv,
 }
 int ref_octree_bytes(void) { return (int)sizeof(octree); }
 
+void ref_set_threaded(int on) { g_threaded = on; }
+
 // Builds the reference tree and dumps it in pre-order (children 0..7).
-// Returns the node count; writes at most `cap` nodes.  Per node: depth,
-// leaf particle index (or -1), first / count in the DFS leaf order, side,
-// mass, COM; `order` gets the particle indices in DFS leaf order.
+// Returns the node count; writes at most `cap` nodes (and at most `cap`
+// entries of order).  Per node: depth, leaf particle index (or -1), first /
+// count in the DFS leaf order, side, mass, COM; `order` gets the particle
+// indices in DFS leaf order.
 long ref_tree_dump(size_t n, const double* pos, const double* mass,
                    long cap, int* depth, int* leaf, long* first, long* count,
                    double* side, double* nmass, double* com, int* order) {
@@ -250,7 +272,7 @@ long ref_tree_dump(size_t n, const double* pos, const double* mass,
         com[3 * o + 2] = nd->get_com()->get_z();
       }
       if (p != NULL) {
-        if (nleaf < (long)n) order[nleaf] = li;
+        if (nleaf < cap) order[nleaf] = li;
         nleaf++;
         if (w) count[o] = 1;
       } else {

@@ -304,3 +304,23 @@ def test_float_records_write_and_resume(tmp_path, monkeypatch):
             got = host.read_binary(host.frame_filename("./data", frame, True), prec="f32")
             for g, w in zip(got[:3], s.state()):
                 np.testing.assert_array_equal(g, f32(w))
+
+
+def test_frame_loop_threads_selects_the_threaded_tree(tmp_path, monkeypatch):
+    """threads 8 (the reference's sample settings.cfg:21) selects
+    gen_root_threaded's tree in the frame loop (src/main.cpp:1423): the
+    frame files equal the library's steps with threaded_tree set."""
+    monkeypatch.chdir(tmp_path)
+    with open("settings.cfg", "w") as f:
+        f.write(SETTINGS.format(prec="f64", keep="true") + "threads 8\n")
+    assert host.run_frames("settings.cfg") == 0
+    settings = host.read_settings("settings.cfg")[0]
+    pos, vel, mass = host.generate(3000, gen_type="cube", seed=0xDEADBEEF, settings=settings)
+    import nbodysim_amd
+    with nbodysim_amd.Simulation(theta=0.5, dt=0.0333, threaded_tree=True) as s:
+        s.set_state(pos, vel, mass)
+        for frame in range(4):
+            s.step(1)
+            got = host.read_binary(host.frame_filename("./data", frame, True))
+            for g, w in zip(got[:3], s.state()):
+                np.testing.assert_array_equal(g, w)
