@@ -403,6 +403,10 @@ def config_adjoint(args):
                 "grad_norms": {"x0": float(np.linalg.norm(gx)), "v0": float(np.linalg.norm(gv)),
                                "m": float(np.linalg.norm(gm))}})
     log("adjoint timing", out)
+    if args.adjoint_timing_only:
+        out["value"] = out["forward_ms"] + out["backward_ms"]
+        print(json.dumps(out), flush=True)
+        return
 
     def fd_study(steps, ncoord, ndir):
         """FP64 central differences of the frozen map (tree topology and
@@ -726,6 +730,8 @@ def main():
     ap.add_argument("--steps5", type=int, default=200)
     ap.add_argument("--every5", type=int, default=20)
     ap.add_argument("--fd-steps5", type=int, default=10)
+    ap.add_argument("--adjoint-timing-only", action="store_true",
+                    help="--config 5: the forward and backward timing only (for profiling)")
     args = ap.parse_args()
     if args.fp64_steps is None:
         args.fp64_steps = args.steps

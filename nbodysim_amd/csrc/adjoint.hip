@@ -45,6 +45,33 @@ __device__ __forceinline__ bool mac_open_f64_a(double d2, double side,
   return side / __builtin_sqrt(d2) > theta;
 }
 
+// Totals of four values over the wave: lane l ends with the total of value
+// l >> 4 (lanes 0-15 a, 16-31 b, 32-47 c, 48-63 d).  Each exchange halves
+// the values a lane carries (two at xor 32, one at xor 16), so the four
+// sums take seven shuffles instead of twenty-four.
+__device__ __forceinline__ double wave_sum4(double a, double b, double c, double d) {
+  const int lane = lane_id();
+  const bool h32 = (lane & 32) != 0, h16 = (lane & 16) != 0;
+  double k0 = h32 ? c : a, k1 = h32 ? d : b;
+  const double s0 = h32 ? a : c, s1 = h32 ? b : d;
+  k0 += __shfl_xor(s0, 32, 64);
+  k1 += __shfl_xor(s1, 32, 64);
+  double k = h16 ? k1 : k0;
+  k += __shfl_xor(h16 ? k0 : k1, 16, 64);
+#pragma unroll
+  for (int q = 8; q >= 1; q >>= 1) k += __shfl_xor(k, q, 64);
+  return k;
+}
+
+// 1 / sqrt(d2) in FP64: the hardware estimate and one Newton-Raphson step
+// (relative error ~1e-15).
+__device__ __forceinline__ double rsqrt_f64_a(double d2) {
+  const double r = __builtin_amdgcn_rsq(d2);
+  const double h = d2 * r;
+  const double e = fma(-h, r, 1.0);
+  return fma(0.5 * r, e, r);
+}
+
 // Eight wavefronts per SIMD (at most 64 VGPRs and 80 SGPRs each on gfx950)
 // where that needs no spills: without damping (54-56 VGPRs; 74-76 before
 // the bound); the damped variants keep their registers.
@@ -130,8 +157,8 @@ adj_walk_kernel(AdjArgs a) {
     if (I) {
       double wx = 0, wy = 0, wz = 0, wm = 0;
       if (inter) {
-        const double d = __builtin_sqrt(d2);
-        const double rd = 1.0 / d;
+        const double rd = rsqrt_f64_a(d2);
+        const double d = d2 * rd;
         double f, fp_over_d;  // f(d), f'(d) / d
         if (DAMP) {
           const double e = exp(-d);
@@ -153,16 +180,10 @@ adj_walk_kernel(AdjArgs a) {
         gy -= wy;
         gz -= wz;
       }
-      wx = wave_sum(wx);
-      wy = wave_sum(wy);
-      wz = wave_sum(wz);
-      wm = wave_sum(wm);
-      if (lane == 0) {
-        atomicAdd(a.gcx + n, wx);
-        atomicAdd(a.gcy + n, wy);
-        atomicAdd(a.gcz + n, wz);
-        atomicAdd(a.gm_node + n, wm);
-      }
+      // the wave's totals: lanes 0, 16, 32, 48 add gc_n (x, y, z) and gM_n,
+      // four words of one 32-byte record
+      const double t = wave_sum4(wx, wy, wz, wm);
+      if ((lane & 15) == 0) atomicAdd(a.gnode + 4 * (size_t)n + (lane >> 4), t);
     }
     n = max(next, n + 1);
     if (top > 0 && __builtin_amdgcn_readlane(st_pos, top - 1) == n) {
@@ -184,10 +205,7 @@ __global__ void __launch_bounds__(kAdjBlock)
     adj_propagate_kernel(const int* __restrict__ offsets, int n,
                          const WalkNode* __restrict__ nodes,
                          const int* __restrict__ parent,
-                         const double* __restrict__ gcx,
-                         const double* __restrict__ gcy,
-                         const double* __restrict__ gcz,
-                         const double* __restrict__ gm_node,
+                         const double* __restrict__ gnode,
                          const double* __restrict__ x,
                          const double* __restrict__ y,
                          const double* __restrict__ z,
@@ -202,11 +220,12 @@ __global__ void __launch_bounds__(kAdjBlock)
     const double M = nodes[k].m;
     if (M != 0) {
       const double inv = 1.0 / M;
-      const double cx = gcx[k], cy = gcy[k], cz = gcz[k];
+      const double cx = gnode[4 * (size_t)k], cy = gnode[4 * (size_t)k + 1],
+                   cz = gnode[4 * (size_t)k + 2];
       Ax += cx * inv;
       Ay += cy * inv;
       Az += cz * inv;
-      B += gm_node[k] - (cx * nodes[k].cx + cy * nodes[k].cy + cz * nodes[k].cz) * inv;
+      B += gnode[4 * (size_t)k + 3] - (cx * nodes[k].cx + cy * nodes[k].cy + cz * nodes[k].cz) * inv;
     }
     k = parent[k];
   }
@@ -292,7 +311,7 @@ void launch_adj_propagate(const AdjArgs& a, const int* offsets,
                           int n, hipStream_t s) {
   if (n <= 0) return;
   adj_propagate_kernel<<<(n + kAdjBlock - 1) / kAdjBlock, kAdjBlock, 0, s>>>(
-      offsets, n, a.nodes, parent, a.gcx, a.gcy, a.gcz, a.gm_node, a.x, a.y,
+      offsets, n, a.nodes, parent, a.gnode, a.x, a.y,
       a.z, m, a.gx, a.gy, a.gz, gm, a.flags);
 }
 

@@ -2317,10 +2317,7 @@ int adj_reverse_step(nbs_ctx* c, const double* x3) {
   a.gx = c->s_gx;
   a.gy = c->s_gx + n;
   a.gz = c->s_gx + 2 * (size_t)n;
-  a.gcx = c->node_g;
-  a.gcy = c->node_g + c->node_cap;
-  a.gcz = c->node_g + 2 * (size_t)c->node_cap;
-  a.gm_node = c->node_g + 3 * (size_t)c->node_cap;
+  a.gnode = c->node_g;
   a.flags = c->flags;
   a.theta = c->p.theta;
   launch_adj_walk(a, c->cfg.force_precision == NBS_FORCE_F32, c->p.damping, s);

@@ -262,8 +262,7 @@ struct AdjArgs {
   const double *x, *y, *z;       // slot order
   const double *ux, *uy, *uz;    // adjoint of the acceleration, slot order
   double *gx, *gy, *gz;          // out: position adjoint, slot order
-  double *gcx, *gcy, *gcz;       // per node: adjoint of the COM
-  double* gm_node;               // per node: adjoint of the mass
+  double* gnode;  // per node [4]: adjoint of the COM (x, y, z), of the mass
   const int* flags;
   double theta;
 };
