@@ -24,7 +24,8 @@
 //     side / sqrt(dx*dx + dy*dy + dz*dz) > theta (correctly rounded sqrt and
 //     division, no contraction); a squared test with a 1e-9 relative margin
 //     settles all but the near-ties without the sqrt and division.  Forces:
-//     1/r via v_rsq_f64 and two Newton steps.
+//     1/r via v_rsq_f64 and one Newton step (the estimate's ~2^-26 relative
+//     error squares to ~1e-15).
 //   FP32: com - x is formed in FP64 (full rate on CDNA4) and rounded to
 //     FP32, which keeps near-field differences exact to FP32 precision
 //     (node-relative coordinates); d^2, the MAC (d^2 < (side/theta)^2) and
