@@ -57,7 +57,7 @@ def log(*a):
 
 
 def time_path(sim, pos, vel, mass, precision, steps, warmup, shape, graph, order=False,
-              theta=None):
+              theta=None, offsets=None):
     """Timed region: `steps` graph-replayed steps (nbs_step_graph; eager
     launches with graph=False) after `warmup` steps.  Before it (untimed)
     the walk's launch shape is tuned on the initial state (nbs_autotune,
@@ -67,12 +67,15 @@ def time_path(sim, pos, vel, mass, precision, steps, warmup, shape, graph, order
     on the context's stream (kernel times for the phases and the roofline,
     and the interactions those walks counted).  The trajectory depends only
     on positions and caller order, so both runs take the same steps; the
-    end states are compared bit for bit."""
+    end states are compared bit for bit.  `offsets` batches the state into
+    systems (nbs_set_systems) for both runs."""
     kw = {"precision": precision}
     if theta is not None:
         kw["theta"] = theta
     sim.configure(**kw)
     sim.set_state(pos, vel, mass)
+    if offsets is not None:
+        sim.set_systems(offsets)
     sim.set_walk_order(order)
     if shape is None:
         shape, _ = sim.autotune(reps=3)
@@ -87,9 +90,12 @@ def time_path(sim, pos, vel, mass, precision, steps, warmup, shape, graph, order
     sim.synchronize()
     t1 = time.perf_counter()
     info = sim.tree_info()
+    sim_time = float(sim.time()[0])
     end_pos, end_vel, _ = sim.state()
     # the same window, eager and profiled
     sim.set_state(start_pos, start_vel, mass)
+    if offsets is not None:
+        sim.set_systems(offsets)
     sim.interactions_total(reset=True)
     sim.profile(True)
     sim.profile_read(reset=True)
@@ -106,7 +112,8 @@ def time_path(sim, pos, vel, mass, precision, steps, warmup, shape, graph, order
             "prof_steps": steps, "eager_ms_per_step": (t3 - t2) * 1e3 / steps,
             "same_window_bit_identical": bool(np.array_equal(replay_pos, end_pos) and
                                               np.array_equal(replay_vel, end_vel)),
-            "tree": info, "walk_shape": list(shape), "end_state": (end_pos, end_vel)}
+            "tree": info, "walk_shape": list(shape), "end_state": (end_pos, end_vel),
+            "sim_time": sim_time}
 
 
 def accuracy_gated(sim, pos, vel, mass, ids, ref, rms, args, shape, graph):
@@ -226,21 +233,13 @@ SPHERE = dict(r_sphere=100.0, rotation_magnitude=0.1, rotation_vector=(0.0, 0.0,
               min_mass=5e10, max_mass=5e11)
 
 
-def timed_steps(sim, steps, warmup):
-    if warmup:
-        sim.step(warmup, graph=True)
-    sim.synchronize()
-    t0 = time.perf_counter()
-    sim.step(steps, graph=True)
-    sim.synchronize()
-    return (time.perf_counter() - t0) * 1e3 / steps
-
-
 def config_shell(args):
     """Shell of N = 4,194,304 at r = 100 with 1 % radial jitter, rotation
     0.1 about z, FP32 forces, adaptive dt (min 0.00333, max position /
-    velocity change 3), theta 0.3 / 0.5 / 0.7, 50 steps each from the same
-    initial state."""
+    velocity change 3), theta 0.3 / 0.5 / 0.7, from the same initial
+    state each: 2 warm-up steps, `--steps3` steps timed as a HIP graph
+    (eager launches with --eager), then the same window replayed eagerly
+    with HIP events per phase (time_path)."""
     import nbodysim_amd
     from nbodysim_amd import host
     n = args.n3
@@ -251,22 +250,21 @@ def config_shell(args):
         sim = nbodysim_amd.Simulation(theta=theta, dt=0.0333, precision="f32", adaptive=True,
                                       min_adaptive_dt=0.00333, max_pos_change=3.0,
                                       max_vel_change=3.0)
-        sim.set_state(pos, vel, mass)
-        sim.set_walk_shape(64, 8, 1)
-        ms = timed_steps(sim, args.steps3, 2)
-        t_sim = float(sim.time()[0])
-        sim.interactions_total(reset=True)
-        sim.profile(True)
-        sim.profile_read(reset=True)
-        sim.step(2)
-        prof = sim.profile_read(reset=True)
-        inter = sim.interactions_total(reset=True) / 2
+        r = time_path(sim, pos, vel, mass, "f32", args.steps3, 2, (64, 8, 1),
+                      not args.eager)
+        prof, steps = r["prof"], r["prof_steps"]
         walk_ms = prof["walk_ms"] / max(prof["walk_launches"], 1)
-        res.append({"theta": theta, "value": n * 1e3 / ms, "ms_per_step": ms,
-                    "simulated_time": t_sim, "steps": args.steps3 + 2,
-                    "interactions_per_particle": inter / n, "walk_ms": walk_ms,
-                    "build_ms": prof["build_ms"] / 2,
-                    "walk_tflops": FLOP_PER_INTERACTION * inter / (walk_ms * 1e-3) / 1e12})
+        res.append({"theta": theta, "value": n * 1e3 / r["ms_per_step"],
+                    "ms_per_step": r["ms_per_step"],
+                    "eager_ms_per_step": r["eager_ms_per_step"],
+                    "simulated_time": r["sim_time"], "steps": steps, "warmup": 2,
+                    "interactions_per_particle": r["interactions"] / (n * steps),
+                    "walk_ms": walk_ms, "build_ms": prof["build_ms"] / steps,
+                    "update_ms": prof["update_ms"] / steps,
+                    "same_window_bit_identical": r["same_window_bit_identical"],
+                    "walk_tflops": FLOP_PER_INTERACTION * r["interactions"] / steps /
+                                   (walk_ms * 1e-3) / 1e12,
+                    "tree_end": r["tree"]})
         log("shell theta", theta, res[-1])
         sim.close()
     print(json.dumps({
@@ -275,8 +273,10 @@ def config_shell(args):
         "data": "synthetic (seeded gen_type shell, 1% radial jitter)",
         "config": {"workload": "gen_type shell N=%d r=100 jitter 1%% rotation 0.1 about z, "
                                "masses U[5e10,5e11], adaptive dt (dt 0.0333, min 0.00333, "
-                               "max_pos_change 3, max_vel_change 3), %d timed steps per theta"
-                               % (n, args.steps3)},
+                               "max_pos_change 3, max_vel_change 3), %d timed steps per theta "
+                               "after 2 warm-up; phases from the same window replayed eagerly"
+                               % (n, args.steps3),
+                   "launch": "eager" if args.eager else "graph"},
         "results": res}), flush=True)
 
 
@@ -327,12 +327,15 @@ def config_batched(args):
     gen.close()
     log("ICs host %.1f ms, device %.1f ms" % (ic_host_ms, ic_dev_ms))
     sim = nbodysim_amd.Simulation(theta=0.5, dt=0.0333, precision="f32")
-    sim.set_state(pos, vel, mass)
-    sim.set_systems(off)
-    shape, _ = sim.autotune(reps=3)
-    ms_b = timed_steps(sim, args.steps4, 2)
+    r = time_path(sim, pos, vel, mass, "f32", args.steps4, 2, None, not args.eager,
+                  offsets=off)
     sim.close()
-    log("batched", shape, ms_b)
+    shape, ms_b, prof = r["walk_shape"], r["ms_per_step"], r["prof"]
+    phases = {"build": prof["build_ms"] / args.steps4, "walk": prof["walk_ms"] / args.steps4,
+              "update": prof["update_ms"] / args.steps4,
+              "eager_step": r["eager_ms_per_step"],
+              "same_window_bit_identical": r["same_window_bit_identical"]}
+    log("batched", shape, ms_b, phases)
     # one context per system, each system's steps launched on its own
     sims = []
     for p in parts:
@@ -361,7 +364,10 @@ def config_batched(args):
         "data": "synthetic (seeded per-system cube/sphere/shell, the reference's generator)",
         "config": {"workload": "%d systems, N_s log-uniform in [1024, 65536], theta 0.5, "
                                "dt 0.0333, FP32 forces" % len(parts),
-                   "walk_shape": list(shape), "launch": "graph, one batched context"},
+                   "walk_shape": list(shape),
+                   "launch": ("eager" if args.eager else "graph") + ", one batched context"},
+        "phases_ms_per_step": phases,
+        "interactions_per_particle": r["interactions"] / (n * args.steps4),
         "per_system_launches": {"value": n * 1e3 / ms_s, "ms_per_step": ms_s,
                                 "steps": steps_s},
         "speedup_batched": ms_s / ms_b,
