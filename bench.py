@@ -34,6 +34,8 @@ FP32_PEAK_TFLOPS = 157.3   # MI355X vector FP32 (AMD spec)
 FP64_PEAK_TFLOPS = 78.6    # MI355X vector FP64
 HBM_PEAK_GBS = 8000.0
 FLOP_PER_INTERACTION = 20
+# interaction counting of the timed walks (--counting)
+COUNTING = "total"
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "walk_traffic_sphere1M.md")
 
 
@@ -68,11 +70,15 @@ def time_path(sim, pos, vel, mass, precision, steps, warmup, shape, graph, order
     and the interactions those walks counted).  The trajectory depends only
     on positions and caller order, so both runs take the same steps; the
     end states are compared bit for bit.  `offsets` batches the state into
-    systems (nbs_set_systems) for both runs."""
+    systems (nbs_set_systems) for both runs.  Both runs count interactions
+    as COUNTING says (by default the running total only, as the frame loop
+    steps: per-particle counts are a diagnostic the reference's step does
+    not have)."""
     kw = {"precision": precision}
     if theta is not None:
         kw["theta"] = theta
     sim.configure(**kw)
+    sim.set_counting(COUNTING)
     sim.set_state(pos, vel, mass)
     if offsets is not None:
         sim.set_systems(offsets)
@@ -340,6 +346,7 @@ def config_batched(args):
     sims = []
     for p in parts:
         t = nbodysim_amd.Simulation(theta=0.5, dt=0.0333, precision="f32")
+        t.set_counting(COUNTING)
         t.set_state(*p)
         t.set_walk_shape(*shape)
         t.step(2)
@@ -525,6 +532,7 @@ def config_frames(args):
     pos, vel, mass = host.generate(n, gen_type="sphere", seed=0xDEADBEEF, **SPHERE)
     d = tempfile.mkdtemp(prefix="nbs_frames_", dir="/tmp")
     sim = nbodysim_amd.Simulation(theta=0.5, dt=0.0333, precision="f32")
+    sim.set_counting(COUNTING)
     sim.set_state(pos, vel, mass)
     sim.set_walk_shape(64, 8, 1)
     sim.step(2)
@@ -614,6 +622,7 @@ def config_collide(args):
     for prec in ("f32", "f64"):
         for coll in (False, True):
             with nbodysim_amd.Simulation(theta=0.5, dt=0.0333, precision=prec) as sim:
+                sim.set_counting(COUNTING)
                 sim.set_state(pos, vel, mass)
                 sim.set_walk_shape(64, 8, 1)
                 if coll:
@@ -711,6 +720,9 @@ def main():
                     help="let nbs_autotune pick the walk shape before the timed steps")
     ap.add_argument("--walk-order", action="store_true",
                     help="longest-first dispatch of the walk's workgroups (nbs_set_walk_order)")
+    ap.add_argument("--counting", default="total", choices=("total", "particles", "none"),
+                    help="interaction counting of the timed walks (nbs_set_counting); the "
+                         "interaction figures need total or particles")
     ap.add_argument("--eager", action="store_true",
                     help="timed steps launched one kernel at a time, not by graph replay")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
@@ -739,6 +751,8 @@ def main():
     ap.add_argument("--adjoint-timing-only", action="store_true",
                     help="--config 5: the forward and backward timing only (for profiling)")
     args = ap.parse_args()
+    global COUNTING
+    COUNTING = args.counting
     if args.fp64_steps is None:
         args.fp64_steps = args.steps
     if args.gpus != 1:
@@ -831,7 +845,8 @@ def main():
                                "forces, FP64 state/keys/COM" % n,
                    "n": n, "theta": theta, "dt": dt, "parallelism": "single GPU",
                    "walk_shape": r32["walk_shape"], "launch": "graph" if graph else "eager",
-                   "walk_order": "longest-first" if args.walk_order else "plain"},
+                   "walk_order": "longest-first" if args.walk_order else "plain",
+                   "counting": COUNTING},
         "interactions_per_s": inter_per_launch / (r32["ms_per_step"] * 1e-3),
         "interactions_per_particle": inter_per_launch / n,
         "fp64": {"value": value64, "ms_per_step": r64["ms_per_step"],

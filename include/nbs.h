@@ -261,12 +261,25 @@ int nbs_synchronize(nbs_ctx* ctx);
  * reference for the tree walk. */
 int nbs_direct(nbs_ctx* ctx, size_t m, const int64_t* ids, double* acc);
 
+/* Interaction counting of the walks (no counterpart in the reference, whose
+ * step counts nothing): NBS_COUNT_PARTICLES (default) keeps every
+ * particle's count of the last walk (nbs_interactions) and the running
+ * total (nbs_interactions_total); NBS_COUNT_TOTAL only the running total,
+ * counted per wavefront on the scalar unit; NBS_COUNT_NONE nothing.  The
+ * per-particle count is one vector instruction per interacting node visit
+ * of the walk, so the frame loop steps with NBS_COUNT_TOTAL.  Changes no
+ * force, position or velocity bit. */
+enum { NBS_COUNT_NONE = 0, NBS_COUNT_TOTAL = 1, NBS_COUNT_PARTICLES = 2 };
+int nbs_set_counting(nbs_ctx* ctx, int mode);
+
 /* Interaction count (accepted nodes per particle) of the last walk, summed
- * over all particles; per_particle (host, caller order) may be NULL. */
+ * over all particles; per_particle (host, caller order) may be NULL.
+ * NBS_ERR_STATE when the last walk ran without NBS_COUNT_PARTICLES. */
 int nbs_interactions(nbs_ctx* ctx, int64_t* total, int64_t* per_particle);
 
 /* Interactions summed over every walk since the last reset (reset != 0
- * clears the counter after reading). */
+ * clears the counter after reading); walks under NBS_COUNT_NONE add
+ * nothing. */
 int nbs_interactions_total(nbs_ctx* ctx, int64_t* total, int reset);
 
 /* Simulation time advanced so far, summed over steps (differs from

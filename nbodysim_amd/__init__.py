@@ -275,6 +275,16 @@ class Simulation:
         compiler-generated loop (generic=True); results are bit-identical."""
         self._check(self._L.nbs_set_walk_code(self._h, int(bool(generic))))
 
+    def set_counting(self, mode="particles"):
+        """Interaction counting of the walks: "particles" (default; per
+        particle and the running total), "total" (the running total only,
+        no per-lane work in the walk) or "none".  Changes no result bit."""
+        modes = {"none": _native.NBS_COUNT_NONE, "total": _native.NBS_COUNT_TOTAL,
+                 "particles": _native.NBS_COUNT_PARTICLES}
+        if mode not in modes:
+            raise ValueError("counting mode must be one of %s" % sorted(modes))
+        self._check(self._L.nbs_set_counting(self._h, modes[mode]))
+
     def synchronize(self):
         self._check(self._L.nbs_synchronize(self._h))
 

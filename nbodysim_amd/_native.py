@@ -18,6 +18,7 @@ NBS_OK, NBS_ERR_ARG, NBS_ERR_HIP, NBS_ERR_STATE, NBS_ERR_OVERFLOW, NBS_ERR_NOMEM
     NBS_ERR_IO = range(7)
 NBS_FORCE_F64, NBS_FORCE_F32 = 0, 1
 NBS_ROOT_BBOX, NBS_ROOT_FIXED = 0, 1
+NBS_COUNT_NONE, NBS_COUNT_TOTAL, NBS_COUNT_PARTICLES = 0, 1, 2
 
 
 class Config(ctypes.Structure):
@@ -99,6 +100,7 @@ SIGNATURES = [
                                           ctypes.c_int]),
     ("nbs_set_walk_order", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("nbs_set_walk_code", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("nbs_set_counting", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("nbs_synchronize", ctypes.c_int, [ctypes.c_void_p]),
     ("nbs_direct", ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, c_i64p, c_dp]),
     ("nbs_interactions", ctypes.c_int, [ctypes.c_void_p, c_i64p, c_i64p]),

@@ -227,6 +227,10 @@ struct nbs_ctx {
   int walk_order = 0;
   // 1: the compiler-generated walk loop even where the hand-written one applies
   int walk_generic = 0;
+  // interaction counting of the walks (nbs_set_counting): per particle by
+  // default; lane_counts: `inter` holds the last walk's per-particle counts
+  int count_mode = kCountLanes;
+  bool lane_counts = false;
   unsigned* tile_cost = nullptr;
   int* tile_order = nullptr;
   int cost_block = -1, cost_pack = -1, cost_tiles = -1;
@@ -763,6 +767,8 @@ void enqueue_walk(nbs_ctx* c, int fused) {
   a.inter_total = c->inter_total;
   a.fused_update = fused;
   a.groups = c->walk_groups;
+  a.count_mode = c->count_mode;
+  c->lane_counts = c->count_mode == kCountLanes;
   // the hand-written loop addresses node records with 32-bit byte offsets
   a.asm_loop = !c->walk_generic && ((size_t)c->node_cap + 1) * sizeof(WalkNode) < (1ull << 32);
   if (c->walk_order) {
@@ -1814,6 +1820,16 @@ int nbs_set_walk_code(nbs_ctx* c, int generic) {
   return NBS_OK;
 }
 
+int nbs_set_counting(nbs_ctx* c, int mode) {
+  if (!c) return NBS_ERR_ARG;
+  if (mode < NBS_COUNT_NONE || mode > NBS_COUNT_PARTICLES)
+    return fail(c, NBS_ERR_ARG, "counting mode must be 0 (none), 1 (total) or 2 (per particle)");
+  drop_graph(c);
+  c->count_mode = mode == NBS_COUNT_PARTICLES ? kCountLanes
+                                              : (mode == NBS_COUNT_TOTAL ? kCountWave : kCountNone);
+  return NBS_OK;
+}
+
 int nbs_set_walk_order(nbs_ctx* c, int on) {
   if (!c) return NBS_ERR_ARG;
   drop_graph(c);
@@ -2019,6 +2035,9 @@ int nbs_direct(nbs_ctx* c, size_t m, const int64_t* ids, double* acc) {
 int nbs_interactions(nbs_ctx* c, int64_t* total, int64_t* per_particle) {
   if (!c) return NBS_ERR_ARG;
   if (!c->acc_valid) return fail(c, NBS_ERR_STATE, "no walk has run");
+  if (!c->lane_counts)
+    return fail(c, NBS_ERR_STATE,
+                "the last walk did not count per particle (nbs_set_counting(ctx, 2))");
   const int n = c->n;
   std::vector<int> h(n);
   std::vector<unsigned> id(n);
@@ -2528,6 +2547,7 @@ int nbs_frozen_forward(nbs_ctx* c, const double* pos0, const double* vel0,
     a.steps_done = c->steps_done;
     a.inter_total = c->inter_total;
     a.fused_update = 1;
+    a.count_mode = kCountLanes;
     a.mac_nodes = c->mac_nodes;
     a.mac_x = c->mac_x;
     a.mac_y = c->mac_x + n;

@@ -157,6 +157,9 @@ extern "C" int nbs_run_frames(const char* cfgfile) {
     nbs_destroy(ctx);
     return 1;
   }
+  // the reference's step counts no interactions: only the running total
+  // (scalar work in the walk) is kept
+  nbs_set_counting(ctx, NBS_COUNT_TOTAL);
   // the reference applies min_node_size only when verbose is set too
   // (src/main.cpp:1425)
   if (s.min_node_size > 0 && s.verbose &&

@@ -173,6 +173,8 @@ void launch_entry_patch(const int* offsets, const int* sp, const int* new_slot, 
                         const int* flags, hipStream_t s);
 
 // ----------------------------------------------------------------- walk --
+enum CountMode { kCountNone = 0, kCountWave = 1, kCountLanes = 2 };
+
 struct WalkArgs {
   const WalkNode* nodes;
   const int* node_start;
@@ -197,6 +199,9 @@ struct WalkArgs {
   int fused_update;
   int groups;  // workgroup -> tile interleave over the XCDs (1 = plain)
   int asm_loop;  // FP32 walk: the hand-written loop where it applies
+  // interaction counting: kCountLanes per particle (interactions[] and the
+  // per-wave totals), kCountWave the per-wave totals only, kCountNone none
+  int count_mode;
   const int* order;     // workgroup -> tile (longest first), or null
   unsigned* tile_cost;  // per tile: node visits of this walk added, or null
   // frozen decisions (FP64 only; null otherwise): base tree + positions

@@ -189,12 +189,27 @@ __device__ __forceinline__ void interact(const NodeRegs& nd, double dx, double d
 // when the walk ends: the node array holds one record beyond its capacity.
 // Record offsets are 32-bit: callers keep (capacity + 1) * 64 below 2^32.
 
+// Interaction counting on the lanes of MASK (the interacting lanes), one
+// of three forms chosen per launch: a per-lane count (one VALU instruction
+// per interacting visit), the wave's total on the scalar unit (the mask's
+// population count, no VALU), or none.  Each form sits between v_rsq and
+// the first instruction that reads its result, and each is at least one
+// instruction: on gfx950 a VALU instruction may read a transcendental
+// result only after one wait state, which the compiler inserts for its own
+// code but not inside this asm (without it the multiply reads the old
+// register).
+#define NBS_CNT_LANE(MASK) "v_add_u32 %[cnt], 1, %[cnt]\n\t"
+#define NBS_CNT_WAVE(MASK)                                            \
+  "s_bcnt1_i32_b64 %[tc], " MASK "\n\t"                               \
+  "s_add_u32 %[tot], %[tot], %[tc]\n\t"
+#define NBS_CNT_NONE(MASK) "s_nop 0\n\t"
+
 // The force on the lanes of MASK and the loop test (the next record has
 // been requested); NBS_WALK_TAIL requests it first.
-#define NBS_WALK_FORCE_PLAIN(MASK)                                    \
+#define NBS_WALK_FORCE_PLAIN(MASK, CNT)                               \
   "s_and_saveexec_b64 s[40:41], " MASK "\n\t"                          \
   "v_rsq_f32 %[tv], %[d2]\n\t"                                        \
-  "v_add_u32 %[cnt], 1, %[cnt]\n\t"                                   \
+  CNT(MASK)                                                           \
   "v_mul_f32 %[d2], %[tv], %[tv]\n\t"                                 \
   "v_mul_f32 %[d2], %[d2], %[tv]\n\t"                                 \
   "v_mul_f32 %[d2], %[gm], %[d2]\n\t"                                 \
@@ -206,10 +221,10 @@ __device__ __forceinline__ void interact(const NodeRegs& nd, double dx, double d
   "s_cbranch_scc1 L_loop%=\n\t"                                       \
   "s_branch L_slow%=\n"
 
-#define NBS_WALK_TAIL(FORCE, MASK)                                    \
+#define NBS_WALK_TAIL(FORCE, MASK, CNT)                               \
   "s_lshl_b32 %[t], %[n], 6\n\t"                                      \
   "s_load_dwordx16 s[48:63], %[base], %[t]\n\t"                       \
-  FORCE(MASK)
+  FORCE(MASK, CNT)
 
 // The damped force (src/main.cpp:318: |a| = G m / (r^2 + exp(-r))), in the
 // generic loop's operations: d = r^2 * (1/r) = r, e = exp2(-d log2 e) (the
@@ -217,10 +232,10 @@ __device__ __forceinline__ void interact(const NodeRegs& nd, double dx, double d
 // rounded by the scale / reciprocal / Newton / fixup division sequence.
 // s[38:39] receives the first scale's flag (unused); vcc the second's,
 // which v_div_fmas reads six instructions later.
-#define NBS_WALK_FORCE_DAMPED(MASK)                                   \
+#define NBS_WALK_FORCE_DAMPED(MASK, CNT)                              \
   "s_and_saveexec_b64 s[40:41], " MASK "\n\t"                          \
   "v_rsq_f32 %[tv], %[d2]\n\t"                                        \
-  "v_add_u32 %[cnt], 1, %[cnt]\n\t"                                   \
+  CNT(MASK)                                                           \
   "v_mul_f32 %[te], %[d2], %[tv]\n\t"                                 \
   "v_mul_f32 %[te], 0xbfb8aa3b, %[te]\n\t"                            \
   "v_exp_f32 %[te], %[te]\n\t"                                        \
@@ -245,7 +260,7 @@ __device__ __forceinline__ void interact(const NodeRegs& nd, double dx, double d
   "s_cbranch_scc1 L_loop%=\n\t"                                       \
   "s_branch L_slow%=\n"
 
-#define NBS_WALK_F32_LOOP(FORCE) \
+#define NBS_WALK_F32_LOOP(FORCE, CNT) \
       "s_mov_b32 %[m0save], m0\n\t" \
       "v_mov_b32 %[sp], -1\n\t" \
       "v_mov_b32 %[slo], 0\n\t" \
@@ -283,7 +298,7 @@ __device__ __forceinline__ void interact(const NodeRegs& nd, double dx, double d
       /* subtree (never backwards, even on a corrupt skip) */ \
       "s_add_i32 %[n], %[n], 1\n\t" \
       "s_max_i32 %[n], %[n], s54\n\t" \
-      NBS_WALK_TAIL(FORCE, "s[46:47]") \
+      NBS_WALK_TAIL(FORCE, "s[46:47]", CNT) \
       "L_opened%=:\n\t" \
       /* I = A & ~O interact and wait at skip; O descend (A = O); with */ \
       /* no lane in I the force is skipped */ \
@@ -325,7 +340,7 @@ __device__ __forceinline__ void interact(const NodeRegs& nd, double dx, double d
       "s_cbranch_scc1 L_loop%=\n\t" \
       "s_branch L_slow%=\n" \
       "L_int%=:\n\t" \
-      NBS_WALK_TAIL(FORCE, "s[44:45]") \
+      NBS_WALK_TAIL(FORCE, "s[44:45]", CNT) \
       "L_leaf%=:\n\t" \
       /* leaf: the next entry is known at once, so its record is requested */ \
       /* as soon as the three differences have read this one's position; */ \
@@ -352,7 +367,7 @@ __device__ __forceinline__ void interact(const NodeRegs& nd, double dx, double d
       "v_mul_f32 %[d2], %[fz], %[fz]\n\t" \
       "v_fmac_f32 %[d2], %[fy], %[fy]\n\t" \
       "v_fmac_f32 %[d2], %[fx], %[fx]\n\t" \
-      FORCE("s[44:45]") \
+      FORCE("s[44:45]", CNT) \
       "L_slow%=:\n\t" \
       /* n reached lim = min(stack top, nend): the lanes waiting at n */ \
       /* rejoin (the stack top is never passed), or the walk is done */ \
@@ -373,51 +388,72 @@ __device__ __forceinline__ void interact(const NodeRegs& nd, double dx, double d
       "L_done%=:\n\t" \
       "s_mov_b32 m0, %[m0save]"
 
+// Operands of the FP32 loops: the shared ones, then those of each counting
+// form (the lane count in a VGPR, the wave total and a temporary in SGPRs).
+#define NBS_F32_OUTS                                                                      \
+  [n] "+s"(n), [ovf] "+s"(overflow), [ax] "+v"(ax), [ay] "+v"(ay), [az] "+v"(az),         \
+      [top] "=&s"(top), [tpos] "=&s"(tpos), [lim] "=&s"(lim), [t] "=&s"(t), [gm] "=&s"(gm), \
+      [m0save] "=&s"(m0save), [sp] "=&v"(sp), [slo] "=&v"(slo), [shi] "=&v"(shi),          \
+      [dx] "=&v"(dx), [dy] "=&v"(dy), [dz] "=&v"(dz), [fx] "=&v"(fx), [fy] "=&v"(fy),       \
+      [fz] "=&v"(fz), [d2] "=&v"(d2), [tv] "=&v"(tv)
+#define NBS_F32_DAMPED_OUTS                                                               \
+  NBS_F32_OUTS, [te] "=&v"(te), [tn] "=&v"(tn), [td] "=&v"(td), [q0] "=&v"(q0),          \
+      [q1] "=&v"(q1), [q2] "=&v"(q2), [q3] "=&v"(q3), [q4] "=&v"(q4)
+#define NBS_F32_INS                                                                       \
+  [base] "s"(nodes), [nend] "s"(nend), [pbeg] "s"(pbeg), [A0] "s"(A), [x] "v"(x),          \
+      [y] "v"(y), [z] "v"(z)
+#define NBS_WALK_CLOBBERS                                                                 \
+  "s38", "s39", "s40", "s41", "s42", "s43", "s44", "s45", "s46", "s47", "s48", "s49",     \
+      "s50", "s51", "s52", "s53", "s54", "s55", "s56", "s57", "s58", "s59", "s60", "s61", \
+      "s62", "s63", "vcc", "scc", "memory"
+
+// CM: kCountLanes adds to cnt on the interacting lanes, kCountWave adds the
+// interacting lanes' number to tot, kCountNone counts nothing.
+template <int CM>
 __device__ __forceinline__ void walk_loop_f32(ConstNodes nodes, int n, int nend, int pbeg,
                                               unsigned long long A, double x, double y,
                                               double z, float& ax, float& ay, float& az,
-                                              int& cnt, int& overflow) {
-  int top, tpos, lim, t, gm, m0save;
+                                              int& cnt, int& tot, int& overflow) {
+  int top, tpos, lim, t, gm, m0save, tc;
   int sp, slo, shi;
   double dx, dy, dz;
   float fx, fy, fz, d2, tv;
-  __asm__ volatile(
-      NBS_WALK_F32_LOOP(NBS_WALK_FORCE_PLAIN)
-      : [n] "+s"(n), [ovf] "+s"(overflow), [ax] "+v"(ax), [ay] "+v"(ay), [az] "+v"(az),
-        [cnt] "+v"(cnt), [top] "=&s"(top), [tpos] "=&s"(tpos), [lim] "=&s"(lim), [t] "=&s"(t),
-        [gm] "=&s"(gm), [m0save] "=&s"(m0save), [sp] "=&v"(sp), [slo] "=&v"(slo),
-        [shi] "=&v"(shi), [dx] "=&v"(dx), [dy] "=&v"(dy), [dz] "=&v"(dz), [fx] "=&v"(fx),
-        [fy] "=&v"(fy), [fz] "=&v"(fz), [d2] "=&v"(d2), [tv] "=&v"(tv)
-      : [base] "s"(nodes), [nend] "s"(nend), [pbeg] "s"(pbeg), [A0] "s"(A), [x] "v"(x),
-        [y] "v"(y), [z] "v"(z)
-      : "s38", "s39", "s40", "s41", "s42", "s43", "s44", "s45", "s46", "s47", "s48", "s49",
-        "s50", "s51", "s52", "s53", "s54", "s55", "s56", "s57", "s58", "s59", "s60", "s61",
-        "s62", "s63", "vcc", "scc", "memory");
+  if constexpr (CM == kCountLanes)
+    __asm__ volatile(NBS_WALK_F32_LOOP(NBS_WALK_FORCE_PLAIN, NBS_CNT_LANE)
+                     : NBS_F32_OUTS, [cnt] "+v"(cnt) : NBS_F32_INS : NBS_WALK_CLOBBERS);
+  else if constexpr (CM == kCountWave)
+    __asm__ volatile(NBS_WALK_F32_LOOP(NBS_WALK_FORCE_PLAIN, NBS_CNT_WAVE)
+                     : NBS_F32_OUTS, [tot] "+s"(tot), [tc] "=&s"(tc)
+                     : NBS_F32_INS : NBS_WALK_CLOBBERS);
+  else
+    __asm__ volatile(NBS_WALK_F32_LOOP(NBS_WALK_FORCE_PLAIN, NBS_CNT_NONE)
+                     : NBS_F32_OUTS : NBS_F32_INS : NBS_WALK_CLOBBERS);
 }
 
+template <int CM>
 __device__ __forceinline__ void walk_loop_f32_damped(ConstNodes nodes, int n, int nend,
                                                      int pbeg, unsigned long long A, double x,
                                                      double y, double z, float& ax, float& ay,
-                                                     float& az, int& cnt, int& overflow) {
-  int top, tpos, lim, t, gm, m0save;
+                                                     float& az, int& cnt, int& tot,
+                                                     int& overflow) {
+  int top, tpos, lim, t, gm, m0save, tc;
   int sp, slo, shi;
   double dx, dy, dz;
   float fx, fy, fz, d2, tv, te, tn, td, q0, q1, q2, q3, q4;
-  __asm__ volatile(
-      NBS_WALK_F32_LOOP(NBS_WALK_FORCE_DAMPED)
-      : [n] "+s"(n), [ovf] "+s"(overflow), [ax] "+v"(ax), [ay] "+v"(ay), [az] "+v"(az),
-        [cnt] "+v"(cnt), [top] "=&s"(top), [tpos] "=&s"(tpos), [lim] "=&s"(lim), [t] "=&s"(t),
-        [gm] "=&s"(gm), [m0save] "=&s"(m0save), [sp] "=&v"(sp), [slo] "=&v"(slo),
-        [shi] "=&v"(shi), [dx] "=&v"(dx), [dy] "=&v"(dy), [dz] "=&v"(dz), [fx] "=&v"(fx),
-        [fy] "=&v"(fy), [fz] "=&v"(fz), [d2] "=&v"(d2), [tv] "=&v"(tv),
-        [te] "=&v"(te), [tn] "=&v"(tn), [td] "=&v"(td), [q0] "=&v"(q0), [q1] "=&v"(q1),
-        [q2] "=&v"(q2), [q3] "=&v"(q3), [q4] "=&v"(q4)
-      : [base] "s"(nodes), [nend] "s"(nend), [pbeg] "s"(pbeg), [A0] "s"(A), [x] "v"(x),
-        [y] "v"(y), [z] "v"(z)
-      : "s38", "s39", "s40", "s41", "s42", "s43", "s44", "s45", "s46", "s47", "s48", "s49",
-        "s50", "s51", "s52", "s53", "s54", "s55", "s56", "s57", "s58", "s59", "s60", "s61",
-        "s62", "s63", "vcc", "scc", "memory");
+  if constexpr (CM == kCountLanes)
+    __asm__ volatile(NBS_WALK_F32_LOOP(NBS_WALK_FORCE_DAMPED, NBS_CNT_LANE)
+                     : NBS_F32_DAMPED_OUTS, [cnt] "+v"(cnt) : NBS_F32_INS : NBS_WALK_CLOBBERS);
+  else if constexpr (CM == kCountWave)
+    __asm__ volatile(NBS_WALK_F32_LOOP(NBS_WALK_FORCE_DAMPED, NBS_CNT_WAVE)
+                     : NBS_F32_DAMPED_OUTS, [tot] "+s"(tot), [tc] "=&s"(tc)
+                     : NBS_F32_INS : NBS_WALK_CLOBBERS);
+  else
+    __asm__ volatile(NBS_WALK_F32_LOOP(NBS_WALK_FORCE_DAMPED, NBS_CNT_NONE)
+                     : NBS_F32_DAMPED_OUTS : NBS_F32_INS : NBS_WALK_CLOBBERS);
 }
+#undef NBS_F32_OUTS
+#undef NBS_F32_DAMPED_OUTS
+#undef NBS_F32_INS
 #undef NBS_WALK_F32_LOOP
 #undef NBS_WALK_TAIL
 #undef NBS_WALK_FORCE_DAMPED
@@ -436,10 +472,10 @@ __device__ __forceinline__ void walk_loop_f32_damped(ConstNodes nodes, int n, in
 // the caller then visits it with the exact test (walk_visit_f64) and calls
 // the loop again.  It returns 0 when the walk is done.  The record's FP64
 // mass-times-G is copied to `gm` before the next record is requested.
-#define NBS_WALK64_FORCE(MASK)                                        \
+#define NBS_WALK64_FORCE(MASK, CNT)                                   \
   "s_and_saveexec_b64 s[40:41], " MASK "\n\t"                          \
   "v_rsq_f64 %[tv], %[d2]\n\t"                                        \
-  "v_add_u32 %[cnt], 1, %[cnt]\n\t"                                   \
+  CNT(MASK)                                                           \
   "v_mul_f64 %[tmp], %[d2], %[tv]\n\t"                                \
   "v_fma_f64 %[tmp], -%[tmp], %[tv], 1.0\n\t"                         \
   "v_mul_f64 %[s2], 0.5, %[tv]\n\t"                                   \
@@ -455,10 +491,10 @@ __device__ __forceinline__ void walk_loop_f32_damped(ConstNodes nodes, int n, in
   "s_cbranch_scc1 L_loop%=\n\t"                                       \
   "s_branch L_slow%=\n"
 
-#define NBS_WALK64_TAIL(MASK)                                         \
+#define NBS_WALK64_TAIL(MASK, CNT)                                    \
   "s_lshl_b32 %[t], %[n], 6\n\t"                                      \
   "s_load_dwordx16 s[48:63], %[base], %[t]\n\t"                       \
-  NBS_WALK64_FORCE(MASK)
+  NBS_WALK64_FORCE(MASK, CNT)
 
 #define NBS_WALK64_D2                                                 \
   "v_mul_f64 %[d2], %[dx], %[dx]\n\t"                                 \
@@ -467,144 +503,164 @@ __device__ __forceinline__ void walk_loop_f32_damped(ConstNodes nodes, int n, in
   "v_mul_f64 %[tmp], %[dz], %[dz]\n\t"                                \
   "v_add_f64 %[d2], %[d2], %[tmp]\n\t"
 
+#define NBS_WALK64_LOOP(CNT) \
+      "s_mov_b32 %[m0save], m0\n\t" \
+      "s_mov_b32 %[tie], 0\n\t" \
+      "s_mov_b64 s[46:47], %[A]\n\t" \
+      "s_min_u32 %[lim], %[tpos], %[nend]\n\t" \
+      "s_cmp_lt_i32 %[n], %[nend]\n\t" \
+      "s_cbranch_scc0 L_done%=\n\t" \
+      "s_lshl_b32 %[t], %[n], 6\n\t" \
+      "s_load_dwordx16 s[48:63], %[base], %[t]\n" \
+      "L_loop%=:\n\t" \
+      "s_waitcnt lgkmcnt(0)\n\t" \
+      "s_cmp_gt_i32 s55, -1\n\t" \
+      "s_cbranch_scc1 L_leaf%=\n\t" \
+      "v_add_f64 %[dx], s[48:49], -%[x]\n\t" \
+      "v_add_f64 %[dy], s[50:51], -%[y]\n\t" \
+      "v_add_f64 %[dz], s[52:53], -%[z]\n\t" \
+      "s_mov_b64 %[gm], s[56:57]\n\t" \
+      NBS_WALK64_D2 \
+      "s_cmp_eq_u32 s55, -2\n\t" \
+      "s_cbranch_scc1 L_bucket%=\n\t" \
+      /* cell: t = side^2 - theta^2 d^2, open where t > 1e-9 side^2; a */ \
+      /* taking lane with |t| <= 1e-9 side^2 (or NaN) needs the exact test */ \
+      "v_mul_f64 %[s2], s[58:59], s[58:59]\n\t" \
+      "v_mul_f64 %[tmp], %[th2], %[d2]\n\t" \
+      "v_add_f64 %[tmp], %[s2], -%[tmp]\n\t" \
+      "v_mul_f64 %[s2], %[tolf], %[s2]\n\t" \
+      "v_cmp_gt_f64 vcc, %[tmp], %[s2]\n\t" \
+      "s_and_b64 s[42:43], vcc, s[46:47]\n\t" \
+      "v_cmp_ngt_f64 s[38:39], %[tmp], %[s2]\n\t" \
+      "v_cmp_nlt_f64 vcc, %[tmp], -%[s2]\n\t" \
+      "s_and_b64 s[38:39], s[38:39], vcc\n\t" \
+      "s_and_b64 s[38:39], s[38:39], s[46:47]\n\t" \
+      "s_cbranch_scc1 L_tie%=\n\t" \
+      "s_cmp_lg_u64 s[42:43], 0\n\t" \
+      "s_cbranch_scc1 L_opened%=\n\t" \
+      "s_add_i32 %[n], %[n], 1\n\t" \
+      "s_max_i32 %[n], %[n], s54\n\t" \
+      NBS_WALK64_TAIL("s[46:47]", CNT) \
+      "L_opened%=:\n\t" \
+      "s_add_i32 %[n], %[n], 1\n\t" \
+      "s_andn2_b64 s[44:45], s[46:47], s[42:43]\n\t" \
+      "s_mov_b64 s[46:47], s[42:43]\n\t" \
+      "s_cbranch_scc0 L_noint%=\n\t" \
+      "s_cmp_eq_u32 %[tpos], s54\n\t" \
+      "s_cbranch_scc1 L_merge%=\n\t" \
+      "s_cmp_gt_u32 %[top], 62\n\t" \
+      "s_cbranch_scc1 L_ovf%=\n\t" \
+      "s_mov_b32 m0, %[top]\n\t" \
+      "v_writelane_b32 %[sp], s54, m0\n\t" \
+      "v_writelane_b32 %[slo], s44, m0\n\t" \
+      "v_writelane_b32 %[shi], s45, m0\n\t" \
+      "s_add_u32 %[top], %[top], 1\n\t" \
+      "s_mov_b32 %[tpos], s54\n\t" \
+      "s_min_u32 %[lim], %[tpos], %[nend]\n\t" \
+      "s_branch L_int%=\n" \
+      "L_merge%=:\n\t" \
+      "s_sub_u32 m0, %[top], 1\n\t" \
+      "v_readlane_b32 s38, %[slo], m0\n\t" \
+      "v_readlane_b32 s39, %[shi], m0\n\t" \
+      "s_or_b64 s[38:39], s[38:39], s[44:45]\n\t" \
+      "v_writelane_b32 %[slo], s38, m0\n\t" \
+      "v_writelane_b32 %[shi], s39, m0\n\t" \
+      "s_branch L_int%=\n" \
+      "L_ovf%=:\n\t" \
+      "s_mov_b32 %[ovf], 1\n\t" \
+      "s_branch L_int%=\n" \
+      "L_bucket%=:\n\t" \
+      "s_add_i32 %[n], %[n], 1\n" \
+      /* no lane interacts: the next record, no force */ \
+      "L_noint%=:\n\t" \
+      "s_lshl_b32 %[t], %[n], 6\n\t" \
+      "s_load_dwordx16 s[48:63], %[base], %[t]\n" \
+      "L_next%=:\n\t" \
+      "s_cmp_lt_u32 %[n], %[lim]\n\t" \
+      "s_cbranch_scc1 L_loop%=\n\t" \
+      "s_branch L_slow%=\n" \
+      "L_int%=:\n\t" \
+      NBS_WALK64_TAIL("s[44:45]", CNT) \
+      "L_leaf%=:\n\t" \
+      "v_add_f64 %[dx], s[48:49], -%[x]\n\t" \
+      "v_add_f64 %[dy], s[50:51], -%[y]\n\t" \
+      "v_add_f64 %[dz], s[52:53], -%[z]\n\t" \
+      "s_mov_b64 %[gm], s[56:57]\n\t" \
+      "s_sub_u32 %[t], s55, %[pbeg]\n\t" \
+      "s_lshl_b64 s[38:39], 1, %[t]\n\t" \
+      "s_cmp_lt_u32 %[t], 64\n\t" \
+      "s_cselect_b64 s[38:39], s[38:39], 0\n\t" \
+      "s_andn2_b64 s[44:45], s[46:47], s[38:39]\n\t" \
+      "s_add_i32 %[n], %[n], 1\n\t" \
+      "s_lshl_b32 %[t], %[n], 6\n\t" \
+      "s_load_dwordx16 s[48:63], %[base], %[t]\n\t" \
+      "s_cmp_eq_u64 s[44:45], 0\n\t" \
+      "s_cbranch_scc1 L_next%=\n\t" \
+      NBS_WALK64_D2 \
+      NBS_WALK64_FORCE("s[44:45]", CNT) \
+      "L_tie%=:\n\t" \
+      "s_mov_b32 %[tie], 1\n\t" \
+      "s_branch L_end%=\n" \
+      "L_slow%=:\n\t" \
+      "s_cmp_eq_u32 %[n], %[tpos]\n\t" \
+      "s_cbranch_scc0 L_end%=\n\t" \
+      "s_sub_u32 %[top], %[top], 1\n\t" \
+      "v_readlane_b32 s38, %[slo], %[top]\n\t" \
+      "v_readlane_b32 s39, %[shi], %[top]\n\t" \
+      "s_add_u32 %[t], %[top], 63\n\t" \
+      "s_and_b32 %[t], %[t], 63\n\t" \
+      "v_readlane_b32 %[tpos], %[sp], %[t]\n\t" \
+      "s_or_b64 s[46:47], s[46:47], s[38:39]\n\t" \
+      "s_min_u32 %[lim], %[tpos], %[nend]\n\t" \
+      "s_cmp_lt_u32 %[n], %[lim]\n\t" \
+      "s_cbranch_scc1 L_loop%=\n" \
+      "L_end%=:\n\t" \
+      "s_waitcnt lgkmcnt(0)\n" \
+      "L_done%=:\n\t" \
+      "s_mov_b64 %[A], s[46:47]\n\t" \
+      "s_mov_b32 m0, %[m0save]"
+
+#define NBS_F64_OUTS                                                                      \
+  [n] "+s"(n), [ovf] "+s"(overflow), [A] "+s"(A), [top] "+s"(top), [tpos] "+s"(tpos),     \
+      [ax] "+v"(ax), [ay] "+v"(ay), [az] "+v"(az), [sp] "+v"(sp), [slo] "+v"(slo),        \
+      [shi] "+v"(shi), [lim] "=&s"(lim), [t] "=&s"(t), [tie] "=&s"(tie), [gm] "=&s"(gm),  \
+      [m0save] "=&s"(m0save), [dx] "=&v"(dx), [dy] "=&v"(dy), [dz] "=&v"(dz),              \
+      [d2] "=&v"(d2), [tmp] "=&v"(tmp), [tv] "=&v"(tv), [s2] "=&v"(s2)
+#define NBS_F64_INS                                                                       \
+  [base] "s"(nodes), [nend] "s"(nend), [pbeg] "s"(pbeg), [x] "v"(x), [y] "v"(y),          \
+      [z] "v"(z), [th2] "s"(theta2), [tolf] "s"(1e-9)
+
+template <int CM>
 __device__ __forceinline__ int walk_loop_f64(ConstNodes nodes, int& n, int nend, int pbeg,
                                              unsigned long long& A, int& top, int& tpos,
                                              int& sp, int& slo, int& shi, double x, double y,
                                              double z, double theta2, double& ax, double& ay,
-                                             double& az, int& cnt, int& overflow) {
-  int lim, t, m0save, tie;
+                                             double& az, int& cnt, int& tot, int& overflow) {
+  int lim, t, m0save, tie, tc;
   double gm, dx, dy, dz, d2, tmp, tv, s2;
-  __asm__ volatile(
-      "s_mov_b32 %[m0save], m0\n\t"
-      "s_mov_b32 %[tie], 0\n\t"
-      "s_mov_b64 s[46:47], %[A]\n\t"
-      "s_min_u32 %[lim], %[tpos], %[nend]\n\t"
-      "s_cmp_lt_i32 %[n], %[nend]\n\t"
-      "s_cbranch_scc0 L_done%=\n\t"
-      "s_lshl_b32 %[t], %[n], 6\n\t"
-      "s_load_dwordx16 s[48:63], %[base], %[t]\n"
-      "L_loop%=:\n\t"
-      "s_waitcnt lgkmcnt(0)\n\t"
-      "s_cmp_gt_i32 s55, -1\n\t"
-      "s_cbranch_scc1 L_leaf%=\n\t"
-      "v_add_f64 %[dx], s[48:49], -%[x]\n\t"
-      "v_add_f64 %[dy], s[50:51], -%[y]\n\t"
-      "v_add_f64 %[dz], s[52:53], -%[z]\n\t"
-      "s_mov_b64 %[gm], s[56:57]\n\t"
-      NBS_WALK64_D2
-      "s_cmp_eq_u32 s55, -2\n\t"
-      "s_cbranch_scc1 L_bucket%=\n\t"
-      // cell: t = side^2 - theta^2 d^2, open where t > 1e-9 side^2; a
-      // taking lane with |t| <= 1e-9 side^2 (or NaN) needs the exact test
-      "v_mul_f64 %[s2], s[58:59], s[58:59]\n\t"
-      "v_mul_f64 %[tmp], %[th2], %[d2]\n\t"
-      "v_add_f64 %[tmp], %[s2], -%[tmp]\n\t"
-      "v_mul_f64 %[s2], %[tolf], %[s2]\n\t"
-      "v_cmp_gt_f64 vcc, %[tmp], %[s2]\n\t"
-      "s_and_b64 s[42:43], vcc, s[46:47]\n\t"
-      "v_cmp_ngt_f64 s[38:39], %[tmp], %[s2]\n\t"
-      "v_cmp_nlt_f64 vcc, %[tmp], -%[s2]\n\t"
-      "s_and_b64 s[38:39], s[38:39], vcc\n\t"
-      "s_and_b64 s[38:39], s[38:39], s[46:47]\n\t"
-      "s_cbranch_scc1 L_tie%=\n\t"
-      "s_cmp_lg_u64 s[42:43], 0\n\t"
-      "s_cbranch_scc1 L_opened%=\n\t"
-      "s_add_i32 %[n], %[n], 1\n\t"
-      "s_max_i32 %[n], %[n], s54\n\t"
-      NBS_WALK64_TAIL("s[46:47]")
-      "L_opened%=:\n\t"
-      "s_add_i32 %[n], %[n], 1\n\t"
-      "s_andn2_b64 s[44:45], s[46:47], s[42:43]\n\t"
-      "s_mov_b64 s[46:47], s[42:43]\n\t"
-      "s_cbranch_scc0 L_noint%=\n\t"
-      "s_cmp_eq_u32 %[tpos], s54\n\t"
-      "s_cbranch_scc1 L_merge%=\n\t"
-      "s_cmp_gt_u32 %[top], 62\n\t"
-      "s_cbranch_scc1 L_ovf%=\n\t"
-      "s_mov_b32 m0, %[top]\n\t"
-      "v_writelane_b32 %[sp], s54, m0\n\t"
-      "v_writelane_b32 %[slo], s44, m0\n\t"
-      "v_writelane_b32 %[shi], s45, m0\n\t"
-      "s_add_u32 %[top], %[top], 1\n\t"
-      "s_mov_b32 %[tpos], s54\n\t"
-      "s_min_u32 %[lim], %[tpos], %[nend]\n\t"
-      "s_branch L_int%=\n"
-      "L_merge%=:\n\t"
-      "s_sub_u32 m0, %[top], 1\n\t"
-      "v_readlane_b32 s38, %[slo], m0\n\t"
-      "v_readlane_b32 s39, %[shi], m0\n\t"
-      "s_or_b64 s[38:39], s[38:39], s[44:45]\n\t"
-      "v_writelane_b32 %[slo], s38, m0\n\t"
-      "v_writelane_b32 %[shi], s39, m0\n\t"
-      "s_branch L_int%=\n"
-      "L_ovf%=:\n\t"
-      "s_mov_b32 %[ovf], 1\n\t"
-      "s_branch L_int%=\n"
-      "L_bucket%=:\n\t"
-      "s_add_i32 %[n], %[n], 1\n"
-      // no lane interacts: the next record, no force
-      "L_noint%=:\n\t"
-      "s_lshl_b32 %[t], %[n], 6\n\t"
-      "s_load_dwordx16 s[48:63], %[base], %[t]\n"
-      "L_next%=:\n\t"
-      "s_cmp_lt_u32 %[n], %[lim]\n\t"
-      "s_cbranch_scc1 L_loop%=\n\t"
-      "s_branch L_slow%=\n"
-      "L_int%=:\n\t"
-      NBS_WALK64_TAIL("s[44:45]")
-      "L_leaf%=:\n\t"
-      "v_add_f64 %[dx], s[48:49], -%[x]\n\t"
-      "v_add_f64 %[dy], s[50:51], -%[y]\n\t"
-      "v_add_f64 %[dz], s[52:53], -%[z]\n\t"
-      "s_mov_b64 %[gm], s[56:57]\n\t"
-      "s_sub_u32 %[t], s55, %[pbeg]\n\t"
-      "s_lshl_b64 s[38:39], 1, %[t]\n\t"
-      "s_cmp_lt_u32 %[t], 64\n\t"
-      "s_cselect_b64 s[38:39], s[38:39], 0\n\t"
-      "s_andn2_b64 s[44:45], s[46:47], s[38:39]\n\t"
-      "s_add_i32 %[n], %[n], 1\n\t"
-      "s_lshl_b32 %[t], %[n], 6\n\t"
-      "s_load_dwordx16 s[48:63], %[base], %[t]\n\t"
-      "s_cmp_eq_u64 s[44:45], 0\n\t"
-      "s_cbranch_scc1 L_next%=\n\t"
-      NBS_WALK64_D2
-      NBS_WALK64_FORCE("s[44:45]")
-      "L_tie%=:\n\t"
-      "s_mov_b32 %[tie], 1\n\t"
-      "s_branch L_end%=\n"
-      "L_slow%=:\n\t"
-      "s_cmp_eq_u32 %[n], %[tpos]\n\t"
-      "s_cbranch_scc0 L_end%=\n\t"
-      "s_sub_u32 %[top], %[top], 1\n\t"
-      "v_readlane_b32 s38, %[slo], %[top]\n\t"
-      "v_readlane_b32 s39, %[shi], %[top]\n\t"
-      "s_add_u32 %[t], %[top], 63\n\t"
-      "s_and_b32 %[t], %[t], 63\n\t"
-      "v_readlane_b32 %[tpos], %[sp], %[t]\n\t"
-      "s_or_b64 s[46:47], s[46:47], s[38:39]\n\t"
-      "s_min_u32 %[lim], %[tpos], %[nend]\n\t"
-      "s_cmp_lt_u32 %[n], %[lim]\n\t"
-      "s_cbranch_scc1 L_loop%=\n"
-      "L_end%=:\n\t"
-      "s_waitcnt lgkmcnt(0)\n"
-      "L_done%=:\n\t"
-      "s_mov_b64 %[A], s[46:47]\n\t"
-      "s_mov_b32 m0, %[m0save]"
-      : [n] "+s"(n), [ovf] "+s"(overflow), [A] "+s"(A), [top] "+s"(top), [tpos] "+s"(tpos),
-        [ax] "+v"(ax), [ay] "+v"(ay), [az] "+v"(az), [cnt] "+v"(cnt), [sp] "+v"(sp),
-        [slo] "+v"(slo), [shi] "+v"(shi), [lim] "=&s"(lim), [t] "=&s"(t), [tie] "=&s"(tie),
-        [gm] "=&s"(gm), [m0save] "=&s"(m0save), [dx] "=&v"(dx), [dy] "=&v"(dy),
-        [dz] "=&v"(dz), [d2] "=&v"(d2), [tmp] "=&v"(tmp), [tv] "=&v"(tv), [s2] "=&v"(s2)
-      : [base] "s"(nodes), [nend] "s"(nend), [pbeg] "s"(pbeg), [x] "v"(x), [y] "v"(y),
-        [z] "v"(z), [th2] "s"(theta2), [tolf] "s"(1e-9)
-      : "s38", "s39", "s40", "s41", "s42", "s43", "s44", "s45", "s46", "s47", "s48", "s49",
-        "s50", "s51", "s52", "s53", "s54", "s55", "s56", "s57", "s58", "s59", "s60", "s61",
-        "s62", "s63", "vcc", "scc", "memory");
+  if constexpr (CM == kCountLanes)
+    __asm__ volatile(NBS_WALK64_LOOP(NBS_CNT_LANE)
+                     : NBS_F64_OUTS, [cnt] "+v"(cnt) : NBS_F64_INS : NBS_WALK_CLOBBERS);
+  else if constexpr (CM == kCountWave)
+    __asm__ volatile(NBS_WALK64_LOOP(NBS_CNT_WAVE)
+                     : NBS_F64_OUTS, [tot] "+s"(tot), [tc] "=&s"(tc)
+                     : NBS_F64_INS : NBS_WALK_CLOBBERS);
+  else
+    __asm__ volatile(NBS_WALK64_LOOP(NBS_CNT_NONE)
+                     : NBS_F64_OUTS : NBS_F64_INS : NBS_WALK_CLOBBERS);
   return tie;
 }
+#undef NBS_F64_OUTS
+#undef NBS_F64_INS
+#undef NBS_WALK64_LOOP
+
 #undef NBS_WALK64_D2
 #undef NBS_WALK64_TAIL
 #undef NBS_WALK64_FORCE
+#undef NBS_CNT_LANE
+#undef NBS_CNT_WAVE
+#undef NBS_CNT_NONE
+#undef NBS_WALK_CLOBBERS
 
 // One visit of the FP64 walk (no damping, one particle per lane) at node
 // n with the exact opening test: the generic loop's body for one node,
@@ -756,16 +812,29 @@ walk_kernel(WalkArgs a) {
   }
 
   unsigned visits = 0;
+  // interactions the hand-written loops counted on the scalar unit
+  // (kCountWave); cnt holds the per-lane counts of every other path
+  int wave_count = 0;
   // (the hand-written loops do not count visits: per-tile costs come from
   // the generic loop)
   if (F32 && !FROZEN && P == 1 && a.asm_loop && !a.tile_cost) {
     int ovf = 0;
     float fax = 0.f, fay = 0.f, faz = 0.f;
-    if (DAMP)
-      walk_loop_f32_damped(nodes, n, nend, pbeg, A[0], x[0], y[0], z[0], fax, fay, faz, cnt[0],
-                           ovf);
-    else
-      walk_loop_f32(nodes, n, nend, pbeg, A[0], x[0], y[0], z[0], fax, fay, faz, cnt[0], ovf);
+#define NBS_F32_WALK(CM)                                                                    \
+  if constexpr (DAMP)                                                                       \
+    walk_loop_f32_damped<CM>(nodes, n, nend, pbeg, A[0], x[0], y[0], z[0], fax, fay, faz,   \
+                             cnt[0], wave_count, ovf);                                      \
+  else                                                                                      \
+    walk_loop_f32<CM>(nodes, n, nend, pbeg, A[0], x[0], y[0], z[0], fax, fay, faz, cnt[0],  \
+                      wave_count, ovf);
+    if (a.count_mode == kCountLanes) {
+      NBS_F32_WALK(kCountLanes)
+    } else if (a.count_mode == kCountWave) {
+      NBS_F32_WALK(kCountWave)
+    } else {
+      NBS_F32_WALK(kCountNone)
+    }
+#undef NBS_F32_WALK
     ax[0] = fax;
     ay[0] = fay;
     az[0] = faz;
@@ -774,10 +843,20 @@ walk_kernel(WalkArgs a) {
     int ovf = 0, top = 0, top_pos = -1, st_pos = -1, st_lo = 0, st_hi = 0;
     double fax = 0, fay = 0, faz = 0;
     unsigned long long Acur = A[0];
-    while (walk_loop_f64(nodes, n, nend, pbeg, Acur, top, top_pos, st_pos, st_lo, st_hi, x[0],
-                         y[0], z[0], theta2, fax, fay, faz, cnt[0], ovf))
-      walk_visit_f64(nodes, n, i[0], Acur, top, top_pos, st_pos, st_lo, st_hi, ovf, x[0], y[0],
-                     z[0], theta, theta2, fax, fay, faz, cnt[0]);
+#define NBS_F64_WALK(CM)                                                                    \
+  while (walk_loop_f64<CM>(nodes, n, nend, pbeg, Acur, top, top_pos, st_pos, st_lo, st_hi,  \
+                           x[0], y[0], z[0], theta2, fax, fay, faz, cnt[0], wave_count,     \
+                           ovf))                                                            \
+    walk_visit_f64(nodes, n, i[0], Acur, top, top_pos, st_pos, st_lo, st_hi, ovf, x[0], y[0], \
+                   z[0], theta, theta2, fax, fay, faz, cnt[0]);
+    if (a.count_mode == kCountLanes) {
+      NBS_F64_WALK(kCountLanes)
+    } else if (a.count_mode == kCountWave) {
+      NBS_F64_WALK(kCountWave)
+    } else {
+      NBS_F64_WALK(kCountNone)
+    }
+#undef NBS_F64_WALK
     ax[0] = fax;
     ay[0] = fay;
     az[0] = faz;
@@ -910,7 +989,7 @@ walk_kernel(WalkArgs a) {
       a.ax[ip] = fax;
       a.ay[ip] = fay;
       a.az[ip] = faz;
-      a.interactions[ip] = cnt[p];
+      if (a.count_mode == kCountLanes) a.interactions[ip] = cnt[p];
     }
     if (a.fused_update) {
       if (valid[p]) {
@@ -931,14 +1010,15 @@ walk_kernel(WalkArgs a) {
       if (w0 + p < wend)
         wave_bbox_partial(valid[p], x[p], y[p], z[p], w0 + p, a.n_waves, a.bbox);
     }
-    wtot += wave_sum(valid[p] ? cnt[p] : 0);
+    if (a.count_mode != kCountNone) wtot += wave_sum(valid[p] ? cnt[p] : 0);
   }
+  wtot += wave_count;
   if (a.fused_update && lane == 0 && k == 0) {
     a.sys_dt[s] = a.dt;
     a.sys_time[s] += a.dt;
   }
   // per-wave running interaction count (no contended atomics)
-  if (lane == 0) a.inter_total[w0] += (unsigned long long)wtot;
+  if (lane == 0 && a.count_mode != kCountNone) a.inter_total[w0] += (unsigned long long)wtot;
   if (a.tile_cost && lane == 0) atomicAdd(&a.tile_cost[tile], visits);
   if (hw == 0 && lane == 0) atomicAdd(a.steps_done, 1);
 }
