@@ -116,6 +116,19 @@ int nbs_set_systems(nbs_ctx* ctx, int nsys, const int64_t* offsets);
  * in particle::acceleration, src/particle.h:35). */
 int nbs_get_state(nbs_ctx* ctx, double* pos, double* vel, double* acc);
 
+/* Device-memory forms of nbs_set_state and nbs_get_state (no counterpart in
+ * the reference; for callers whose particles already live on the GPU, such
+ * as PyTorch tensors): the same layouts and meaning, every array in device
+ * memory of the HIP runtime this library is bound to (checked with
+ * hipPointerGetAttributes: a host pointer, or memory of a second HIP
+ * runtime loaded in the same process, is NBS_ERR_ARG before any copy).
+ * The arrays must be ready when the call is made (their producers'
+ * streams synchronized); both return after their copies have completed,
+ * with no transfer over PCIe. */
+int nbs_set_state_device(nbs_ctx* ctx, size_t n, const double* pos,
+                         const double* vel, const double* mass);
+int nbs_get_state_device(nbs_ctx* ctx, double* pos, double* vel, double* acc);
+
 /* Downloads the masses in the caller's order (the mass field of the
  * reference's particle record, src/particle.h:36). */
 int nbs_get_mass(nbs_ctx* ctx, double* mass);
@@ -320,6 +333,11 @@ int nbs_roots(nbs_ctx* ctx, double* center, double* side, int nsys_cap);
 int nbs_adjoint_forward(nbs_ctx* ctx, int nsteps, int every, int record_all);
 int nbs_adjoint_backward(nbs_ctx* ctx, const double* gx_T, const double* gv_T,
                          double* gx0, double* gv0, double* gm);
+/* The same with every array in device memory of this library's HIP runtime
+ * (checked as for nbs_set_state_device); returns after the gradients are
+ * written. */
+int nbs_adjoint_backward_device(nbs_ctx* ctx, const double* gx_T, const double* gv_T,
+                                double* gx0, double* gv0, double* gm);
 
 /* Replays the recorded forward from perturbed initial data with the tree
  * topology and cell decisions of the recorded trajectory (FP64 only): the

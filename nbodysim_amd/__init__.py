@@ -125,6 +125,21 @@ class Simulation:
         self._check(self._L.nbs_set_state(self._h, n, _dp(pos), _dp(vel), _dp(mass)))
         self.n = n
 
+    def set_state_device(self, n, pos_ptr, vel_ptr, mass_ptr):
+        """nbs_set_state_device: n particles from device arrays (addresses
+        of float64 [n][3], [n][3] or 0 for zero velocities, [n]) of the HIP
+        runtime this library is bound to; they must be ready (their streams
+        synchronized) when called."""
+        self._check(self._L.nbs_set_state_device(self._h, int(n), int(pos_ptr),
+                                                 int(vel_ptr) or None, int(mass_ptr)))
+        self.n = int(n)
+
+    def state_device(self, pos_ptr, vel_ptr=0, acc_ptr=0):
+        """nbs_get_state_device: the state in caller order into device
+        arrays (addresses, 0 to skip); returns once they are written."""
+        self._check(self._L.nbs_get_state_device(self._h, int(pos_ptr) or None,
+                                                 int(vel_ptr) or None, int(acc_ptr) or None))
+
     def generate_state(self, n=0, gen_type="cube", seed=0xDEADBEEF, settings=None,
                        systems=None, **kw):
         """Seeded particles generated on the device straight into the state
@@ -329,6 +344,13 @@ class Simulation:
         self._check(self._L.nbs_adjoint_backward(self._h, _dp(gx), _dp(gv), _dp(gx0),
                                                  _dp(gv0), _dp(gm)))
         return gx0, gv0, gm
+
+    def adjoint_backward_device(self, gx_T_ptr, gv_T_ptr, gx0_ptr, gv0_ptr, gm_ptr):
+        """nbs_adjoint_backward_device: seeds and gradients as device array
+        addresses (0 = zero seed / not wanted)."""
+        p = lambda a: int(a) or None
+        self._check(self._L.nbs_adjoint_backward_device(self._h, p(gx_T_ptr), p(gv_T_ptr),
+                                                        p(gx0_ptr), p(gv0_ptr), p(gm_ptr)))
 
     def frozen_forward(self, pos0, vel0, mass):
         """Replays the recorded forward at perturbed inputs (FP64, fixed

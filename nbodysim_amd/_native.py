@@ -76,6 +76,10 @@ SIGNATURES = [
     ("nbs_set_state", ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, c_dp, c_dp, c_dp]),
     ("nbs_set_systems", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_i64p]),
     ("nbs_get_state", ctypes.c_int, [ctypes.c_void_p, c_dp, c_dp, c_dp]),
+    ("nbs_set_state_device", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    ("nbs_get_state_device", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     ("nbs_get_mass", ctypes.c_int, [ctypes.c_void_p, c_dp]),
     ("nbs_snapshot_begin", ctypes.c_int, [ctypes.c_void_p]),
     ("nbs_snapshot_end", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(c_dp),
@@ -114,6 +118,7 @@ SIGNATURES = [
     ("nbs_adjoint_forward", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                            ctypes.c_int]),
     ("nbs_adjoint_backward", ctypes.c_int, [ctypes.c_void_p, c_dp, c_dp, c_dp, c_dp, c_dp]),
+    ("nbs_adjoint_backward_device", ctypes.c_int, [ctypes.c_void_p] * 6),
     ("nbs_frozen_forward", ctypes.c_int, [ctypes.c_void_p, c_dp, c_dp, c_dp, c_dp, c_dp]),
     ("nbs_profile_enable", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("nbs_profile_read", ctypes.c_int, [ctypes.c_void_p, c_dp, ctypes.c_int]),

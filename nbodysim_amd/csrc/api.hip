@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <initializer_list>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -62,6 +63,20 @@ __global__ void scatter_kernel(const double* __restrict__ a, const unsigned* __r
                                int n, double* __restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[id[i]] = a[i];
+}
+
+// [n][3] <-> [3][n] (caller order both ways): the adjoint's seeds and
+// gradients between the caller's device arrays and the engine's layout.
+__global__ void aos_soa_kernel(const double* __restrict__ in, double* __restrict__ out,
+                               int n, int to_soa) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int k = 0; k < 3; k++) {
+    if (to_soa)
+      out[(size_t)k * n + i] = in[3 * (size_t)i + k];
+    else
+      out[3 * (size_t)i + k] = in[(size_t)k * n + i];
+  }
 }
 
 // slot order -> caller order, int widened to int64
@@ -1161,6 +1176,101 @@ void free_snapshots(nbs_ctx* c) {
   c->snap_pending = -1;
 }
 
+int set_state_from(nbs_ctx* c, size_t n, const double* pos, const double* vel,
+                   const double* mass, hipMemcpyKind kind) {
+  if (!c) return NBS_ERR_ARG;
+  if (n > 0 && (!pos || !mass)) return fail(c, NBS_ERR_ARG, "pos and mass are required");
+  if (n > (size_t)0x3fffffff) return fail(c, NBS_ERR_ARG, "n too large");
+  HIPCHK(c, hipSetDevice(c->device));
+  drop_graph(c);
+  int r = alloc_particles(c, (int)n);
+  if (r) return r;
+  c->n = (int)n;
+  c->cur = 0;
+  if (n > 0) {
+    HIPCHK(c, hipMemcpyAsync(c->stage, pos, 3 * n * sizeof(double),
+                             kind, c->stream));
+    if (vel)
+      HIPCHK(c, hipMemcpyAsync(c->stage_v, vel, 3 * n * sizeof(double),
+                               kind, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->stage_m, mass, n * sizeof(double),
+                             kind, c->stream));
+    deinterleave_kernel<<<(int)((n + 255) / 256), 256, 0, c->stream>>>(
+        c->stage, c->stage_v, c->stage_m, (int)n, c->st[0], vel ? 1 : 0);
+    launch_fill_int(c->inter, 0, n, c->stream);
+  }
+  c->nsys_req = 1;
+  c->sys_off_req.clear();
+  r = setup_systems(c);
+  if (r) return r;
+  c->has_state = true;
+  c->acc_valid = false;
+  return NBS_OK;
+}
+
+
+// The state in caller order, [n][3] each, into host arrays or (device)
+// into device arrays of this library's runtime: interleaved straight into
+// the caller's array on the device, through the staging buffers to the
+// host.  Returns after the copies have completed.
+int get_state_into(nbs_ctx* c, double* pos, double* vel, double* acc, bool device) {
+  if (!c) return NBS_ERR_ARG;
+  if (!c->has_state) return fail(c, NBS_ERR_STATE, "no particle state set");
+  HIPCHK(c, hipSetDevice(c->device));
+  const int n = c->n;
+  if (n == 0) return NBS_OK;
+  State& st = c->st[c->cur];
+  const int blocks = (n + 255) / 256;
+  const size_t bytes = 3 * (size_t)n * sizeof(double);
+  if (pos) {
+    interleave_kernel<<<blocks, 256, 0, c->stream>>>(st.x, st.y, st.z, st.id, n,
+                                                     device ? pos : c->stage);
+    if (!device)
+      HIPCHK(c, hipMemcpyAsync(pos, c->stage, bytes, hipMemcpyDeviceToHost, c->stream));
+  }
+  if (vel) {
+    interleave_kernel<<<blocks, 256, 0, c->stream>>>(st.vx, st.vy, st.vz, st.id, n,
+                                                     device ? vel : c->stage_v);
+    if (!device)
+      HIPCHK(c, hipMemcpyAsync(vel, c->stage_v, bytes, hipMemcpyDeviceToHost, c->stream));
+  }
+  if (acc) {
+    if (!c->acc_valid) {
+      if (device)
+        launch_fill_u64((unsigned long long*)acc, 0, 3 * (size_t)n, c->stream);
+      else
+        std::memset(acc, 0, bytes);
+    } else {
+      interleave_kernel<<<blocks, 256, 0, c->stream>>>(c->ax, c->ay, c->az, st.id, n,
+                                                       device ? acc : c->stage);
+      if (!device)
+        HIPCHK(c, hipMemcpyAsync(acc, c->stage, bytes, hipMemcpyDeviceToHost, c->stream));
+    }
+  }
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return NBS_OK;
+}
+
+// Every non-null array is device memory of the HIP runtime this library is
+// bound to (a pointer of another runtime in the same process -- e.g. a
+// PyTorch tensor when PyTorch loaded its own HIP after this library -- or a
+// host pointer is refused before any copy).
+int check_device_arrays(nbs_ctx* c, std::initializer_list<const void*> ptrs) {
+  for (const void* p : ptrs) {
+    if (!p) continue;
+    hipPointerAttribute_t at{};
+    const hipError_t e = hipPointerGetAttributes(&at, p);
+    if (e != hipSuccess || (at.type != hipMemoryTypeDevice && at.type != hipMemoryTypeManaged)) {
+      (void)hipGetLastError();
+      return fail(c, NBS_ERR_ARG,
+                  "not device memory of this library's HIP runtime (a host pointer, or "
+                  "memory of another HIP runtime loaded in the same process)");
+    }
+  }
+  return NBS_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1265,33 +1375,20 @@ int nbs_configure(nbs_ctx* c, const nbs_config* cfg) {
 
 int nbs_set_state(nbs_ctx* c, size_t n, const double* pos, const double* vel,
                   const double* mass) {
+  return set_state_from(c, n, pos, vel, mass, hipMemcpyHostToDevice);
+}
+
+int nbs_set_state_device(nbs_ctx* c, size_t n, const double* pos, const double* vel,
+                         const double* mass) {
   if (!c) return NBS_ERR_ARG;
-  if (n > 0 && (!pos || !mass)) return fail(c, NBS_ERR_ARG, "pos and mass are required");
-  if (n > (size_t)0x3fffffff) return fail(c, NBS_ERR_ARG, "n too large");
-  HIPCHK(c, hipSetDevice(c->device));
-  drop_graph(c);
-  int r = alloc_particles(c, (int)n);
-  if (r) return r;
-  c->n = (int)n;
-  c->cur = 0;
   if (n > 0) {
-    HIPCHK(c, hipMemcpyAsync(c->stage, pos, 3 * n * sizeof(double),
-                             hipMemcpyHostToDevice, c->stream));
-    if (vel)
-      HIPCHK(c, hipMemcpyAsync(c->stage_v, vel, 3 * n * sizeof(double),
-                               hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->stage_m, mass, n * sizeof(double),
-                             hipMemcpyHostToDevice, c->stream));
-    deinterleave_kernel<<<(int)((n + 255) / 256), 256, 0, c->stream>>>(
-        c->stage, c->stage_v, c->stage_m, (int)n, c->st[0], vel ? 1 : 0);
-    launch_fill_int(c->inter, 0, n, c->stream);
+    int r = check_device_arrays(c, {pos, vel, mass});
+    if (r) return r;
   }
-  c->nsys_req = 1;
-  c->sys_off_req.clear();
-  r = setup_systems(c);
+  int r = set_state_from(c, n, pos, vel, mass, hipMemcpyDeviceToDevice);
   if (r) return r;
-  c->has_state = true;
-  c->acc_valid = false;
+  // the caller may reuse its arrays as soon as this returns
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return NBS_OK;
 }
 
@@ -1399,36 +1496,16 @@ int nbs_generate_state(nbs_ctx* c, const nbs_settings* s, int nsys, const int64_
 }
 
 int nbs_get_state(nbs_ctx* c, double* pos, double* vel, double* acc) {
+  return get_state_into(c, pos, vel, acc, false);
+}
+
+int nbs_get_state_device(nbs_ctx* c, double* pos, double* vel, double* acc) {
   if (!c) return NBS_ERR_ARG;
-  if (!c->has_state) return fail(c, NBS_ERR_STATE, "no particle state set");
-  HIPCHK(c, hipSetDevice(c->device));
-  const int n = c->n;
-  if (n == 0) return NBS_OK;
-  State& st = c->st[c->cur];
-  const int blocks = (n + 255) / 256;
-  if (pos) {
-    interleave_kernel<<<blocks, 256, 0, c->stream>>>(st.x, st.y, st.z, st.id, n, c->stage);
-    HIPCHK(c, hipMemcpyAsync(pos, c->stage, 3 * (size_t)n * sizeof(double),
-                             hipMemcpyDeviceToHost, c->stream));
+  if (c->has_state && c->n > 0) {
+    int r = check_device_arrays(c, {pos, vel, acc});
+    if (r) return r;
   }
-  if (vel) {
-    interleave_kernel<<<blocks, 256, 0, c->stream>>>(st.vx, st.vy, st.vz, st.id, n,
-                                                     c->stage_v);
-    HIPCHK(c, hipMemcpyAsync(vel, c->stage_v, 3 * (size_t)n * sizeof(double),
-                             hipMemcpyDeviceToHost, c->stream));
-  }
-  if (acc) {
-    if (!c->acc_valid) {
-      std::memset(acc, 0, 3 * (size_t)n * sizeof(double));
-    } else {
-      interleave_kernel<<<blocks, 256, 0, c->stream>>>(c->ax, c->ay, c->az, st.id, n,
-                                                       c->stage);
-      HIPCHK(c, hipMemcpyAsync(acc, c->stage, 3 * (size_t)n * sizeof(double),
-                               hipMemcpyDeviceToHost, c->stream));
-    }
-  }
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  return NBS_OK;
+  return get_state_into(c, pos, vel, acc, true);
 }
 
 int nbs_get_mass(nbs_ctx* c, double* mass) {
@@ -2356,6 +2433,82 @@ void to_aos(const std::vector<double>& soa, double* aos, size_t n) {
     for (int k = 0; k < 3; k++) aos[3 * i + k] = soa[k * n + i];
 }
 
+// dL/dx_0, dL/dv_0, dL/dm from dL/dx_T, dL/dv_T: host [n][3] / [n] arrays,
+// or (device) device arrays of this library's runtime.
+int adjoint_backward(nbs_ctx* c, const double* gx_T, const double* gv_T, double* gx0,
+                     double* gv0, double* gm, bool device) {
+  if (!c) return NBS_ERR_ARG;
+  {
+    const int r = check_threaded(c, "the adjoint");
+    if (r) return r;
+  }
+  if (!c->adj_steps || c->adj_n != c->n)
+    return fail(c, NBS_ERR_STATE, "run nbs_adjoint_forward first");
+  HIPCHK(c, hipSetDevice(c->device));
+  const size_t n = c->n;
+  hipStream_t s = c->stream;
+  const int nb = (int)((n + 255) / 256);
+  if (device) {
+    for (int k = 0; k < 2; k++) {
+      const double* g = k ? gv_T : gx_T;
+      double* dst = k ? c->g_v : c->g_x;
+      if (g)
+        aos_soa_kernel<<<nb, 256, 0, s>>>(g, dst, (int)n, 1);
+      else
+        launch_fill_u64((unsigned long long*)dst, 0, 3 * n, s);
+    }
+  } else {
+    std::vector<double> h(3 * n, 0.0);
+    if (gx_T) to_soa(gx_T, h, n); else std::fill(h.begin(), h.end(), 0.0);
+    HIPCHK(c, hipMemcpyAsync(c->g_x, h.data(), 3 * n * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    if (gv_T) to_soa(gv_T, h, n); else std::fill(h.begin(), h.end(), 0.0);
+    HIPCHK(c, hipMemcpyAsync(c->g_v, h.data(), 3 * n * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+  }
+  launch_fill_u64((unsigned long long*)c->g_m, 0, n, s);
+  const int T = c->adj_steps, K = c->adj_every;
+  const int nck = (T + K - 1) / K;
+  for (int seg = nck - 1; seg >= 0; seg--) {
+    const int t0 = seg * K, t1 = std::min(T, t0 + K);
+    // recompute the segment's positions from its checkpoint (bit-exact:
+    // the build sorts canonically, so every step repeats the forward)
+    load_state(c, c->ck_x[seg], c->ck_v[seg], c->adj_m);
+    for (int t = t0; t < t1; t++) {
+      save_state(c, c->seg_x + 3 * n * (t - t0), nullptr);
+      if (t + 1 < t1) {
+        int r = run(c, 1);
+        if (r) return r;
+      }
+    }
+    for (int t = t1 - 1; t >= t0; t--) {
+      int r = adj_reverse_step(c, c->seg_x + 3 * n * (t - t0));
+      if (r) return r;
+    }
+  }
+  if (device) {
+    if (gx0) aos_soa_kernel<<<nb, 256, 0, s>>>(c->g_x, gx0, (int)n, 0);
+    if (gv0) aos_soa_kernel<<<nb, 256, 0, s>>>(c->g_v, gv0, (int)n, 0);
+    if (gm) HIPCHK(c, hipMemcpyAsync(gm, c->g_m, n * 8, hipMemcpyDeviceToDevice, s));
+    HIPCHK(c, hipGetLastError());
+    // leave the context at the forward's final state
+    load_state(c, c->fin_x, c->fin_v, c->adj_m);
+    HIPCHK(c, hipStreamSynchronize(s));
+    return NBS_OK;
+  }
+  std::vector<double> hx(3 * n), hv(3 * n), hm(n);
+  HIPCHK(c, hipMemcpyAsync(hx.data(), c->g_x, 3 * n * 8, hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipMemcpyAsync(hv.data(), c->g_v, 3 * n * 8, hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipMemcpyAsync(hm.data(), c->g_m, n * 8, hipMemcpyDeviceToHost, s));
+  // leave the context at the forward's final state
+  load_state(c, c->fin_x, c->fin_v, c->adj_m);
+  HIPCHK(c, hipStreamSynchronize(s));
+  if (gx0) to_aos(hx, gx0, n);
+  if (gv0) to_aos(hv, gv0, n);
+  if (gm) std::memcpy(gm, hm.data(), n * 8);
+  return NBS_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -2422,53 +2575,17 @@ int nbs_adjoint_forward(nbs_ctx* c, int nsteps, int every, int record_all) {
 
 int nbs_adjoint_backward(nbs_ctx* c, const double* gx_T, const double* gv_T,
                          double* gx0, double* gv0, double* gm) {
+  return adjoint_backward(c, gx_T, gv_T, gx0, gv0, gm, false);
+}
+
+int nbs_adjoint_backward_device(nbs_ctx* c, const double* gx_T, const double* gv_T,
+                                double* gx0, double* gv0, double* gm) {
   if (!c) return NBS_ERR_ARG;
-  {
-    const int r = check_threaded(c, "the adjoint");
+  if (c->n > 0) {
+    int r = check_device_arrays(c, {gx_T, gv_T, gx0, gv0, gm});
     if (r) return r;
   }
-  if (!c->adj_steps || c->adj_n != c->n)
-    return fail(c, NBS_ERR_STATE, "run nbs_adjoint_forward first");
-  HIPCHK(c, hipSetDevice(c->device));
-  const size_t n = c->n;
-  hipStream_t s = c->stream;
-  std::vector<double> h(3 * n, 0.0);
-  if (gx_T) to_soa(gx_T, h, n); else std::fill(h.begin(), h.end(), 0.0);
-  HIPCHK(c, hipMemcpyAsync(c->g_x, h.data(), 3 * n * 8, hipMemcpyHostToDevice, s));
-  HIPCHK(c, hipStreamSynchronize(s));
-  if (gv_T) to_soa(gv_T, h, n); else std::fill(h.begin(), h.end(), 0.0);
-  HIPCHK(c, hipMemcpyAsync(c->g_v, h.data(), 3 * n * 8, hipMemcpyHostToDevice, s));
-  launch_fill_u64((unsigned long long*)c->g_m, 0, n, s);
-  const int T = c->adj_steps, K = c->adj_every;
-  const int nck = (T + K - 1) / K;
-  for (int seg = nck - 1; seg >= 0; seg--) {
-    const int t0 = seg * K, t1 = std::min(T, t0 + K);
-    // recompute the segment's positions from its checkpoint (bit-exact:
-    // the build sorts canonically, so every step repeats the forward)
-    load_state(c, c->ck_x[seg], c->ck_v[seg], c->adj_m);
-    for (int t = t0; t < t1; t++) {
-      save_state(c, c->seg_x + 3 * n * (t - t0), nullptr);
-      if (t + 1 < t1) {
-        int r = run(c, 1);
-        if (r) return r;
-      }
-    }
-    for (int t = t1 - 1; t >= t0; t--) {
-      int r = adj_reverse_step(c, c->seg_x + 3 * n * (t - t0));
-      if (r) return r;
-    }
-  }
-  std::vector<double> hx(3 * n), hv(3 * n), hm(n);
-  HIPCHK(c, hipMemcpyAsync(hx.data(), c->g_x, 3 * n * 8, hipMemcpyDeviceToHost, s));
-  HIPCHK(c, hipMemcpyAsync(hv.data(), c->g_v, 3 * n * 8, hipMemcpyDeviceToHost, s));
-  HIPCHK(c, hipMemcpyAsync(hm.data(), c->g_m, n * 8, hipMemcpyDeviceToHost, s));
-  // leave the context at the forward's final state
-  load_state(c, c->fin_x, c->fin_v, c->adj_m);
-  HIPCHK(c, hipStreamSynchronize(s));
-  if (gx0) to_aos(hx, gx0, n);
-  if (gv0) to_aos(hv, gv0, n);
-  if (gm) std::memcpy(gm, hm.data(), n * 8);
-  return NBS_OK;
+  return adjoint_backward(c, gx_T, gv_T, gx0, gv0, gm, true);
 }
 
 int nbs_frozen_forward(nbs_ctx* c, const double* pos0, const double* vel0,
