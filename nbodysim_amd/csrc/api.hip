@@ -451,7 +451,8 @@ int alloc_build(nbs_ctx* c, int cap) {
   b.blkstats = cbuf<int>(c, o, 2 * (size_t)emit_blocks(cap), &e);
   b.offsets = cbuf<int>(c, o, (size_t)cap + 1, &e);
   b.sort_tmp = cbuf<int>(c, o, radix_tmp_ints(cap), &e);
-  b.runs = cbuf<int>(c, o, 2 + 2 * ((size_t)cap / 2 + 1), &e);
+  b.runs = cbuf<int>(c, o, run_list_ints(cap), &e);
+  b.runs_big = run_list_big(cap);
   b.dhist = cbuf<int>(c, o, depth_hist_ints(cap), &e);
   b.doff = cbuf<int>(c, o, depth_hist_ints(cap), &e);
   b.scan_tmp = cbuf<int>(

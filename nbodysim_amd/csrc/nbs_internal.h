@@ -111,7 +111,8 @@ struct BuildBuffers {
   int* close_at;                      // [n] last slot of each cell, by opening rank
   int* sort_tmp;
   int* scan_tmp;
-  int* runs;                          // long runs of equal primary keys
+  int* runs;                          // runs of equal primary keys: run_list_ints(cap)
+  int runs_big;                       // index in runs of the list of runs over 256 entries
   int* flags;                         // device flags word
   int* stats;                         // [2]: nodes, internal cells
   int* blkstats;                      // [2 * emit_blocks(n)]: per emit block deepest leaf, buckets
@@ -132,6 +133,10 @@ int max_prim_levels(int nsys);
 // Gathers the full keys into sorted order and sorts runs of equal primary
 // keys by (full key, caller index).  perm is updated in place; free_perm
 // (the sort's other value array) is scratch.
+// Ints of the run lists for up to cap keys (runs of 9-256 and of more than
+// 256 equal primary keys); the second list starts at run_list_big(cap).
+size_t run_list_ints(size_t cap);
+int run_list_big(size_t cap);
 void launch_fixup(BuildBuffers& b, const unsigned long long* prim,
                   unsigned* perm, unsigned* free_perm, const State& st, int n,
                   hipStream_t s);

@@ -189,51 +189,72 @@ __global__ void gather_keys_kernel(const unsigned* __restrict__ perm, int n,
 // resolution, common only while a cold system collapses) are sorted by
 // (full key, caller index).  One thread per run start: a run of up to
 // kShortRun entries is insertion-sorted in place; a longer one is listed
-// for long_run_kernel.  (Runs are many and short when they occur: a cold
-// 4M-particle shell passing through its centre puts ~90 % of its
-// particles into ~10^5 runs, the longest a few hundred entries.)  runs[0] counts listed runs, runs[1] is
-// long_run_kernel's work counter, runs[2 + 2r], runs[3 + 2r] = start,
-// length of run r.
+// for long_run_kernel, runs of up to kWaveRun entries in one list (one
+// wavefront sorts each) and longer ones in another (one workgroup each).
+// (A cold 4M-particle shell falling through its centre goes from ~10^6
+// runs of 2-8 entries through ~10^5 of 9-64 to ~10^4 of 65-1024 and a
+// thousand longer ones.)  Layout of `runs`: [0] entries of the short list,
+// [1] long_run_kernel's queue counter over the long list, [2] entries of
+// the long list, [3] unused; the short list's (start, length) pairs from
+// [4], the long list's from [big].  Appends are aggregated per wavefront:
+// one atomic per wavefront and list instead of one per run.
 constexpr int kShortRun = 8;
+constexpr int kWaveRun = 256;
+
+__device__ __forceinline__ void append_run(bool on, int beg, int len, int* count,
+                                           int* list) {
+  const unsigned long long m = ballot(on);
+  if (!m) return;
+  const int leader = __ffsll((long long)m) - 1;
+  int base = 0;
+  if (lane_id() == leader) base = atomicAdd(count, __popcll(m));
+  base = __shfl(base, leader, 64);
+  if (on) {
+    const int r = base + mbcnt(m);
+    list[2 * r] = beg;
+    list[2 * r + 1] = len;
+  }
+}
 
 __global__ void fixup_kernel(const unsigned long long* __restrict__ prim,
                              unsigned* __restrict__ perm,
                              unsigned long long* __restrict__ khi,
                              unsigned long long* __restrict__ klo,
                              const unsigned* __restrict__ id, int n,
-                             int* __restrict__ runs) {
-  int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j + 1 >= n) return;
-  const unsigned long long pk = prim[j];
-  if (pk != prim[j + 1]) return;
-  if (j > 0 && prim[j - 1] == pk) return;
-  // last index of the run: galloping, then bisection (prim is sorted)
-  int good = j + 1, bad = n, step = 1;
-  for (;;) {
-    const int probe = good + step;
-    if (probe >= n) break;
-    if (prim[probe] == pk) {
-      good = probe;
-      step <<= 1;
-    } else {
-      bad = probe;
-      break;
+                             int* __restrict__ runs, int big) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  // every lane reaches the appends below (they take a wavefront ballot)
+  int len = 0;
+  if (j + 1 < n) {
+    const unsigned long long pk = prim[j];
+    if (pk == prim[j + 1] && !(j > 0 && prim[j - 1] == pk)) {
+      // last index of the run: galloping, then bisection (prim is sorted)
+      int good = j + 1, bad = n, step = 1;
+      for (;;) {
+        const int probe = good + step;
+        if (probe >= n) break;
+        if (prim[probe] == pk) {
+          good = probe;
+          step <<= 1;
+        } else {
+          bad = probe;
+          break;
+        }
+      }
+      while (bad - good > 1) {
+        const int mid = good + ((bad - good) >> 1);
+        if (prim[mid] == pk)
+          good = mid;
+        else
+          bad = mid;
+      }
+      len = good - j + 1;
     }
   }
-  while (bad - good > 1) {
-    const int mid = good + ((bad - good) >> 1);
-    if (prim[mid] == pk)
-      good = mid;
-    else
-      bad = mid;
-  }
-  const int e = good;
-  if (e - j + 1 > kShortRun) {
-    const int r = atomicAdd(&runs[0], 1);
-    runs[2 + 2 * r] = j;
-    runs[3 + 2 * r] = e - j + 1;
-    return;
-  }
+  append_run(len > kShortRun && len <= kWaveRun, j, len, &runs[0], runs + 4);
+  append_run(len > kWaveRun, j, len, &runs[2], runs + big);
+  if (len < 2 || len > kShortRun) return;
+  const int e = j + len - 1;
   for (int a = j + 1; a <= e; a++) {
     unsigned long long h = khi[a], l = klo[a];
     unsigned p = perm[a];
@@ -255,15 +276,101 @@ __global__ void fixup_kernel(const unsigned long long* __restrict__ prim,
   }
 }
 
-// A listed run sorted by (key_hi, key_lo, caller index) by one workgroup.
-// Up to kRunBitonic entries: a bitonic sort in LDS (the triple is unique,
-// so stability does not arise).  Longer runs: stable LSD radix sort, 8-bit
+// (h, l, q) < (h', l', q') lexicographically: key_hi, key_lo, caller index.
+__device__ __forceinline__ bool run_less(unsigned long long h, unsigned long long l, unsigned q,
+                                         unsigned long long h2, unsigned long long l2,
+                                         unsigned q2) {
+  return h < h2 || (h == h2 && (l < l2 || (l == l2 && q < q2)));
+}
+
+// One wavefront sorts a run of up to 64 * E entries by (key_hi, key_lo,
+// caller index): a bitonic network over elements i = lane * E + e (E
+// consecutive elements per lane), exchanging with shuffles where the
+// partner sits in another lane (j >= E) and within the lane's registers
+// otherwise.  The triple is unique, so the order is total;
+// padding entries (all ones) sort behind the run.
+template <int E>
+__device__ __forceinline__ void wave_sort_run(int len, unsigned long long* __restrict__ h0,
+                                              unsigned long long* __restrict__ l0,
+                                              unsigned* __restrict__ p0,
+                                              const unsigned* __restrict__ id) {
+  const int lane = lane_id();
+  unsigned long long h[E], l[E];
+  unsigned q[E], p[E];
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const int i = lane * E + e;
+    const bool v = i < len;
+    p[e] = v ? p0[i] : 0u;
+    h[e] = v ? h0[i] : ~0ull;
+    l[e] = v ? l0[i] : ~0ull;
+    q[e] = v ? id[p[e]] : ~0u;
+  }
+#pragma unroll
+  for (int k = 2; k <= 64 * E; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      if (j >= E) {
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+          const int i = lane * E + e;
+          const unsigned long long oh = __shfl_xor(h[e], j / E, 64);
+          const unsigned long long ol = __shfl_xor(l[e], j / E, 64);
+          const unsigned oq = __shfl_xor(q[e], j / E, 64);
+          const unsigned op = __shfl_xor(p[e], j / E, 64);
+          const bool want_min = ((i & j) == 0) == ((i & k) == 0);
+          if (want_min == run_less(oh, ol, oq, h[e], l[e], q[e])) {
+            h[e] = oh;
+            l[e] = ol;
+            q[e] = oq;
+            p[e] = op;
+          }
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+          if (e & j) continue;
+          const int f = e | j;
+          const bool asc = ((lane * E + e) & k) == 0;
+          if (asc == run_less(h[f], l[f], q[f], h[e], l[e], q[e])) {
+            const unsigned long long th = h[e], tl = l[e];
+            const unsigned tq = q[e], tp = p[e];
+            h[e] = h[f];
+            l[e] = l[f];
+            q[e] = q[f];
+            p[e] = p[f];
+            h[f] = th;
+            l[f] = tl;
+            q[f] = tq;
+            p[f] = tp;
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const int i = lane * E + e;
+    if (i < len) {
+      h0[i] = h[e];
+      l0[i] = l[e];
+      p0[i] = p[e];
+    }
+  }
+}
+
+// The listed runs sorted by (key_hi, key_lo, caller index).  First the
+// short list: wavefront g of the grid takes runs g, g + G, ... (G
+// wavefronts in the grid) and sorts each in registers (wave_sort_run, no
+// barrier, no atomic).  Then the long list, one run per workgroup from a
+// queue (runs[1]).  Up to kRunBitonic entries: a bitonic sort in LDS (the
+// triple is unique, so stability does not arise).  Longer runs: stable LSD radix sort, 8-bit
 // digits, least significant first (caller index bytes 0-3, key_lo bytes
 // 0-7, key_hi bytes 0-7), skipping digits that are equal over the whole
 // run; entries ping-pong between the run's range of (khi, klo, perm) and
 // the same range of three free arrays.  Each chunk of 1024 entries is
 // ranked stably with ballots (equal digits within a wave) and per-wave
-// digit counts.  Workgroups take runs from a queue (runs[1]).
+// digit counts.
 constexpr int kRunBlock = 1024;
 constexpr int kRunWaves = kRunBlock / 64;
 constexpr int kRunGrid = 1024;
@@ -273,7 +380,7 @@ __global__ void __launch_bounds__(kRunBlock)
     long_run_kernel(int* __restrict__ runs, unsigned long long* __restrict__ khi,
                     unsigned long long* __restrict__ klo, unsigned* __restrict__ perm,
                     unsigned long long* __restrict__ shi, unsigned long long* __restrict__ slo,
-                    unsigned* __restrict__ sperm, const unsigned* __restrict__ id) {
+                    unsigned* __restrict__ sperm, const unsigned* __restrict__ id, int big) {
   __shared__ int hist[256];
   __shared__ int wcnt[kRunWaves][256];
   __shared__ int total[256];
@@ -282,16 +389,27 @@ __global__ void __launch_bounds__(kRunBlock)
   __shared__ unsigned long long bh[kRunBitonic], bl[kRunBitonic];
   __shared__ unsigned bid[kRunBitonic], bp[kRunBitonic];
   const int t = threadIdx.x, lane = lane_id(), w = t >> 6;
-  // no listed run (the usual case): leave without touching the shared queue
+  const int nshort = runs[0], nlong = runs[2];
+  for (int r = blockIdx.x * kRunWaves + w; r < nshort; r += gridDim.x * kRunWaves) {
+    const int beg = runs[4 + 2 * r], len = runs[5 + 2 * r];
+    if (len <= 64)
+      wave_sort_run<1>(len, khi + beg, klo + beg, perm + beg, id);
+    else if (len <= 128)
+      wave_sort_run<2>(len, khi + beg, klo + beg, perm + beg, id);
+    else
+      wave_sort_run<4>(len, khi + beg, klo + beg, perm + beg, id);
+  }
+  // no long run (the usual case): leave without touching the shared queue
   // counter (1024 workgroups' atomics on one word cost ~10 us)
-  if (runs[0] == 0) return;
+  if (nlong == 0) return;
+  const int* lst = runs + big;
   for (;;) {
     if (t == 0) run_sh = atomicAdd(&runs[1], 1);
     __syncthreads();
     const int r = run_sh;
     __syncthreads();
-    if (r >= runs[0]) return;
-    const int beg = runs[2 + 2 * r], len = runs[3 + 2 * r];
+    if (r >= nlong) return;
+    const int beg = lst[2 * r], len = lst[2 * r + 1];
     unsigned long long *h0 = khi + beg, *l0 = klo + beg, *h1 = shi + beg, *l1 = slo + beg;
     unsigned *p0 = perm + beg, *p1 = sperm + beg;
     if (len <= kRunBitonic) {
@@ -814,14 +932,22 @@ void launch_fixup(BuildBuffers& b, const unsigned long long* prim,
   if (n <= 0) return;
   gather_keys_kernel<<<blocks_for(n, kBlock), kBlock, 0, s>>>(
       perm, n, b.khi_u, b.klo_u, b.khi, b.klo);
-  launch_fill_int(b.runs, 0, 2, s);
+  launch_fill_int(b.runs, 0, 4, s);
   fixup_kernel<<<blocks_for(n, kBlock), kBlock, 0, s>>>(prim, perm, b.khi,
                                                         b.klo, st.id, n,
-                                                        b.runs);
+                                                        b.runs, b.runs_big);
   // khi_u / klo_u (already gathered) and the sort's other value array are
   // free until the next build: scratch for the long runs
   long_run_kernel<<<kRunGrid, kRunBlock, 0, s>>>(b.runs, b.khi, b.klo, perm, b.khi_u,
-                                                 b.klo_u, free_perm, st.id);
+                                                 b.klo_u, free_perm, st.id, b.runs_big);
+}
+
+// Runs listed for long_run_kernel have more than kShortRun entries: at
+// most cap / (kShortRun + 1) in the short list, cap / (kWaveRun + 1) in
+// the long one.
+int run_list_big(size_t cap) { return (int)(4 + 2 * (cap / (kShortRun + 1) + 1)); }
+size_t run_list_ints(size_t cap) {
+  return (size_t)run_list_big(cap) + 2 * (cap / (kWaveRun + 1) + 1);
 }
 
 void launch_permute(const State& src, State& dst, const unsigned* perm, int n,

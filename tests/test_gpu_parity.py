@@ -533,6 +533,33 @@ def test_many_long_runs_sorted_concurrently():
     assert rel_err(acc, oacc).max() < 1e-12
 
 
+def test_runs_at_the_sort_path_boundaries():
+    """Runs of equal primary keys whose lengths straddle every sort path:
+    in place (up to 8), one wavefront (9-64, 65-128, 129-256 entries: one,
+    two and four entries per lane), a workgroup's LDS bitonic sort
+    (257-1024) and its radix sort (1025 and more).  Tree as the
+    reference's, forces within 1e-12.  (Equal full keys, where the caller
+    index decides, make depth-42 buckets, which the reference's tree does
+    not have: the buckets case of test_hand_written_walk_equals_generic
+    sorts such a run.)"""
+    rng = np.random.default_rng(33)
+    parts = [rng.uniform(-500, 500, size=(3000, 3))]
+    sizes = [8, 9, 10, 63, 64, 65, 127, 128, 129, 255, 256, 257, 1024, 1025]
+    for m in sizes:
+        c = rng.uniform(-400, 400, size=3)
+        parts.append(c + rng.uniform(0, 2e-6, size=(m, 3)))
+    pos = np.concatenate(parts)
+    pos = pos[rng.permutation(len(pos))]
+    mass = rng.uniform(1e9, 1e10, size=len(pos))
+    print("sort path boundaries: n =", len(pos), flush=True)
+    with sim(theta=0.5) as s:
+        s.set_state(pos, None, mass)
+        acc = s.forces()
+        assert_tree_equal(s.tree_dump(), O.Tree(pos, mass).dump())
+    oacc, _ = O.Tree(pos, mass).forces(theta=0.5)
+    assert rel_err(acc, oacc).max() < 1e-12
+
+
 def test_many_runs_of_equal_primary_keys():
     """Many moderate runs of equal primary sort keys, as a cold shell makes
     while it passes through its centre: 150 clusters of 9-400 particles,
