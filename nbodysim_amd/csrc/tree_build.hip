@@ -359,12 +359,33 @@ __device__ __forceinline__ void wave_sort_run(int len, unsigned long long* __res
   }
 }
 
-// The listed runs sorted by (key_hi, key_lo, caller index).  First the
-// short list: wavefront g of the grid takes runs g, g + G, ... (G
-// wavefronts in the grid) and sorts each in registers (wave_sort_run, no
-// barrier, no atomic).  Then the long list, one run per workgroup from a
-// queue (runs[1]).  Up to kRunBitonic entries: a bitonic sort in LDS (the
-// triple is unique, so stability does not arise).  Longer runs: stable LSD radix sort, 8-bit
+// The short list's runs sorted by (key_hi, key_lo, caller index):
+// wavefront g of the grid takes runs g, g + G, ... (G wavefronts in the
+// grid) and sorts each in registers (wave_sort_run: no barrier, no
+// atomic).  A kernel of its own: the unrolled register sort takes ~128
+// VGPRs, which would cost the workgroup sorts below their occupancy.
+constexpr int kShortBlock = 256;
+constexpr int kShortGrid = 1024;
+__global__ void __launch_bounds__(kShortBlock)
+    short_run_kernel(const int* __restrict__ runs, unsigned long long* __restrict__ khi,
+                     unsigned long long* __restrict__ klo, unsigned* __restrict__ perm,
+                     const unsigned* __restrict__ id) {
+  const int nshort = runs[0];
+  const int waves = gridDim.x * (kShortBlock / 64);
+  for (int r = (blockIdx.x * kShortBlock + threadIdx.x) / 64; r < nshort; r += waves) {
+    const int beg = runs[4 + 2 * r], len = runs[5 + 2 * r];
+    if (len <= 64)
+      wave_sort_run<1>(len, khi + beg, klo + beg, perm + beg, id);
+    else if (len <= 128)
+      wave_sort_run<2>(len, khi + beg, klo + beg, perm + beg, id);
+    else
+      wave_sort_run<4>(len, khi + beg, klo + beg, perm + beg, id);
+  }
+}
+
+// The long list's runs sorted by (key_hi, key_lo, caller index), one run
+// per workgroup from a queue (runs[1]).  Up to kRunBitonic entries: a
+// bitonic sort in LDS (the triple is unique, so stability does not arise).  Longer runs: stable LSD radix sort, 8-bit
 // digits, least significant first (caller index bytes 0-3, key_lo bytes
 // 0-7, key_hi bytes 0-7), skipping digits that are equal over the whole
 // run; entries ping-pong between the run's range of (khi, klo, perm) and
@@ -389,16 +410,7 @@ __global__ void __launch_bounds__(kRunBlock)
   __shared__ unsigned long long bh[kRunBitonic], bl[kRunBitonic];
   __shared__ unsigned bid[kRunBitonic], bp[kRunBitonic];
   const int t = threadIdx.x, lane = lane_id(), w = t >> 6;
-  const int nshort = runs[0], nlong = runs[2];
-  for (int r = blockIdx.x * kRunWaves + w; r < nshort; r += gridDim.x * kRunWaves) {
-    const int beg = runs[4 + 2 * r], len = runs[5 + 2 * r];
-    if (len <= 64)
-      wave_sort_run<1>(len, khi + beg, klo + beg, perm + beg, id);
-    else if (len <= 128)
-      wave_sort_run<2>(len, khi + beg, klo + beg, perm + beg, id);
-    else
-      wave_sort_run<4>(len, khi + beg, klo + beg, perm + beg, id);
-  }
+  const int nlong = runs[2];
   // no long run (the usual case): leave without touching the shared queue
   // counter (1024 workgroups' atomics on one word cost ~10 us)
   if (nlong == 0) return;
@@ -936,6 +948,7 @@ void launch_fixup(BuildBuffers& b, const unsigned long long* prim,
   fixup_kernel<<<blocks_for(n, kBlock), kBlock, 0, s>>>(prim, perm, b.khi,
                                                         b.klo, st.id, n,
                                                         b.runs, b.runs_big);
+  short_run_kernel<<<kShortGrid, kShortBlock, 0, s>>>(b.runs, b.khi, b.klo, perm, st.id);
   // khi_u / klo_u (already gathered) and the sort's other value array are
   // free until the next build: scratch for the long runs
   long_run_kernel<<<kRunGrid, kRunBlock, 0, s>>>(b.runs, b.khi, b.klo, perm, b.khi_u,
