@@ -533,6 +533,37 @@ def test_many_long_runs_sorted_concurrently():
     assert rel_err(acc, oacc).max() < 1e-12
 
 
+def test_graph_steps_as_the_tree_deepens():
+    """Steps captured while the tree is shallow keep building correct trees
+    when it deepens: a pair closing in on itself adds ~9 levels within the
+    replayed steps.  The end state equals eager steps bit for bit."""
+    rng = np.random.default_rng(44)
+    cloud = rng.uniform(-500, 500, size=(20000, 3))
+    pair = np.array([[0.3, 0.2, 0.1], [1.3, 0.2, 0.1]])
+    pos = np.concatenate([cloud, pair])
+    vel = np.zeros_like(pos)
+    # 2.1407 * 0.0333 per step each: 1.0 apart at first, 0.002 at the
+    # eighth build (after 7 steps), past each other after the eighth step
+    vel[-2] = [2.1407, 0.0, 0.0]
+    vel[-1] = [-2.1407, 0.0, 0.0]
+    mass = np.full(len(pos), 1e3)
+    with sim(theta=0.5, dt=0.0333) as s:
+        s.set_state(pos, vel, mass)
+        s.step(2, graph=True)
+        depth0 = s.tree_info()["max_depth"]
+        s.step(6, graph=True)
+        depth1 = s.tree_info()["max_depth"]
+        got = s.state()
+    print("levels: depth %d -> %d" % (depth0, depth1), flush=True)
+    assert depth1 > depth0 + 5
+    with sim(theta=0.5, dt=0.0333) as s:
+        s.set_state(pos, vel, mass)
+        s.step(8)
+        want = s.state()
+    for a, b in zip(got, want):
+        np.testing.assert_array_equal(a, b)
+
+
 def test_runs_at_the_sort_path_boundaries():
     """Runs of equal primary keys whose lengths straddle every sort path:
     in place (up to 8), one wavefront (9-64, 65-128, 129-256 entries: one,
