@@ -260,11 +260,14 @@ int nbs_set_walk_order(nbs_ctx* ctx, int on);
 
 /* Which instruction stream runs the FP32 walk (no counterpart in the
  * reference; for tests and diagnostics): 0 (default) = the hand-written
- * gfx950 loop where it applies (FP32 forces, no damping, one particle per
- * lane), 1 = the compiler-generated loop everywhere.  Both make the same
- * decisions and the same arithmetic in the same order: changes the speed
- * only, never a result bit. */
-int nbs_set_walk_code(nbs_ctx* ctx, int generic);
+ * gfx950 loop where it applies (FP32 forces, one particle per lane),
+ * reading 32-byte node records from 2^21 particles on and the 64-byte
+ * ones below; 1 = the compiler-generated loop everywhere; 2 / 3 = the
+ * hand-written loop on 64-byte / 32-byte records at any size (32-byte
+ * records need fewer than 2^25 nodes).  All make the same decisions and
+ * the same arithmetic in the same order: changes the speed only, never a
+ * result bit. */
+int nbs_set_walk_code(nbs_ctx* ctx, int code);
 
 /* Blocks until the context's stream has drained. */
 int nbs_synchronize(nbs_ctx* ctx);

@@ -285,10 +285,15 @@ class Simulation:
         """Longest-first workgroup dispatch of the walk (default off)."""
         self._check(self._L.nbs_set_walk_order(self._h, int(bool(on))))
 
-    def set_walk_code(self, generic=False):
+    def set_walk_code(self, generic=False, records="auto"):
         """FP32 walk loop: hand-written gfx950 code (default) or the
-        compiler-generated loop (generic=True); results are bit-identical."""
-        self._check(self._L.nbs_set_walk_code(self._h, int(bool(generic))))
+        compiler-generated loop (generic=True); the hand-written loop reads
+        32- or 64-byte node records ("auto": by particle count).  Results
+        are bit-identical."""
+        codes = {"auto": 0, 64: 2, 32: 3}
+        if records not in codes:
+            raise ValueError("records must be 'auto', 32 or 64")
+        self._check(self._L.nbs_set_walk_code(self._h, 1 if generic else codes[records]))
 
     def set_counting(self, mode="particles"):
         """Interaction counting of the walks: "particles" (default; per

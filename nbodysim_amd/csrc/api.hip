@@ -203,6 +203,7 @@ struct nbs_ctx {
   int* slot_of_id = nullptr;
   BuildBuffers bb{};
   WalkNode* nodes = nullptr;
+  WalkNode32* nodes32 = nullptr;  // the FP32 walk's records, or null
   TreeAux aux{};
   Systems sys{};
   std::vector<int> h_start, h_wave_start, h_wave2_start;
@@ -242,6 +243,9 @@ struct nbs_ctx {
   int walk_order = 0;
   // 1: the compiler-generated walk loop even where the hand-written one applies
   int walk_generic = 0;
+  // records of the FP32 hand-written loop: 0 by particle count
+  // (kWalk32MinN), 32 or 64 bytes (nbs_set_walk_code 3 / 2)
+  int walk_records = 0;
   // interaction counting of the walks (nbs_set_counting): per particle by
   // default; lane_counts: `inter` holds the last walk's per-particle counts
   int count_mode = kCountLanes;
@@ -466,6 +470,24 @@ int alloc_build(nbs_ctx* c, int cap) {
   return NBS_OK;
 }
 
+// Particle count from which the FP32 walk reads 32-byte records.  The
+// 2^20-particle sphere's 1.6M 64-byte records (100 MB) fit in MI355X's
+// 256 MB of last-level cache and its walk gains nothing from the smaller
+// ones (which also cost the build 32 B per node); the 2^22-particle shell's
+// 6.2M (400 MB) do not, and its walk takes 4-8 % less time with them.
+constexpr int kWalk32MinN = 1 << 21;
+
+// Whether node arrays of capacity `cap` come with the 32-byte copy: FP32
+// forces only, while its 25-bit skip links can address every node, not in a
+// context carved from the fixed scratch array (its budget holds the full
+// records only), and as the walk code asks.
+bool want_walk32(const nbs_ctx* c, int cap) {
+  if (c->cfg.force_precision != NBS_FORCE_F32 || c->arena.base ||
+      (size_t)cap + 1 >= (size_t)kLink32MaxNodes || c->walk_generic)
+    return false;
+  return c->walk_records == 32 || (c->walk_records == 0 && c->n >= kWalk32MinN);
+}
+
 int alloc_nodes(nbs_ctx* c, int cap) {
   drop_graph(c);
   free_all(c->node_bufs);
@@ -474,6 +496,10 @@ int alloc_nodes(nbs_ctx* c, int cap) {
   // one record beyond the capacity: the hand-written FP32 walk requests the
   // record after a system's last node (unused) without a bounds test
   c->nodes = cbuf<WalkNode>(c, o, (size_t)cap + 1, &e);
+  // the FP32 walk's 32-byte copy of the records
+  c->nodes32 = nullptr;
+  if (want_walk32(c, cap)) c->nodes32 = cbuf<WalkNode32>(c, o, (size_t)cap + 1, &e);
+  c->aux.walk32 = c->nodes32;
   c->aux.parent = cbuf<int>(c, o, cap, &e);
   c->bb.bucket = cbuf<int>(c, o, cap, &e);
   if (e != hipSuccess) {
@@ -561,8 +587,9 @@ void prof_end(nbs_ctx* c, int phase, hipEvent_t ev0) {
 
 int ensure_nodes(nbs_ctx* c) {
   const int want = std::max(c->cfg.node_factor > 0 ? c->cfg.node_factor : 4, 2) * c->n + 4096;
-  if (c->node_cap >= want) return NBS_OK;
-  return alloc_nodes(c, want);
+  if (c->node_cap >= want && (c->nodes32 != nullptr) == want_walk32(c, c->node_cap))
+    return NBS_OK;
+  return alloc_nodes(c, std::max(want, c->node_cap));
 }
 
 // Buffers of the threaded-root build for the current particle count and
@@ -694,7 +721,7 @@ void enqueue_build_threaded(nbs_ctx* c, bool keep_root) {
   if (!c->want_parent) aux.parent = nullptr;
   launch_emit(b, es, c->p, leaves, c->nodes, aux, ne, c->node_cap, s);
   launch_entry_patch(b.offsets, c->t_sp, c->t_new_slot, ne, c->p.theta, n_empty, c->nodes,
-                     b.stats, c->flags, s);
+                     c->nodes32, b.stats, c->flags, s);
   launch_bottom_up(b, c->nodes, aux, ne, 1, s);
   c->n_entries = ne;
 }
@@ -755,6 +782,8 @@ void enqueue_walk(nbs_ctx* c, int fused) {
   State& st = c->st[c->cur];
   WalkArgs a{};
   a.nodes = c->nodes;
+  a.nodes32 = c->nodes32;
+  a.root = c->sys.root;
   a.node_start = c->sys.node_start;
   a.sys_start = c->sys.start;
   a.wave_start = c->sys.wave_start;
@@ -1891,10 +1920,14 @@ int nbs_set_walk_shape(nbs_ctx* c, int block, int groups, int pack) {
   return NBS_OK;
 }
 
-int nbs_set_walk_code(nbs_ctx* c, int generic) {
+int nbs_set_walk_code(nbs_ctx* c, int code) {
   if (!c) return NBS_ERR_ARG;
   drop_graph(c);
-  c->walk_generic = generic ? 1 : 0;
+  if (code < 0 || code > 3)
+    return fail(c, NBS_ERR_ARG, "walk code must be 0 (hand-written), 1 (generic), "
+                                "2 (hand-written, 64-byte records) or 3 (32-byte records)");
+  c->walk_generic = code == 1;
+  c->walk_records = code == 2 ? 64 : (code == 3 ? 32 : 0);
   return NBS_OK;
 }
 
@@ -2634,7 +2667,7 @@ int nbs_frozen_forward(nbs_ctx* c, const double* pos0, const double* vel0,
     gather3_kernel<<<b, 256, 0, s>>>(c->p_x, st.id, (int)n, st.x, st.y, st.z);
     gather3_kernel<<<b, 256, 0, s>>>(c->p_v, st.id, (int)n, st.vx, st.vy, st.vz);
     launch_gather_comp(c->p_m, st.m, st.id, (int)n, 1, s);
-    launch_leaf_refresh(c->bb, st, c->nodes, (int)n, s);
+    launch_leaf_refresh(c->bb, st, c->nodes, c->nodes32, (int)n, s);
     launch_bottom_up(c->bb, c->nodes, c->aux, (int)n, c->sys.nsys, s);
     WalkArgs a{};
     a.nodes = c->nodes;

@@ -22,9 +22,8 @@ constexpr int kLevels = 42;         // key depth: 21 levels per u64 half
 constexpr int kDepthBins = kLevels + 1;  // internal-cell depths 0..42
 constexpr int kWave = 64;
 
-// One tree entry, one 64-byte line.  The walk reads it with wave-uniform
-// scalar loads: the FP32 walk reads bytes 0-31 and 48-55, the FP64 walk
-// bytes 0-47, two loads each.
+// One tree entry, one 64-byte line.  The FP64 walk reads it with one
+// wave-uniform scalar load.
 struct alignas(64) WalkNode {
   double cx, cy, cz;  // centre of mass (src/octree.cpp:252-278)
   int skip;           // pre-order index after this subtree
@@ -37,12 +36,35 @@ struct alignas(64) WalkNode {
 };
 static_assert(sizeof(WalkNode) == 64, "WalkNode must be one 64 B line");
 
+// What the FP32 walk reads of a tree entry, 32 bytes: half a line, so twice
+// the entries per cache line and per L2.  The centre of mass and G * m are
+// the WalkNode's, bit for bit.  `link`: a leaf has bit 31 set and its
+// particle slot in bits 30..0; a cell has its depth in bits 30..25 (a
+// depth-42 bucket is opened by every lane) and its skip in bits 24..0, so
+// the copy exists only while the node capacity stays below 2^25.  The
+// opening radius is not stored: (side / theta)^2 in f32 is the root's
+// value times 4^-depth, exactly, as long as every such value is a normal
+// float, which the walk checks per system.
+struct alignas(32) WalkNode32 {
+  double cx, cy, cz;
+  unsigned link;
+  float gm32;
+};
+static_assert(sizeof(WalkNode32) == 32, "WalkNode32 must be half a 64 B line");
+constexpr unsigned kLink32Leaf = 0x80000000u;
+constexpr int kLink32DepthShift = 25;
+constexpr int kLink32MaxNodes = 1 << 25;  // capacity + 1 must stay below
+__host__ __device__ inline unsigned link32_cell(int depth, int skip) {
+  return ((unsigned)depth << kLink32DepthShift) | (unsigned)skip;
+}
+
 // Per-build tree bookkeeping beside the WalkNode array: each node's parent
 // (-1 for a root), written only when the adjoint needs it (null otherwise).
 // A node's depth, first particle and particle count follow from the
 // pre-order array and its skip links (nbs_tree_dump derives them).
 struct TreeAux {
   int* parent;  // [cap] or null
+  WalkNode32* walk32;  // [cap + 1]: the FP32 walk's copy, or null
 };
 
 // Device-side per-system data (batched mode; one system otherwise).
@@ -155,6 +177,7 @@ void launch_bottom_up(BuildBuffers& b, WalkNode* nodes, TreeAux& aux, int n,
                       int nsys, hipStream_t s);
 // Leaves of the current topology take new particle data.
 void launch_leaf_refresh(BuildBuffers& b, const State& st, WalkNode* nodes,
+                         WalkNode32* walk32,
                          int n, hipStream_t s);
 
 // -------------------------------------------------- threaded-root tree --
@@ -174,8 +197,8 @@ void launch_entry_slots(const int* sp, const int* first, const int* pslot, const
                         const int* orank, int ne, int n, int* new_slot, unsigned* gperm,
                         hipStream_t s);
 void launch_entry_patch(const int* offsets, const int* sp, const int* new_slot, int ne,
-                        double theta, int n_empty, WalkNode* nodes, int* stats,
-                        const int* flags, hipStream_t s);
+                        double theta, int n_empty, WalkNode* nodes, WalkNode32* walk32,
+                        int* stats, const int* flags, hipStream_t s);
 
 // ----------------------------------------------------------------- walk --
 enum CountMode { kCountNone = 0, kCountWave = 1, kCountLanes = 2 };
@@ -212,6 +235,10 @@ struct WalkArgs {
   // frozen decisions (FP64 only; null otherwise): base tree + positions
   const WalkNode* mac_nodes;
   const double *mac_x, *mac_y, *mac_z;
+  // the FP32 hand-written loop's 32-byte records (null: 64-byte records)
+  // and the roots ([4 s + 3] = side of system s) their opening radii follow
+  const WalkNode32* nodes32;
+  const double* root;
 };
 
 // Collision pairs: every (particle i, leaf j) that i's walk reaches -- the

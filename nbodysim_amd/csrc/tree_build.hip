@@ -694,7 +694,9 @@ __global__ void sys_nodes_kernel(const int* __restrict__ start,
 
 // Writes the cells and the leaf that start at slot i (pre-order), files
 // each internal cell into its depth bucket, and fills the leaf's mass/COM
-// from its particle.  Same grid as emit_count_kernel.
+// from its particle.  Same grid as emit_count_kernel.  W32: the FP32
+// walk's 32-byte records too (aux.walk32).
+template <bool W32>
 __global__ void __launch_bounds__(kBlock)
     emit_kernel(const int* __restrict__ start, int nsys, int n,
                 const unsigned long long* __restrict__ khi,
@@ -742,6 +744,7 @@ __global__ void __launch_bounds__(kBlock)
       w.skip = offsets[b + 1];
       w.leaf = bkt ? -2 : -1;
       nodes[node] = w;
+      if constexpr (W32) aux.walk32[node].link = link32_cell(d, w.skip);
       if (d == 0 && aux.parent) aux.parent[node] = -1;
       bucket[r] = node;
       if (bkt) my_bucket++;
@@ -761,6 +764,15 @@ __global__ void __launch_bounds__(kBlock)
     w.skip = leafn + 1;
     w.leaf = i;
     nodes[leafn] = w;
+    if constexpr (W32) {
+      WalkNode32 v;
+      v.cx = w.cx;
+      v.cy = w.cy;
+      v.cz = w.cz;
+      v.link = kLink32Leaf | (unsigned)i;
+      v.gm32 = w.gm32;
+      aux.walk32[leafn] = v;
+    }
     if (ld == 0 && aux.parent) aux.parent[leafn] = -1;
     my_depth = ld;
   }
@@ -801,6 +813,7 @@ __global__ void __launch_bounds__(kBlock)
                         const double* __restrict__ z,
                         const double* __restrict__ m,
                         WalkNode* __restrict__ nodes,
+                        WalkNode32* __restrict__ walk32,
                         const int* __restrict__ flags) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n || *flags) return;
@@ -812,6 +825,12 @@ __global__ void __launch_bounds__(kBlock)
   nodes[leafn].m = mi;
   nodes[leafn].gm = kG * mi;
   nodes[leafn].gm32 = (float)(kG * mi);
+  if (walk32) {
+    walk32[leafn].cx = x[i];
+    walk32[leafn].cy = y[i];
+    walk32[leafn].cz = z[i];
+    walk32[leafn].gm32 = (float)(kG * mi);
+  }
 }
 
 // One internal cell p sums its children (already final) in child order
@@ -824,8 +843,10 @@ __global__ void __launch_bounds__(kBlock)
 // The parent links are written only when parent is not null (adjoint).
 // (Running sums instead of the children's values: 30 VGPRs instead of 106,
 // eight wavefronts per SIMD instead of four for this latency-bound chain.)
+template <bool W32>
 __device__ __forceinline__ void reduce_cell(int p, WalkNode* __restrict__ nodes,
-                                            int* __restrict__ parent) {
+                                            int* __restrict__ parent,
+                                            WalkNode32* __restrict__ walk32) {
 #pragma clang fp contract(off)
   const int end = nodes[p].skip;
   double M = 0, tx = 0, ty = 0, tz = 0;
@@ -854,33 +875,45 @@ __device__ __forceinline__ void reduce_cell(int p, WalkNode* __restrict__ nodes,
   nodes[p].m = M;
   nodes[p].gm = kG * M;
   nodes[p].gm32 = (float)(kG * M);
+  if constexpr (W32) {
+    walk32[p].cx = cxv;
+    walk32[p].cy = cyv;
+    walk32[p].cz = czv;
+    walk32[p].gm32 = (float)(kG * M);
+  }
 }
 
 // One tree level: every internal cell of depth d.  Launched from the
 // deepest level up; kernel boundaries order the levels, so no fences are
-// needed.
+// needed.  W32: the FP32 walk's 32-byte records are written too (a
+// variant of its own, so that the pass without them costs what it did).
+template <bool W32>
 __global__ void __launch_bounds__(kBlock)
     level_kernel(int d, int nblocks, const int* __restrict__ doff,
                  const int* __restrict__ bucket, WalkNode* __restrict__ nodes,
-                 int* __restrict__ parent, const int* __restrict__ flags) {
+                 int* __restrict__ parent, WalkNode32* __restrict__ walk32,
+                 const int* __restrict__ flags) {
   if (*flags) return;
   const int b0 = doff[d * nblocks], b1 = doff[(d + 1) * nblocks];
   for (int q = b0 + blockIdx.x * kBlock + threadIdx.x; q < b1; q += gridDim.x * kBlock)
-    reduce_cell(bucket[q], nodes, parent);
+    reduce_cell<W32>(bucket[q], nodes, parent, walk32);
 }
 
 // Levels d1 down to d0 in one workgroup, a barrier between levels (which
 // makes each level's results visible to the workgroup): the top levels,
 // which hold few cells, in one launch instead of one per level.
 constexpr int kFewBlock = 1024;
+template <bool W32>
 __global__ void __launch_bounds__(kFewBlock)
     few_levels_kernel(int d1, int d0, int nblocks, const int* __restrict__ doff,
                       const int* __restrict__ bucket, WalkNode* __restrict__ nodes,
-                      int* __restrict__ parent, const int* __restrict__ flags) {
+                      int* __restrict__ parent, WalkNode32* __restrict__ walk32,
+                      const int* __restrict__ flags) {
   if (*flags) return;
   for (int d = d1; d >= d0; d--) {
     const int b0 = doff[d * nblocks], b1 = doff[(d + 1) * nblocks];
-    for (int q = b0 + threadIdx.x; q < b1; q += kFewBlock) reduce_cell(bucket[q], nodes, parent);
+    for (int q = b0 + threadIdx.x; q < b1; q += kFewBlock)
+      reduce_cell<W32>(bucket[q], nodes, parent, walk32);
     __syncthreads();
   }
 }
@@ -983,10 +1016,16 @@ void launch_emit(BuildBuffers& b, Systems& sys, const StepParams& p,
                                      b.close_at);
   sys_nodes_kernel<<<blocks_for(sys.nsys + 1, 64), 64, 0, s>>>(
       sys.start, b.offsets, sys.nsys, sys.node_start);
-  emit_kernel<<<nb, kBlock, 0, s>>>(
-      sys.start, sys.nsys, n, b.khi, b.klo, b.offsets, b.doff, b.close_at, sys.root,
-      p.theta, cap, st.x, st.y, st.z, st.m, nodes, aux, b.bucket, b.flags,
-      b.stats, b.blkstats);
+  if (aux.walk32)
+    emit_kernel<true><<<nb, kBlock, 0, s>>>(
+        sys.start, sys.nsys, n, b.khi, b.klo, b.offsets, b.doff, b.close_at, sys.root,
+        p.theta, cap, st.x, st.y, st.z, st.m, nodes, aux, b.bucket, b.flags,
+        b.stats, b.blkstats);
+  else
+    emit_kernel<false><<<nb, kBlock, 0, s>>>(
+        sys.start, sys.nsys, n, b.khi, b.klo, b.offsets, b.doff, b.close_at, sys.root,
+        p.theta, cap, st.x, st.y, st.z, st.m, nodes, aux, b.bucket, b.flags,
+        b.stats, b.blkstats);
 }
 
 void launch_bottom_up(BuildBuffers& b, WalkNode* nodes, TreeAux& aux, int n,
@@ -1000,19 +1039,30 @@ void launch_bottom_up(BuildBuffers& b, WalkNode* nodes, TreeAux& aux, int n,
   // and one workgroup would then reduce millions of cells alone.)
   int top = -1;
   for (long long c = nsys; c <= kFewBlock; c *= 8) top++;
-  for (int d = kLevels; d > top; d--)
-    level_kernel<<<kLevelBlocks, kBlock, 0, s>>>(d, nb, b.doff, b.bucket, nodes,
-                                                 aux.parent, b.flags);
-  if (top >= 0)
-    few_levels_kernel<<<1, kFewBlock, 0, s>>>(top, 0, nb, b.doff, b.bucket, nodes, aux.parent,
-                                              b.flags);
+  const bool w32 = aux.walk32 != nullptr;
+  for (int d = kLevels; d > top; d--) {
+    if (w32)
+      level_kernel<true><<<kLevelBlocks, kBlock, 0, s>>>(d, nb, b.doff, b.bucket, nodes,
+                                                         aux.parent, aux.walk32, b.flags);
+    else
+      level_kernel<false><<<kLevelBlocks, kBlock, 0, s>>>(d, nb, b.doff, b.bucket, nodes,
+                                                          aux.parent, nullptr, b.flags);
+  }
+  if (top >= 0) {
+    if (w32)
+      few_levels_kernel<true><<<1, kFewBlock, 0, s>>>(top, 0, nb, b.doff, b.bucket, nodes,
+                                                      aux.parent, aux.walk32, b.flags);
+    else
+      few_levels_kernel<false><<<1, kFewBlock, 0, s>>>(top, 0, nb, b.doff, b.bucket, nodes,
+                                                       aux.parent, nullptr, b.flags);
+  }
 }
 
 void launch_leaf_refresh(BuildBuffers& b, const State& st, WalkNode* nodes,
-                         int n, hipStream_t s) {
+                         WalkNode32* walk32, int n, hipStream_t s) {
   if (n > 0)
     leaf_refresh_kernel<<<blocks_for(n, kBlock), kBlock, 0, s>>>(
-        b.offsets, n, st.x, st.y, st.z, st.m, nodes, b.flags);
+        b.offsets, n, st.x, st.y, st.z, st.m, nodes, walk32, b.flags);
 }
 
 }  // namespace nbs

@@ -213,7 +213,8 @@ __global__ void __launch_bounds__(kBlock)
 __global__ void __launch_bounds__(kBlock)
     entry_patch_kernel(const int* __restrict__ offsets, const int* __restrict__ sp,
                        const int* __restrict__ new_slot, int ne, double theta,
-                       int n_empty, WalkNode* __restrict__ nodes, int* __restrict__ stats,
+                       int n_empty, WalkNode* __restrict__ nodes,
+                       WalkNode32* __restrict__ walk32, int* __restrict__ stats,
                        const int* __restrict__ flags) {
   const int j = blockIdx.x * kBlock + threadIdx.x;
   if (j >= ne || *flags) return;
@@ -221,6 +222,7 @@ __global__ void __launch_bounds__(kBlock)
   const int p = sp[j];
   if (p >= 0) {
     nodes[leafn].leaf = new_slot[p];
+    if (walk32) walk32[leafn].link = kLink32Leaf | (unsigned)new_slot[p];
   } else {
     WalkNode w = nodes[leafn];
     w.cx = w.cy = w.cz = 0.0;
@@ -230,6 +232,13 @@ __global__ void __launch_bounds__(kBlock)
     w.r2open32 = (float)(ro * ro);
     w.leaf = -1;
     nodes[leafn] = w;
+    if (walk32) {  // a root child: depth 1, no children (skip = next)
+      WalkNode32 v;
+      v.cx = v.cy = v.cz = 0.0;
+      v.link = link32_cell(1, w.skip);
+      v.gm32 = 0.0f;
+      walk32[leafn] = v;
+    }
   }
   if (j == 0) stats[1] += n_empty;
 }
@@ -278,11 +287,11 @@ void launch_entry_slots(const int* sp, const int* first, const int* pslot, const
 }
 
 void launch_entry_patch(const int* offsets, const int* sp, const int* new_slot, int ne,
-                        double theta, int n_empty, WalkNode* nodes, int* stats,
-                        const int* flags, hipStream_t s) {
+                        double theta, int n_empty, WalkNode* nodes, WalkNode32* walk32,
+                        int* stats, const int* flags, hipStream_t s) {
   if (ne > 0)
     entry_patch_kernel<<<blocks_for(ne, kBlock), kBlock, 0, s>>>(
-        offsets, sp, new_slot, ne, theta, n_empty, nodes, stats, flags);
+        offsets, sp, new_slot, ne, theta, n_empty, nodes, walk32, stats, flags);
 }
 
 }  // namespace nbs

@@ -41,6 +41,7 @@ namespace {
 
 
 typedef __attribute__((address_space(4))) const WalkNode* ConstNodes;
+typedef __attribute__((address_space(4))) const WalkNode32* ConstNodes32;
 
 // The fields of a WalkNode the walk reads, loaded as 64-bit words with
 // scalar loads (the node index is wave-uniform); the compiler merges the
@@ -169,14 +170,20 @@ __device__ __forceinline__ void interact(const NodeRegs& nd, double dx, double d
 // (FP64, rounded to FP32), d^2 and the opening test, 8 for the force on
 // the interacting lanes, and scalar code for everything else; the common
 // case (a cell every lane accepts) takes 10 scalar ALU instructions and 3
-// branches.  The walk waits on each next record (the scalar cache misses
-// ~90 % of them), so the next record is requested as early as it is known:
-// for a cell after the opening test, before the force arithmetic; for a
-// leaf (next = this + 1) as soon as the position differences have read the
+// branches on the 64-byte records (WalkNode), 15 on the 32-byte ones
+// (WalkNode32), which decode the skip link and the opening radius from a
+// packed link word but put twice as many records in a cache line and in
+// the L2.  The walk waits on each next record (the scalar cache misses most
+// of them), so the next record is requested as early as it is known: for
+// a cell after the opening test, before the force arithmetic; for a leaf
+// (next = this + 1) as soon as the position differences have read the
 // current record.
 //
-// Fixed SGPRs (clobbered): s[48:63] the 64-byte node record (cx s[48:49],
-// cy s[50:51], cz s[52:53], skip s54, leaf s55, gm32 s60, r2open32 s61);
+// Fixed SGPRs (clobbered): the 64-byte record in s[48:63] (cx s[48:49],
+// cy s[50:51], cz s[52:53], skip s54, leaf s55, gm32 s60, r2open32 s61),
+// or the 32-byte one in s[48:55] (cx s[48:49], cy s[50:51], cz s[52:53],
+// link s54, gm32 s55) with, from a cell's link, its skip s56, its
+// (side/theta)^2 s57 and its depth s58;
 // s[46:47] A (lanes taking the node), s[44:45] I (lanes interacting),
 // s[42:43] O (lanes opening), s[40:41] saved exec, s[38:39] a popped
 // stack mask.  The stack of waiting lanes is entry `slot` in lane `slot`
@@ -186,8 +193,9 @@ __device__ __forceinline__ void interact(const NodeRegs& nd, double dx, double d
 // scalar load is waited for inside the block.
 //
 // The record after the last node of a system is requested (and not used)
-// when the walk ends: the node array holds one record beyond its capacity.
-// Record offsets are 32-bit: callers keep (capacity + 1) * 64 below 2^32.
+// when the walk ends: each record array holds one beyond its capacity.
+// Record offsets are 32-bit: callers keep (capacity + 1) * 64 below 2^32,
+// and the 32-byte records exist only below 2^25 nodes (their skip links).
 
 // Interaction counting on the lanes of MASK (the interacting lanes), one
 // of three forms chosen per launch: a per-lane count (one VALU instruction
@@ -221,9 +229,44 @@ __device__ __forceinline__ void interact(const NodeRegs& nd, double dx, double d
   "s_cbranch_scc1 L_loop%=\n\t"                                       \
   "s_branch L_slow%=\n"
 
-#define NBS_WALK_TAIL(FORCE, MASK, CNT)                               \
+// The two record formats (FMT W64: WalkNode, W32: WalkNode32): the load of
+// record n, the leaf test, where G*m, the skip link and (side/theta)^2
+// are, the decoding a 32-byte cell needs (its skip s56, depth s58 and
+// (side/theta)^2 s57 = the root's value minus 2 * depth in the exponent;
+// rmask 0 keeps an infinite root value), the bucket test and a leaf's
+// particle slot minus pbeg into t.
+#define NBS_REC_LOAD_W64                                              \
   "s_lshl_b32 %[t], %[n], 6\n\t"                                      \
-  "s_load_dwordx16 s[48:63], %[base], %[t]\n\t"                       \
+  "s_load_dwordx16 s[48:63], %[base], %[t]\n\t"
+#define NBS_REC_LOAD_W32                                              \
+  "s_lshl_b32 %[t], %[n], 5\n\t"                                      \
+  "s_load_dwordx8 s[48:55], %[base], %[t]\n\t"
+#define NBS_REC_LEAF_W64 "s_cmp_gt_i32 s55, -1\n\t"
+#define NBS_REC_LEAF_W32 "s_cmp_lt_i32 s54, 0\n\t"
+#define NBS_REC_GM_W64 "s60"
+#define NBS_REC_GM_W32 "s55"
+#define NBS_REC_SKIP_W64 "s54"
+#define NBS_REC_SKIP_W32 "s56"
+#define NBS_REC_R2_W64 "s61"
+#define NBS_REC_R2_W32 "s57"
+#define NBS_REC_DECODE1_W64 ""
+#define NBS_REC_DECODE1_W32                                           \
+  "s_bfe_u32 s58, s54, 0x60019\n\t"                                   \
+  "s_and_b32 s56, s54, 0x1ffffff\n\t"
+#define NBS_REC_DECODE2_W64 ""
+#define NBS_REC_DECODE2_W32                                           \
+  "s_lshl_b32 s57, s58, 24\n\t"                                       \
+  "s_and_b32 s57, s57, %[rmask]\n\t"                                  \
+  "s_sub_u32 s57, %[rbase], s57\n\t"
+#define NBS_REC_BUCKET_W64 "s_cmp_eq_u32 s55, -2\n\t"
+#define NBS_REC_BUCKET_W32 "s_cmp_eq_u32 s58, 42\n\t"
+#define NBS_REC_SLOT_W64 "s_sub_u32 %[t], s55, %[pbeg]\n\t"
+#define NBS_REC_SLOT_W32                                              \
+  "s_and_b32 %[t], s54, 0x7fffffff\n\t"                               \
+  "s_sub_u32 %[t], %[t], %[pbeg]\n\t"
+
+#define NBS_WALK_TAIL(FORCE, MASK, CNT, FMT)                          \
+  NBS_REC_LOAD_##FMT                                                  \
   FORCE(MASK, CNT)
 
 // The damped force (src/main.cpp:318: |a| = G m / (r^2 + exp(-r))), in the
@@ -260,7 +303,7 @@ __device__ __forceinline__ void interact(const NodeRegs& nd, double dx, double d
   "s_cbranch_scc1 L_loop%=\n\t"                                       \
   "s_branch L_slow%=\n"
 
-#define NBS_WALK_F32_LOOP(FORCE, CNT) \
+#define NBS_WALK_F32_LOOP(FORCE, CNT, FMT) \
       "s_mov_b32 %[m0save], m0\n\t" \
       "v_mov_b32 %[sp], -1\n\t" \
       "v_mov_b32 %[slo], 0\n\t" \
@@ -271,34 +314,34 @@ __device__ __forceinline__ void interact(const NodeRegs& nd, double dx, double d
       "s_mov_b32 %[lim], %[nend]\n\t" \
       "s_cmp_lt_i32 %[n], %[nend]\n\t" \
       "s_cbranch_scc0 L_done%=\n\t" \
-      "s_lshl_b32 %[t], %[n], 6\n\t" \
-      "s_load_dwordx16 s[48:63], %[base], %[t]\n" \
+      NBS_REC_LOAD_##FMT \
       "L_loop%=:\n\t" \
       "s_waitcnt lgkmcnt(0)\n\t" \
-      /* leaf field: >= 0 a leaf, -1 a cell, -2 a depth-42 bucket */ \
-      "s_cmp_gt_i32 s55, -1\n\t" \
+      NBS_REC_LEAF_##FMT \
       "s_cbranch_scc1 L_leaf%=\n\t" \
       "v_add_f64 %[dx], s[48:49], -%[x]\n\t" \
       "v_add_f64 %[dy], s[50:51], -%[y]\n\t" \
       "v_add_f64 %[dz], s[52:53], -%[z]\n\t" \
-      "s_mov_b32 %[gm], s60\n\t" \
+      "s_mov_b32 %[gm], " NBS_REC_GM_##FMT "\n\t" \
+      NBS_REC_DECODE1_##FMT \
       "v_cvt_f32_f64 %[fx], %[dx]\n\t" \
       "v_cvt_f32_f64 %[fy], %[dy]\n\t" \
       "v_cvt_f32_f64 %[fz], %[dz]\n\t" \
+      NBS_REC_DECODE2_##FMT \
       "v_mul_f32 %[d2], %[fz], %[fz]\n\t" \
       "v_fmac_f32 %[d2], %[fy], %[fy]\n\t" \
       "v_fmac_f32 %[d2], %[fx], %[fx]\n\t" \
-      "s_cmp_eq_u32 s55, -2\n\t" \
+      NBS_REC_BUCKET_##FMT \
       "s_cbranch_scc1 L_bucket%=\n\t" \
       /* cell: open where d^2 < (side/theta)^2 */ \
-      "v_cmp_gt_f32 vcc, s61, %[d2]\n\t" \
+      "v_cmp_gt_f32 vcc, " NBS_REC_R2_##FMT ", %[d2]\n\t" \
       "s_and_b64 s[42:43], vcc, s[46:47]\n\t" \
       "s_cbranch_scc1 L_opened%=\n\t" \
       /* every lane that takes it accepts it: all interact, jump the */ \
       /* subtree (never backwards, even on a corrupt skip) */ \
       "s_add_i32 %[n], %[n], 1\n\t" \
-      "s_max_i32 %[n], %[n], s54\n\t" \
-      NBS_WALK_TAIL(FORCE, "s[46:47]", CNT) \
+      "s_max_i32 %[n], %[n], " NBS_REC_SKIP_##FMT "\n\t" \
+      NBS_WALK_TAIL(FORCE, "s[46:47]", CNT, FMT) \
       "L_opened%=:\n\t" \
       /* I = A & ~O interact and wait at skip; O descend (A = O); with */ \
       /* no lane in I the force is skipped */ \
@@ -306,16 +349,16 @@ __device__ __forceinline__ void interact(const NodeRegs& nd, double dx, double d
       "s_andn2_b64 s[44:45], s[46:47], s[42:43]\n\t" \
       "s_mov_b64 s[46:47], s[42:43]\n\t" \
       "s_cbranch_scc0 L_noint%=\n\t" \
-      "s_cmp_eq_u32 %[tpos], s54\n\t" \
+      "s_cmp_eq_u32 %[tpos], " NBS_REC_SKIP_##FMT "\n\t" \
       "s_cbranch_scc1 L_merge%=\n\t" \
       "s_cmp_gt_u32 %[top], 62\n\t" \
       "s_cbranch_scc1 L_ovf%=\n\t" \
       "s_mov_b32 m0, %[top]\n\t" \
-      "v_writelane_b32 %[sp], s54, m0\n\t" \
+      "v_writelane_b32 %[sp], " NBS_REC_SKIP_##FMT ", m0\n\t" \
       "v_writelane_b32 %[slo], s44, m0\n\t" \
       "v_writelane_b32 %[shi], s45, m0\n\t" \
       "s_add_u32 %[top], %[top], 1\n\t" \
-      "s_mov_b32 %[tpos], s54\n\t" \
+      "s_mov_b32 %[tpos], " NBS_REC_SKIP_##FMT "\n\t" \
       "s_min_u32 %[lim], %[tpos], %[nend]\n\t" \
       "s_branch L_int%=\n" \
       "L_merge%=:\n\t" \
@@ -333,31 +376,29 @@ __device__ __forceinline__ void interact(const NodeRegs& nd, double dx, double d
       /* every lane opens a bucket; nobody interacts */ \
       "s_add_i32 %[n], %[n], 1\n" \
       "L_noint%=:\n\t" \
-      "s_lshl_b32 %[t], %[n], 6\n\t" \
-      "s_load_dwordx16 s[48:63], %[base], %[t]\n" \
+      NBS_REC_LOAD_##FMT \
       "L_next%=:\n\t" \
       "s_cmp_lt_u32 %[n], %[lim]\n\t" \
       "s_cbranch_scc1 L_loop%=\n\t" \
       "s_branch L_slow%=\n" \
       "L_int%=:\n\t" \
-      NBS_WALK_TAIL(FORCE, "s[44:45]", CNT) \
+      NBS_WALK_TAIL(FORCE, "s[44:45]", CNT, FMT) \
       "L_leaf%=:\n\t" \
       /* leaf: the next entry is known at once, so its record is requested */ \
       /* as soon as the three differences have read this one's position; */ \
       /* every lane that takes the leaf interacts, except its own */ \
-      /* particle's (bit leaf - pbeg when that is below 64) */ \
+      /* particle's (bit slot - pbeg when that is below 64) */ \
       "v_add_f64 %[dx], s[48:49], -%[x]\n\t" \
       "v_add_f64 %[dy], s[50:51], -%[y]\n\t" \
       "v_add_f64 %[dz], s[52:53], -%[z]\n\t" \
-      "s_mov_b32 %[gm], s60\n\t" \
-      "s_sub_u32 %[t], s55, %[pbeg]\n\t" \
+      "s_mov_b32 %[gm], " NBS_REC_GM_##FMT "\n\t" \
+      NBS_REC_SLOT_##FMT \
       "s_lshl_b64 s[38:39], 1, %[t]\n\t" \
       "s_cmp_lt_u32 %[t], 64\n\t" \
       "s_cselect_b64 s[38:39], s[38:39], 0\n\t" \
       "s_andn2_b64 s[44:45], s[46:47], s[38:39]\n\t" \
       "s_add_i32 %[n], %[n], 1\n\t" \
-      "s_lshl_b32 %[t], %[n], 6\n\t" \
-      "s_load_dwordx16 s[48:63], %[base], %[t]\n\t" \
+      NBS_REC_LOAD_##FMT \
       /* only its own particle's lane takes the leaf: no force */ \
       "s_cmp_eq_u64 s[44:45], 0\n\t" \
       "s_cbranch_scc1 L_next%=\n\t" \
@@ -399,61 +440,79 @@ __device__ __forceinline__ void interact(const NodeRegs& nd, double dx, double d
 #define NBS_F32_DAMPED_OUTS                                                               \
   NBS_F32_OUTS, [te] "=&v"(te), [tn] "=&v"(tn), [td] "=&v"(td), [q0] "=&v"(q0),          \
       [q1] "=&v"(q1), [q2] "=&v"(q2), [q3] "=&v"(q3), [q4] "=&v"(q4)
-#define NBS_F32_INS                                                                       \
+#define NBS_F32_INS_W64                                                                   \
   [base] "s"(nodes), [nend] "s"(nend), [pbeg] "s"(pbeg), [A0] "s"(A), [x] "v"(x),          \
       [y] "v"(y), [z] "v"(z)
+#define NBS_F32_INS_W32                                                                   \
+  NBS_F32_INS_W64, [rbase] "s"(__builtin_amdgcn_readfirstlane(rbase)),                     \
+      [rmask] "s"(__builtin_amdgcn_readfirstlane(rmask))
+#define NBS_F32_CLOBBERS_W32                                                              \
+  "s38", "s39", "s40", "s41", "s42", "s43", "s44", "s45", "s46", "s47", "s48", "s49",     \
+      "s50", "s51", "s52", "s53", "s54", "s55", "s56", "s57", "s58", "vcc", "scc", "memory"
 #define NBS_WALK_CLOBBERS                                                                 \
   "s38", "s39", "s40", "s41", "s42", "s43", "s44", "s45", "s46", "s47", "s48", "s49",     \
       "s50", "s51", "s52", "s53", "s54", "s55", "s56", "s57", "s58", "s59", "s60", "s61", \
       "s62", "s63", "vcc", "scc", "memory"
+#define NBS_F32_CLOBBERS_W64 NBS_WALK_CLOBBERS
 
-// CM: kCountLanes adds to cnt on the interacting lanes, kCountWave adds the
-// interacting lanes' number to tot, kCountNone counts nothing.
-template <int CM>
-__device__ __forceinline__ void walk_loop_f32(ConstNodes nodes, int n, int nend, int pbeg,
+// One loop: force law FORCE, counting form CM (kCountLanes adds to cnt on
+// the interacting lanes, kCountWave adds the interacting lanes' number to
+// tot, kCountNone counts nothing), record format FMT.
+#define NBS_F32_ASM(FORCE, OUTS, FMT)                                                     \
+  if constexpr (CM == kCountLanes)                                                        \
+    __asm__ volatile(NBS_WALK_F32_LOOP(FORCE, NBS_CNT_LANE, FMT)                           \
+                     : OUTS, [cnt] "+v"(cnt) : NBS_F32_INS_##FMT : NBS_F32_CLOBBERS_##FMT);  \
+  else if constexpr (CM == kCountWave)                                                    \
+    __asm__ volatile(NBS_WALK_F32_LOOP(FORCE, NBS_CNT_WAVE, FMT)                           \
+                     : OUTS, [tot] "+s"(tot), [tc] "=&s"(tc)                              \
+                     : NBS_F32_INS_##FMT : NBS_F32_CLOBBERS_##FMT);                       \
+  else                                                                                    \
+    __asm__ volatile(NBS_WALK_F32_LOOP(FORCE, NBS_CNT_NONE, FMT)                           \
+                     : OUTS : NBS_F32_INS_##FMT : NBS_F32_CLOBBERS_##FMT);
+
+// W32: `nodes` is the 32-byte record array (rbase, rmask: the opening radii's
+// root value and mask); otherwise the 64-byte one.
+template <int CM, bool W32, class Nodes>
+__device__ __forceinline__ void walk_loop_f32(Nodes nodes, int n, int nend, int pbeg,
                                               unsigned long long A, double x, double y,
                                               double z, float& ax, float& ay, float& az,
-                                              int& cnt, int& tot, int& overflow) {
+                                              int& cnt, int& tot, int& overflow, unsigned rbase,
+                                              unsigned rmask) {
   int top, tpos, lim, t, gm, m0save, tc;
   int sp, slo, shi;
   double dx, dy, dz;
   float fx, fy, fz, d2, tv;
-  if constexpr (CM == kCountLanes)
-    __asm__ volatile(NBS_WALK_F32_LOOP(NBS_WALK_FORCE_PLAIN, NBS_CNT_LANE)
-                     : NBS_F32_OUTS, [cnt] "+v"(cnt) : NBS_F32_INS : NBS_WALK_CLOBBERS);
-  else if constexpr (CM == kCountWave)
-    __asm__ volatile(NBS_WALK_F32_LOOP(NBS_WALK_FORCE_PLAIN, NBS_CNT_WAVE)
-                     : NBS_F32_OUTS, [tot] "+s"(tot), [tc] "=&s"(tc)
-                     : NBS_F32_INS : NBS_WALK_CLOBBERS);
-  else
-    __asm__ volatile(NBS_WALK_F32_LOOP(NBS_WALK_FORCE_PLAIN, NBS_CNT_NONE)
-                     : NBS_F32_OUTS : NBS_F32_INS : NBS_WALK_CLOBBERS);
+  if constexpr (W32) {
+    NBS_F32_ASM(NBS_WALK_FORCE_PLAIN, NBS_F32_OUTS, W32)
+  } else {
+    NBS_F32_ASM(NBS_WALK_FORCE_PLAIN, NBS_F32_OUTS, W64)
+  }
 }
 
-template <int CM>
-__device__ __forceinline__ void walk_loop_f32_damped(ConstNodes nodes, int n, int nend,
-                                                     int pbeg, unsigned long long A, double x,
-                                                     double y, double z, float& ax, float& ay,
+template <int CM, bool W32, class Nodes>
+__device__ __forceinline__ void walk_loop_f32_damped(Nodes nodes, int n, int nend, int pbeg,
+                                                     unsigned long long A, double x, double y,
+                                                     double z, float& ax, float& ay,
                                                      float& az, int& cnt, int& tot,
-                                                     int& overflow) {
+                                                     int& overflow, unsigned rbase,
+                                                     unsigned rmask) {
   int top, tpos, lim, t, gm, m0save, tc;
   int sp, slo, shi;
   double dx, dy, dz;
   float fx, fy, fz, d2, tv, te, tn, td, q0, q1, q2, q3, q4;
-  if constexpr (CM == kCountLanes)
-    __asm__ volatile(NBS_WALK_F32_LOOP(NBS_WALK_FORCE_DAMPED, NBS_CNT_LANE)
-                     : NBS_F32_DAMPED_OUTS, [cnt] "+v"(cnt) : NBS_F32_INS : NBS_WALK_CLOBBERS);
-  else if constexpr (CM == kCountWave)
-    __asm__ volatile(NBS_WALK_F32_LOOP(NBS_WALK_FORCE_DAMPED, NBS_CNT_WAVE)
-                     : NBS_F32_DAMPED_OUTS, [tot] "+s"(tot), [tc] "=&s"(tc)
-                     : NBS_F32_INS : NBS_WALK_CLOBBERS);
-  else
-    __asm__ volatile(NBS_WALK_F32_LOOP(NBS_WALK_FORCE_DAMPED, NBS_CNT_NONE)
-                     : NBS_F32_DAMPED_OUTS : NBS_F32_INS : NBS_WALK_CLOBBERS);
+  if constexpr (W32) {
+    NBS_F32_ASM(NBS_WALK_FORCE_DAMPED, NBS_F32_DAMPED_OUTS, W32)
+  } else {
+    NBS_F32_ASM(NBS_WALK_FORCE_DAMPED, NBS_F32_DAMPED_OUTS, W64)
+  }
 }
+#undef NBS_F32_ASM
 #undef NBS_F32_OUTS
 #undef NBS_F32_DAMPED_OUTS
-#undef NBS_F32_INS
+#undef NBS_F32_INS_W64
+#undef NBS_F32_INS_W32
+#undef NBS_F32_CLOBBERS_W32
+#undef NBS_F32_CLOBBERS_W64
 #undef NBS_WALK_F32_LOOP
 #undef NBS_WALK_TAIL
 #undef NBS_WALK_FORCE_DAMPED
@@ -755,7 +814,7 @@ __device__ __forceinline__ int tile_of_block(int b, int nblocks, int groups) {
 // values it cannot keep into VGPR lanes outside the loop) the walk uses 78;
 // at the 95 it used before, only seven fitted and the FP32 walk took 10 %
 // longer.
-template <bool F32, bool DAMP, bool FROZEN, int BLOCK, int P>
+template <bool F32, bool DAMP, bool FROZEN, int BLOCK, int P, bool W32 = false>
 __global__ void __launch_bounds__(BLOCK)
 __attribute__((amdgpu_waves_per_eu(!F32 && (DAMP || P == 2) ? 1 : 8, 8)))
 walk_kernel(WalkArgs a) {
@@ -815,25 +874,62 @@ walk_kernel(WalkArgs a) {
   // interactions the hand-written loops counted on the scalar unit
   // (kCountWave); cnt holds the per-lane counts of every other path
   int wave_count = 0;
+  // The FP32 hand-written loop reads the 32-byte records in the W32
+  // variant (launched when the context keeps them), else the 64-byte
+  // ones.  A 32-byte cell carries its depth
+  // instead of (side / theta)^2 in f32: that is the root's value
+  // r0 = (float)((side_0 / theta)^2) times 4^-depth exactly (dividing a
+  // power-of-two multiple of side_0 by theta and squaring it scale
+  // exactly, in FP64 and in the rounding to f32) while every value down to
+  // depth 41 is a normal float: r0's exponent field at least 85.  So the
+  // loop subtracts 2 * depth from r0's exponent.  theta = 0 makes r0
+  // infinite, as every cell's value is: rmask then keeps r0.  Any other
+  // root (an overflowing r0, or one too small) takes the generic loop.
   // (the hand-written loops do not count visits: per-tile costs come from
   // the generic loop)
-  if (F32 && !FROZEN && P == 1 && a.asm_loop && !a.tile_cost) {
+  unsigned rbase = 0, rmask = 0;
+  bool hand32 = F32 && !FROZEN && P == 1 && a.asm_loop && !a.tile_cost;
+  if constexpr (W32) {
+    if (hand32) {
+      const double ro = a.root[4 * s + 3] / a.theta;
+      rbase = __float_as_uint((float)(ro * ro));
+      const unsigned ex = (rbase >> 23) & 255u;
+      if (a.theta == 0.0 && rbase == 0x7f800000u) {
+        rmask = 0u;
+      } else if (ex >= 85u && ex < 255u) {
+        rmask = ~0u;
+      } else {
+        hand32 = false;
+      }
+      rbase = __builtin_amdgcn_readfirstlane(rbase);
+      rmask = __builtin_amdgcn_readfirstlane(rmask);
+    }
+  }
+  if (hand32) {
     int ovf = 0;
     float fax = 0.f, fay = 0.f, faz = 0.f;
-#define NBS_F32_WALK(CM)                                                                    \
+    ConstNodes32 nodes32 = (ConstNodes32)a.nodes32;
+#define NBS_F32_WALK(CM, W, ND)                                                             \
   if constexpr (DAMP)                                                                       \
-    walk_loop_f32_damped<CM>(nodes, n, nend, pbeg, A[0], x[0], y[0], z[0], fax, fay, faz,   \
-                             cnt[0], wave_count, ovf);                                      \
+    walk_loop_f32_damped<CM, W>(ND, n, nend, pbeg, A[0], x[0], y[0], z[0], fax, fay, faz,   \
+                                cnt[0], wave_count, ovf, rbase, rmask);                     \
   else                                                                                      \
-    walk_loop_f32<CM>(nodes, n, nend, pbeg, A[0], x[0], y[0], z[0], fax, fay, faz, cnt[0],  \
-                      wave_count, ovf);
+    walk_loop_f32<CM, W>(ND, n, nend, pbeg, A[0], x[0], y[0], z[0], fax, fay, faz, cnt[0],  \
+                         wave_count, ovf, rbase, rmask);
+#define NBS_F32_WALK_FMT(CM)                                                                \
+  if constexpr (W32) {                                                                      \
+    NBS_F32_WALK(CM, true, nodes32)                                                         \
+  } else {                                                                                  \
+    NBS_F32_WALK(CM, false, nodes)                                                          \
+  }
     if (a.count_mode == kCountLanes) {
-      NBS_F32_WALK(kCountLanes)
+      NBS_F32_WALK_FMT(kCountLanes)
     } else if (a.count_mode == kCountWave) {
-      NBS_F32_WALK(kCountWave)
+      NBS_F32_WALK_FMT(kCountWave)
     } else {
-      NBS_F32_WALK(kCountNone)
+      NBS_F32_WALK_FMT(kCountNone)
     }
+#undef NBS_F32_WALK_FMT
 #undef NBS_F32_WALK
     ax[0] = fax;
     ay[0] = fay;
@@ -1299,7 +1395,12 @@ template <int BLOCK, int P>
 void launch_walk_shape(const WalkArgs& a, int f32, int damping, hipStream_t s) {
   const int nhw = P == 1 ? a.n_waves : a.n_waves2;
   const int blocks = (nhw * kWave + BLOCK - 1) / BLOCK;
-  if (f32) {
+  if (f32 && P == 1 && a.nodes32) {
+    if (damping)
+      walk_kernel<true, true, false, BLOCK, 1, true><<<blocks, BLOCK, 0, s>>>(a);
+    else
+      walk_kernel<true, false, false, BLOCK, 1, true><<<blocks, BLOCK, 0, s>>>(a);
+  } else if (f32) {
     if (damping)
       walk_kernel<true, true, false, BLOCK, P><<<blocks, BLOCK, 0, s>>>(a);
     else

@@ -19,10 +19,10 @@ def sim(**kw):
     return nbodysim_amd.Simulation(**kw)
 
 
-def run(mode, pos, vel, mass, steps, graph=False, generic=False, **kw):
+def run(mode, pos, vel, mass, steps, graph=False, generic=False, records="auto", **kw):
     with sim(**kw) as s:
         s.set_counting(mode)
-        s.set_walk_code(generic)
+        s.set_walk_code(generic, records)
         s.set_state(pos, vel, mass)
         acc = s.forces()
         total_forces = s.interactions_total(reset=True)
@@ -33,11 +33,13 @@ def run(mode, pos, vel, mass, steps, graph=False, generic=False, **kw):
     return acc, per, total_forces, (p, v, a), total_steps
 
 
-CASES = [dict(precision="f32"), dict(precision="f32", damping=True),
+CASES = [dict(precision="f32"), dict(precision="f32", records=32),
+         dict(precision="f32", damping=True), dict(precision="f32", damping=True, records=32),
          dict(precision="f64"), dict(precision="f32", generic=True)]
 
 
-@pytest.mark.parametrize("case", CASES, ids=["f32", "f32-damped", "f64", "f32-generic"])
+@pytest.mark.parametrize("case", CASES, ids=["f32", "f32-rec32", "f32-damped",
+                                             "f32-damped-rec32", "f64", "f32-generic"])
 def test_counting_modes_change_no_result_bit(case):
     pos, vel, mass = host.generate(20000, gen_type="sphere", seed=11, **SPHERE)
     kw = dict(theta=0.5, dt=0.0333, **case)

@@ -419,14 +419,14 @@ def test_long_runs_of_equal_primary_keys():
 
 
 def _walk_code_outputs(pos, vel, mass, generic, offsets=None, shape=None, theta=0.5,
-                       precision="f32", damping=False):
+                       precision="f32", damping=False, records="auto"):
     with sim(theta=theta, precision=precision, damping=damping) as s:
         s.set_state(pos, vel, mass)
         if offsets is not None:
             s.set_systems(offsets)
         if shape is not None:
             s.set_walk_shape(*shape)
-        s.set_walk_code(generic)
+        s.set_walk_code(generic, records)
         acc = s.forces()
         cnt = s.interactions(per_particle=True)[1]
         info = s.tree_info()
@@ -438,8 +438,9 @@ def _walk_code_outputs(pos, vel, mass, generic, offsets=None, shape=None, theta=
 @pytest.mark.parametrize("precision", ["f32", "f64", "f32-damped"])
 @pytest.mark.parametrize("case", ["sphere", "cluster", "buckets", "batched", "theta0"])
 def test_hand_written_walk_equals_generic(case, precision):
-    """The hand-written gfx950 walk loops (FP32, FP32 with damping and FP64)
-    make the generic loop's lane decisions and arithmetic in the same order: forces,
+    """The hand-written gfx950 walk loops (FP32 on 64- and on 32-byte node
+    records, FP32 with damping, FP64) make the generic loop's lane
+    decisions and arithmetic in the same order: forces,
     interaction counts and four steps (eager and graph-replayed) agree bit
     for bit.  The cases
     reach every branch of the loop: a partly empty last wave (30001
@@ -474,16 +475,18 @@ def test_hand_written_walk_equals_generic(case, precision):
         theta = 0.0
     damping = precision.endswith("-damped")
     precision = precision.split("-")[0]
-    got, info = _walk_code_outputs(pos, vel, mass, False, offsets, shape, theta, precision,
-                                   damping)
     want, _ = _walk_code_outputs(pos, vel, mass, True, offsets, shape, theta, precision,
                                  damping)
-    if case == "buckets":
-        assert info["n_buckets"] > 0
-    for a, b in zip(got, want):
-        np.testing.assert_array_equal(a, b)
-    if case == "theta0":
-        assert (got[1] == len(pos) - 1).all()
+    # the FP32 loop on both record formats (the FP64 loop reads 64 bytes)
+    for records in ((64, 32) if precision == "f32" else ("auto",)):
+        got, info = _walk_code_outputs(pos, vel, mass, False, offsets, shape, theta,
+                                       precision, damping, records)
+        if case == "buckets":
+            assert info["n_buckets"] > 0
+        for a, b in zip(got, want):
+            np.testing.assert_array_equal(a, b, err_msg=str(records))
+        if case == "theta0":
+            assert (got[1] == len(pos) - 1).all()
 
 
 def test_fp64_walk_exact_opening_tie():

@@ -79,6 +79,8 @@ def test_threaded_tree_and_forces_match_oracle():
             assert_tree_equal(s.tree_dump(), tree.dump(), name)
             s.configure(precision="f32")
             acc32 = s.forces()
+            s.set_walk_code(records=32)  # the walk's 32-byte records, patched too
+            np.testing.assert_array_equal(s.forces(), acc32, err_msg=name)
         np.testing.assert_array_equal(cnt, ocnt, err_msg=name)
         assert rel_err(acc, oacc).max() < 1e-12, name
         e = rel_err(acc32, oacc)
