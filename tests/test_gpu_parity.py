@@ -82,6 +82,25 @@ def test_forces_f32_precision():
     assert e.max() < 1e-3
 
 
+def test_forces_f32_far_from_the_origin():
+    """FP32 differences are formed in FP64 and then rounded, so a system far
+    from the origin keeps the precision of one at the origin (in absolute
+    f32 coordinates a 3e4 offset would leave ~2e-3 of rounding in every
+    position, ~1e-3 relative in the near field)."""
+    pos, vel, mass = O.generate(8192, "sphere")
+    errs = []
+    for shift in ((0.0, 0.0, 0.0), (3e4, -2e4, 1e4)):
+        p = pos + np.array(shift)
+        oacc, _ = O.Tree(p, mass).forces(theta=0.5)
+        with sim(theta=0.5, precision="f32") as s:
+            s.set_state(p, vel, mass)
+            acc = s.forces()
+        e = rel_err(acc, oacc)
+        errs.append(np.sqrt(np.mean(e ** 2)))
+        assert errs[-1] < 1e-5 and e.max() < 1e-3, shift
+    assert errs[1] < 3 * errs[0], errs
+
+
 @pytest.mark.parametrize("theta", [0.0, 0.3, 0.7, 1.0])
 def test_theta_sweep_f64(theta):
     pos, vel, mass = O.generate(3000, "shell")
