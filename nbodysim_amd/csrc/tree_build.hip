@@ -188,42 +188,86 @@ __global__ void gather_keys_kernel(const unsigned* __restrict__ perm, int n,
 // Runs of equal primary keys (particles closer than the primary key's
 // resolution, common only while a cold system collapses) are sorted by
 // (full key, caller index).  One thread per run start: a run of up to
-// kShortRun entries is insertion-sorted in place; a longer one is listed
-// for long_run_kernel, runs of up to kWaveRun entries in one list (one
-// wavefront sorts each) and longer ones in another (one workgroup each).
+// kShortRun entries is sorted by that thread in registers; a longer one is
+// listed, runs of up to kWaveRun entries in one list (short_run_kernel, one
+// wavefront each) and longer ones in another (long_run_kernel, one
+// workgroup each).
 // (A cold 4M-particle shell falling through its centre goes from ~10^6
 // runs of 2-8 entries through ~10^5 of 9-64 to ~10^4 of 65-1024 and a
 // thousand longer ones.)  Layout of `runs`: [0] entries of the short list,
 // [1] long_run_kernel's queue counter over the long list, [2] entries of
 // the long list, [3] unused; the short list's (start, length) pairs from
-// [4], the long list's from [big].  Appends are aggregated per wavefront:
-// one atomic per wavefront and list instead of one per run.
+// [4], the long list's from [big].  Appends are aggregated per workgroup
+// (append_runs).
 constexpr int kShortRun = 8;
 constexpr int kWaveRun = 256;
 
-__device__ __forceinline__ void append_run(bool on, int beg, int len, int* count,
-                                           int* list) {
-  const unsigned long long m = ballot(on);
-  if (!m) return;
-  const int leader = __ffsll((long long)m) - 1;
-  int base = 0;
-  if (lane_id() == leader) base = atomicAdd(count, __popcll(m));
-  base = __shfl(base, leader, 64);
-  if (on) {
-    const int r = base + mbcnt(m);
-    list[2 * r] = beg;
-    list[2 * r + 1] = len;
+// Appends go through the workgroup: per-wave counts in LDS, one
+// device-scope atomic per workgroup and list, offsets back through LDS.
+// (One atomic per wavefront on the two shared counters cost ~0.6 ms per
+// build in a 4M-particle collapse, where nearly every wavefront holds a
+// run: atomics on one word serialise across the XCDs.)  Every thread of
+// the workgroup calls it.
+constexpr int kFixBlock = 1024;
+constexpr int kFixWaves = kFixBlock / kWave;
+
+__device__ __forceinline__ void append_runs(bool to_short, bool to_long, int beg, int len,
+                                            int* runs, int big) {
+  __shared__ int sh[2][kFixWaves + 1];
+  const unsigned long long m0 = ballot(to_short), m1 = ballot(to_long);
+  const int w = threadIdx.x >> 6;
+  if (lane_id() == 0) {
+    sh[0][w] = __popcll(m0);
+    sh[1][w] = __popcll(m1);
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {  // list q: wave offsets, then the workgroup's base
+    const int q = threadIdx.x;
+    int t = 0;
+    for (int k = 0; k < kFixWaves; k++) {
+      const int c = sh[q][k];
+      sh[q][k] = t;
+      t += c;
+    }
+    sh[q][kFixWaves] = t ? atomicAdd(&runs[2 * q], t) : 0;
+  }
+  __syncthreads();
+  if (to_short) {
+    const int r = sh[0][kFixWaves] + sh[0][w] + mbcnt(m0);
+    runs[4 + 2 * r] = beg;
+    runs[4 + 2 * r + 1] = len;
+  }
+  if (to_long) {
+    const int r = sh[1][kFixWaves] + sh[1][w] + mbcnt(m1);
+    runs[big + 2 * r] = beg;
+    runs[big + 2 * r + 1] = len;
   }
 }
 
-__global__ void fixup_kernel(const unsigned long long* __restrict__ prim,
-                             unsigned* __restrict__ perm,
-                             unsigned long long* __restrict__ khi,
-                             unsigned long long* __restrict__ klo,
-                             const unsigned* __restrict__ id, int n,
-                             int* __restrict__ runs, int big) {
+// (h, l, q) < (h', l', q') lexicographically: key_hi, key_lo, caller index.
+__device__ __forceinline__ bool run_less(unsigned long long h, unsigned long long l, unsigned q,
+                                         unsigned long long h2, unsigned long long l2,
+                                         unsigned q2) {
+  return h < h2 || (h == h2 && (l < l2 || (l == l2 && q < q2)));
+}
+
+// Compare-exchange of slots a < b of a run held in registers.
+__device__ __forceinline__ void run_cx(unsigned long long* H, unsigned long long* L, unsigned* Q,
+                                       unsigned* P, int a, int b) {
+  if (run_less(H[b], L[b], Q[b], H[a], L[a], Q[a])) {
+    const unsigned long long th = H[a], tl = L[a];
+    const unsigned tq = Q[a], tp = P[a];
+    H[a] = H[b]; L[a] = L[b]; Q[a] = Q[b]; P[a] = P[b];
+    H[b] = th; L[b] = tl; Q[b] = tq; P[b] = tp;
+  }
+}
+
+__global__ void __launch_bounds__(kFixBlock)
+    fixup_kernel(const unsigned long long* __restrict__ prim, unsigned* __restrict__ perm,
+                 unsigned long long* __restrict__ khi, unsigned long long* __restrict__ klo,
+                 const unsigned* __restrict__ id, int n, int* __restrict__ runs, int big) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  // every lane reaches the appends below (they take a wavefront ballot)
+  // every thread reaches the appends below (workgroup barriers)
   int len = 0;
   if (j + 1 < n) {
     const unsigned long long pk = prim[j];
@@ -251,36 +295,38 @@ __global__ void fixup_kernel(const unsigned long long* __restrict__ prim,
       len = good - j + 1;
     }
   }
-  append_run(len > kShortRun && len <= kWaveRun, j, len, &runs[0], runs + 4);
-  append_run(len > kWaveRun, j, len, &runs[2], runs + big);
+  append_runs(len > kShortRun && len <= kWaveRun, len > kWaveRun, j, len, runs, big);
   if (len < 2 || len > kShortRun) return;
-  const int e = j + len - 1;
-  for (int a = j + 1; a <= e; a++) {
-    unsigned long long h = khi[a], l = klo[a];
-    unsigned p = perm[a];
-    unsigned pid = id[p];
-    int b = a - 1;
-    while (b >= j) {
-      unsigned long long bh = khi[b], bl = klo[b];
-      unsigned bid = id[perm[b]];
-      bool greater = bh > h || (bh == h && (bl > l || (bl == l && bid > pid)));
-      if (!greater) break;
-      khi[b + 1] = bh;
-      klo[b + 1] = bl;
-      perm[b + 1] = perm[b];
-      b--;
-    }
-    khi[b + 1] = h;
-    klo[b + 1] = l;
-    perm[b + 1] = p;
+  // The run in registers (its loads issued together, not one dependent
+  // chain per insertion), eight slots, those past the run holding a key
+  // above every real one (key_hi never has bit 63 set), sorted by a fixed
+  // 19-comparator network of depth 6 and written back.  The key includes
+  // the caller index, so the order is total and equals any other sort's.
+  static_assert(kShortRun == 8, "the network sorts eight slots");
+  unsigned long long H[kShortRun], L[kShortRun];
+  unsigned Q[kShortRun], P[kShortRun];
+#pragma unroll
+  for (int k = 0; k < kShortRun; k++) {
+    const bool in = k < len;
+    H[k] = in ? khi[j + k] : ~0ull;
+    L[k] = in ? klo[j + k] : ~0ull;
+    P[k] = in ? perm[j + k] : 0u;
   }
-}
-
-// (h, l, q) < (h', l', q') lexicographically: key_hi, key_lo, caller index.
-__device__ __forceinline__ bool run_less(unsigned long long h, unsigned long long l, unsigned q,
-                                         unsigned long long h2, unsigned long long l2,
-                                         unsigned q2) {
-  return h < h2 || (h == h2 && (l < l2 || (l == l2 && q < q2)));
+#pragma unroll
+  for (int k = 0; k < kShortRun; k++) Q[k] = k < len ? id[P[k]] : ~0u;
+  constexpr int kNet[19][2] = {{0, 2}, {1, 3}, {4, 6}, {5, 7}, {0, 4}, {1, 5}, {2, 6},
+                               {3, 7}, {0, 1}, {2, 3}, {4, 5}, {6, 7}, {2, 4}, {3, 5},
+                               {1, 4}, {3, 6}, {1, 2}, {3, 4}, {5, 6}};
+#pragma unroll
+  for (int c = 0; c < 19; c++) run_cx(H, L, Q, P, kNet[c][0], kNet[c][1]);
+#pragma unroll
+  for (int k = 0; k < kShortRun; k++) {
+    if (k < len) {
+      khi[j + k] = H[k];
+      klo[j + k] = L[k];
+      perm[j + k] = P[k];
+    }
+  }
 }
 
 // One wavefront sorts a run of up to 64 * E entries by (key_hi, key_lo,
@@ -978,7 +1024,7 @@ void launch_fixup(BuildBuffers& b, const unsigned long long* prim,
   gather_keys_kernel<<<blocks_for(n, kBlock), kBlock, 0, s>>>(
       perm, n, b.khi_u, b.klo_u, b.khi, b.klo);
   launch_fill_int(b.runs, 0, 4, s);
-  fixup_kernel<<<blocks_for(n, kBlock), kBlock, 0, s>>>(prim, perm, b.khi,
+  fixup_kernel<<<blocks_for(n, kFixBlock), kFixBlock, 0, s>>>(prim, perm, b.khi,
                                                         b.klo, st.id, n,
                                                         b.runs, b.runs_big);
   short_run_kernel<<<kShortGrid, kShortBlock, 0, s>>>(b.runs, b.khi, b.klo, perm, st.id);
