@@ -86,6 +86,14 @@ __device__ __forceinline__ double sq_sum_ref(double dx, double dy, double dz) {
   return dx * dx + dy * dy + dz * dz;  // src/vector.cpp:146 rounding
 }
 
+// |d|^2 of the FP64 force: two fused multiply-adds (the opening decision
+// keeps the reference's rounding, sq_sum_ref; the force needs no bit
+// pattern of the reference's, only its value to well within 1e-12).
+__device__ __forceinline__ double sq_sum_fused(double dx, double dy, double dz) {
+#pragma clang fp contract(off)
+  return fma(dx, dx, fma(dy, dy, dz * dz));
+}
+
 // Called wave-uniformly (every lane of the wave, divergence-free).  The
 // exact test runs only when some lane is within the margin; the empty asm
 // keeps the compiler from hoisting the sqrt and division (cheap-looking
@@ -146,7 +154,7 @@ __device__ __forceinline__ void interact(const NodeRegs& nd, double dx, double d
     ay = fmaf(sc, fdy, (float)ay);
     az = fmaf(sc, fdz, (float)az);
   } else {
-    const double d2 = sq_sum_ref(dx, dy, dz);
+    const double d2 = sq_sum_fused(dx, dy, dz);
     const double r = rsqrt_f64(d2);
     double sc;
     if (DAMP) {
@@ -520,17 +528,28 @@ __device__ __forceinline__ void walk_loop_f32_damped(Nodes nodes, int n, int nen
 
 // The FP64 walk loop (no damping, one particle per lane) written out for
 // gfx950, the FP64 counterpart of walk_loop_f32 with the same registers,
-// traversal and stack.  Per visit: 3 FP64 subtracts and the unfused
-// d^2 = (dx^2 + dy^2) + dz^2 of the reference's rounding (5), the opening
-// test s^2 - theta^2 d^2 > 1e-9 s^2 (4 and two compares) and, for the
-// interacting lanes, 1/r from v_rsq_f64 with one Newton step and the force
-// (12), the operations and their order those of the generic loop
-// (interact<false, false>, mac_open_f64).  When a lane that takes a cell
-// is within the margin of the opening boundary the loop returns 1 before
-// touching the cell (n, A and the stack describe the walk at that cell);
-// the caller then visits it with the exact test (walk_visit_f64) and calls
-// the loop again.  It returns 0 when the walk is done.  The record's FP64
-// mass-times-G is copied to `gm` before the next record is requested.
+// traversal and stack.  Per visit: 3 FP64 subtracts, d^2 fused (3), and
+// for a cell the margin test below (a conversion and two compares); for
+// the interacting lanes 1/r from v_rsq_f64 with one Newton step and the
+// force (12), the operations and their order those of the generic loop
+// (interact<false, false>).
+//
+// The margin test settles every lane whose d^2 is clearly on one side of
+// the opening boundary r2 = (side / theta)^2, with no FP64 arithmetic
+// beyond the conversion: d^2 rounded to f32 is compared with the record's
+// r2 in f32 lowered and raised by 8 units in the last place (integer
+// arithmetic on its bits, on the scalar unit).  Both roundings to f32 are
+// within half a unit of the values, so a lane below the lowered bound is
+// inside the exact boundary and one above the raised bound outside it, in
+// the normal and the subnormal range alike; a bound that wraps to a NaN
+// pattern (an r2 of 0 or infinity) decides nothing on its side.  When a
+// lane that takes a cell is between the bounds, or its d^2 is a NaN, the
+// loop returns 1 before touching the cell (n, A and the stack describe the
+// walk at that cell); the caller then visits it with the reference's exact
+// test (walk_visit_f64) and calls the loop again.  So the decisions are
+// the reference's whatever the margin.  It returns 0 when the walk is
+// done.  The record's FP64 mass-times-G is copied to `gm` before the next
+// record is requested.
 #define NBS_WALK64_FORCE(MASK, CNT)                                   \
   "s_and_saveexec_b64 s[40:41], " MASK "\n\t"                          \
   "v_rsq_f64 %[tv], %[d2]\n\t"                                        \
@@ -555,12 +574,11 @@ __device__ __forceinline__ void walk_loop_f32_damped(Nodes nodes, int n, int nen
   "s_load_dwordx16 s[48:63], %[base], %[t]\n\t"                       \
   NBS_WALK64_FORCE(MASK, CNT)
 
+// d^2 = fma(dx, dx, fma(dy, dy, dz * dz)) (sq_sum_fused)
 #define NBS_WALK64_D2                                                 \
-  "v_mul_f64 %[d2], %[dx], %[dx]\n\t"                                 \
-  "v_mul_f64 %[tmp], %[dy], %[dy]\n\t"                                \
-  "v_add_f64 %[d2], %[d2], %[tmp]\n\t"                                \
-  "v_mul_f64 %[tmp], %[dz], %[dz]\n\t"                                \
-  "v_add_f64 %[d2], %[d2], %[tmp]\n\t"
+  "v_mul_f64 %[d2], %[dz], %[dz]\n\t"                                 \
+  "v_fma_f64 %[d2], %[dy], %[dy], %[d2]\n\t"                          \
+  "v_fma_f64 %[d2], %[dx], %[dx], %[d2]\n\t"
 
 #define NBS_WALK64_LOOP(CNT) \
       "s_mov_b32 %[m0save], m0\n\t" \
@@ -582,18 +600,17 @@ __device__ __forceinline__ void walk_loop_f32_damped(Nodes nodes, int n, int nen
       NBS_WALK64_D2 \
       "s_cmp_eq_u32 s55, -2\n\t" \
       "s_cbranch_scc1 L_bucket%=\n\t" \
-      /* cell: t = side^2 - theta^2 d^2, open where t > 1e-9 side^2; a */ \
-      /* taking lane with |t| <= 1e-9 side^2 (or NaN) needs the exact test */ \
-      "v_mul_f64 %[s2], s[58:59], s[58:59]\n\t" \
-      "v_mul_f64 %[tmp], %[th2], %[d2]\n\t" \
-      "v_add_f64 %[tmp], %[s2], -%[tmp]\n\t" \
-      "v_mul_f64 %[s2], %[tolf], %[s2]\n\t" \
-      "v_cmp_gt_f64 vcc, %[tmp], %[s2]\n\t" \
+      /* cell: open where f32(d^2) < r2 - 8 ulp, accept where it is */ \
+      /* > r2 + 8 ulp (s60, s62: the record's unused gm32 and mass); a */ \
+      /* taking lane in neither (or NaN) needs the exact test */ \
+      "v_cvt_f32_f64 %[f2], %[d2]\n\t" \
+      "s_sub_u32 s60, s61, 8\n\t" \
+      "s_add_u32 s62, s61, 8\n\t" \
+      "v_cmp_gt_f32 vcc, s60, %[f2]\n\t" \
       "s_and_b64 s[42:43], vcc, s[46:47]\n\t" \
-      "v_cmp_ngt_f64 s[38:39], %[tmp], %[s2]\n\t" \
-      "v_cmp_nlt_f64 vcc, %[tmp], -%[s2]\n\t" \
-      "s_and_b64 s[38:39], s[38:39], vcc\n\t" \
-      "s_and_b64 s[38:39], s[38:39], s[46:47]\n\t" \
+      "v_cmp_lt_f32 s[38:39], s62, %[f2]\n\t" \
+      "s_or_b64 s[38:39], s[38:39], vcc\n\t" \
+      "s_andn2_b64 s[38:39], s[46:47], s[38:39]\n\t" \
       "s_cbranch_scc1 L_tie%=\n\t" \
       "s_cmp_lg_u64 s[42:43], 0\n\t" \
       "s_cbranch_scc1 L_opened%=\n\t" \
@@ -684,19 +701,20 @@ __device__ __forceinline__ void walk_loop_f32_damped(Nodes nodes, int n, int nen
       [ax] "+v"(ax), [ay] "+v"(ay), [az] "+v"(az), [sp] "+v"(sp), [slo] "+v"(slo),        \
       [shi] "+v"(shi), [lim] "=&s"(lim), [t] "=&s"(t), [tie] "=&s"(tie), [gm] "=&s"(gm),  \
       [m0save] "=&s"(m0save), [dx] "=&v"(dx), [dy] "=&v"(dy), [dz] "=&v"(dz),              \
-      [d2] "=&v"(d2), [tmp] "=&v"(tmp), [tv] "=&v"(tv), [s2] "=&v"(s2)
+      [d2] "=&v"(d2), [tmp] "=&v"(tmp), [tv] "=&v"(tv), [s2] "=&v"(s2), [f2] "=&v"(f2)
 #define NBS_F64_INS                                                                       \
   [base] "s"(nodes), [nend] "s"(nend), [pbeg] "s"(pbeg), [x] "v"(x), [y] "v"(y),          \
-      [z] "v"(z), [th2] "s"(theta2), [tolf] "s"(1e-9)
+      [z] "v"(z)
 
 template <int CM>
 __device__ __forceinline__ int walk_loop_f64(ConstNodes nodes, int& n, int nend, int pbeg,
                                              unsigned long long& A, int& top, int& tpos,
                                              int& sp, int& slo, int& shi, double x, double y,
-                                             double z, double theta2, double& ax, double& ay,
-                                             double& az, int& cnt, int& tot, int& overflow) {
+                                             double z, double& ax, double& ay, double& az, int& cnt,
+                                             int& tot, int& overflow) {
   int lim, t, m0save, tie, tc;
   double gm, dx, dy, dz, d2, tmp, tv, s2;
+  float f2;
   if constexpr (CM == kCountLanes)
     __asm__ volatile(NBS_WALK64_LOOP(NBS_CNT_LANE)
                      : NBS_F64_OUTS, [cnt] "+v"(cnt) : NBS_F64_INS : NBS_WALK_CLOBBERS);
@@ -941,7 +959,7 @@ walk_kernel(WalkArgs a) {
     unsigned long long Acur = A[0];
 #define NBS_F64_WALK(CM)                                                                    \
   while (walk_loop_f64<CM>(nodes, n, nend, pbeg, Acur, top, top_pos, st_pos, st_lo, st_hi,  \
-                           x[0], y[0], z[0], theta2, fax, fay, faz, cnt[0], wave_count,     \
+                           x[0], y[0], z[0], fax, fay, faz, cnt[0], wave_count,             \
                            ovf))                                                            \
     walk_visit_f64(nodes, n, i[0], Acur, top, top_pos, st_pos, st_lo, st_hi, ovf, x[0], y[0], \
                    z[0], theta, theta2, fax, fay, faz, cnt[0]);
