@@ -508,6 +508,27 @@ def test_hand_written_walk_equals_generic(case, precision):
             assert (got[1] == len(pos) - 1).all()
 
 
+@pytest.mark.parametrize("scale", [1e-21, 1e-25, 1e21])
+def test_fp64_decisions_at_extreme_scales(scale):
+    """The FP64 loop's margin test compares f32(d^2) with the record's f32
+    opening radius.  On a fixed root of side 512 * scale the radii are in
+    f32: at 1e-21 mostly subnormal, at 1e-25 all but the top levels 0, at
+    1e21 all infinite (every cell then goes to the exact test).  Every
+    particle's accepted set still equals the oracle's, and the forces stay
+    within the FP64 bar."""
+    pos, vel, mass = cube(3000, seed=17)
+    pos = pos * scale
+    tree = O.Tree(pos, mass, center=np.zeros(3), side=512.0 * scale)
+    oacc, ocnt = tree.forces(theta=0.5)
+    with sim(theta=0.5, root_center=(0.0, 0.0, 0.0), root_size=512.0 * scale) as s:
+        s.set_state(pos, None, mass)
+        acc = s.forces()
+        _, cnt = s.interactions(per_particle=True)
+        assert_tree_equal(s.tree_dump(), tree.dump())
+    np.testing.assert_array_equal(cnt, ocnt)
+    assert rel_err(acc, oacc).max() < 1e-12
+
+
 def test_fp64_walk_exact_opening_tie():
     """A cell whose opening test is an exact tie: the FP64 walk's margin test
     cannot decide it, so the hand-written loop hands the cell to the exact
