@@ -113,6 +113,20 @@ __device__ __forceinline__ bool mac_open_f64(double d2, double side,
   return open;
 }
 
+// G m / d^3 from d^2 in FP64: with the hardware estimate r = R (1 + eps)
+// of R = 1 / sqrt(d^2) and e = 1 - d^2 r^2 = -2 eps - eps^2,
+// r^3 (1 + 1.5 e) = R^3 (1 + O(7.5 eps^2)): relative error ~2e-15 for the
+// estimate's ~2^-26 (the same order as a Newton step on r, cubed), in
+// seven operations.
+__device__ __forceinline__ double inv_cube_gm(double d2, double gm) {
+#pragma clang fp contract(off)
+  const double r = __builtin_amdgcn_rsq(d2);
+  const double r2 = r * r;
+  const double e = fma(-d2, r2, 1.0);
+  const double g = gm * (r2 * r);
+  return fma(g, fma(e, 0.5, e), g);
+}
+
 // 1/sqrt(d2) in FP64: the hardware estimate refined by one Newton-Raphson
 // step (relative error ~1e-15 after it; the estimate's ~2^-26 squares).
 __device__ __forceinline__ double rsqrt_f64(double d2) {
@@ -155,13 +169,13 @@ __device__ __forceinline__ void interact(const NodeRegs& nd, double dx, double d
     az = fmaf(sc, fdz, (float)az);
   } else {
     const double d2 = sq_sum_fused(dx, dy, dz);
-    const double r = rsqrt_f64(d2);
     double sc;
     if (DAMP) {
+      const double r = rsqrt_f64(d2);
       const double d = d2 * r;
       sc = nd.gm * r / (d2 + exp(-d));
     } else {
-      sc = nd.gm * (r * r * r);
+      sc = inv_cube_gm(d2, nd.gm);
     }
     ax = fma(sc, dx, (double)ax);
     ay = fma(sc, dy, (double)ay);
@@ -530,9 +544,9 @@ __device__ __forceinline__ void walk_loop_f32_damped(Nodes nodes, int n, int nen
 // gfx950, the FP64 counterpart of walk_loop_f32 with the same registers,
 // traversal and stack.  Per visit: 3 FP64 subtracts, d^2 fused (3), and
 // for a cell the margin test below (a conversion and two compares); for
-// the interacting lanes 1/r from v_rsq_f64 with one Newton step and the
-// force (12), the operations and their order those of the generic loop
-// (interact<false, false>).
+// the interacting lanes G m / |d|^3 from v_rsq_f64 with a first-order
+// correction, and the force (10), the operations and their order those of
+// the generic loop (interact<false, false>).
 //
 // The margin test settles every lane whose d^2 is clearly on one side of
 // the opening boundary r2 = (side / theta)^2, with no FP64 arithmetic
@@ -542,7 +556,9 @@ __device__ __forceinline__ void walk_loop_f32_damped(Nodes nodes, int n, int nen
 // within half a unit of the values, so a lane below the lowered bound is
 // inside the exact boundary and one above the raised bound outside it, in
 // the normal and the subnormal range alike; a bound that wraps to a NaN
-// pattern (an r2 of 0 or infinity) decides nothing on its side.  When a
+// pattern (an r2 of 0 or infinity) decides nothing on its side.  The force
+// takes G m / |d|^3 as (G m r^3)(1 + 1.5 e) from the hardware estimate
+// r = v_rsq_f64(d^2) and e = 1 - d^2 r^2 (inv_cube_gm).  When a
 // lane that takes a cell is between the bounds, or its d^2 is a NaN, the
 // loop returns 1 before touching the cell (n, A and the stack describe the
 // walk at that cell); the caller then visits it with the reference's exact
@@ -554,13 +570,12 @@ __device__ __forceinline__ void walk_loop_f32_damped(Nodes nodes, int n, int nen
   "s_and_saveexec_b64 s[40:41], " MASK "\n\t"                          \
   "v_rsq_f64 %[tv], %[d2]\n\t"                                        \
   CNT(MASK)                                                           \
-  "v_mul_f64 %[tmp], %[d2], %[tv]\n\t"                                \
-  "v_fma_f64 %[tmp], -%[tmp], %[tv], 1.0\n\t"                         \
-  "v_mul_f64 %[s2], 0.5, %[tv]\n\t"                                   \
-  "v_fma_f64 %[tv], %[s2], %[tmp], %[tv]\n\t"                         \
-  "v_mul_f64 %[tmp], %[tv], %[tv]\n\t"                                \
-  "v_mul_f64 %[tmp], %[tmp], %[tv]\n\t"                               \
-  "v_mul_f64 %[tmp], %[gm], %[tmp]\n\t"                               \
+  "v_mul_f64 %[s2], %[tv], %[tv]\n\t"                                 \
+  "v_fma_f64 %[tmp], -%[d2], %[s2], 1.0\n\t"                          \
+  "v_mul_f64 %[s2], %[s2], %[tv]\n\t"                                 \
+  "v_mul_f64 %[s2], %[gm], %[s2]\n\t"                                 \
+  "v_fma_f64 %[tmp], %[tmp], 0.5, %[tmp]\n\t"                         \
+  "v_fma_f64 %[tmp], %[s2], %[tmp], %[s2]\n\t"                        \
   "v_fma_f64 %[ax], %[tmp], %[dx], %[ax]\n\t"                         \
   "v_fma_f64 %[ay], %[tmp], %[dy], %[ay]\n\t"                         \
   "v_fma_f64 %[az], %[tmp], %[dz], %[az]\n\t"                         \
