@@ -454,6 +454,19 @@ def _walk_code_outputs(pos, vel, mass, generic, offsets=None, shape=None, theta=
         return (acc, cnt) + s.state(), info
 
 
+@pytest.mark.parametrize("damping", [False, True])
+def test_hand_written_walk_equals_generic_on_32_byte_records(damping):
+    """From 2^21 particles on, the FP32 loop reads the 32-byte records by
+    default: a sphere just above that size gives forces, counts and four
+    steps (eager and graph-replayed) bit-identical to the generic loop,
+    which reads the 64-byte records."""
+    pos, vel, mass = O.generate((1 << 21) + 77, "sphere", seed=5)
+    want, _ = _walk_code_outputs(pos, vel, mass, True, damping=damping)
+    got, _ = _walk_code_outputs(pos, vel, mass, False, damping=damping)
+    for a, b in zip(got, want):
+        np.testing.assert_array_equal(a, b)
+
+
 @pytest.mark.parametrize("precision", ["f32", "f64", "f32-damped"])
 @pytest.mark.parametrize("case", ["sphere", "cluster", "buckets", "batched", "theta0"])
 def test_hand_written_walk_equals_generic(case, precision):
