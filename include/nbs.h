@@ -165,7 +165,7 @@ int nbs_forces(nbs_ctx* ctx, double* acc);
  * (device, [n] int64, may be NULL) each particle's interactions.  It calls
  * no hipMalloc / hipFree: every buffer comes from one fixed library-owned
  * device array per GPU, sized for at most NBS_NOALLOC_MAX_N particles at
- * node_factor <= 4 (about 40 MiB, held until the process ends).  Larger n
+ * node_factor <= 4 (56 MiB, held until the process ends).  Larger n
  * (or a tree that outgrows the fixed node capacity) is rejected with
  * NBS_ERR_ARG / NBS_ERR_OVERFLOW and a message in msg (may be NULL).  Work
  * is enqueued on `stream` (hipStream_t, NULL = default stream) and the call
@@ -268,6 +268,18 @@ int nbs_set_walk_order(nbs_ctx* ctx, int on);
  * the same arithmetic in the same order: changes the speed only, never a
  * result bit. */
 int nbs_set_walk_code(nbs_ctx* ctx, int code);
+
+/* Pieces per walk traversal (no counterpart in the reference): 0 (default)
+ * = by the wavefront count, so that a system too small to fill the GPU's
+ * wavefront slots (fewer than ~4096 64-particle wavefronts) walks each
+ * traversal as 2, 4 or 8 pieces over equal ranges of its nodes, whose
+ * partial sums are added in piece order; 1, 2, 4 or 8 = that many.  Every
+ * particle's accepted cells and interaction count are those of the whole
+ * traversal; the forces differ from it only by the summation order, and a
+ * given state and setting always give the same bits.  nbs_walk_split
+ * returns the pieces in use. */
+int nbs_set_walk_split(nbs_ctx* ctx, int pieces);
+int nbs_walk_split(nbs_ctx* ctx);
 
 /* Blocks until the context's stream has drained. */
 int nbs_synchronize(nbs_ctx* ctx);

@@ -295,6 +295,18 @@ class Simulation:
             raise ValueError("records must be 'auto', 32 or 64")
         self._check(self._L.nbs_set_walk_code(self._h, 1 if generic else codes[records]))
 
+    def set_walk_split(self, pieces=0):
+        """Pieces per walk traversal: 0 (default) by the wavefront count
+        (small systems split each traversal to fill the GPU), or 1, 2, 4, 8.
+        Interaction counts never change; forces differ only by the order of
+        the pieces' sums."""
+        self._check(self._L.nbs_set_walk_split(self._h, int(pieces)))
+
+    @property
+    def walk_split(self):
+        """Pieces per walk traversal in use."""
+        return self._L.nbs_walk_split(self._h)
+
     def set_counting(self, mode="particles"):
         """Interaction counting of the walks: "particles" (default; per
         particle and the running total), "total" (the running total only,

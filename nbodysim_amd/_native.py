@@ -104,6 +104,8 @@ SIGNATURES = [
                                           ctypes.c_int]),
     ("nbs_set_walk_order", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("nbs_set_walk_code", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("nbs_set_walk_split", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("nbs_walk_split", ctypes.c_int, [ctypes.c_void_p]),
     ("nbs_set_counting", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("nbs_synchronize", ctypes.c_int, [ctypes.c_void_p]),
     ("nbs_direct", ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, c_i64p, c_dp]),

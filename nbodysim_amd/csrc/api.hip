@@ -191,7 +191,7 @@ struct nbs_ctx {
   int n = 0, cap_n = 0;
   int node_cap = 0;
   int build_cap = 0;  // keys the build buffers hold
-  std::vector<void*> part_bufs, build_bufs, node_bufs, sys_bufs, misc_bufs;
+  std::vector<void*> part_bufs, build_bufs, node_bufs, sys_bufs, misc_bufs, split_bufs;
 
   State st[2];
   int cur = 0;
@@ -243,6 +243,12 @@ struct nbs_ctx {
   int walk_order = 0;
   // 1: the compiler-generated walk loop even where the hand-written one applies
   int walk_generic = 0;
+  // pieces per walk traversal: requested (0: by wavefront count) and in use,
+  // with the pieces' partial sums ([split][3][n] doubles, [split][n] ints)
+  int walk_split_req = 0;
+  int walk_split = 1;
+  double* walk_part = nullptr;
+  int* walk_cpart = nullptr;
   // records of the FP32 hand-written loop: 0 by particle count
   // (kWalk32MinN), 32 or 64 bytes (nbs_set_walk_code 3 / 2)
   int walk_records = 0;
@@ -511,6 +517,41 @@ int alloc_nodes(nbs_ctx* c, int cap) {
   return NBS_OK;
 }
 
+// Pieces per walk traversal: a system of few wavefronts (a small N) leaves
+// most of the GPU's wavefront slots (1024 SIMDs x 8) empty and each
+// wavefront latency-bound, so each traversal is walked as up to 8 pieces,
+// the fewest that give about kWalkWaveSlots wavefronts.  Fixed by the
+// wavefront count, so a given state always takes the same pieces.
+constexpr long long kWalkWaveSlots = 8192;
+int walk_split_for(const nbs_ctx* c) {
+  if (c->walk_split_req > 0) return c->walk_split_req;
+  int k = 1;
+  while (k < 8 && (long long)c->sys.n_waves * k * 2 <= kWalkWaveSlots) k *= 2;
+  return k;
+}
+
+// The pieces' partial-sum buffers for the current systems and request.
+int alloc_walk_split(nbs_ctx* c) {
+  drop_graph(c);
+  free_all(c->split_bufs);
+  c->walk_part = nullptr;
+  c->walk_cpart = nullptr;
+  c->walk_split = walk_split_for(c);
+  if (c->walk_split > 1 && c->n > 0) {
+    hipError_t e = hipSuccess;
+    c->walk_part = cbuf<double>(c, c->split_bufs, (size_t)c->walk_split * 3 * c->n, &e);
+    c->walk_cpart = cbuf<int>(c, c->split_bufs, (size_t)c->walk_split * c->n, &e);
+    if (e != hipSuccess) {
+      free_all(c->split_bufs);
+      c->walk_part = nullptr;
+      c->walk_cpart = nullptr;
+      c->walk_split = 1;
+      return hip_fail(c, e, "hipMalloc(walk pieces)");
+    }
+  }
+  return NBS_OK;
+}
+
 // Per-system arrays from caller offsets (one system when offsets empty).
 int setup_systems(nbs_ctx* c) {
   const int nsys = c->nsys_req;
@@ -552,6 +593,10 @@ int setup_systems(nbs_ctx* c) {
   c->cost_tiles = -1;
   if (e != hipSuccess) return hip_fail(c, e, "hipMalloc(system buffers)");
   launch_fill_u64(c->inter_total, 0, std::max(s.n_waves, 1), c->stream);
+  {
+    const int r = alloc_walk_split(c);
+    if (r) return r;
+  }
   HIPCHK(c, hipMemcpyAsync(s.start, c->h_start.data(), (nsys + 1) * sizeof(int),
                            hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(s.wave_start, c->h_wave_start.data(),
@@ -816,7 +861,14 @@ void enqueue_walk(nbs_ctx* c, int fused) {
   c->lane_counts = c->count_mode == kCountLanes;
   // the hand-written loop addresses node records with 32-bit byte offsets
   a.asm_loop = !c->walk_generic && ((size_t)c->node_cap + 1) * sizeof(WalkNode) < (1ull << 32);
-  if (c->walk_order) {
+  if (c->walk_split > 1) {
+    // the pieces take their own order (the traversal's pieces on
+    // consecutive workgroups); no tile costs
+    a.split = c->walk_split;
+    a.part = c->walk_part;
+    a.cpart = c->walk_cpart;
+    a.n = c->n;
+  } else if (c->walk_order) {
     const int tiles = sync_tile_costs(c);
     launch_tile_order(c->tile_cost, tiles, c->walk_groups, c->tile_order, c->stream);
     a.order = c->tile_order;
@@ -1374,6 +1426,7 @@ void nbs_destroy(nbs_ctx* c) {
   free_all(c->thr_ebufs);
   free_all(c->node_bufs);
   free_all(c->sys_bufs);
+  free_all(c->split_bufs);
   free_all(c->misc_bufs);
   free_all(c->col_bufs);
   free_all(c->col_pair_bufs);
@@ -1716,7 +1769,7 @@ int nbs_forces(nbs_ctx* c, double* acc) {
 // 580 B per particle plus fixed parts).  Each device holds its own copy
 // until the process ends; all calls on one device share it.
 namespace {
-constexpr size_t kScratchBytes = 40ull << 20;
+constexpr size_t kScratchBytes = 56ull << 20;
 constexpr int kMaxDevices = 64;
 }  // namespace
 
@@ -1919,6 +1972,17 @@ int nbs_set_walk_shape(nbs_ctx* c, int block, int groups, int pack) {
   c->walk_pack = pack;
   return NBS_OK;
 }
+
+int nbs_set_walk_split(nbs_ctx* c, int pieces) {
+  if (!c) return NBS_ERR_ARG;
+  if (pieces != 0 && pieces != 1 && pieces != 2 && pieces != 4 && pieces != 8)
+    return fail(c, NBS_ERR_ARG, "walk pieces must be 0 (by wavefront count), 1, 2, 4 or 8");
+  c->walk_split_req = pieces;
+  if (!c->has_state) return NBS_OK;
+  return alloc_walk_split(c);
+}
+
+int nbs_walk_split(nbs_ctx* c) { return c ? c->walk_split : 0; }
 
 int nbs_set_walk_code(nbs_ctx* c, int code) {
   if (!c) return NBS_ERR_ARG;
@@ -2703,6 +2767,12 @@ int nbs_frozen_forward(nbs_ctx* c, const double* pos0, const double* vel0,
     a.mac_x = c->mac_x;
     a.mac_y = c->mac_x + n;
     a.mac_z = c->mac_x + 2 * n;
+    if (c->walk_split > 1) {  // the pieces of the recorded forward's walks
+      a.split = c->walk_split;
+      a.part = c->walk_part;
+      a.cpart = c->walk_cpart;
+      a.n = (int)n;
+    }
     launch_walk(a, 0, c->p.damping, c->walk_block, c->walk_pack, s);
     scatter3_kernel<<<b, 256, 0, s>>>(st.x, st.y, st.z, st.id, (int)n, c->p_x);
     scatter3_kernel<<<b, 256, 0, s>>>(st.vx, st.vy, st.vz, st.id, (int)n, c->p_v);

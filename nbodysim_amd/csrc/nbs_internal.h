@@ -239,6 +239,14 @@ struct WalkArgs {
   // and the roots ([4 s + 3] = side of system s) their opening radii follow
   const WalkNode32* nodes32;
   const double* root;
+  // split > 1: each traversal walked as `split` pieces over equal ranges of
+  // its system's nodes; part [split][3][n] partial accelerations (f32 or
+  // f64, the force precision's) and cpart [split][n] partial counts, added
+  // by the combine kernel; n particles in all systems
+  int split;
+  void* part;
+  int* cpart;
+  int n;
 };
 
 // Collision pairs: every (particle i, leaf j) that i's walk reaches -- the

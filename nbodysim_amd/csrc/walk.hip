@@ -327,13 +327,8 @@ __device__ __forceinline__ void interact(const NodeRegs& nd, double dx, double d
 
 #define NBS_WALK_F32_LOOP(FORCE, CNT, FMT) \
       "s_mov_b32 %[m0save], m0\n\t" \
-      "v_mov_b32 %[sp], -1\n\t" \
-      "v_mov_b32 %[slo], 0\n\t" \
-      "v_mov_b32 %[shi], 0\n\t" \
       "s_mov_b64 s[46:47], %[A0]\n\t" \
-      "s_mov_b32 %[top], 0\n\t" \
-      "s_mov_b32 %[tpos], -1\n\t" \
-      "s_mov_b32 %[lim], %[nend]\n\t" \
+      "s_min_u32 %[lim], %[tpos], %[nend]\n\t" \
       "s_cmp_lt_i32 %[n], %[nend]\n\t" \
       "s_cbranch_scc0 L_done%=\n\t" \
       NBS_REC_LOAD_##FMT \
@@ -455,8 +450,8 @@ __device__ __forceinline__ void interact(const NodeRegs& nd, double dx, double d
 // form (the lane count in a VGPR, the wave total and a temporary in SGPRs).
 #define NBS_F32_OUTS                                                                      \
   [n] "+s"(n), [ovf] "+s"(overflow), [ax] "+v"(ax), [ay] "+v"(ay), [az] "+v"(az),         \
-      [top] "=&s"(top), [tpos] "=&s"(tpos), [lim] "=&s"(lim), [t] "=&s"(t), [gm] "=&s"(gm), \
-      [m0save] "=&s"(m0save), [sp] "=&v"(sp), [slo] "=&v"(slo), [shi] "=&v"(shi),          \
+      [top] "+s"(top), [tpos] "+s"(tpos), [lim] "=&s"(lim), [t] "=&s"(t), [gm] "=&s"(gm),   \
+      [m0save] "=&s"(m0save), [sp] "+v"(sp), [slo] "+v"(slo), [shi] "+v"(shi),             \
       [dx] "=&v"(dx), [dy] "=&v"(dy), [dz] "=&v"(dz), [fx] "=&v"(fx), [fy] "=&v"(fy),       \
       [fz] "=&v"(fz), [d2] "=&v"(d2), [tv] "=&v"(tv)
 #define NBS_F32_DAMPED_OUTS                                                               \
@@ -493,15 +488,18 @@ __device__ __forceinline__ void interact(const NodeRegs& nd, double dx, double d
                      : OUTS : NBS_F32_INS_##FMT : NBS_F32_CLOBBERS_##FMT);
 
 // W32: `nodes` is the 32-byte record array (rbase, rmask: the opening radii's
-// root value and mask); otherwise the 64-byte one.
+// root value and mask); otherwise the 64-byte one.  The walk starts at node
+// n with the lanes of A and the stack (top entries, the top's position
+// tpos, entry k in lane k of sp, slo, shi; lane 63 of sp -1) a piece's
+// entry left (empty for a whole traversal) and ends at nend.
 template <int CM, bool W32, class Nodes>
 __device__ __forceinline__ void walk_loop_f32(Nodes nodes, int n, int nend, int pbeg,
-                                              unsigned long long A, double x, double y,
+                                              unsigned long long A, int top, int tpos, int sp,
+                                              int slo, int shi, double x, double y,
                                               double z, float& ax, float& ay, float& az,
                                               int& cnt, int& tot, int& overflow, unsigned rbase,
                                               unsigned rmask) {
-  int top, tpos, lim, t, gm, m0save, tc;
-  int sp, slo, shi;
+  int lim, t, gm, m0save, tc;
   double dx, dy, dz;
   float fx, fy, fz, d2, tv;
   if constexpr (W32) {
@@ -513,13 +511,13 @@ __device__ __forceinline__ void walk_loop_f32(Nodes nodes, int n, int nend, int 
 
 template <int CM, bool W32, class Nodes>
 __device__ __forceinline__ void walk_loop_f32_damped(Nodes nodes, int n, int nend, int pbeg,
-                                                     unsigned long long A, double x, double y,
-                                                     double z, float& ax, float& ay,
+                                                     unsigned long long A, int top, int tpos,
+                                                     int sp, int slo, int shi, double x,
+                                                     double y, double z, float& ax, float& ay,
                                                      float& az, int& cnt, int& tot,
                                                      int& overflow, unsigned rbase,
                                                      unsigned rmask) {
-  int top, tpos, lim, t, gm, m0save, tc;
-  int sp, slo, shi;
+  int lim, t, gm, m0save, tc;
   double dx, dy, dz;
   float fx, fy, fz, d2, tv, te, tn, td, q0, q1, q2, q3, q4;
   if constexpr (W32) {
@@ -827,6 +825,104 @@ __device__ __forceinline__ int tile_of_block(int b, int nblocks, int groups) {
   return v * q + min(v, r) + j;
 }
 
+// Where a piece of a wavefront's traversal starts.  With split > 1 the
+// system's node range is cut into `split` equal ranges of pre-order index
+// and each wavefront walks each range as a separate piece (more wavefronts
+// for a system too small to fill the GPU).  The walk state at the first
+// node b of a range is what the whole traversal has there: descending from
+// the system's root to b, each cell on the path is opened or accepted by
+// the lanes that take it with the walk's own test, lanes that accept one
+// wait at its skip (the stack, merged as the walk merges), and the rest
+// take b.  Each lane's decisions, and so its interactions and their order
+// within the range, are the whole traversal's; a piece only adds the
+// range's contribution.  When no lane takes b the piece resumes at the
+// stack top, or has nothing to do (n = e).
+template <bool F32, bool FROZEN, int P>
+__device__ __forceinline__ void walk_enter(ConstNodes nodes, ConstNodes mac_nodes, int root,
+                                           int b, int e, const double* x, const double* y,
+                                           const double* z, const double* mx,
+                                           const double* my, const double* mz, double theta,
+                                           double theta2, unsigned long long* A, int& n,
+                                           int& top, int& top_pos, int& st_pos, int* st_lo,
+                                           int* st_hi, bool& overflow) {
+  const int lane = lane_id();
+  int X = root;
+  while (X < b) {
+    const NodeRegs cur = load_node(nodes, X);
+    unsigned long long O[P], C[P], any_wait = 0;
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+      bool open;
+      if (cur.leaf == -2) {
+        open = true;
+      } else if (F32) {
+        const float fdx = (float)(cur.cx - x[p]), fdy = (float)(cur.cy - y[p]),
+                    fdz = (float)(cur.cz - z[p]);
+        open = fmaf(fdx, fdx, fmaf(fdy, fdy, fdz * fdz)) < cur.r2open32;
+      } else {
+        const double d2 = FROZEN ? sq_sum_ref(mac_nodes[X].cx - mx[p], mac_nodes[X].cy - my[p],
+                                              mac_nodes[X].cz - mz[p])
+                                 : sq_sum_ref(cur.cx - x[p], cur.cy - y[p], cur.cz - z[p]);
+        open = mac_open_f64(d2, cur.side, theta, theta2);
+      }
+      O[p] = ballot(open && __builtin_amdgcn_inverse_ballot_w64(A[p]));
+      C[p] = A[p] & ~O[p];
+      any_wait |= C[p];
+    }
+    if (any_wait) {
+      if (top > 0 && top_pos == cur.skip) {
+        const int slot = top - 1;
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+          st_lo[p] = lane == slot ? (st_lo[p] | (int)(unsigned)C[p]) : st_lo[p];
+          st_hi[p] = lane == slot ? (st_hi[p] | (int)(unsigned)(C[p] >> 32)) : st_hi[p];
+        }
+      } else if (top < kWave - 1) {
+        const int slot = top;
+        st_pos = lane == slot ? cur.skip : st_pos;
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+          st_lo[p] = lane == slot ? (int)(unsigned)C[p] : st_lo[p];
+          st_hi[p] = lane == slot ? (int)(unsigned)(C[p] >> 32) : st_hi[p];
+        }
+        top_pos = cur.skip;
+        top++;
+      } else {
+        overflow = true;
+      }
+    }
+#pragma unroll
+    for (int p = 0; p < P; p++) A[p] = O[p];
+    // the child of X whose subtree holds b (siblings are chained by skip)
+    int c = X + 1;
+    for (;;) {
+      const int sk = max(load_node(nodes, c).skip, c + 1);
+      if (sk > b) break;
+      c = sk;
+    }
+    X = c;
+  }
+  n = b;
+  unsigned long long any = 0;
+#pragma unroll
+  for (int p = 0; p < P; p++) any |= A[p];
+  if (!any) {
+    if (top > 0 && top_pos < e) {
+      n = top_pos;
+#pragma unroll
+      for (int p = 0; p < P; p++) {
+        const unsigned lo = (unsigned)__builtin_amdgcn_readlane(st_lo[p], top - 1);
+        const unsigned hi = (unsigned)__builtin_amdgcn_readlane(st_hi[p], top - 1);
+        A[p] = ((unsigned long long)hi << 32) | lo;
+      }
+      top--;
+      top_pos = top > 0 ? __builtin_amdgcn_readlane(st_pos, top - 1) : -1;
+    } else {
+      n = e;
+    }
+  }
+}
+
 // One wavefront walks P * 64 particles that are consecutive in octree
 // order (lane l holds particles pbeg + l, pbeg + 64 + l, ...), each set
 // with its own lane mask; a node is visited if any particle of any set
@@ -847,15 +943,26 @@ __device__ __forceinline__ int tile_of_block(int b, int nblocks, int groups) {
 // values it cannot keep into VGPR lanes outside the loop) the walk uses 78;
 // at the 95 it used before, only seven fitted and the FP32 walk took 10 %
 // longer.
-template <bool F32, bool DAMP, bool FROZEN, int BLOCK, int P, bool W32 = false>
+template <bool F32, bool DAMP, bool FROZEN, int BLOCK, int P, bool W32 = false,
+          bool SPLIT = false>
 __global__ void __launch_bounds__(BLOCK)
 __attribute__((amdgpu_waves_per_eu(!F32 && (DAMP || P == 2) ? 1 : 8, 8)))
 walk_kernel(WalkArgs a) {
   if (*a.flags) return;
   const int lane = lane_id();
-  const int tile = a.order ? a.order[blockIdx.x] : tile_of_block(blockIdx.x, gridDim.x, a.groups);
-  const int hw = __builtin_amdgcn_readfirstlane(
-      (int)((tile * BLOCK + threadIdx.x) >> 6));
+  // split > 1: wavefront q walks piece q % split of traversal q / split
+  // (the pieces of one traversal on consecutive workgroups, which the
+  // hardware hands to different XCDs: each XCD walks its own node range)
+  const int split = SPLIT ? a.split : 1;
+  int tile = 0, hw, piece = 0;
+  if constexpr (SPLIT) {
+    const int q = (int)((blockIdx.x * BLOCK + threadIdx.x) >> 6);
+    hw = __builtin_amdgcn_readfirstlane(q / split);
+    piece = __builtin_amdgcn_readfirstlane(q - (q / split) * split);
+  } else {
+    tile = a.order ? a.order[blockIdx.x] : tile_of_block(blockIdx.x, gridDim.x, a.groups);
+    hw = __builtin_amdgcn_readfirstlane((int)((tile * BLOCK + threadIdx.x) >> 6));
+  }
   const int* hw_start = P == 1 ? a.wave_start : a.wave2_start;
   if (hw >= (P == 1 ? a.n_waves : a.n_waves2)) return;
   const int s = a.nsys > 1 ? __builtin_amdgcn_readfirstlane(
@@ -888,11 +995,30 @@ walk_kernel(WalkArgs a) {
   ConstNodes nodes = (ConstNodes)a.nodes;
   ConstNodes mac_nodes = (ConstNodes)a.mac_nodes;
   int n = a.node_start[s];
-  const int nend = a.node_start[s + 1];
+  int nend = a.node_start[s + 1];
   const double theta = a.theta;
   const double theta2 = theta * theta;
 
   bool overflow = false;
+  // the stack of waiting lanes: entry `slot` lives in lane `slot` of
+  // st_pos, st_lo, st_hi (lane 63 of st_pos stays -1); top_pos is the top
+  // entry's position, kept in an SGPR
+  int top = 0, top_pos = -1, st_pos = -1;
+  int st_lo[P], st_hi[P];
+#pragma unroll
+  for (int p = 0; p < P; p++) st_lo[p] = st_hi[p] = 0;
+  if constexpr (SPLIT) {
+    const long long len = nend - n;
+    const int b = n + (int)(len * piece / split);
+    const int e = n + (int)(len * (piece + 1) / split);
+    walk_enter<F32, FROZEN, P>(nodes, mac_nodes, n, b, e, x, y, z, mx, my, mz, theta, theta2,
+                               A, n, top, top_pos, st_pos, st_lo, st_hi, overflow);
+    nend = e;
+  }
+  if constexpr (SPLIT) {
+    top = __builtin_amdgcn_readfirstlane(top);
+    top_pos = __builtin_amdgcn_readfirstlane(top_pos);
+  }
 
   typedef typename std::conditional<F32, float, double>::type acc_t;
   acc_t ax[P], ay[P], az[P];
@@ -939,16 +1065,18 @@ walk_kernel(WalkArgs a) {
     }
   }
   if (hand32) {
-    int ovf = 0;
+    int ovf = overflow ? 1 : 0;
     float fax = 0.f, fay = 0.f, faz = 0.f;
     ConstNodes32 nodes32 = (ConstNodes32)a.nodes32;
 #define NBS_F32_WALK(CM, W, ND)                                                             \
   if constexpr (DAMP)                                                                       \
-    walk_loop_f32_damped<CM, W>(ND, n, nend, pbeg, A[0], x[0], y[0], z[0], fax, fay, faz,   \
-                                cnt[0], wave_count, ovf, rbase, rmask);                     \
+    walk_loop_f32_damped<CM, W>(ND, n, nend, pbeg, A[0], top, top_pos, st_pos, st_lo[0],    \
+                                st_hi[0], x[0], y[0], z[0], fax, fay, faz, cnt[0],          \
+                                wave_count, ovf, rbase, rmask);                             \
   else                                                                                      \
-    walk_loop_f32<CM, W>(ND, n, nend, pbeg, A[0], x[0], y[0], z[0], fax, fay, faz, cnt[0],  \
-                         wave_count, ovf, rbase, rmask);
+    walk_loop_f32<CM, W>(ND, n, nend, pbeg, A[0], top, top_pos, st_pos, st_lo[0], st_hi[0], \
+                         x[0], y[0], z[0], fax, fay, faz, cnt[0], wave_count, ovf, rbase,   \
+                         rmask);
 #define NBS_F32_WALK_FMT(CM)                                                                \
   if constexpr (W32) {                                                                      \
     NBS_F32_WALK(CM, true, nodes32)                                                         \
@@ -969,14 +1097,14 @@ walk_kernel(WalkArgs a) {
     az[0] = faz;
     overflow = ovf != 0;
   } else if (!F32 && !DAMP && !FROZEN && P == 1 && a.asm_loop && !a.tile_cost) {
-    int ovf = 0, top = 0, top_pos = -1, st_pos = -1, st_lo = 0, st_hi = 0;
+    int ovf = overflow ? 1 : 0, slo = st_lo[0], shi = st_hi[0];
     double fax = 0, fay = 0, faz = 0;
     unsigned long long Acur = A[0];
 #define NBS_F64_WALK(CM)                                                                    \
-  while (walk_loop_f64<CM>(nodes, n, nend, pbeg, Acur, top, top_pos, st_pos, st_lo, st_hi,  \
+  while (walk_loop_f64<CM>(nodes, n, nend, pbeg, Acur, top, top_pos, st_pos, slo, shi,      \
                            x[0], y[0], z[0], fax, fay, faz, cnt[0], wave_count,             \
                            ovf))                                                            \
-    walk_visit_f64(nodes, n, i[0], Acur, top, top_pos, st_pos, st_lo, st_hi, ovf, x[0], y[0], \
+    walk_visit_f64(nodes, n, i[0], Acur, top, top_pos, st_pos, slo, shi, ovf, x[0], y[0],     \
                    z[0], theta, theta2, fax, fay, faz, cnt[0]);
     if (a.count_mode == kCountLanes) {
       NBS_F64_WALK(kCountLanes)
@@ -991,14 +1119,6 @@ walk_kernel(WalkArgs a) {
     az[0] = faz;
     overflow = ovf != 0;
   } else {
-    // waiting lanes: stack entry `slot` lives in lane `slot` of these
-    int st_pos = 0;
-    int st_lo[P], st_hi[P];
-#pragma unroll
-    for (int p = 0; p < P; p++) st_lo[p] = st_hi[p] = 0;
-    int top = 0;
-    int top_pos = -1;  // st_pos of the top entry, kept in an SGPR
-
     // One scalar load of the node record per visit.  (Loading both possible
     // successors a visit ahead measured slower: the doubled scalar-memory
     // traffic costs more than the latency it hides.)
@@ -1109,6 +1229,28 @@ walk_kernel(WalkArgs a) {
   }
   if (overflow && lane == 0) atomicOr(a.flags_out, kFlagStack);
 
+  if constexpr (SPLIT) {
+    // a piece: its partial sums and counts, which walk_combine_kernel adds
+    // in piece order and finishes (update, bbox partials, step counters)
+    acc_t* part = (acc_t*)a.part;
+    const size_t np = (size_t)a.n;
+    int wt = wave_count;
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+      const int ip = i[p];
+      if (valid[p]) {
+        part[((size_t)piece * 3 + 0) * np + ip] = ax[p];
+        part[((size_t)piece * 3 + 1) * np + ip] = ay[p];
+        part[((size_t)piece * 3 + 2) * np + ip] = az[p];
+        if (a.count_mode == kCountLanes) a.cpart[(size_t)piece * np + ip] = cnt[p];
+      }
+      if (a.count_mode == kCountWave) wt += wave_sum(valid[p] ? cnt[p] : 0);
+    }
+    if (a.count_mode == kCountWave && lane == 0 && wt)
+      atomicAdd(&a.inter_total[w0], (unsigned long long)wt);
+    return;
+  }
+
   int wtot = 0;
 #pragma unroll
   for (int p = 0; p < P; p++) {
@@ -1149,6 +1291,71 @@ walk_kernel(WalkArgs a) {
   // per-wave running interaction count (no contended atomics)
   if (lane == 0 && a.count_mode != kCountNone) a.inter_total[w0] += (unsigned long long)wtot;
   if (a.tile_cost && lane == 0) atomicAdd(&a.tile_cost[tile], visits);
+  if (hw == 0 && lane == 0) atomicAdd(a.steps_done, 1);
+}
+
+// The sums of a split walk: per particle the pieces' partial sums added in
+// piece order (FP32 sums in FP32), then what the whole walk does after its
+// loop: accelerations and counts, the fused kick-drift and the next bbox
+// partials, the system's time, the per-wave interaction totals and the
+// completed-walk counter.  One thread per particle, 64-particle waves as in
+// the walk.
+template <bool F32>
+__global__ void __launch_bounds__(256) walk_combine_kernel(WalkArgs a) {
+  if (*a.flags) return;
+  typedef typename std::conditional<F32, float, double>::type acc_t;
+  const int lane = lane_id();
+  const int hw = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * 256 + threadIdx.x) >> 6));
+  if (hw >= a.n_waves) return;
+  const int s = a.nsys > 1 ? __builtin_amdgcn_readfirstlane(sys_of_wave(hw, a.wave_start, a.nsys))
+                           : 0;
+  const int k = hw - a.wave_start[s];
+  const int i = a.sys_start[s] + k * kWave + lane;
+  const bool valid = i < a.sys_start[s + 1];
+  const acc_t* part = (const acc_t*)a.part;
+  const size_t np = (size_t)a.n;
+  acc_t sx = 0, sy = 0, sz = 0;
+  int cnt = 0;
+  if (valid) {
+    for (int q = 0; q < a.split; q++) {
+      sx = sx + part[((size_t)q * 3 + 0) * np + i];
+      sy = sy + part[((size_t)q * 3 + 1) * np + i];
+      sz = sz + part[((size_t)q * 3 + 2) * np + i];
+      if (a.count_mode == kCountLanes) cnt += a.cpart[(size_t)q * np + i];
+    }
+  }
+  const double fax = (double)sx, fay = (double)sy, faz = (double)sz;
+  double x = valid ? a.x[i] : 0.0, y = valid ? a.y[i] : 0.0, z = valid ? a.z[i] : 0.0;
+  if (valid) {
+    a.ax[i] = fax;
+    a.ay[i] = fay;
+    a.az[i] = faz;
+    if (a.count_mode == kCountLanes) a.interactions[i] = cnt;
+  }
+  if (a.fused_update) {
+    if (valid) {
+      const double dt = a.dt;
+      double vx = a.vx[i], vy = a.vy[i], vz = a.vz[i];
+      kick_drift(x, vx, fax, dt);
+      kick_drift(y, vy, fay, dt);
+      kick_drift(z, vz, faz, dt);
+      a.x[i] = x;
+      a.y[i] = y;
+      a.z[i] = z;
+      a.vx[i] = vx;
+      a.vy[i] = vy;
+      a.vz[i] = vz;
+    }
+    wave_bbox_partial(valid, x, y, z, hw, a.n_waves, a.bbox);
+    if (lane == 0 && k == 0) {
+      a.sys_dt[s] = a.dt;
+      a.sys_time[s] += a.dt;
+    }
+  }
+  if (a.count_mode == kCountLanes) {
+    const int wt = wave_sum(valid ? cnt : 0);
+    if (lane == 0) a.inter_total[hw] += (unsigned long long)wt;
+  }
   if (hw == 0 && lane == 0) atomicAdd(a.steps_done, 1);
 }
 
@@ -1475,11 +1682,44 @@ void launch_walk_pack(const WalkArgs& a, int f32, int damping, int block,
 void launch_walk(const WalkArgs& a, int f32, int damping, int block, int pack,
                  hipStream_t s) {
   if (a.n_waves <= 0) return;
-  // the frozen-decision walk has a one-particle-per-lane form only
-  if (pack == 2 && !a.mac_nodes)
+  if (a.split > 1) {
+    // pieces: 64-thread workgroups, one particle per lane (the pieces of
+    // one traversal on consecutive workgroups, spread over the XCDs)
+    const int blocks = (int)(((long long)a.n_waves * a.split * kWave + 63) / 64);
+    if (f32 && a.nodes32) {
+      if (damping)
+        walk_kernel<true, true, false, 64, 1, true, true><<<blocks, 64, 0, s>>>(a);
+      else
+        walk_kernel<true, false, false, 64, 1, true, true><<<blocks, 64, 0, s>>>(a);
+    } else if (f32) {
+      if (damping)
+        walk_kernel<true, true, false, 64, 1, false, true><<<blocks, 64, 0, s>>>(a);
+      else
+        walk_kernel<true, false, false, 64, 1, false, true><<<blocks, 64, 0, s>>>(a);
+    } else if (a.mac_nodes) {
+      if (damping)
+        walk_kernel<false, true, true, 64, 1, false, true><<<blocks, 64, 0, s>>>(a);
+      else
+        walk_kernel<false, false, true, 64, 1, false, true><<<blocks, 64, 0, s>>>(a);
+    } else {
+      if (damping)
+        walk_kernel<false, true, false, 64, 1, false, true><<<blocks, 64, 0, s>>>(a);
+      else
+        walk_kernel<false, false, false, 64, 1, false, true><<<blocks, 64, 0, s>>>(a);
+    }
+  } else if (pack == 2 && !a.mac_nodes) {
+    // the frozen-decision walk has a one-particle-per-lane form only
     launch_walk_pack<2>(a, f32, damping, block, s);
-  else
+  } else {
     launch_walk_pack<1>(a, f32, damping, block, s);
+  }
+  if (a.split > 1) {
+    const int blocks = (int)(((long long)a.n_waves * kWave + 255) / 256);
+    if (f32)
+      walk_combine_kernel<true><<<blocks, 256, 0, s>>>(a);
+    else
+      walk_combine_kernel<false><<<blocks, 256, 0, s>>>(a);
+  }
 }
 
 void launch_tile_order(unsigned* tile_cost, int n_tiles, int groups, int* order,
