@@ -72,16 +72,85 @@ __device__ __forceinline__ double rsqrt_f64_a(double d2) {
   return fma(0.5 * r, e, r);
 }
 
+// Where a piece of the reverse walk starts (split > 1): the walk state at
+// node b of the whole traversal, from a descent from the root with this
+// walk's opening tests (lanes that accept an ancestor wait at its skip,
+// the rest take b), as the forward walk's pieces start.  With no lane
+// taking b the piece resumes at the stack top, or has nothing to do.
+template <bool MAC32>
+__device__ __forceinline__ void adj_enter(ConstNodes nodes, int root, int b, int e, double x,
+                                          double y, double z, double theta, double theta2,
+                                          unsigned long long& A, int& n, int& top,
+                                          int& st_pos, int& st_lo, int& st_hi) {
+  const int lane = lane_id();
+  int X = root;
+  while (X < b) {
+    const int leaf = nodes[X].leaf;
+    const int skip = nodes[X].skip;
+    bool open;
+    if (leaf == -2) {
+      open = true;
+    } else {
+      const double dx = nodes[X].cx - x, dy = nodes[X].cy - y, dz = nodes[X].cz - z;
+      if (MAC32) {
+        const float fdx = (float)dx, fdy = (float)dy, fdz = (float)dz;
+        open = fmaf(fdx, fdx, fmaf(fdy, fdy, fdz * fdz)) < nodes[X].r2open32;
+      } else {
+        open = mac_open_f64_a(sq_sum_ref_a(dx, dy, dz), nodes[X].side, theta, theta2);
+      }
+    }
+    const bool in = (A >> lane) & 1ull;
+    const unsigned long long O = ballot(open && in);
+    const unsigned long long C = A & ~O;
+    if (C) {
+      if (top > 0 && __builtin_amdgcn_readlane(st_pos, top - 1) == skip) {
+        const int slot = top - 1;
+        st_lo = lane == slot ? (st_lo | (int)(unsigned)C) : st_lo;
+        st_hi = lane == slot ? (st_hi | (int)(unsigned)(C >> 32)) : st_hi;
+      } else {
+        const int slot = top;
+        st_pos = lane == slot ? skip : st_pos;
+        st_lo = lane == slot ? (int)(unsigned)C : st_lo;
+        st_hi = lane == slot ? (int)(unsigned)(C >> 32) : st_hi;
+        if (top < kWave - 1) top++;
+      }
+    }
+    A = O;
+    int c = X + 1;  // the child of X whose subtree holds b
+    for (;;) {
+      const int sk = max(nodes[c].skip, c + 1);
+      if (sk > b) break;
+      c = sk;
+    }
+    X = c;
+  }
+  n = b;
+  if (!A) {
+    const int p = top > 0 ? __builtin_amdgcn_readlane(st_pos, top - 1) : e;
+    if (p < e) {
+      n = p;
+      const unsigned lo = (unsigned)__builtin_amdgcn_readlane(st_lo, top - 1);
+      const unsigned hi = (unsigned)__builtin_amdgcn_readlane(st_hi, top - 1);
+      A = ((unsigned long long)hi << 32) | lo;
+      top--;
+    } else {
+      n = e;
+    }
+  }
+}
+
 // Eight wavefronts per SIMD (at most 64 VGPRs and 80 SGPRs each on gfx950)
 // where that needs no spills: without damping (54-56 VGPRs; 74-76 before
-// the bound); the damped variants keep their registers.
-template <bool MAC32, bool DAMP>
+// the bound); the damped variants keep their registers.  SPLIT: wavefront
+// q walks piece q % split of traversal q / split.
+template <bool MAC32, bool DAMP, bool SPLIT>
 __global__ void __launch_bounds__(kAdjBlock) __attribute__((amdgpu_waves_per_eu(DAMP ? 1 : 8, 8)))
 adj_walk_kernel(AdjArgs a) {
   if (*a.flags) return;
   const int lane = lane_id();
-  const int wave = __builtin_amdgcn_readfirstlane(
-      (int)((blockIdx.x * kAdjBlock + threadIdx.x) >> 6));
+  const int q = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * kAdjBlock + threadIdx.x) >> 6));
+  const int wave = SPLIT ? q / a.split : q;
+  const int piece = SPLIT ? q - wave * a.split : 0;
   if (wave >= a.n_waves) return;
   const int s = a.nsys > 1 ? __builtin_amdgcn_readfirstlane(
                                  sys_of_wave(wave, a.wave_start, a.nsys))
@@ -100,10 +169,17 @@ adj_walk_kernel(AdjArgs a) {
   ConstNodes nodes = (ConstNodes)a.nodes;
   unsigned long long A = ballot(valid);
   int n = a.node_start[s];
-  const int nend = a.node_start[s + 1];
+  int nend = a.node_start[s + 1];
   const double theta = a.theta, theta2 = theta * theta;
   int st_pos = 0, st_lo = 0, st_hi = 0, top = 0;
   double gx = 0, gy = 0, gz = 0;
+  if constexpr (SPLIT) {
+    const long long len = nend - n;
+    const int b = n + (int)(len * piece / a.split);
+    const int e = n + (int)(len * (piece + 1) / a.split);
+    adj_enter<MAC32>(nodes, n, b, e, x, y, z, theta, theta2, A, n, top, st_pos, st_lo, st_hi);
+    nend = e;
+  }
 
   while (n < nend) {
     const int leaf = nodes[n].leaf;
@@ -194,10 +270,33 @@ adj_walk_kernel(AdjArgs a) {
     }
   }
   if (valid) {
-    a.gx[i] = gx;
-    a.gy[i] = gy;
-    a.gz[i] = gz;
+    if constexpr (SPLIT) {
+      const size_t np = (size_t)a.n;
+      a.part[((size_t)piece * 3 + 0) * np + i] = gx;
+      a.part[((size_t)piece * 3 + 1) * np + i] = gy;
+      a.part[((size_t)piece * 3 + 2) * np + i] = gz;
+    } else {
+      a.gx[i] = gx;
+      a.gy[i] = gy;
+      a.gz[i] = gz;
+    }
   }
+}
+
+// The pieces' position adjoints added in piece order.
+__global__ void __launch_bounds__(kAdjBlock) adj_combine_kernel(AdjArgs a) {
+  const int i = blockIdx.x * kAdjBlock + threadIdx.x;
+  if (i >= a.n || *a.flags) return;
+  const size_t np = (size_t)a.n;
+  double gx = 0, gy = 0, gz = 0;
+  for (int q = 0; q < a.split; q++) {
+    gx = gx + a.part[((size_t)q * 3 + 0) * np + i];
+    gy = gy + a.part[((size_t)q * 3 + 1) * np + i];
+    gz = gz + a.part[((size_t)q * 3 + 2) * np + i];
+  }
+  a.gx[i] = gx;
+  a.gy[i] = gy;
+  a.gz[i] = gz;
 }
 
 // Leaf-to-root sums of the cell adjoints, per particle (slot order).
@@ -292,17 +391,33 @@ void launch_scatter_comp(const double* src, double* dst, const unsigned* id,
 
 void launch_adj_walk(const AdjArgs& a, int mac32, int damping, hipStream_t s) {
   if (a.n_waves <= 0) return;
+  if (a.split > 1) {
+    const int blocks = (int)(((long long)a.n_waves * a.split * kWave + kAdjBlock - 1) / kAdjBlock);
+    if (mac32) {
+      if (damping)
+        adj_walk_kernel<true, true, true><<<blocks, kAdjBlock, 0, s>>>(a);
+      else
+        adj_walk_kernel<true, false, true><<<blocks, kAdjBlock, 0, s>>>(a);
+    } else {
+      if (damping)
+        adj_walk_kernel<false, true, true><<<blocks, kAdjBlock, 0, s>>>(a);
+      else
+        adj_walk_kernel<false, false, true><<<blocks, kAdjBlock, 0, s>>>(a);
+    }
+    adj_combine_kernel<<<(a.n + kAdjBlock - 1) / kAdjBlock, kAdjBlock, 0, s>>>(a);
+    return;
+  }
   const int blocks = (a.n_waves * kWave + kAdjBlock - 1) / kAdjBlock;
   if (mac32) {
     if (damping)
-      adj_walk_kernel<true, true><<<blocks, kAdjBlock, 0, s>>>(a);
+      adj_walk_kernel<true, true, false><<<blocks, kAdjBlock, 0, s>>>(a);
     else
-      adj_walk_kernel<true, false><<<blocks, kAdjBlock, 0, s>>>(a);
+      adj_walk_kernel<true, false, false><<<blocks, kAdjBlock, 0, s>>>(a);
   } else {
     if (damping)
-      adj_walk_kernel<false, true><<<blocks, kAdjBlock, 0, s>>>(a);
+      adj_walk_kernel<false, true, false><<<blocks, kAdjBlock, 0, s>>>(a);
     else
-      adj_walk_kernel<false, false><<<blocks, kAdjBlock, 0, s>>>(a);
+      adj_walk_kernel<false, false, false><<<blocks, kAdjBlock, 0, s>>>(a);
   }
 }
 

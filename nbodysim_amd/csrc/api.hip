@@ -2514,6 +2514,11 @@ int adj_reverse_step(nbs_ctx* c, const double* x3) {
   a.gnode = c->node_g;
   a.flags = c->flags;
   a.theta = c->p.theta;
+  if (c->walk_split > 1) {
+    a.split = c->walk_split;
+    a.part = c->walk_part;
+    a.n = n;
+  }
   launch_adj_walk(a, c->cfg.force_precision == NBS_FORCE_F32, c->p.damping, s);
   launch_adj_propagate(a, c->bb.offsets, c->aux.parent, st.m, c->s_gm, n, s);
   launch_scatter_comp(c->s_gx, c->g_x, st.id, n, 3, 1, s);

@@ -310,6 +310,12 @@ struct AdjArgs {
   double* gnode;  // per node [4]: adjoint of the COM (x, y, z), of the mass
   const int* flags;
   double theta;
+  // split > 1: each traversal as `split` pieces over equal node ranges (as
+  // the forward walk), partial position adjoints part [split][3][n] added
+  // in piece order into gx, gy, gz
+  int split;
+  double* part;
+  int n;
 };
 
 void launch_adj_walk(const AdjArgs& a, int mac32, int damping, hipStream_t s);
