@@ -278,6 +278,8 @@ struct nbs_ctx {
   double *adj_m = nullptr, *seg_x = nullptr, *base_x = nullptr;
   double *fin_x = nullptr, *fin_v = nullptr;
   double *g_x = nullptr, *g_v = nullptr, *g_m = nullptr, *g_u = nullptr;
+  // g_x, g_v, g_m at the start of the segment being reversed
+  double *b_x = nullptr, *b_v = nullptr, *b_m = nullptr;
   double *s_u = nullptr, *s_gx = nullptr, *s_gm = nullptr;
   double* node_g = nullptr;
   int node_g_cap = 0;
@@ -2481,13 +2483,16 @@ int build_checked(nbs_ctx* c) {
 }
 
 // Reverse step at positions x3 (caller order): g_x += dF/dx^T u,
-// g_m += dF/dm^T u, with u = dt * (g_v + dt g_x) and g_v updated.
-int adj_reverse_step(nbs_ctx* c, const double* x3) {
+// g_m += dF/dm^T u, with u = dt * (g_v + dt g_x) and g_v updated.  checked:
+// the build's flags are read (and the node arrays grown) before the walk;
+// otherwise the caller reads them after a stretch of steps.
+int adj_reverse_step(nbs_ctx* c, const double* x3, bool checked) {
   const int n = c->n;
   hipStream_t s = c->stream;
   launch_adj_kick(c->g_v, c->g_x, c->g_u, c->adj_dt, n, s);
   load_state(c, x3, nullptr, c->adj_m);
-  int r = build_checked(c);
+  int r = checked ? build_checked(c) : NBS_OK;
+  if (!checked) enqueue_build(c);
   if (r) return r;
   r = ensure_node_adjoints(c);
   if (r) return r;
@@ -2536,6 +2541,43 @@ void to_aos(const std::vector<double>& soa, double* aos, size_t n) {
     for (int k = 0; k < 3; k++) aos[3 * i + k] = soa[k * n + i];
 }
 
+// Steps t0 .. t1 - 1 of an adjoint pass from the current state, save(t)
+// enqueued before step t: enqueued together with one read of the flags at
+// the end instead of one per step.  A node overflow at step t leaves the
+// state as it was before step t (every later kernel of the stretch is a
+// no-op; the builds only permute it), so the node arrays grow and the
+// stretch resumes at t, saves included.
+template <class Save>
+int adj_run_steps(nbs_ctx* c, int t0, int t1, Save save) {
+  int r = ensure_nodes(c);
+  if (r) return r;
+  for (int attempt = 0; attempt < 8; attempt++) {
+    if (t0 >= t1) return NBS_OK;
+    reset_flags(c);
+    for (int t = t0; t < t1; t++) {
+      save(t);
+      enqueue_build(c);
+      enqueue_step_tail(c);
+    }
+    int flags = 0, done = 0;
+    r = read_flags(c, &flags, &done);
+    if (r) return r;
+    if (!flags) {
+      c->tree_valid = true;
+      c->acc_valid = true;
+      return NBS_OK;
+    }
+    c->bbox_valid = false;
+    c->tree_valid = false;
+    r = fatal_flags(c, flags);
+    if (r) return r;
+    t0 += done;
+    r = alloc_nodes(c, c->node_cap * 2);
+    if (r) return r;
+  }
+  return fail(c, NBS_ERR_OVERFLOW, "node capacity overflow after retries");
+}
+
 // dL/dx_0, dL/dv_0, dL/dm from dL/dx_T, dL/dv_T: host [n][3] / [n] arrays,
 // or (device) device arrays of this library's runtime.
 int adjoint_backward(nbs_ctx* c, const double* gx_T, const double* gv_T, double* gx0,
@@ -2577,16 +2619,36 @@ int adjoint_backward(nbs_ctx* c, const double* gx_T, const double* gv_T, double*
     // recompute the segment's positions from its checkpoint (bit-exact:
     // the build sorts canonically, so every step repeats the forward)
     load_state(c, c->ck_x[seg], c->ck_v[seg], c->adj_m);
-    for (int t = t0; t < t1; t++) {
+    int r = adj_run_steps(c, t0, t1 - 1, [&](int t) {
       save_state(c, c->seg_x + 3 * n * (t - t0), nullptr);
-      if (t + 1 < t1) {
-        int r = run(c, 1);
+    });
+    if (r) return r;
+    save_state(c, c->seg_x + 3 * n * (t1 - 1 - t0), nullptr);
+    // the reverse steps, their flags read once: the trees are the
+    // forward's, which the node arrays already hold, so an overflow is not
+    // expected; if one comes, the segment is redone from saved adjoints
+    // with a checked build per step
+    HIPCHK(c, hipMemcpyAsync(c->b_x, c->g_x, 3 * n * 8, hipMemcpyDeviceToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(c->b_v, c->g_v, 3 * n * 8, hipMemcpyDeviceToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(c->b_m, c->g_m, n * 8, hipMemcpyDeviceToDevice, s));
+    reset_flags(c);
+    for (int t = t1 - 1; t >= t0; t--) {
+      r = adj_reverse_step(c, c->seg_x + 3 * n * (t - t0), false);
+      if (r) return r;
+    }
+    int flags = 0, done = 0;
+    r = read_flags(c, &flags, &done);
+    if (r) return r;
+    if (flags) {
+      r = fatal_flags(c, flags);
+      if (r) return r;
+      HIPCHK(c, hipMemcpyAsync(c->g_x, c->b_x, 3 * n * 8, hipMemcpyDeviceToDevice, s));
+      HIPCHK(c, hipMemcpyAsync(c->g_v, c->b_v, 3 * n * 8, hipMemcpyDeviceToDevice, s));
+      HIPCHK(c, hipMemcpyAsync(c->g_m, c->b_m, n * 8, hipMemcpyDeviceToDevice, s));
+      for (int t = t1 - 1; t >= t0; t--) {
+        r = adj_reverse_step(c, c->seg_x + 3 * n * (t - t0), true);
         if (r) return r;
       }
-    }
-    for (int t = t1 - 1; t >= t0; t--) {
-      int r = adj_reverse_step(c, c->seg_x + 3 * n * (t - t0));
-      if (r) return r;
     }
   }
   if (device) {
@@ -2645,6 +2707,9 @@ int nbs_adjoint_forward(nbs_ctx* c, int nsteps, int every, int record_all) {
   c->g_v = dalloc<double>(o, 3 * (size_t)n, &e);
   c->g_u = dalloc<double>(o, 3 * (size_t)n, &e);
   c->g_m = dalloc<double>(o, n, &e);
+  c->b_x = dalloc<double>(o, 3 * (size_t)n, &e);
+  c->b_v = dalloc<double>(o, 3 * (size_t)n, &e);
+  c->b_m = dalloc<double>(o, n, &e);
   c->s_u = dalloc<double>(o, 3 * (size_t)n, &e);
   c->s_gx = dalloc<double>(o, 3 * (size_t)n, &e);
   c->s_gm = dalloc<double>(o, n, &e);
@@ -2660,10 +2725,11 @@ int nbs_adjoint_forward(nbs_ctx* c, int nsteps, int every, int record_all) {
     State& st = c->st[c->cur];
     launch_scatter_comp(st.m, c->adj_m, st.id, n, 1, 0, c->stream);
   }
-  for (int t = 0; t < nsteps; t++) {
-    if (t % every == 0) save_state(c, c->ck_x[t / every], c->ck_v[t / every]);
-    if (record_all) save_state(c, c->base_x + 3 * (size_t)n * t, nullptr);
-    int r = run(c, 1);
+  {
+    const int r = adj_run_steps(c, 0, nsteps, [&](int t) {
+      if (t % every == 0) save_state(c, c->ck_x[t / every], c->ck_v[t / every]);
+      if (record_all) save_state(c, c->base_x + 3 * (size_t)n * t, nullptr);
+    });
     if (r) return r;
   }
   save_state(c, c->fin_x, c->fin_v);
