@@ -36,6 +36,8 @@ HBM_PEAK_GBS = 8000.0
 FLOP_PER_INTERACTION = 20
 # interaction counting of the timed walks (--counting)
 COUNTING = "total"
+# pieces per walk traversal (--walk-split; 0: the library's choice)
+WALK_SPLIT = 0
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "walk_traffic_sphere1M.md")
 
 
@@ -79,6 +81,7 @@ def time_path(sim, pos, vel, mass, precision, steps, warmup, shape, graph, order
         kw["theta"] = theta
     sim.configure(**kw)
     sim.set_counting(COUNTING)
+    sim.set_walk_split(WALK_SPLIT)
     sim.set_state(pos, vel, mass)
     if offsets is not None:
         sim.set_systems(offsets)
@@ -404,6 +407,7 @@ def config_adjoint(args):
            "data": "synthetic (seeded gen_type sphere)"}
     # timing: FP32 forces, FP64 state and adjoint
     with nbodysim_amd.Simulation(theta=0.5, dt=0.0333, precision="f32") as sim:
+        sim.set_walk_split(WALK_SPLIT)
         sim.set_state(pos, vel, mass)
         t0 = time.perf_counter()
         sim.adjoint_forward(args.steps5, args.every5)
@@ -723,6 +727,8 @@ def main():
     ap.add_argument("--counting", default="total", choices=("total", "particles", "none"),
                     help="interaction counting of the timed walks (nbs_set_counting); the "
                          "interaction figures need total or particles")
+    ap.add_argument("--walk-split", type=int, default=0, choices=(0, 1, 2, 4, 8, 16, 32),
+                    help="pieces per walk traversal (nbs_set_walk_split; 0: by wavefront count)")
     ap.add_argument("--eager", action="store_true",
                     help="timed steps launched one kernel at a time, not by graph replay")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
@@ -751,8 +757,9 @@ def main():
     ap.add_argument("--adjoint-timing-only", action="store_true",
                     help="--config 5: the forward and backward timing only (for profiling)")
     args = ap.parse_args()
-    global COUNTING
+    global COUNTING, WALK_SPLIT
     COUNTING = args.counting
+    WALK_SPLIT = args.walk_split
     if args.fp64_steps is None:
         args.fp64_steps = args.steps
     if args.gpus != 1:

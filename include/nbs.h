@@ -273,7 +273,8 @@ int nbs_set_walk_code(nbs_ctx* ctx, int code);
  * = by the wavefront count, so that a system too small to fill the GPU's
  * wavefront slots (fewer than ~4096 64-particle wavefronts) walks each
  * traversal as 2, 4 or 8 pieces over equal ranges of its nodes, whose
- * partial sums are added in piece order; 1, 2, 4 or 8 = that many.  Every
+ * partial sums are added in piece order; a power of two up to 32 = that
+ * many (more pieces than the GPU has room for only add work).  Every
  * particle's accepted cells and interaction count are those of the whole
  * traversal; the forces differ from it only by the summation order, and a
  * given state and setting always give the same bits.  nbs_walk_split

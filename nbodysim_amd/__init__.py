@@ -297,7 +297,8 @@ class Simulation:
 
     def set_walk_split(self, pieces=0):
         """Pieces per walk traversal: 0 (default) by the wavefront count
-        (small systems split each traversal to fill the GPU), or 1, 2, 4, 8.
+        (small systems split each traversal to fill the GPU), or a power of
+        two up to 32.
         Interaction counts never change; forces differ only by the order of
         the pieces' sums."""
         self._check(self._L.nbs_set_walk_split(self._h, int(pieces)))

@@ -525,6 +525,7 @@ int alloc_nodes(nbs_ctx* c, int cap) {
 // the fewest that give about kWalkWaveSlots wavefronts.  Fixed by the
 // wavefront count, so a given state always takes the same pieces.
 constexpr long long kWalkWaveSlots = 8192;
+constexpr int kWalkMaxPieces = 32;
 int walk_split_for(const nbs_ctx* c) {
   if (c->walk_split_req > 0) return c->walk_split_req;
   int k = 1;
@@ -1977,8 +1978,9 @@ int nbs_set_walk_shape(nbs_ctx* c, int block, int groups, int pack) {
 
 int nbs_set_walk_split(nbs_ctx* c, int pieces) {
   if (!c) return NBS_ERR_ARG;
-  if (pieces != 0 && pieces != 1 && pieces != 2 && pieces != 4 && pieces != 8)
-    return fail(c, NBS_ERR_ARG, "walk pieces must be 0 (by wavefront count), 1, 2, 4 or 8");
+  if (pieces < 0 || pieces > kWalkMaxPieces || (pieces & (pieces - 1)))
+    return fail(c, NBS_ERR_ARG, "walk pieces must be 0 (by wavefront count) or a power of two "
+                                "up to " + std::to_string(kWalkMaxPieces));
   c->walk_split_req = pieces;
   if (!c->has_state) return NBS_OK;
   return alloc_walk_split(c);
