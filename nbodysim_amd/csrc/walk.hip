@@ -22,10 +22,13 @@
 // Precision modes:
 //   FP64: the MAC decision is bit-exact with the reference's
 //     side / sqrt(dx*dx + dy*dy + dz*dz) > theta (correctly rounded sqrt and
-//     division, no contraction); a squared test with a 1e-9 relative margin
-//     settles all but the near-ties without the sqrt and division.  Forces:
-//     1/r via v_rsq_f64 and one Newton step (the estimate's ~2^-26 relative
-//     error squares to ~1e-15).
+//     division, no contraction); a margin test (the generic loop: squared,
+//     1e-9 relative; the hand-written loop: f32(d^2) against the record's
+//     f32 radius widened by 8 ulps) settles all but the near-ties without
+//     the sqrt and division.  Forces: G m / |d|^3 from the v_rsq_f64
+//     estimate with a first-order correction of its cube (inv_cube_gm);
+//     the damped law takes 1/r with one Newton step (rsqrt_f64).  Either
+//     way the estimate's ~2^-26 relative error enters squared, ~1e-15.
 //   FP32: com - x is formed in FP64 (full rate on CDNA4) and rounded to
 //     FP32, which keeps near-field differences exact to FP32 precision
 //     (node-relative coordinates); d^2, the MAC (d^2 < (side/theta)^2) and

@@ -36,8 +36,12 @@
 //     an exact-integer restatement of generate_canonical, and a g++
 //     argument-evaluation-order probe (right to left: z is drawn first).
 //     The reference's generate_particle itself lives in main.cpp, which
-//     does not compile here (it needs FreeImage.h), so the end-to-end IC
-//     draw sequence is "parity unpinned" beyond those three checks.
+//     does not compile here (it needs FreeImage.h); its sphere and shell
+//     branch is pinned from the raw engine words by draw accounting (8n /
+//     6n words in FP64, 4n / 3n in FP32), by the geometry each draw fixes
+//     (radius, polar cosine, azimuth), by v = omega x p, and by planted
+//     mistakes (draws exchanged, sin/cos exchanged, p x omega) that those
+//     checks catch (tests/test_oracle_pins.py).
 
 #include <algorithm>
 #include <atomic>
