@@ -194,11 +194,12 @@ __device__ __forceinline__ void interact(const NodeRegs& nd, double dx, double d
 // instruction stream.  Per visit: 10 VALU instructions for d = com - x
 // (FP64, rounded to FP32), d^2 and the opening test, 8 for the force on
 // the interacting lanes, and scalar code for everything else; the common
-// case (a cell every lane accepts) takes 10 scalar ALU instructions and 3
-// branches on the 64-byte records (WalkNode), 15 on the 32-byte ones
-// (WalkNode32), which decode the skip link and the opening radius from a
-// packed link word but put twice as many records in a cache line and in
-// the L2.  The walk waits on each next record (the scalar cache misses most
+// case (a cell every lane accepts) takes 9 scalar ALU instructions (11 with
+// the wave's interaction count) and 3 branches on the 64-byte records
+// (WalkNode), whose buckets need no test of their own, 5 more and a branch
+// on the 32-byte ones (WalkNode32), which decode the skip link and the
+// opening radius from a packed link word but put twice as many records in
+// a cache line and in the L2.  The walk waits on each next record (the scalar cache misses most
 // of them), so the next record is requested as early as it is known: for
 // a cell after the opening test, before the force arithmetic; for a leaf
 // (next = this + 1) as soon as the position differences have read the
@@ -283,8 +284,15 @@ __device__ __forceinline__ void interact(const NodeRegs& nd, double dx, double d
   "s_lshl_b32 s57, s58, 24\n\t"                                       \
   "s_and_b32 s57, s57, %[rmask]\n\t"                                  \
   "s_sub_u32 s57, %[rbase], s57\n\t"
-#define NBS_REC_BUCKET_W64 "s_cmp_eq_u32 s55, -2\n\t"
-#define NBS_REC_BUCKET_W32 "s_cmp_eq_u32 s58, 42\n\t"
+// A depth-42 bucket is opened by every lane that takes it.  The 64-byte
+// record of a bucket carries an infinite opening radius, so the opening
+// test itself opens it for every lane with a finite d^2 and no bucket test
+// is needed; the 32-byte record derives its radius from the depth and tests
+// the depth.
+#define NBS_REC_BUCKET_W64 ""
+#define NBS_REC_BUCKET_W32                                            \
+  "s_cmp_eq_u32 s58, 42\n\t"                                         \
+  "s_cbranch_scc1 L_bucket%=\n\t"
 #define NBS_REC_SLOT_W64 "s_sub_u32 %[t], s55, %[pbeg]\n\t"
 #define NBS_REC_SLOT_W32                                              \
   "s_and_b32 %[t], s54, 0x7fffffff\n\t"                               \
@@ -352,7 +360,6 @@ __device__ __forceinline__ void interact(const NodeRegs& nd, double dx, double d
       "v_fmac_f32 %[d2], %[fy], %[fy]\n\t" \
       "v_fmac_f32 %[d2], %[fx], %[fx]\n\t" \
       NBS_REC_BUCKET_##FMT \
-      "s_cbranch_scc1 L_bucket%=\n\t" \
       /* cell: open where d^2 < (side/theta)^2 */ \
       "v_cmp_gt_f32 vcc, " NBS_REC_R2_##FMT ", %[d2]\n\t" \
       "s_and_b64 s[42:43], vcc, s[46:47]\n\t" \
