@@ -370,22 +370,25 @@ __device__ __forceinline__ void interact(const NodeRegs& nd, double dx, double d
       "s_max_i32 %[n], %[n], " NBS_REC_SKIP_##FMT "\n\t" \
       NBS_WALK_TAIL(FORCE, "s[46:47]", CNT, FMT) \
       "L_opened%=:\n\t" \
-      /* I = A & ~O interact and wait at skip; O descend (A = O); with */ \
-      /* no lane in I the force is skipped */ \
+      /* I = A & ~O interact and wait at skip; O descend (A = O), so the */ \
+      /* next record is n + 1: requested at once, the skip link kept in */ \
+      /* sk for the stack; with no lane in I the force is skipped */ \
       "s_add_i32 %[n], %[n], 1\n\t" \
+      "s_mov_b32 %[sk], " NBS_REC_SKIP_##FMT "\n\t" \
+      NBS_REC_LOAD_##FMT \
       "s_andn2_b64 s[44:45], s[46:47], s[42:43]\n\t" \
       "s_mov_b64 s[46:47], s[42:43]\n\t" \
-      "s_cbranch_scc0 L_noint%=\n\t" \
-      "s_cmp_eq_u32 %[tpos], " NBS_REC_SKIP_##FMT "\n\t" \
+      "s_cbranch_scc0 L_next%=\n\t" \
+      "s_cmp_eq_u32 %[tpos], %[sk]\n\t" \
       "s_cbranch_scc1 L_merge%=\n\t" \
       "s_cmp_gt_u32 %[top], 62\n\t" \
       "s_cbranch_scc1 L_ovf%=\n\t" \
       "s_mov_b32 m0, %[top]\n\t" \
-      "v_writelane_b32 %[sp], " NBS_REC_SKIP_##FMT ", m0\n\t" \
+      "v_writelane_b32 %[sp], %[sk], m0\n\t" \
       "v_writelane_b32 %[slo], s44, m0\n\t" \
       "v_writelane_b32 %[shi], s45, m0\n\t" \
       "s_add_u32 %[top], %[top], 1\n\t" \
-      "s_mov_b32 %[tpos], " NBS_REC_SKIP_##FMT "\n\t" \
+      "s_mov_b32 %[tpos], %[sk]\n\t" \
       "s_min_u32 %[lim], %[tpos], %[nend]\n\t" \
       "s_branch L_int%=\n" \
       "L_merge%=:\n\t" \
@@ -409,7 +412,7 @@ __device__ __forceinline__ void interact(const NodeRegs& nd, double dx, double d
       "s_cbranch_scc1 L_loop%=\n\t" \
       "s_branch L_slow%=\n" \
       "L_int%=:\n\t" \
-      NBS_WALK_TAIL(FORCE, "s[44:45]", CNT, FMT) \
+      FORCE("s[44:45]", CNT) \
       "L_leaf%=:\n\t" \
       /* leaf: the next entry is known at once, so its record is requested */ \
       /* as soon as the three differences have read this one's position; */ \
@@ -463,7 +466,7 @@ __device__ __forceinline__ void interact(const NodeRegs& nd, double dx, double d
       [top] "+s"(top), [tpos] "+s"(tpos), [lim] "=&s"(lim), [t] "=&s"(t), [gm] "=&s"(gm),   \
       [m0save] "=&s"(m0save), [sp] "+v"(sp), [slo] "+v"(slo), [shi] "+v"(shi),             \
       [dx] "=&v"(dx), [dy] "=&v"(dy), [dz] "=&v"(dz), [fx] "=&v"(fx), [fy] "=&v"(fy),       \
-      [fz] "=&v"(fz), [d2] "=&v"(d2), [tv] "=&v"(tv)
+      [fz] "=&v"(fz), [d2] "=&v"(d2), [tv] "=&v"(tv), [sk] "=&s"(sk)
 #define NBS_F32_DAMPED_OUTS                                                               \
   NBS_F32_OUTS, [te] "=&v"(te), [tn] "=&v"(tn), [td] "=&v"(td), [q0] "=&v"(q0),          \
       [q1] "=&v"(q1), [q2] "=&v"(q2), [q3] "=&v"(q3), [q4] "=&v"(q4)
@@ -509,7 +512,7 @@ __device__ __forceinline__ void walk_loop_f32(Nodes nodes, int n, int nend, int 
                                               double z, float& ax, float& ay, float& az,
                                               int& cnt, int& tot, int& overflow, unsigned rbase,
                                               unsigned rmask) {
-  int lim, t, gm, m0save, tc;
+  int lim, t, gm, m0save, tc, sk;
   double dx, dy, dz;
   float fx, fy, fz, d2, tv;
   if constexpr (W32) {
@@ -527,7 +530,7 @@ __device__ __forceinline__ void walk_loop_f32_damped(Nodes nodes, int n, int nen
                                                      float& az, int& cnt, int& tot,
                                                      int& overflow, unsigned rbase,
                                                      unsigned rmask) {
-  int lim, t, gm, m0save, tc;
+  int lim, t, gm, m0save, tc, sk;
   double dx, dy, dz;
   float fx, fy, fz, d2, tv, te, tn, td, q0, q1, q2, q3, q4;
   if constexpr (W32) {
