@@ -624,11 +624,11 @@ __device__ __forceinline__ void walk_loop_f32_damped(Nodes nodes, int n, int nen
       "v_add_f64 %[dz], s[52:53], -%[z]\n\t" \
       "s_mov_b64 %[gm], s[56:57]\n\t" \
       NBS_WALK64_D2 \
-      "s_cmp_eq_u32 s55, -2\n\t" \
-      "s_cbranch_scc1 L_bucket%=\n\t" \
       /* cell: open where f32(d^2) < r2 - 8 ulp, accept where it is */ \
       /* > r2 + 8 ulp (s60, s62: the record's unused gm32 and mass); a */ \
-      /* taking lane in neither (or NaN) needs the exact test */ \
+      /* taking lane in neither (or NaN) needs the exact test.  A bucket */ \
+      /* (r2 infinite: its lower bound the largest float, its upper one */ \
+      /* a NaN) is opened here by every lane with a finite d^2 */ \
       "v_cvt_f32_f64 %[f2], %[d2]\n\t" \
       "s_sub_u32 s60, s61, 8\n\t" \
       "s_add_u32 s62, s61, 8\n\t" \
@@ -671,8 +671,6 @@ __device__ __forceinline__ void walk_loop_f32_damped(Nodes nodes, int n, int nen
       "L_ovf%=:\n\t" \
       "s_mov_b32 %[ovf], 1\n\t" \
       "s_branch L_int%=\n" \
-      "L_bucket%=:\n\t" \
-      "s_add_i32 %[n], %[n], 1\n" \
       /* no lane interacts: the next record, no force */ \
       "L_noint%=:\n\t" \
       "s_lshl_b32 %[t], %[n], 6\n\t" \
