@@ -267,6 +267,13 @@ __device__ __forceinline__ void interact(const NodeRegs& nd, double dx, double d
 #define NBS_REC_LOAD_W32                                              \
   "s_lshl_b32 %[t], %[n], 5\n\t"                                      \
   "s_load_dwordx8 s[48:55], %[base], %[t]\n\t"
+// The same, the offset formed in sk (t holds a leaf's slot meanwhile).
+#define NBS_REC_LOADSK_W64                                            \
+  "s_lshl_b32 %[sk], %[n], 6\n\t"                                     \
+  "s_load_dwordx16 s[48:63], %[base], %[sk]\n\t"
+#define NBS_REC_LOADSK_W32                                            \
+  "s_lshl_b32 %[sk], %[n], 5\n\t"                                     \
+  "s_load_dwordx8 s[48:55], %[base], %[sk]\n\t"
 #define NBS_REC_LEAF_W64 "s_cmp_gt_i32 s55, -1\n\t"
 #define NBS_REC_LEAF_W32 "s_cmp_lt_i32 s54, 0\n\t"
 #define NBS_REC_GM_W64 "s60"
@@ -415,20 +422,21 @@ __device__ __forceinline__ void interact(const NodeRegs& nd, double dx, double d
       FORCE("s[44:45]", CNT) \
       "L_leaf%=:\n\t" \
       /* leaf: the next entry is known at once, so its record is requested */ \
-      /* as soon as the three differences have read this one's position; */ \
-      /* every lane that takes the leaf interacts, except its own */ \
-      /* particle's (bit slot - pbeg when that is below 64) */ \
+      /* as soon as the three differences have read this one's position */ \
+      /* and its slot is taken (the offset in sk); every lane that takes */ \
+      /* the leaf interacts, except its own particle's (bit slot - pbeg */ \
+      /* when that is below 64) */ \
       "v_add_f64 %[dx], s[48:49], -%[x]\n\t" \
       "v_add_f64 %[dy], s[50:51], -%[y]\n\t" \
       "v_add_f64 %[dz], s[52:53], -%[z]\n\t" \
       "s_mov_b32 %[gm], " NBS_REC_GM_##FMT "\n\t" \
       NBS_REC_SLOT_##FMT \
+      "s_add_i32 %[n], %[n], 1\n\t" \
+      NBS_REC_LOADSK_##FMT \
       "s_lshl_b64 s[38:39], 1, %[t]\n\t" \
       "s_cmp_lt_u32 %[t], 64\n\t" \
       "s_cselect_b64 s[38:39], s[38:39], 0\n\t" \
       "s_andn2_b64 s[44:45], s[46:47], s[38:39]\n\t" \
-      "s_add_i32 %[n], %[n], 1\n\t" \
-      NBS_REC_LOAD_##FMT \
       /* only its own particle's lane takes the leaf: no force */ \
       "s_cmp_eq_u64 s[44:45], 0\n\t" \
       "s_cbranch_scc1 L_next%=\n\t" \
@@ -548,6 +556,8 @@ __device__ __forceinline__ void walk_loop_f32_damped(Nodes nodes, int n, int nen
 #undef NBS_F32_CLOBBERS_W64
 #undef NBS_WALK_F32_LOOP
 #undef NBS_WALK_TAIL
+#undef NBS_REC_LOADSK_W64
+#undef NBS_REC_LOADSK_W32
 #undef NBS_WALK_FORCE_DAMPED
 #undef NBS_WALK_FORCE_PLAIN
 
