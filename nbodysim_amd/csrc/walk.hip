@@ -201,9 +201,11 @@ __device__ __forceinline__ void interact(const NodeRegs& nd, double dx, double d
 // opening radius from a packed link word but put twice as many records in
 // a cache line and in the L2.  The walk waits on each next record (the scalar cache misses most
 // of them), so the next record is requested as early as it is known: for
-// a cell after the opening test, before the force arithmetic; for a leaf
-// (next = this + 1) as soon as the position differences have read the
-// current record.
+// a cell every taking lane accepts, after the opening test, before the
+// force arithmetic; for an opened cell (next = this + 1) right after the
+// opening test, before the stack push; for a leaf (next = this + 1) as
+// soon as the position differences have read the current record and its
+// particle slot is taken.
 //
 // Fixed SGPRs (clobbered): the 64-byte record in s[48:63] (cx s[48:49],
 // cy s[50:51], cz s[52:53], skip s54, leaf s55, gm32 s60, r2open32 s61),
